@@ -1,0 +1,106 @@
+/*
+ * orc.h -- CPU oracle for the AMRClaw per-level patch update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (amrclaw_amd/) may include,
+ * link or call this code: it is the checker that tests/, the smoke entry point
+ * and the CPU-baseline leg of bench.py compare the HIP path against.
+ *
+ * The oracle is a plain-C restatement of the reference's FP64 Fortran, in the
+ * reference's own evaluation order and with the reference's array layout
+ * (AoS, component fastest: q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc)).  Every function
+ * cites the reference file:line it follows.  Build with -ffp-contract=off so
+ * that no multiply-add is fused (the reference is built the same way).
+ *
+ * Riemann solvers: the reference links rpn2/rpt2 from clawpack/riemann
+ * (commit 48a46d3, see tests/acoustics_2d_radial/regression_data/
+ * claw_git_status.txt), which is not part of the reference tree.  The solvers
+ * here restate the published algorithms (LeVeque, "Finite Volume Methods for
+ * Hyperbolic Problems", 2002: advection ch. 20, acoustics ch. 3/21, Euler Roe
+ * solver with Harten-Hyman entropy fix sec. 15.3) behind the reference's own
+ * rpn2/rpt2 calling convention (flux2.f:125,202,219; rpn2from1d.f:5-17).
+ */
+#ifndef AMR_ORACLE_H
+#define AMR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* solver ids */
+enum { ORC_ADVECTION = 1, ORC_ACOUSTICS = 2, ORC_EULER = 3 };
+
+/* Method parameters: the fields of amr_module.f90 that the hot path reads
+ * (method(7), mthlim(:), mwaves, mcapa, use_fwaves, cfl, cflv1) plus the
+ * physics parameters the reference passes through common /cparam/. */
+typedef struct {
+    int solver;        /* ORC_* */
+    int meqn, mwaves, maux, mbc;
+    int method[8];     /* 1-based like the Fortran: method[1..7] */
+    int mthlim[10];    /* 0-based: mthlim[mw-1] */
+    int mcapa;         /* 0 or the aux index (1-based) of the capacity */
+    int use_fwaves;
+    double cfl;        /* cfl_desired (amr_module cfl)   */
+    double cflv1;      /* cfl_max     (amr_module cflv1) */
+    /* /cparam/ */
+    double ubar, vbar;              /* advection            */
+    double rho, bulk, cc, zz;       /* acoustics            */
+    double gamma;                   /* Euler                */
+    int efix;                       /* Euler entropy fix    */
+} orc_params;
+
+/* ---- Riemann solvers, Fortran calling convention over a 1-D slice -------
+ * All arrays are Fortran column-major with first index = component and
+ * second index i in [1-mbc, maxm+mbc]; pointers address element (1, 1-mbc).
+ * Interface i uses qr(:,i-1) (left state) and ql(:,i) (right state). */
+void orc_rpn2(const orc_params* p, int ixy, int maxm, int meqn, int mwaves,
+              int maux, int mbc, int mx, const double* ql, const double* qr,
+              const double* auxl, const double* auxr, double* wave, double* s,
+              double* amdq, double* apdq);
+void orc_rpt2(const orc_params* p, int ixy, int imp, int maxm, int meqn,
+              int mwaves, int maux, int mbc, int mx, const double* ql,
+              const double* qr, const double* aux1, const double* aux2,
+              const double* aux3, const double* asdq, double* bmasdq,
+              double* bpasdq);
+
+/* Physical flux of one state in direction ixy (used only by closed-form
+ * checks of the Roe property; the reference has no such routine). */
+void orc_flux(const orc_params* p, int ixy, const double* q, double* f);
+
+/* ---- sweep ---------------------------------------------------------------
+ * limiter   : inlinelimiter.f:4-108
+ * flux2     : flux2.f:25-237
+ * step2     : step2.f90:7-169
+ * stepgrid  : stepgrid.f:24-234 (update, cfl, dtnew; b4step2/src2 are the
+ *             library no-ops)                                              */
+void orc_limiter(int maxm, int meqn, int mwaves, int mbc, int mx, double* wave,
+                 const double* s, const int* mthlim);
+void orc_flux2(const orc_params* p, int ixy, int maxm, int mx,
+               const double* q1d, const double* dtdx1d, const double* aux1,
+               const double* aux2, const double* aux3, double* faddm,
+               double* faddp, double* gaddm, double* gaddp, double* cfl1d,
+               double* wave, double* s, double* amdq, double* apdq,
+               double* cqxx, double* bmasdq, double* bpasdq);
+/* qold, aux, fm, fp, gm, gp: (meqn|maux, mx+2mbc, my+2mbc) */
+void orc_step2(const orc_params* p, int mx, int my, const double* qold,
+               const double* aux, double dx, double dy, double dt,
+               double* cflgrid, double* fm, double* fp, double* gm,
+               double* gp);
+/* q updated in place (interior only); fluxes returned; returns cflgrid and
+ * dtnew like stepgrid.f:206-212 (rinfinity = 10.e32 when cflgrid == 0). */
+void orc_stepgrid(const orc_params* p, int mx, int my, double* q,
+                  const double* aux, double dx, double dy, double dt,
+                  double* fm, double* fp, double* gm, double* gp,
+                  double* cflgrid, double* dtnew);
+
+/* amr_module.f90 declares these real(kind=8) parameters from default-real
+ * (single precision) literals 10.e32 and 10.e33, so their values are the
+ * float roundings widened to double. */
+#define ORC_RINFINITY ((double)10.e32f)
+#define ORC_NEEDS_TO_BE_SET ((double)10.e33f)
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,213 @@
+"""ctypes front end of the C oracle (liborc.so) and of the reference build
+(_ref/libamrref.so).  TEST INFRASTRUCTURE ONLY.
+
+Arrays follow the reference layout: q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc) in
+Fortran order, i.e. numpy arrays of shape (my+2mbc, mx+2mbc, meqn) in C order.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ADVECTION, ACOUSTICS, EULER = 1, 2, 3
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("solver", C.c_int), ("meqn", C.c_int), ("mwaves", C.c_int),
+        ("maux", C.c_int), ("mbc", C.c_int),
+        ("method", C.c_int * 8), ("mthlim", C.c_int * 10),
+        ("mcapa", C.c_int), ("use_fwaves", C.c_int),
+        ("cfl", C.c_double), ("cflv1", C.c_double),
+        ("ubar", C.c_double), ("vbar", C.c_double),
+        ("rho", C.c_double), ("bulk", C.c_double), ("cc", C.c_double),
+        ("zz", C.c_double), ("gamma", C.c_double), ("efix", C.c_int),
+    ]
+
+
+def make_params(solver, *, order=2, transverse=2, limiter=4, mwaves=None,
+                maux=0, mcapa=0, cfl=0.9, cflv1=1.0, ubar=0.5, vbar=1.0,
+                rho=1.0, bulk=4.0, gamma=1.4, efix=True, mbc=2):
+    """Build the params of one configuration.  limiter: mthlim code applied
+    to every wave (0 none, 1 minmod, 2 superbee, 3 van Leer, 4 MC, 5 BW)."""
+    p = OrcParams()
+    p.solver = solver
+    p.meqn, nw = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4)}[solver]
+    p.mwaves = nw if mwaves is None else mwaves
+    p.maux, p.mbc = maux, mbc
+    for k in range(8):
+        p.method[k] = 0
+    p.method[2], p.method[3] = order, transverse
+    lim = limiter if isinstance(limiter, (list, tuple)) else [limiter] * p.mwaves
+    for k in range(10):
+        p.mthlim[k] = lim[k] if k < len(lim) else 0
+    p.mcapa, p.use_fwaves = mcapa, 0
+    p.cfl, p.cflv1 = cfl, cflv1
+    p.ubar, p.vbar = ubar, vbar
+    p.rho, p.bulk = rho, bulk
+    p.cc = float(np.sqrt(bulk / rho))
+    p.zz = rho * p.cc
+    p.gamma, p.efix = gamma, int(bool(efix))
+    return p
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        L = C.CDLL(path)
+        P = C.POINTER(OrcParams)
+        L.orc_step2.argtypes = [P, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                C.c_double, C.c_double, C.POINTER(C.c_double),
+                                _dp, _dp, _dp, _dp]
+        L.orc_stepgrid.argtypes = [P, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                   C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+                                   C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+        L.orc_rpn2.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                               _dp, _dp]
+        L.orc_rpt2.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp,
+                               _dp, _dp, _dp, _dp]
+        L.orc_flux.argtypes = [P, C.c_int, _dp, _dp]
+        L.orc_set_plugin.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libamrref.so"))
+
+
+def ref():
+    """The reference's own routines (built by `make -C oracle ref`)."""
+    global _ref
+    if _ref is None:
+        lib()  # libamrref links liborc; load it first
+        R = C.CDLL(os.path.join(HERE, "_ref", "libamrref.so"))
+        R.ref_set_method.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, C.c_int]
+        R.ref_step2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                _dp, _dp, C.c_double, C.c_double, C.c_double,
+                                C.POINTER(C.c_double), _dp, _dp, _dp, _dp]
+        R.ref_stepgrid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   _dp, _dp, C.c_double, C.c_double,
+                                   C.c_double, C.c_double, _dp, _dp, _dp, _dp,
+                                   C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+        _ref = R
+    return _ref
+
+
+def _aux_or_dummy(p, aux, mx, my):
+    if aux is None:
+        return np.zeros((my + 2 * p.mbc, mx + 2 * p.mbc, max(p.maux, 1)))
+    return np.ascontiguousarray(aux, dtype=np.float64)
+
+
+def step2(p, q, dx, dy, dt, aux=None):
+    """orc_step2 on one patch; q of shape (my+2mbc, mx+2mbc, meqn)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    my, mx = q.shape[0] - 2 * p.mbc, q.shape[1] - 2 * p.mbc
+    aux = _aux_or_dummy(p, aux, mx, my)
+    f = [np.zeros_like(q) for _ in range(4)]
+    cfl = C.c_double()
+    lib().orc_step2(C.byref(p), mx, my, q, aux, dx, dy, dt, C.byref(cfl), *f)
+    return f, cfl.value
+
+
+def stepgrid(p, q, dx, dy, dt, aux=None):
+    """orc_stepgrid on one patch: returns (qnew, fluxes, cflgrid, dtnew)."""
+    qn = np.array(q, dtype=np.float64, order="C", copy=True)
+    my, mx = qn.shape[0] - 2 * p.mbc, qn.shape[1] - 2 * p.mbc
+    aux = _aux_or_dummy(p, aux, mx, my)
+    f = [np.zeros_like(qn) for _ in range(4)]
+    cfl, dtn = C.c_double(), C.c_double()
+    lib().orc_stepgrid(C.byref(p), mx, my, qn, aux, dx, dy, dt, *f,
+                       C.byref(cfl), C.byref(dtn))
+    return qn, f, cfl.value, dtn.value
+
+
+def _ref_setup(p):
+    R = ref()
+    lib().orc_set_plugin(C.byref(p))
+    meth = (C.c_int * 7)(*[p.method[k] for k in range(1, 8)])
+    lim = (C.c_int * 10)(*list(p.mthlim))
+    R.ref_set_method(meth, lim, p.mwaves, p.mcapa, p.use_fwaves, p.cfl,
+                     p.cflv1, p.mbc)
+    return R
+
+
+def ref_step2(p, q, dx, dy, dt, aux=None):
+    """The reference's step2.f90 on one patch (same conventions as step2)."""
+    R = _ref_setup(p)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    my, mx = q.shape[0] - 2 * p.mbc, q.shape[1] - 2 * p.mbc
+    aux = _aux_or_dummy(p, aux, mx, my)
+    f = [np.zeros_like(q) for _ in range(4)]
+    cfl = C.c_double()
+    R.ref_step2(p.meqn, p.maux, p.mbc, mx, my, q, aux, dx, dy, dt,
+                C.byref(cfl), *f)
+    return f, cfl.value
+
+
+def ref_stepgrid(p, q, dx, dy, dt, aux=None, time=0.0):
+    """The reference's stepgrid.f on one patch."""
+    R = _ref_setup(p)
+    qn = np.array(q, dtype=np.float64, order="C", copy=True)
+    my, mx = qn.shape[0] - 2 * p.mbc, qn.shape[1] - 2 * p.mbc
+    aux = _aux_or_dummy(p, aux, mx, my)
+    f = [np.zeros_like(qn) for _ in range(4)]
+    cfl, dtn = C.c_double(), C.c_double()
+    R.ref_stepgrid(p.meqn, p.maux, p.mbc, mx, my, qn, aux, dx, dy, dt, time,
+                   *f, C.byref(dtn), C.byref(cfl))
+    return qn, f, cfl.value, dtn.value
+
+
+def rpn2(p, ixy, ql, qr):
+    """Solve the Riemann problems between consecutive slice cells.
+    ql, qr: (n, meqn) slices (cell index 1-mbc.. as rows); returns
+    wave (n, mwaves, meqn), s (n, mwaves), amdq, apdq (n, meqn)."""
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    n = ql.shape[0]
+    mx = n - 2 * p.mbc
+    wave = np.zeros((n, p.mwaves, p.meqn))
+    s = np.zeros((n, p.mwaves))
+    amdq = np.zeros((n, p.meqn))
+    apdq = np.zeros((n, p.meqn))
+    aux = np.zeros((n, max(p.maux, 1)))
+    lib().orc_rpn2(C.byref(p), ixy, mx, p.meqn, p.mwaves, p.maux, p.mbc, mx,
+                   ql, qr, aux, aux, wave, s, amdq, apdq)
+    return wave, s, amdq, apdq
+
+
+def rpt2(p, ixy, imp, ql, qr, asdq):
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    asdq = np.ascontiguousarray(asdq, dtype=np.float64)
+    n = ql.shape[0]
+    mx = n - 2 * p.mbc
+    bm = np.zeros((n, p.meqn))
+    bp = np.zeros((n, p.meqn))
+    aux = np.zeros((n, max(p.maux, 1)))
+    lib().orc_rpt2(C.byref(p), ixy, imp, mx, p.meqn, p.mwaves, p.maux, p.mbc,
+                   mx, ql, qr, aux, aux, aux, asdq, bm, bp)
+    return bm, bp
+
+
+def flux(p, ixy, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    f = np.zeros(p.meqn)
+    lib().orc_flux(C.byref(p), ixy, q, f)
+    return f

@@ -1,0 +1,101 @@
+! ref_harness.f90 -- C-callable harness around the reference's own Fortran
+! (TEST INFRASTRUCTURE ONLY; built into oracle/_ref, never shipped).
+!
+! The reference routines (step2, stepgrid, flag2refine2, bc2amr, qad, fluxsv,
+! fluxad, upbnd, ...) are compiled unmodified from the reference tree by
+! oracle/Makefile.  This file is the caller: it fills the amr_module state
+! those routines read and exposes bind(C) entry points for the tests.
+! rpn2/rpt2 are the physics plugin the reference expects the application to
+! link (flux2.f:125, qad.f:129); here they forward to the oracle's solvers.
+
+subroutine rpn2(ixy,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,auxl,auxr,wave,s,amdq,apdq)
+  use iso_c_binding
+  implicit none
+  integer :: ixy,maxm,meqn,mwaves,maux,mbc,mx
+  real(kind=8) :: ql(*),qr(*),auxl(*),auxr(*),wave(*),s(*),amdq(*),apdq(*)
+  interface
+     subroutine orc_rpn2_byref(ixy,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,auxl,auxr, &
+          wave,s,amdq,apdq) bind(C, name="orc_rpn2_byref")
+       import :: c_int, c_double
+       integer(c_int) :: ixy,maxm,meqn,mwaves,maux,mbc,mx
+       real(c_double) :: ql(*),qr(*),auxl(*),auxr(*),wave(*),s(*),amdq(*),apdq(*)
+     end subroutine orc_rpn2_byref
+  end interface
+  call orc_rpn2_byref(ixy,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,auxl,auxr,wave,s,amdq,apdq)
+end subroutine rpn2
+
+subroutine rpt2(ixy,imp,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,aux1,aux2,aux3,asdq,bmasdq,bpasdq)
+  use iso_c_binding
+  implicit none
+  integer :: ixy,imp,maxm,meqn,mwaves,maux,mbc,mx
+  real(kind=8) :: ql(*),qr(*),aux1(*),aux2(*),aux3(*),asdq(*),bmasdq(*),bpasdq(*)
+  interface
+     subroutine orc_rpt2_byref(ixy,imp,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,aux1,aux2, &
+          aux3,asdq,bmasdq,bpasdq) bind(C, name="orc_rpt2_byref")
+       import :: c_int, c_double
+       integer(c_int) :: ixy,imp,maxm,meqn,mwaves,maux,mbc,mx
+       real(c_double) :: ql(*),qr(*),aux1(*),aux2(*),aux3(*),asdq(*),bmasdq(*),bpasdq(*)
+     end subroutine orc_rpt2_byref
+  end interface
+  call orc_rpt2_byref(ixy,imp,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,aux1,aux2,aux3,asdq,bmasdq,bpasdq)
+end subroutine rpt2
+
+module ref_harness
+  use iso_c_binding
+  use amr_module
+  implicit none
+contains
+
+  ! method(1:7), mthlim(1:mwaves), mcapa, use_fwaves, cfl_desired, cfl_max
+  subroutine ref_set_method(meth, lim, nwaves, ncapa, fwaves, cfl_in, cflv1_in, ng) &
+       bind(C, name="ref_set_method")
+    integer(c_int), intent(in) :: meth(7), lim(*)
+    integer(c_int), value :: nwaves, ncapa, fwaves, ng
+    real(c_double), value :: cfl_in, cflv1_in
+    method(1:7) = meth(1:7)
+    mwaves = nwaves
+    if (allocated(mthlim)) deallocate(mthlim)
+    allocate(mthlim(mwaves))
+    mthlim(1:mwaves) = lim(1:mwaves)
+    mcapa = ncapa
+    use_fwaves = (fwaves /= 0)
+    cfl = cfl_in
+    cflv1 = cflv1_in
+    nghost = ng
+    dimensional_split = 0
+  end subroutine ref_set_method
+
+  ! step2.f90:7 on one patch; arrays are (meqn|maux, mx+2mbc, my+2mbc)
+  subroutine ref_step2(meqn, maux, mbc, mx, my, qold, aux, dx, dy, dt, cflgrid, &
+       fm, fp, gm, gp) bind(C, name="ref_step2")
+    integer(c_int), value :: meqn, maux, mbc, mx, my
+    real(c_double), value :: dx, dy, dt
+    real(c_double) :: qold(meqn,1-mbc:mx+mbc,1-mbc:my+mbc)
+    real(c_double) :: aux(max(maux,1),1-mbc:mx+mbc,1-mbc:my+mbc)
+    real(c_double) :: fm(meqn,1-mbc:mx+mbc,1-mbc:my+mbc), fp(meqn,1-mbc:mx+mbc,1-mbc:my+mbc)
+    real(c_double) :: gm(meqn,1-mbc:mx+mbc,1-mbc:my+mbc), gp(meqn,1-mbc:mx+mbc,1-mbc:my+mbc)
+    real(c_double) :: cflgrid
+    external rpn2, rpt2
+    call step2(max(mx,my), meqn, maux, mbc, mx, my, qold, aux, dx, dy, dt, cflgrid, &
+         fm, fp, gm, gp, rpn2, rpt2)
+  end subroutine ref_step2
+
+  ! stepgrid.f:24 on one patch (q updated in place)
+  subroutine ref_stepgrid(nvar, naux, mbc, mx, my, q, aux, dx, dy, dt, time, &
+       fm, fp, gm, gp, dtnew, cflgrid) bind(C, name="ref_stepgrid")
+    integer(c_int), value :: nvar, naux, mbc, mx, my
+    real(c_double), value :: dx, dy, dt, time
+    real(c_double) :: q(nvar,mx+2*mbc,my+2*mbc), aux(max(naux,1),mx+2*mbc,my+2*mbc)
+    real(c_double) :: fm(nvar,mx+2*mbc,my+2*mbc), fp(nvar,mx+2*mbc,my+2*mbc)
+    real(c_double) :: gm(nvar,mx+2*mbc,my+2*mbc), gp(nvar,mx+2*mbc,my+2*mbc)
+    real(c_double) :: dtnew, cflgrid
+    real(c_double) :: xlow, ylow
+    xlow = 0.d0
+    ylow = 0.d0
+    cfl_level = 0.d0
+    call stepgrid(q, fm, fp, gm, gp, mx+2*mbc, my+2*mbc, mbc, dt, dtnew, dx, dy, &
+         nvar, xlow, ylow, time, 1, naux, aux)
+    cflgrid = cfl_level
+  end subroutine ref_stepgrid
+
+end module ref_harness
