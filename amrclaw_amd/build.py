@@ -1,0 +1,66 @@
+"""Build libamrgpu.so in-tree with hipcc for gfx950.
+
+    python -m amrclaw_amd.build
+
+Every HIP translation unit is built with packed FP32 instructions disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops); the FP64 sweep is built
+with -ffp-contract=off so that no multiply-add is fused and its results match
+the reference's FP64 arithmetic bit for bit.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libamrgpu.so")
+ARCH = os.environ.get("AMRGPU_ARCH", "gfx950")
+
+COMMON = ["-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-I" + os.path.join(HERE, "..", "include")]
+UNITS = [
+    ("sweep_f64.hip", ["-O3", "-ffp-contract=off"]),
+    ("sweep_f32.hip", ["-O3"]),
+    ("util.hip", ["-O2"]),
+    ("capi.cpp", ["-O2", "-x", "hip"]),
+]
+
+
+def _hipcc():
+    for c in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.exists(c) or c == "hipcc":
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(unit, flags):
+    src = os.path.join(CSRC, unit)
+    obj = os.path.join(OBJ, unit.rsplit(".", 1)[0] + ".o")
+    cmd = [_hipcc()] + COMMON + flags + ["-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    return obj
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
+        objs = list(ex.map(lambda u: _compile(*u), UNITS))
+    tmp = LIB + ".tmp"
+    cmd = [_hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

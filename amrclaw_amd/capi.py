@@ -1,0 +1,177 @@
+"""ctypes binding of libamrgpu.so (include/amrgpu.h).
+
+This is the Python mirror of the binding a Fortran driver makes through
+ISO_C_BINDING: the same entry points, the same argument meaning.  Loading
+fails loudly if the HIP library has not been built; there is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libamrgpu.so")
+
+ADVECTION, ACOUSTICS, EULER = 1, 2, 3
+FP64, FP32 = 0, 1
+
+
+class AmrgpuConfig(C.Structure):
+    _fields_ = [
+        ("solver", C.c_int), ("nvar", C.c_int), ("mwaves", C.c_int),
+        ("naux", C.c_int), ("nghost", C.c_int),
+        ("method", C.c_int * 7), ("mthlim", C.c_int * 8),
+        ("mcapa", C.c_int), ("use_fwaves", C.c_int),
+        ("cfl", C.c_double), ("cflv1", C.c_double),
+        ("mthbc", C.c_int * 4),
+        ("xlower", C.c_double), ("xupper", C.c_double),
+        ("ylower", C.c_double), ("yupper", C.c_double),
+        ("ubar", C.c_double), ("vbar", C.c_double),
+        ("rho", C.c_double), ("bulk", C.c_double),
+        ("gamma", C.c_double), ("efix", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class AmrgpuError(RuntimeError):
+    pass
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libamrgpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise AmrgpuError("libamrgpu.so is not built; run "
+                              "`python -m amrclaw_amd.build`")
+        L = C.CDLL(LIBPATH)
+        dpo = C.POINTER(C.c_double)
+        L.amrgpu_init.argtypes = [C.POINTER(AmrgpuConfig)]
+        L.amrgpu_finalize.argtypes = []
+        L.amrgpu_last_error.argtypes = [C.c_char_p, C.c_int]
+        L.amrgpu_set_tiling.argtypes = [C.c_int, C.c_int]
+        L.amrgpu_stepgrid.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double,
+                                      C.c_double, C.c_double, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p,
+                                      dpo, dpo]
+        L.amrgpu_set_level.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, _ip,
+                                       C.c_double, C.c_double]
+        L.amrgpu_put_grid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.amrgpu_get_grid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        L.amrgpu_sweep_level.argtypes = [C.c_int, C.c_double, C.c_void_p,
+                                         C.c_void_p]
+        L.amrgpu_event_record.argtypes = [C.c_int]
+        L.amrgpu_event_elapsed.argtypes = [C.c_int, C.c_int,
+                                           C.POINTER(C.c_float)]
+        L.amrgpu_synchronize.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        buf = C.create_string_buffer(512)
+        lib().amrgpu_last_error(buf, 512)
+        raise AmrgpuError("amrgpu error %d: %s" % (rc, buf.value.decode()))
+
+
+def make_config(solver, *, order=2, transverse=2, limiter=4, cfl=0.9,
+                cflv1=1.0, mthbc=(1, 1, 1, 1), domain=(0.0, 1.0, 0.0, 1.0),
+                ubar=0.5, vbar=1.0, rho=1.0, bulk=4.0, gamma=1.4, efix=True,
+                precision=FP64):
+    """Configuration of one run (claw.data / amr.data fields, setprob
+    constants).  limiter: one mthlim code for every wave, or a list."""
+    c = AmrgpuConfig()
+    c.solver = solver
+    c.nvar, c.mwaves = {ADVECTION: (1, 1), ACOUSTICS: (3, 2),
+                        EULER: (4, 4)}[solver]
+    c.naux, c.nghost = 0, 2
+    meth = [0, order, transverse, 0, 0, 0, 0]
+    for k in range(7):
+        c.method[k] = meth[k]
+    lim = limiter if isinstance(limiter, (list, tuple)) else [limiter] * c.mwaves
+    for k in range(8):
+        c.mthlim[k] = lim[k] if k < len(lim) else 0
+    c.mcapa, c.use_fwaves = 0, 0
+    c.cfl, c.cflv1 = cfl, cflv1
+    for k in range(4):
+        c.mthbc[k] = mthbc[k]
+    c.xlower, c.xupper, c.ylower, c.yupper = domain
+    c.ubar, c.vbar, c.rho, c.bulk = ubar, vbar, rho, bulk
+    c.gamma, c.efix = gamma, int(bool(efix))
+    c.precision = precision
+    return c
+
+
+def init(cfg):
+    _check(lib().amrgpu_init(C.byref(cfg)))
+
+
+def finalize():
+    _check(lib().amrgpu_finalize())
+
+
+def set_tiling(txmax, tymax):
+    _check(lib().amrgpu_set_tiling(txmax, tymax))
+
+
+def stepgrid(q, dx, dy, dt, want_fluxes=False):
+    """amrgpu_stepgrid on one patch.  q: (my+4, mx+4, nvar) array (the
+    reference's q(nvar, mitot, mjtot) in Fortran order).  Returns
+    (qnew, [fm, fp, gm, gp] or None, cflgrid, dtnew)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    my, mx = q.shape[0] - 4, q.shape[1] - 4
+    out = np.empty_like(q)
+    fl = [np.zeros_like(q) for _ in range(4)] if want_fluxes else None
+    ptrs = [f.ctypes.data_as(C.c_void_p) for f in fl] if fl else [None] * 4
+    cfl, dtn = C.c_double(), C.c_double()
+    _check(lib().amrgpu_stepgrid(mx, my, q, out, dx, dy, dt, *ptrs,
+                                 C.byref(cfl), C.byref(dtn)))
+    return out, fl, cfl.value, dtn.value
+
+
+def set_level(level, boxes, hx, hy):
+    """boxes: (n, 4) int array of (ilo, ihi, jlo, jhi) per grid."""
+    b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 4))
+    cols = [np.ascontiguousarray(b[:, k]) for k in range(4)]
+    _check(lib().amrgpu_set_level(level, b.shape[0], *cols, hx, hy))
+
+
+def put_grid(level, grid, q, which=0):
+    _check(lib().amrgpu_put_grid(level, grid, which,
+                                 np.ascontiguousarray(q, dtype=np.float64)))
+
+
+def get_grid(level, grid, shape, which=0):
+    out = np.empty(shape, dtype=np.float64)
+    _check(lib().amrgpu_get_grid(level, grid, which, out))
+    return out
+
+
+def sweep_level(level, dt, want_cfl=True):
+    if want_cfl:
+        cfl, dtn = C.c_double(), C.c_double()
+        _check(lib().amrgpu_sweep_level(level, dt, C.byref(cfl), C.byref(dtn)))
+        return cfl.value, dtn.value
+    _check(lib().amrgpu_sweep_level(level, dt, None, None))
+    return None
+
+
+def event_record(slot):
+    _check(lib().amrgpu_event_record(slot))
+
+
+def event_elapsed(a, b):
+    ms = C.c_float()
+    _check(lib().amrgpu_event_elapsed(a, b, C.byref(ms)))
+    return ms.value
+
+
+def synchronize():
+    _check(lib().amrgpu_synchronize())

@@ -1,0 +1,455 @@
+// capi.cpp -- host side of libamrgpu: the C ABI of include/amrgpu.h.
+//
+// Owns the device copies of every level (patch table, tile table, two
+// solution buffers), plans the tiling of each level and launches the kernels
+// on one stream.  Mirrors the reference's level bookkeeping (node/rnode,
+// listOfGrids; amr_module.f90:30-151) for the data the level update needs.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amrgpu.h"
+#include "kernels.hpp"
+
+namespace amrgpu {
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n,
+                           hipStream_t st);
+}
+
+using namespace amrgpu;
+
+namespace {
+
+constexpr int MAXLV = 12;          // reference maxlv = 10 (+ scratch slot 0)
+constexpr int NEVENTS = 16;
+// rinfinity = 10.e32 declared from a default-real literal (amr_module.f90)
+const double RINFINITY = (double)10.e32f;
+
+struct Level {
+    int ngrids = 0;
+    double hx = 0, hy = 0;
+    std::vector<PatchDesc> patches;
+    std::vector<TileDesc> tiles;
+    PatchDesc* d_patches = nullptr;
+    TileDesc* d_tiles = nullptr;
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;                    // buf[cur]: new time (store1)
+    int64_t nelem = 0;
+    unsigned long long* d_cfl = nullptr;
+    size_t lds = 0;
+};
+
+struct Context {
+    bool inited = false;
+    amrgpu_config cfg{};
+    MethodArgs meth{};
+    size_t esize = 8;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[NEVENTS] = {};
+    int txmax = 28, tymax = 28;
+    Level levels[MAXLV];
+    std::vector<unsigned long long> h_cfl;
+};
+
+Context g;
+std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                            \
+    do {                                                                        \
+        hipError_t e_ = (call);                                                 \
+        if (e_ != hipSuccess)                                                   \
+            return fail(AMRGPU_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+void free_level(Level& L) {
+    if (L.d_patches) hipFree(L.d_patches);
+    if (L.d_tiles) hipFree(L.d_tiles);
+    for (auto& b : L.buf)
+        if (b) hipFree(b);
+    if (L.d_cfl) hipFree(L.d_cfl);
+    L = Level();
+}
+
+int seg_width(int n) { return n + 3 <= 16 ? 16 : (n + 3 <= 32 ? 32 : 64); }
+
+// split n cells into tiles of at most cap cells, boundaries on multiples of 4
+// (so that the r fine faces of a coarse register segment never straddle two
+// tiles for refinement ratios 1, 2, 4)
+std::vector<std::pair<int, int>> split_axis(int n, int cap) {
+    std::vector<std::pair<int, int>> out;
+    int k = (n + cap - 1) / cap;
+    int t = (n + k - 1) / k;
+    t = (t + 3) / 4 * 4;
+    if (t > cap) t = std::max(4, cap / 4 * 4);
+    for (int s = 0; s < n; s += t) out.push_back({s, std::min(t, n - s)});
+    return out;
+}
+
+int plan_level(Level& L) {
+    L.tiles.clear();
+    size_t lds = 0;
+    const int nv = g.cfg.nvar;
+    for (int p = 0; p < L.ngrids; p++) {
+        const PatchDesc& pd = L.patches[p];
+        auto xs = split_axis(pd.mx, g.txmax), ys = split_axis(pd.my, g.tymax);
+        for (auto& y : ys)
+            for (auto& x : xs) {
+                TileDesc t{};
+                t.patch = p;
+                t.i0 = (int16_t)x.first; t.tx = (int16_t)x.second;
+                t.j0 = (int16_t)y.first; t.ty = (int16_t)y.second;
+                t.wx = (int16_t)seg_width(t.tx);
+                t.wy = (int16_t)seg_width(t.ty);
+                L.tiles.push_back(t);
+                size_t b = g.esize == 8 ? sweep_lds_bytes<double>(nv, t.tx, t.ty)
+                                        : sweep_lds_bytes<float>(nv, t.tx, t.ty);
+                lds = std::max(lds, b);
+            }
+    }
+    // largest tiles first (the reference orders grids by cost, regrid.f:132-169)
+    std::stable_sort(L.tiles.begin(), L.tiles.end(), [](const TileDesc& a, const TileDesc& b) {
+        return a.tx * a.ty > b.tx * b.ty;
+    });
+    if (lds > 160 * 1024)
+        return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", g.txmax, g.tymax, lds);
+    L.lds = lds;
+    if (L.d_tiles) hipFree(L.d_tiles);
+    L.d_tiles = nullptr;
+    if (!L.tiles.empty()) {
+        HIPCHK(hipMalloc(&L.d_tiles, L.tiles.size() * sizeof(TileDesc)));
+        HIPCHK(hipMemcpy(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(TileDesc),
+                         hipMemcpyHostToDevice));
+    }
+    return AMRGPU_OK;
+}
+
+int check_level(int level) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (level < 0 || level >= MAXLV) return fail(AMRGPU_ERR_ARG, "level %d out of range", level);
+    return AMRGPU_OK;
+}
+
+int define_level(int level, int ngrids, const int* ilo, const int* ihi, const int* jlo,
+                 const int* jhi, double hx, double hy) {
+    Level& L = g.levels[level];
+    free_level(L);
+    L.ngrids = ngrids;
+    L.hx = hx;
+    L.hy = hy;
+    const int ng = NGHOST, nv = g.cfg.nvar;
+    int64_t off = 0;
+    for (int p = 0; p < ngrids; p++) {
+        PatchDesc pd{};
+        pd.ilo = ilo[p]; pd.ihi = ihi[p]; pd.jlo = jlo[p]; pd.jhi = jhi[p];
+        pd.mx = ihi[p] - ilo[p] + 1;
+        pd.my = jhi[p] - jlo[p] + 1;
+        if (pd.mx < 1 || pd.my < 1)
+            return fail(AMRGPU_ERR_ARG, "grid %d of level %d is empty", p, level);
+        if (pd.mx > 30000 || pd.my > 30000)
+            return fail(AMRGPU_ERR_ARG, "grid %d of level %d too large", p, level);
+        pd.mitot = pd.mx + 2 * ng;
+        pd.plane = pd.mitot * (pd.my + 2 * ng);
+        pd.off = off;
+        off += (int64_t)nv * pd.plane;
+        off = (off + 31) / 32 * 32;   // 256-byte aligned patch starts
+        L.patches.push_back(pd);
+    }
+    L.nelem = off;
+    if (ngrids > 0) {
+        HIPCHK(hipMalloc(&L.d_patches, ngrids * sizeof(PatchDesc)));
+        HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), ngrids * sizeof(PatchDesc),
+                         hipMemcpyHostToDevice));
+        for (auto& b : L.buf) {
+            HIPCHK(hipMalloc(&b, (size_t)L.nelem * g.esize));
+            HIPCHK(hipMemset(b, 0, (size_t)L.nelem * g.esize));
+        }
+        HIPCHK(hipMalloc(&L.d_cfl, ngrids * sizeof(unsigned long long)));
+    }
+    return plan_level(L);
+}
+
+// AoS FP64 host <-> SoA device (storage precision)
+int put_grid(Level& L, int grid, int which, const double* q) {
+    const PatchDesc& pd = L.patches[grid];
+    const int nv = g.cfg.nvar;
+    const int64_t n = (int64_t)nv * pd.plane;
+    void* dst = (char*)L.buf[which == 0 ? L.cur : 1 - L.cur] + pd.off * g.esize;
+    if (g.esize == 8) {
+        std::vector<double> h(n);
+        for (int64_t c = 0; c < pd.plane; c++)
+            for (int m = 0; m < nv; m++) h[m * pd.plane + c] = q[c * nv + m];
+        HIPCHK(hipMemcpy(dst, h.data(), n * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h(n);
+        for (int64_t c = 0; c < pd.plane; c++)
+            for (int m = 0; m < nv; m++) h[m * pd.plane + c] = (float)q[c * nv + m];
+        HIPCHK(hipMemcpy(dst, h.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    return AMRGPU_OK;
+}
+
+int get_grid(Level& L, int grid, int which, double* q) {
+    const PatchDesc& pd = L.patches[grid];
+    const int nv = g.cfg.nvar;
+    const int64_t n = (int64_t)nv * pd.plane;
+    const void* src = (const char*)L.buf[which == 0 ? L.cur : 1 - L.cur] + pd.off * g.esize;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (g.esize == 8) {
+        std::vector<double> h(n);
+        HIPCHK(hipMemcpy(h.data(), src, n * 8, hipMemcpyDeviceToHost));
+        for (int64_t c = 0; c < pd.plane; c++)
+            for (int m = 0; m < nv; m++) q[c * nv + m] = h[m * pd.plane + c];
+    } else {
+        std::vector<float> h(n);
+        HIPCHK(hipMemcpy(h.data(), src, n * 4, hipMemcpyDeviceToHost));
+        for (int64_t c = 0; c < pd.plane; c++)
+            for (int m = 0; m < nv; m++) q[c * nv + m] = (double)h[m * pd.plane + c];
+    }
+    return AMRGPU_OK;
+}
+
+int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev) {
+    if (L.ngrids == 0) return AMRGPU_OK;
+    HIPCHK(launch_fill_u64(L.d_cfl, 0ull, L.ngrids, g.stream));
+    SweepArgs a{};
+    a.qin = L.buf[L.cur];
+    a.qout = L.buf[1 - L.cur];
+    a.patches = L.d_patches;
+    a.tiles = L.d_tiles;
+    a.ntiles = (int)L.tiles.size();
+    a.dt = dt;
+    a.dx = L.hx;
+    a.dy = L.hy;
+    a.meth = g.meth;
+    a.cfl_patch = L.d_cfl;
+    a.flux_out = flux_out_dev;
+    a.flux_off = flux_off_dev;
+    hipError_t e = g.esize == 8 ? launch_sweep_f64(g.cfg.solver, a, L.lds, g.stream)
+                                : launch_sweep_f32(g.cfg.solver, a, L.lds, g.stream);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "sweep launch: %s", hipGetErrorString(e));
+    // the updated interior is now in buf[1-cur]; buf[cur] keeps time t
+    // (with its ghost cells) and becomes the old-time array (store2)
+    L.cur = 1 - L.cur;
+    return AMRGPU_OK;
+}
+
+int level_cfl(Level& L, double dt, double* cfl_level, double* dtlevnew) {
+    g.h_cfl.resize(L.ngrids);
+    HIPCHK(hipMemcpyAsync(g.h_cfl.data(), L.d_cfl, L.ngrids * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    double cmax = 0.0, dmin = RINFINITY;
+    for (int p = 0; p < L.ngrids; p++) {
+        double c;
+        unsigned long long b = g.h_cfl[p];
+        std::memcpy(&c, &b, 8);
+        cmax = std::max(cmax, c);
+        double dtnew = c > 0.0 ? dt * g.cfg.cfl / c : RINFINITY;   // stepgrid.f:206-212
+        dmin = std::min(dmin, dtnew);                              // advanc.f:124-126
+    }
+    if (cfl_level) *cfl_level = cmax;
+    if (dtlevnew) *dtlevnew = dmin;
+    return AMRGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amrgpu_init(const amrgpu_config* cfg) {
+    if (!cfg) return fail(AMRGPU_ERR_ARG, "cfg is NULL");
+    if (g.inited) amrgpu_finalize();
+    const int nv_expect[4] = {0, 1, 3, 4};
+    const int nw_expect[4] = {0, 1, 2, 4};
+    if (cfg->solver < 1 || cfg->solver > 3) return fail(AMRGPU_ERR_ARG, "unknown solver %d", cfg->solver);
+    if (cfg->nvar != nv_expect[cfg->solver])
+        return fail(AMRGPU_ERR_ARG, "solver %d needs nvar=%d", cfg->solver, nv_expect[cfg->solver]);
+    if (cfg->mwaves != nw_expect[cfg->solver])
+        return fail(AMRGPU_ERR_ARG, "solver %d needs mwaves=%d", cfg->solver, nw_expect[cfg->solver]);
+    if (cfg->nghost != NGHOST) return fail(AMRGPU_ERR_UNSUPPORTED, "nghost must be 2");
+    if (cfg->naux != 0 || cfg->mcapa != 0)
+        return fail(AMRGPU_ERR_UNSUPPORTED, "aux arrays / capacity are not supported");
+    if (cfg->precision != AMRGPU_FP64 && cfg->precision != AMRGPU_FP32)
+        return fail(AMRGPU_ERR_ARG, "precision must be AMRGPU_FP64 or AMRGPU_FP32");
+    if (cfg->method[1] < 1 || cfg->method[1] > 2 || cfg->method[2] < 0 || cfg->method[2] > 2)
+        return fail(AMRGPU_ERR_ARG, "method(2)=%d method(3)=%d invalid", cfg->method[1], cfg->method[2]);
+    for (int mw = 0; mw < cfg->mwaves; mw++)
+        if (cfg->mthlim[mw] < 0 || cfg->mthlim[mw] > 5)
+            return fail(AMRGPU_ERR_ARG, "mthlim(%d)=%d invalid", mw + 1, cfg->mthlim[mw]);
+    g = Context();
+    g.cfg = *cfg;
+    g.esize = cfg->precision == AMRGPU_FP64 ? 8 : 4;
+    MethodArgs& M = g.meth;
+    M.ph.ubar = cfg->ubar;
+    M.ph.vbar = cfg->vbar;
+    M.ph.rho = cfg->rho;
+    M.ph.bulk = cfg->bulk;
+    M.ph.cc = cfg->rho > 0 ? std::sqrt(cfg->bulk / cfg->rho) : 0.0;   // setprob: cc = sqrt(K/rho)
+    M.ph.zz = cfg->rho * M.ph.cc;                                     // zz = rho*cc
+    M.ph.gamma = cfg->gamma;
+    M.ph.efix = cfg->efix;
+    M.order = cfg->method[1];
+    M.transverse = cfg->method[2];
+    M.use_fwaves = cfg->use_fwaves;
+    M.mwaves = cfg->mwaves;
+    for (int mw = 0; mw < NWMAX; mw++) M.mthlim[mw] = mw < cfg->mwaves ? cfg->mthlim[mw] : 0;
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    for (auto& e : g.ev) HIPCHK(hipEventCreate(&e));
+    g.txmax = g.esize == 8 ? 28 : 28;
+    g.tymax = g.esize == 8 ? 12 : 28;
+    g.inited = true;
+    return AMRGPU_OK;
+}
+
+int amrgpu_finalize(void) {
+    if (!g.inited) return AMRGPU_OK;
+    hipStreamSynchronize(g.stream);
+    for (auto& L : g.levels) free_level(L);
+    for (auto& e : g.ev)
+        if (e) hipEventDestroy(e);
+    hipStreamDestroy(g.stream);
+    g = Context();
+    return AMRGPU_OK;
+}
+
+int amrgpu_last_error(char* buf, int len) {
+    if (!buf || len <= 0) return AMRGPU_ERR_ARG;
+    std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_tiling(int txmax, int tymax) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (txmax < 0 || tymax < 0 || txmax > 61 || tymax > 61)
+        return fail(AMRGPU_ERR_ARG, "tile caps must be in [4, 61]");
+    if (txmax) g.txmax = std::max(4, txmax);
+    if (tymax) g.tymax = std::max(4, tymax);
+    for (auto& L : g.levels)
+        if (L.ngrids) {
+            int rc = plan_level(L);
+            if (rc) return rc;
+        }
+    return AMRGPU_OK;
+}
+
+int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out, double dx, double dy,
+                    double dt, double* fm, double* fp, double* gm, double* gp, double* cflgrid,
+                    double* dtnew) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (!q_in || !q_out || mx < 1 || my < 1) return fail(AMRGPU_ERR_ARG, "bad stepgrid arguments");
+    int ilo = 0, ihi = mx - 1, jlo = 0, jhi = my - 1;
+    int rc = define_level(0, 1, &ilo, &ihi, &jlo, &jhi, dx, dy);
+    if (rc) return rc;
+    Level& L = g.levels[0];
+    rc = put_grid(L, 0, 0, q_in);
+    if (rc) return rc;
+    const bool want_flux = fm && fp && gm && gp;
+    const int nv = g.cfg.nvar;
+    const int64_t plane = L.patches[0].plane, arr = nv * plane;
+    double* d_flux = nullptr;
+    int64_t* d_off = nullptr;
+    if (want_flux) {
+        HIPCHK(hipMalloc(&d_flux, 4 * arr * sizeof(double)));
+        HIPCHK(hipMemset(d_flux, 0, 4 * arr * sizeof(double)));
+        HIPCHK(hipMalloc(&d_off, sizeof(int64_t)));
+        HIPCHK(hipMemset(d_off, 0, sizeof(int64_t)));
+    }
+    // ghost cells of the output keep the input values, as in stepgrid.f
+    rc = put_grid(L, 0, 1, q_in);
+    if (rc) return rc;
+    rc = sweep(L, dt, d_flux, d_off);
+    if (rc) return rc;
+    rc = get_grid(L, 0, 0, q_out);
+    if (rc) return rc;
+    rc = level_cfl(L, dt, cflgrid, dtnew);
+    if (rc) return rc;
+    if (want_flux) {
+        std::vector<double> h(4 * arr);
+        HIPCHK(hipMemcpy(h.data(), d_flux, 4 * arr * sizeof(double), hipMemcpyDeviceToHost));
+        double* outs[4] = {fm, fp, gm, gp};
+        for (int k = 0; k < 4; k++)
+            for (int64_t c = 0; c < plane; c++)
+                for (int m = 0; m < nv; m++) outs[k][c * nv + m] = h[k * arr + m * plane + c];
+        hipFree(d_flux);
+        hipFree(d_off);
+    }
+    free_level(L);
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_level(int level, int ngrids, const int* ilo, const int* ihi, const int* jlo,
+                     const int* jhi, double hx, double hy) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (level == 0) return fail(AMRGPU_ERR_ARG, "levels are numbered from 1");
+    if (ngrids < 0 || (ngrids > 0 && (!ilo || !ihi || !jlo || !jhi)))
+        return fail(AMRGPU_ERR_ARG, "bad grid arrays");
+    if (!(hx > 0) || !(hy > 0)) return fail(AMRGPU_ERR_ARG, "mesh widths must be positive");
+    return define_level(level, ngrids, ilo, ihi, jlo, jhi, hx, hy);
+}
+
+int amrgpu_put_grid(int level, int grid, int which, const double* q) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Level& L = g.levels[level];
+    if (grid < 0 || grid >= L.ngrids || (which != 0 && which != 1) || !q)
+        return fail(AMRGPU_ERR_ARG, "bad put_grid arguments");
+    return put_grid(L, grid, which, q);
+}
+
+int amrgpu_get_grid(int level, int grid, int which, double* q) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Level& L = g.levels[level];
+    if (grid < 0 || grid >= L.ngrids || (which != 0 && which != 1) || !q)
+        return fail(AMRGPU_ERR_ARG, "bad get_grid arguments");
+    return get_grid(L, grid, which, q);
+}
+
+int amrgpu_sweep_level(int level, double dt, double* cfl_level, double* dtlevnew) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Level& L = g.levels[level];
+    rc = sweep(L, dt, nullptr, nullptr);
+    if (rc) return rc;
+    if (cfl_level || dtlevnew) return level_cfl(L, dt, cfl_level, dtlevnew);
+    return AMRGPU_OK;
+}
+
+int amrgpu_event_record(int slot) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (slot < 0 || slot >= NEVENTS) return fail(AMRGPU_ERR_ARG, "event slot %d", slot);
+    HIPCHK(hipEventRecord(g.ev[slot], g.stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_event_elapsed(int a, int b, float* ms) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (a < 0 || a >= NEVENTS || b < 0 || b >= NEVENTS || !ms)
+        return fail(AMRGPU_ERR_ARG, "bad event arguments");
+    HIPCHK(hipEventSynchronize(g.ev[b]));
+    HIPCHK(hipEventElapsedTime(ms, g.ev[a], g.ev[b]));
+    return AMRGPU_OK;
+}
+
+int amrgpu_synchronize(void) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return AMRGPU_OK;
+}
+
+}  // extern "C"

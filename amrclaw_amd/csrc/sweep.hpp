@@ -1,0 +1,359 @@
+// sweep.hpp -- fused, batched wave-propagation sweep over the tiles of a level.
+//
+// One workgroup (256 lanes) advances one tile of one patch:
+//   1. stage the tile plus its 2-cell halo in LDS (compute precision T);
+//   2. x-phase: one lane per interface, one 16/32/64-lane segment per row
+//      (rows j0..j0+ty+1); normal solve, CFL, wave limiter (neighbour waves
+//      by cross-lane shuffle), second-order correction, both transverse
+//      solves; F fluxes to LDS, transverse G increments accumulated into LDS;
+//   3. y-phase: the same per column, producing G fluxes and the transverse F
+//      increments;
+//   4. conservative update of the interior into the output buffer and the
+//      per-patch CFL maximum.
+// This is step2.f90:7-169 + flux2.f:25-237 + inlinelimiter.f:4-108 +
+// stepgrid.f:146-173 for every patch of a level in one launch.
+//
+// Accumulation order follows the reference exactly, so an FP64 instantiation
+// built without FMA contraction reproduces it bit for bit:
+//   fm(i,j) = (F_x(i,j) + Gy2(col i-1)(j)) + Gy1(col i)(j)   (step2.f90:117,161,163)
+//   gm(i,j) = (Gx2(row j-1)(i) + Gx1(row j)(i)) + F_y(i,j)   (step2.f90:119,121,159)
+//   gadd(k,1) = (0 - 0.5 dtdx B(A-dq)(k+1)) - 0.5 dtdx B(A+dq)(k)  (flux2.f:207-233)
+// Row (column) contributions of different rows are applied in two
+// barrier-separated steps, rows in ascending order, as the reference's
+// j (i) loop does.
+#pragma once
+#include "level.hpp"
+
+namespace amrgpu {
+
+struct SweepArgs {
+    const void* qin;                // level buffer, time t (ghosts filled)
+    void* qout;                     // level buffer, receives the interior at t+dt
+    const PatchDesc* patches;
+    const TileDesc* tiles;
+    int32_t ntiles;
+    int32_t pad0;
+    double dt, dx, dy;
+    MethodArgs meth;
+    unsigned long long* cfl_patch;  // per-patch max Courant number (double bits)
+    double* flux_out;               // optional: fm,fp,gm,gp per patch (tests)
+    const int64_t* flux_off;        // element offset of each patch in flux_out
+};
+
+template <typename T>
+__device__ __forceinline__ T lane_down(T v, int w) { return __shfl_down(v, 1, w); }
+template <typename T>
+__device__ __forceinline__ T lane_up(T v, int w) { return __shfl_up(v, 1, w); }
+
+// limiter function of inlinelimiter.f:58-94
+template <typename T>
+__device__ __forceinline__ T phi_limiter(int mth, T r) {
+    switch (mth) {
+    case 1: return dmax1(T(0), dmin1(T(1), r));
+    case 2: return dmax1(dmax1(T(0), dmin1(T(1), T(2) * r)), dmin1(T(2), r));
+    case 3: return (r + fabs(r)) / (T(1) + fabs(r));
+    case 4: {
+        T c = (T(1) + r) / T(2);
+        return dmax1(T(0), dmin1(dmin1(c, T(2)), T(2) * r));
+    }
+    case 5: return r;
+    default: return T(1);
+    }
+}
+
+// Results of one interface lane of a segment.
+template <class Sol, typename T> struct LaneResult {
+    T fadm[Sol::NV], fadp[Sol::NV];   // faddm / faddp at this interface
+    T g1[Sol::NV], g2[Sol::NV];       // gadd(.,1) / gadd(.,2) of the cell right of it
+    T cfl;
+};
+
+// flux2.f for one interface.  `inner` marks interfaces 1..n+1 of the slice
+// (the ones flux2 limits and uses); lanes outside still run the solver on
+// clamped, valid states so the shuffles see defined values.
+template <class Sol, typename T>
+__device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T* ql,
+                                           const T* qr, bool inner, int w, T dtdx,
+                                           LaneResult<Sol, T>& o) {
+    constexpr int NV = Sol::NV, NW = Sol::NW;
+    typename Sol::Cache cache = Sol::prep(M.ph, ixy, ql, qr);
+    T wave[NW][NV], s[NW], amdq[NV], apdq[NV];
+    Sol::rpn2(M.ph, ixy, ql, qr, nullptr, nullptr, cache, wave, s, amdq, apdq);
+
+#pragma unroll
+    for (int m = 0; m < NV; m++) {
+        o.fadm[m] = T(0) + amdq[m];
+        o.fadp[m] = T(0) - apdq[m];
+    }
+    T cfl = T(0);
+    if (inner) {
+#pragma unroll
+        for (int mw = 0; mw < NW; mw++) cfl = dmax1(dmax1(cfl, dtdx * s[mw]), -dtdx * s[mw]);
+    }
+    o.cfl = cfl;
+
+    T cqxx[NV];
+#pragma unroll
+    for (int m = 0; m < NV; m++) cqxx[m] = T(0);
+    if (M.order != 1) {
+#pragma unroll
+        for (int mw = 0; mw < NW; mw++) {
+            T wn2 = T(0), dotr = T(0);
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+                T wnext = lane_down(wave[mw][m], w);
+                wn2 = wn2 + wave[mw][m] * wave[mw][m];
+                dotr = dotr + wave[mw][m] * wnext;
+            }
+            T dotl = lane_up(dotr, w);
+            const int mth = M.mthlim[mw];
+            if (mth != 0 && inner && wn2 != T(0)) {
+                T r = (s[mw] > T(0)) ? dotl / wn2 : dotr / wn2;
+                T wl = phi_limiter(mth, r);
+#pragma unroll
+                for (int m = 0; m < NV; m++) wave[mw][m] = wl * wave[mw][m];
+            }
+        }
+        const T dtdxave = T(0.5) * (dtdx + dtdx);
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+            T c = T(0);
+#pragma unroll
+            for (int mw = 0; mw < NW; mw++) {
+                T abs_sign = M.use_fwaves ? copysign(T(1), s[mw]) : fabs(s[mw]);
+                c = c + abs_sign * (T(1) - fabs(s[mw]) * dtdxave) * wave[mw][m];
+            }
+            cqxx[m] = c;
+            o.fadm[m] = o.fadm[m] + T(0.5) * c;
+            o.fadp[m] = o.fadp[m] + T(0.5) * c;
+        }
+    }
+
+    if (M.transverse != 0) {
+        if (M.order > 1 && M.transverse == 2) {
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+                amdq[m] = amdq[m] + cqxx[m];
+                apdq[m] = apdq[m] - cqxx[m];
+            }
+        }
+        T bm1[NV], bp1[NV], bm2[NV], bp2[NV];
+        Sol::rpt2(M.ph, ixy, 1, ql, qr, nullptr, nullptr, nullptr, cache, amdq, bm1, bp1);
+        Sol::rpt2(M.ph, ixy, 2, ql, qr, nullptr, nullptr, nullptr, cache, apdq, bm2, bp2);
+        const T hdt = T(0.5) * dtdx;
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+            T nbm = lane_down(bm1[m], w);   // imp=1 result of interface k+1
+            T nbp = lane_down(bp1[m], w);
+            o.g1[m] = (T(0) - hdt * nbm) - hdt * bm2[m];
+            o.g2[m] = (T(0) - hdt * nbp) - hdt * bp2[m];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NV; m++) { o.g1[m] = T(0); o.g2[m] = T(0); }
+    }
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = dmax1(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// LDS bytes needed for a tile of tx * ty cells (compute type T)
+__host__ __device__ inline int sweep_qs_stride(int tx) { return (tx + 4) | 1; }
+template <typename T>
+__host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty) {
+    size_t qs = (size_t)nv * (ty + 4) * sweep_qs_stride(tx);
+    size_t f = 2 * (size_t)nv * ty * (tx + 1);
+    size_t g = 2 * (size_t)nv * (ty + 1) * tx;
+    return (qs + f + g) * sizeof(T);
+}
+
+template <class Sol, typename T, typename S>
+__global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
+    constexpr int NV = Sol::NV;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* smem = reinterpret_cast<T*>(smem_raw);
+
+    const TileDesc td = a.tiles[blockIdx.x];
+    const PatchDesc pd = a.patches[td.patch];
+    const int tx = td.tx, ty = td.ty;
+    const int qsx = sweep_qs_stride(tx), qsy = ty + 4;
+    T* qs = smem;                          // [NV][qsy][qsx], cell (k,r) at [r+1][k+1]
+    T* fm = qs + NV * qsy * qsx;           // [NV][ty][tx+1]  x-edge e=1..tx+1, row j
+    T* fp = fm + NV * ty * (tx + 1);
+    T* gm = fp + NV * ty * (tx + 1);       // [NV][ty+1][tx]  y-edge e=1..ty+1, col k
+    T* gp = gm + NV * (ty + 1) * tx;
+    const int fstr = tx + 1;               // row stride of fm/fp
+    const int fpl = ty * (tx + 1);         // component stride of fm/fp
+    const int gpl = (ty + 1) * tx;         // component stride of gm/gp
+
+    const int tid = threadIdx.x;
+    const S* qin = static_cast<const S*>(a.qin) + pd.off;
+
+    // 1. stage tile + halo; zero gm (the x-phase accumulates into it)
+    {
+        const int w4 = tx + 4, n = NV * qsy * w4;
+        for (int idx = tid; idx < n; idx += 256) {
+            int m = idx / (qsy * w4);
+            int rem = idx - m * qsy * w4;
+            int rr = rem / w4, kk = rem - rr * w4;
+            int li = (td.j0 + rr) * pd.mitot + (td.i0 + kk);   // patch-local, ghost-origin
+            qs[(m * qsy + rr) * qsx + kk] = T(qin[(int64_t)m * pd.plane + li]);
+        }
+        for (int idx = tid; idx < NV * gpl; idx += 256) gm[idx] = T(0);
+    }
+    __syncthreads();
+
+    const double dtdx_d = a.dt / a.dx, dtdy_d = a.dt / a.dy;
+    const T dtdx = T(dtdx_d), dtdy = T(dtdy_d);
+    T cfl = T(0);
+
+    // 2. x-phase: segment = row r in [0, ty+1], lane = interface c in [0, tx+2]
+    {
+        const int w = td.wx, rows_per = 256 / w;
+        const int seg = tid / w, c = tid - seg * w;
+        const int npass = (ty + 2 + rows_per - 1) / rows_per;
+        for (int pass = 0; pass < npass; pass++) {
+            const int r = pass * rows_per + seg;
+            const bool live = (r <= ty + 1) && (c <= tx + 2);
+            const int rc = r <= ty + 1 ? r : ty + 1;
+            const int cc = c <= tx + 2 ? c : tx + 2;
+            T ql[NV], qr[NV];
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+                ql[m] = qs[(m * qsy + rc + 1) * qsx + cc];       // cell c-1
+                qr[m] = qs[(m * qsy + rc + 1) * qsx + cc + 1];   // cell c
+            }
+            const bool inner = live && c >= 1 && c <= tx + 1;
+            LaneResult<Sol, T> o;
+            flux2_lane<Sol, T>(a.meth, 1, ql, qr, inner, w, dtdx, o);
+            cfl = dmax1(cfl, o.cfl);
+            if (inner && r >= 1 && r <= ty) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    fm[m * fpl + (r - 1) * fstr + (c - 1)] = o.fadm[m];
+                    fp[m * fpl + (r - 1) * fstr + (c - 1)] = o.fadp[m];
+                }
+            }
+            const bool cell = live && c >= 1 && c <= tx;       // cell k = c
+            // step A: gadd(.,2) of row r goes to edge r+1
+            if (a.meth.transverse != 0 && cell && r <= ty) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    T* g = &gm[m * gpl + r * tx + (c - 1)];
+                    *g = *g + o.g2[m];
+                }
+            }
+            __syncthreads();
+            // step B: gadd(.,1) of row r goes to edge r
+            if (a.meth.transverse != 0 && cell && r >= 1) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    T* g = &gm[m * gpl + (r - 1) * tx + (c - 1)];
+                    *g = *g + o.g1[m];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // 3. y-phase: segment = column c in [0, tx+1], lane = interface e in [0, ty+2]
+    {
+        const int w = td.wy, cols_per = 256 / w;
+        const int seg = tid / w, e = tid - seg * w;
+        const int npass = (tx + 2 + cols_per - 1) / cols_per;
+        for (int pass = 0; pass < npass; pass++) {
+            const int c = pass * cols_per + seg;
+            const bool live = (c <= tx + 1) && (e <= ty + 2);
+            const int cc = c <= tx + 1 ? c : tx + 1;
+            const int ee = e <= ty + 2 ? e : ty + 2;
+            T ql[NV], qr[NV];
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+                ql[m] = qs[(m * qsy + ee) * qsx + cc + 1];       // cell (c, e-1)
+                qr[m] = qs[(m * qsy + ee + 1) * qsx + cc + 1];   // cell (c, e)
+            }
+            const bool inner = live && e >= 1 && e <= ty + 1;
+            LaneResult<Sol, T> o;
+            flux2_lane<Sol, T>(a.meth, 2, ql, qr, inner, w, dtdy, o);
+            cfl = dmax1(cfl, o.cfl);
+            if (inner && c >= 1 && c <= tx) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    const int gi = m * gpl + (e - 1) * tx + (c - 1);
+                    T xg = gm[gi];
+                    gm[gi] = xg + o.fadm[m];
+                    gp[gi] = xg + o.fadp[m];
+                }
+            }
+            const bool cell = live && e >= 1 && e <= ty;       // cell j = e
+            // step A: gadd(.,2) of column c goes to x-edge c+1
+            if (a.meth.transverse != 0 && cell && c <= tx) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    const int fi = m * fpl + (e - 1) * fstr + c;
+                    fm[fi] = fm[fi] + o.g2[m];
+                    fp[fi] = fp[fi] + o.g2[m];
+                }
+            }
+            __syncthreads();
+            // step B: gadd(.,1) of column c goes to x-edge c
+            if (a.meth.transverse != 0 && cell && c >= 1) {
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    const int fi = m * fpl + (e - 1) * fstr + (c - 1);
+                    fm[fi] = fm[fi] + o.g1[m];
+                    fp[fi] = fp[fi] + o.g1[m];
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // 4. conservative update (stepgrid.f:154-165) and CFL
+    {
+        S* qout = static_cast<S*>(a.qout) + pd.off;
+        const int n = tx * ty;
+        for (int idx = tid; idx < n; idx += 256) {
+            const int j = idx / tx, k = idx - j * tx;   // 0-based interior of tile
+            const int li = (td.j0 + j + 2) * pd.mitot + (td.i0 + k + 2);
+#pragma unroll
+            for (int m = 0; m < NV; m++) {
+                T q = qs[(m * qsy + j + 2) * qsx + k + 2];
+                T dx_ = fm[m * fpl + j * fstr + (k + 1)] - fp[m * fpl + j * fstr + k];
+                T dy_ = gm[m * gpl + (j + 1) * tx + k] - gp[m * gpl + j * tx + k];
+                qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
+            }
+        }
+        if (a.flux_out) {
+            // fm,fp,gm,gp of the patch, each (NV, my+4, mx+4) like the
+            // reference's flux arrays: x-edge e of row j -> fm(e, j)
+            double* fo = a.flux_out + a.flux_off[td.patch];
+            const int64_t fpln = (int64_t)pd.plane;
+            const int64_t arr = NV * fpln;
+            for (int idx = tid; idx < NV * ty * (tx + 1); idx += 256) {
+                int m = idx / fpl, rem = idx - m * fpl, j = rem / fstr, e = rem - j * fstr;
+                int64_t li = (int64_t)(td.j0 + j + 2) * pd.mitot + (td.i0 + e + 2) + m * fpln;
+                fo[li] = double(fm[idx]);
+                fo[arr + li] = double(fp[idx]);
+            }
+            for (int idx = tid; idx < NV * gpl; idx += 256) {
+                int m = idx / gpl, rem = idx - m * gpl, e = rem / tx, k = rem - e * tx;
+                int64_t li = (int64_t)(td.j0 + e + 2) * pd.mitot + (td.i0 + k + 2) + m * fpln;
+                fo[2 * arr + li] = double(gm[idx]);
+                fo[3 * arr + li] = double(gp[idx]);
+            }
+        }
+    }
+    cfl = wave_max(cfl);
+    if ((tid & 63) == 0) atomic_max_nonneg(&a.cfl_patch[td.patch], double(cfl));
+}
+
+}  // namespace amrgpu

@@ -1,0 +1,80 @@
+"""Synthetic workloads for the level update (batched 2-D Euler).
+
+The batched Euler configuration advances one level of N patches whose sizes
+mx, my are drawn independently and uniformly from the even numbers in
+[8, 128] (fine grids are whole coarse cells at refinement ratio 2), with
+numpy default_rng(seed).  Patches are shelf-packed without overlap into a
+square index space on the unit square; the state is the four-quadrant
+Riemann problem of examples/euler_2d_quadrants/qinit.f (rounded states,
+split at (0.8, 0.8)) sampled at cell centres, times a seeded +-1e-3 relative
+perturbation.
+"""
+import numpy as np
+
+GAMMA = 1.4
+QUADRANTS = {1: (1.5, 0.0, 0.0, 1.5), 2: (0.5323, 1.206, 0.0, 0.3),
+             3: (0.138, 1.206, 1.206, 0.029), 4: (0.5323, 0.0, 1.206, 0.3)}
+
+
+def quadrant_state(x, y, gamma=GAMMA, xs=0.8, ys=0.8):
+    """Conserved (rho, rho u, rho v, E) of the four-quadrant problem."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    iq = np.where(x >= xs, np.where(y >= ys, 1, 4), np.where(y >= ys, 2, 3))
+    q = np.zeros(x.shape + (4,))
+    for k, (r, u, v, p) in QUADRANTS.items():
+        m = iq == k
+        q[m, 0] = r
+        q[m, 1] = r * u
+        q[m, 2] = r * v
+        q[m, 3] = p / (gamma - 1.0) + 0.5 * r * (u * u + v * v)
+    return q
+
+
+def patch_sizes(n, seed=20261202, lo=8, hi=128):
+    rng = np.random.default_rng(seed)
+    mx = 2 * rng.integers(lo // 2, hi // 2 + 1, size=n)
+    my = 2 * rng.integers(lo // 2, hi // 2 + 1, size=n)
+    return mx.astype(np.int64), my.astype(np.int64)
+
+
+def shelf_pack(mx, my, width):
+    """Place boxes left to right in shelves of height max(my of the shelf).
+    Returns (n, 4) int array of (ilo, ihi, jlo, jhi) and the used height."""
+    boxes = np.zeros((len(mx), 4), dtype=np.int64)
+    x = y = shelf_h = 0
+    for k in range(len(mx)):
+        if x + mx[k] > width:
+            y += shelf_h
+            x = shelf_h = 0
+        boxes[k] = (x, x + mx[k] - 1, y, y + my[k] - 1)
+        x += mx[k]
+        shelf_h = max(shelf_h, my[k])
+    return boxes, y + shelf_h
+
+
+def batched_euler_level(n=4096, seed=20261202, width=None):
+    """Layout of the batched Euler level: boxes (global indices), mesh width
+    h (index space width -> unit square), sizes."""
+    mx, my = patch_sizes(n, seed)
+    if width is None:
+        # square-ish packing: area / fill factor of the shelves
+        area = float((mx * my).sum())
+        width = int(np.ceil(np.sqrt(area / 0.55) / 64.0) * 64)
+    boxes, height = shelf_pack(mx, my, width)
+    side = max(width, height)
+    return boxes, 1.0 / side, mx, my
+
+
+def patch_initial_state(box, h, seed, nghost=2, perturb=1e-3):
+    """Initial q of one patch including ghost cells, shape (my+4, mx+4, 4)."""
+    ilo, ihi, jlo, jhi = [int(v) for v in box]
+    i = np.arange(ilo - nghost, ihi + nghost + 1)
+    j = np.arange(jlo - nghost, jhi + nghost + 1)
+    xc = (i + 0.5) * h
+    yc = (j + 0.5) * h
+    X, Y = np.meshgrid(xc, yc)
+    q = quadrant_state(X, Y)
+    rng = np.random.default_rng(seed)
+    q *= 1.0 + perturb * (2.0 * rng.random(q.shape[:2] + (1,)) - 1.0)
+    return q

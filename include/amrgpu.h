@@ -1,0 +1,115 @@
+/*
+ * amrgpu.h -- C ABI of the MI355X level-update library (libamrgpu.so).
+ *
+ * Drop-in boundary for AMRClaw's per-level patch update.  The reference
+ * driver (tick.f) calls advanc/update/flagger; a Fortran ISO_C_BINDING shim
+ * replaces those routines and calls the entry points below.  All arguments
+ * are plain C types; arrays are host pointers unless stated otherwise.
+ * Solution arrays cross the boundary in the reference's layout: FP64,
+ * q(nvar, 1-nghost:mx+nghost, 1-nghost:my+nghost), component fastest
+ * (update.f:31).  Indices are the reference's global 0-based cell indices of
+ * a level (node(ndilo..ndjhi, mptr), amr_module.f90:100-112).
+ *
+ * Every function returns 0 on success and a nonzero code on failure; the
+ * message is available from amrgpu_last_error().  No C++ exception crosses
+ * the boundary.  The library is not thread-safe: call it from one host
+ * thread (the shim replaces the reference's OpenMP patch loops, advanc.f:60,
+ * 106, with device parallelism).
+ */
+#ifndef AMRGPU_H
+#define AMRGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AMRGPU_OK = 0,
+    AMRGPU_ERR_ARG = 1,       /* invalid argument                          */
+    AMRGPU_ERR_STATE = 2,     /* call out of order (e.g. before init)      */
+    AMRGPU_ERR_HIP = 3,       /* HIP runtime error                          */
+    AMRGPU_ERR_UNSUPPORTED = 4
+};
+
+enum { AMRGPU_ADVECTION = 1, AMRGPU_ACOUSTICS = 2, AMRGPU_EULER = 3 };
+enum { AMRGPU_FP64 = 0, AMRGPU_FP32 = 1 };
+
+/* Run parameters: the amr_module.f90 fields the level update reads, filled
+ * by the shim from claw.data/amr.data (amr2.f90:152-392), plus the physics
+ * constants the reference's solvers read from common /cparam/. */
+typedef struct amrgpu_config {
+    int solver;          /* AMRGPU_ADVECTION / _ACOUSTICS / _EULER          */
+    int nvar;            /* num_eqn                                           */
+    int mwaves;          /* num_waves                                         */
+    int naux;            /* num_aux (must be 0: no solver here reads aux)    */
+    int nghost;          /* num_ghost (must be 2)                             */
+    int method[7];       /* method(1:7): [1]=order, [2]=transverse            */
+    int mthlim[8];       /* limiter per wave: 0 none 1 minmod 2 superbee
+                            3 van Leer 4 MC 5 Beam-Warming                    */
+    int mcapa;           /* capa_index (must be 0)                            */
+    int use_fwaves;
+    double cfl;          /* cfl_desired                                       */
+    double cflv1;        /* cfl_max                                           */
+    int mthbc[4];        /* bc_lower/upper: 1 extrap 2 periodic 3 wall        */
+    double xlower, xupper, ylower, yupper;
+    double ubar, vbar;   /* advection velocities                              */
+    double rho, bulk;    /* acoustics                                         */
+    double gamma;        /* Euler                                             */
+    int efix;            /* Euler entropy fix                                 */
+    int precision;       /* AMRGPU_FP64 or AMRGPU_FP32 (bulk arithmetic)     */
+} amrgpu_config;
+
+/* Initialise (or re-initialise) the library on the current HIP device.
+ * Replaces the parameter set-up the reference does in amr2.f90:148-392. */
+int amrgpu_init(const amrgpu_config* cfg);
+
+/* Release all device memory and state. */
+int amrgpu_finalize(void);
+
+/* Copy the last error message (NUL-terminated, truncated to len). */
+int amrgpu_last_error(char* buf, int len);
+
+/* Tile-size caps of the sweep (autotuner knob); 0 keeps the current value. */
+int amrgpu_set_tiling(int txmax, int tymax);
+
+/* Single-patch time step: stepgrid.f:24 (step2 + conservative update + dtnew)
+ * on one grid with ghost cells already filled.  q_in/q_out: (nvar, mx+4,
+ * my+4) AoS FP64; fm/fp/gm/gp: same shape, may all be NULL (stepgrid's flux
+ * outputs, only interior-row x-edges and interior-column y-edges are set);
+ * returns cflgrid and dtnew (rinfinity when cflgrid == 0). */
+int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out,
+                    double dx, double dy, double dt, double* fm, double* fp,
+                    double* gm, double* gp, double* cflgrid, double* dtnew);
+
+/* Define the grids of one level (after regrid): global index boxes
+ * [ilo..ihi] x [jlo..jhi] and mesh widths hx, hy (hxposs/hyposs).
+ * Replaces the layout bookkeeping of regrid.f:72-118 on the device side;
+ * allocates the level's two solution buffers (new and old time). */
+int amrgpu_set_level(int level, int ngrids, const int* ilo, const int* ihi,
+                     const int* jlo, const int* jhi, double hx, double hy);
+
+/* Upload / download one grid's solution (which: 0 = new time, store1;
+ * 1 = old time, store2), AoS FP64 (nvar, mx+4, my+4) including ghosts. */
+int amrgpu_put_grid(int level, int grid, int which, const double* q);
+int amrgpu_get_grid(int level, int grid, int which, double* q);
+
+/* Advance every grid of a level by dt from its current ghost-filled
+ * state: the stepgrid part of par_advanc (advanc.f:170-292) for all grids in
+ * one launch.  new -> old, updated interior -> new.  Returns the level's
+ * max Courant number (cfl_level) and min suggested dt (dtlevnew,
+ * advanc.f:124-126). */
+int amrgpu_sweep_level(int level, double dt, double* cfl_level,
+                       double* dtlevnew);
+
+/* Device timing on the library's stream: record event `slot` (0..15) and
+ * read the elapsed milliseconds between two recorded slots. */
+int amrgpu_event_record(int slot);
+int amrgpu_event_elapsed(int slot_a, int slot_b, float* ms);
+int amrgpu_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -105,6 +105,12 @@ def ref():
                                    C.c_double, C.c_double, _dp, _dp, _dp, _dp,
                                    C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
+        R.ref_stepgrid_batch.argtypes = [
+            C.c_int, C.c_int, C.c_int,
+            np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+            np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"),
+            np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+            _dp, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]
         _ref = R
     return _ref
 
@@ -172,6 +178,22 @@ def ref_stepgrid(p, q, dx, dy, dt, aux=None, time=0.0):
     R.ref_stepgrid(p.meqn, p.maux, p.mbc, mx, my, qn, aux, dx, dy, dt, time,
                    *f, C.byref(dtn), C.byref(cfl))
     return qn, f, cfl.value, dtn.value
+
+
+def ref_stepgrid_batch(p, qs, dx, dy, dt):
+    """The reference's stepgrid over a list of patches with its OpenMP patch
+    loop (advanc.f:106-129); patches are updated in place in a packed copy,
+    which is returned with the level's max Courant number."""
+    R = _ref_setup(p)
+    mys = np.array([q.shape[0] - 2 * p.mbc for q in qs], dtype=np.int32)
+    mxs = np.array([q.shape[1] - 2 * p.mbc for q in qs], dtype=np.int32)
+    sizes = np.array([q.size for q in qs], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    packed = np.concatenate([np.ascontiguousarray(q).ravel() for q in qs])
+    cfl = C.c_double()
+    R.ref_stepgrid_batch(len(qs), p.meqn, p.mbc, mxs, mys, offs, packed, dx,
+                         dy, dt, C.byref(cfl))
+    return packed, offs, cfl.value
 
 
 def rpn2(p, ixy, ql, qr):

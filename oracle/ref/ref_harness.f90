@@ -98,4 +98,35 @@ contains
     cflgrid = cfl_level
   end subroutine ref_stepgrid
 
+  ! The reference's OpenMP patch loop (advanc.f:106-129) over a batch of
+  ! patches: stepgrid on each, dynamic schedule, one patch per chunk.
+  ! q holds the patches back to back, each (nvar, mx+2mbc, my+2mbc).
+  subroutine ref_stepgrid_batch(npatch, nvar, mbc, mxs, mys, offs, q, dx, dy, dt, &
+       cflmax) bind(C, name="ref_stepgrid_batch")
+    integer(c_int), value :: npatch, nvar, mbc
+    integer(c_int), intent(in) :: mxs(npatch), mys(npatch)
+    integer(c_int64_t), intent(in) :: offs(npatch)
+    real(c_double) :: q(*)
+    real(c_double), value :: dx, dy, dt
+    real(c_double) :: cflmax
+    integer :: k, mitot, mjtot
+    real(kind=8) :: dtnew, aux1(1), cmax
+    real(kind=8), allocatable :: fm(:), fp(:), gm(:), gp(:)
+    cfl_level = 0.d0
+    cmax = 0.d0
+    !$omp parallel do schedule(dynamic,1) private(k,mitot,mjtot,fm,fp,gm,gp,dtnew,aux1) &
+    !$omp& reduction(max:cmax)
+    do k = 1, npatch
+       mitot = mxs(k) + 2*mbc
+       mjtot = mys(k) + 2*mbc
+       allocate(fm(nvar*mitot*mjtot), fp(nvar*mitot*mjtot), gm(nvar*mitot*mjtot), &
+            gp(nvar*mitot*mjtot))
+       call stepgrid(q(offs(k)+1), fm, fp, gm, gp, mitot, mjtot, mbc, dt, dtnew, &
+            dx, dy, nvar, 0.d0, 0.d0, 0.d0, k, 0, aux1)
+       deallocate(fm, fp, gm, gp)
+    end do
+    !$omp end parallel do
+    cflmax = max(cmax, cfl_level)
+  end subroutine ref_stepgrid_batch
+
 end module ref_harness

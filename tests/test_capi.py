@@ -1,0 +1,65 @@
+"""CPU checks of the C ABI: the built library exports every entry point that
+include/amrgpu.h declares, the ctypes config struct matches the C struct,
+and argument validation fails cleanly before any device call."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from amrclaw_amd import capi
+
+HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "include", "amrgpu.h")
+
+
+def declared_functions():
+    txt = open(HDR).read()
+    return sorted(set(re.findall(r"^int\s+(amrgpu_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "amrgpu_init" in names and "amrgpu_sweep_level" in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_config_struct_layout():
+    """sizeof(amrgpu_config) as laid out by the C compiler equals ctypes'."""
+    import subprocess
+    import tempfile
+    src = '#include "%s"\n#include <stdio.h>\n#include <stddef.h>\n' \
+          'int main(void){printf("%%zu %%zu %%zu\\n", sizeof(amrgpu_config),' \
+          ' offsetof(amrgpu_config, gamma), offsetof(amrgpu_config, precision));return 0;}\n' % HDR
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "t")
+        subprocess.run(["gcc", c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    assert int(out[0]) == C.sizeof(capi.AmrgpuConfig)
+    assert int(out[1]) == capi.AmrgpuConfig.gamma.offset
+    assert int(out[2]) == capi.AmrgpuConfig.precision.offset
+
+
+@pytest.mark.parametrize("field,value", [("solver", 9), ("nvar", 7), ("nghost", 3),
+                                         ("mcapa", 1), ("precision", 5)])
+def test_init_rejects_bad_config(field, value):
+    cfg = capi.make_config(capi.EULER)
+    setattr(cfg, field, value)
+    rc = capi.lib().amrgpu_init(C.byref(cfg))
+    assert rc != 0
+    buf = C.create_string_buffer(256)
+    capi.lib().amrgpu_last_error(buf, 256)
+    assert buf.value
+
+
+def test_calls_before_init_fail_cleanly():
+    capi.lib().amrgpu_finalize()
+    assert capi.lib().amrgpu_sweep_level(1, 0.1, None, None) != 0
+    assert capi.lib().amrgpu_synchronize() != 0
