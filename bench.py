@@ -42,7 +42,23 @@ def parse():
                     help="skip the FP64 comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=8.0,
                     help="CPU baseline budget (0 disables it)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB "
+                         "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: under rocprofv3 --pmc its FETCH_SIZE and
+    WRITE_SIZE per dispatch give the factor from counter units to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "dispatches": 5}))
 
 
 def setup_level(capi, workload, precision, npatch, txmax, tymax):
@@ -121,6 +137,9 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     import torch
     from amrclaw_amd import capi, workload
 
