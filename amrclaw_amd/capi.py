@@ -64,6 +64,9 @@ def lib():
                                        C.c_double, C.c_double]
         L.amrgpu_put_grid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
         L.amrgpu_get_grid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
+        _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.amrgpu_put_level.argtypes = [C.c_int, C.c_int, _dp, _lp]
+        L.amrgpu_get_level.argtypes = [C.c_int, C.c_int, _dp, _lp]
         L.amrgpu_sweep_level.argtypes = [C.c_int, C.c_double, C.c_void_p,
                                          C.c_void_p]
         L.amrgpu_event_record.argtypes = [C.c_int]
@@ -152,6 +155,24 @@ def get_grid(level, grid, shape, which=0):
     out = np.empty(shape, dtype=np.float64)
     _check(lib().amrgpu_get_grid(level, grid, which, out))
     return out
+
+
+def put_level(level, qs, which=0):
+    """Upload all grids of a level: qs is a list of (my+4, mx+4, nvar)
+    arrays in set_level order (packed like the reference's alloc)."""
+    sizes = np.array([q.size for q in qs], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    packed = np.concatenate([np.ascontiguousarray(q, dtype=np.float64).ravel() for q in qs])
+    _check(lib().amrgpu_put_level(level, which, packed, offs))
+
+
+def get_level(level, shapes, which=0):
+    """Download all grids of a level as a list of arrays of the given shapes."""
+    sizes = np.array([int(np.prod(s)) for s in shapes], dtype=np.int64)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    packed = np.empty(int(sizes.sum()), dtype=np.float64)
+    _check(lib().amrgpu_get_level(level, which, packed, offs))
+    return [packed[o:o + n].reshape(s) for o, n, s in zip(offs, sizes, shapes)]
 
 
 def sweep_level(level, dt, want_cfl=True):

@@ -18,6 +18,9 @@
 namespace amrgpu {
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n,
                            hipStream_t st);
+hipError_t launch_aos_soa(bool to_soa, int esize, double* stage, const int64_t* stage_off,
+                          const int64_t* off, const int32_t* plane, const int64_t* cell_start,
+                          int npatch, int nv, int64_t ncell, void* level, hipStream_t st);
 }
 
 using namespace amrgpu;
@@ -41,6 +44,11 @@ struct Level {
     int64_t nelem = 0;
     unsigned long long* d_cfl = nullptr;
     size_t lds = 0;
+    // bulk transfer tables: patch element offsets, planes, first cell
+    int64_t* d_off = nullptr;
+    int32_t* d_plane = nullptr;
+    int64_t* d_cell_start = nullptr;
+    int64_t ncell_total = 0;       // cells incl. ghosts over all patches
 };
 
 struct Context {
@@ -81,6 +89,9 @@ void free_level(Level& L) {
     for (auto& b : L.buf)
         if (b) hipFree(b);
     if (L.d_cfl) hipFree(L.d_cfl);
+    if (L.d_off) hipFree(L.d_off);
+    if (L.d_plane) hipFree(L.d_plane);
+    if (L.d_cell_start) hipFree(L.d_cell_start);
     L = Level();
 }
 
@@ -178,6 +189,22 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
             HIPCHK(hipMemset(b, 0, (size_t)L.nelem * g.esize));
         }
         HIPCHK(hipMalloc(&L.d_cfl, ngrids * sizeof(unsigned long long)));
+        std::vector<int64_t> off(ngrids), cs(ngrids);
+        std::vector<int32_t> pl(ngrids);
+        int64_t c = 0;
+        for (int p = 0; p < ngrids; p++) {
+            off[p] = L.patches[p].off;
+            pl[p] = L.patches[p].plane;
+            cs[p] = c;
+            c += L.patches[p].plane;
+        }
+        L.ncell_total = c;
+        HIPCHK(hipMalloc(&L.d_off, ngrids * sizeof(int64_t)));
+        HIPCHK(hipMalloc(&L.d_plane, ngrids * sizeof(int32_t)));
+        HIPCHK(hipMalloc(&L.d_cell_start, ngrids * sizeof(int64_t)));
+        HIPCHK(hipMemcpy(L.d_off, off.data(), ngrids * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.d_plane, pl.data(), ngrids * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(L.d_cell_start, cs.data(), ngrids * sizeof(int64_t), hipMemcpyHostToDevice));
     }
     return plan_level(L);
 }
@@ -220,6 +247,56 @@ int get_grid(Level& L, int grid, int which, double* q) {
             for (int m = 0; m < nv; m++) q[c * nv + m] = (double)h[m * pd.plane + c];
     }
     return AMRGPU_OK;
+}
+
+// Bulk transfer of a whole level between the reference's heap layout
+// (alloc(loc + ...), one AoS FP64 block per grid at offsets[p]) and the
+// device.  Grids are gathered into one staging buffer, copied in one
+// transfer and transposed on the device.
+int xfer_level(Level& L, bool to_dev, int which, double* alloc_base, const int64_t* offsets) {
+    if (L.ngrids == 0) return AMRGPU_OK;
+    const int nv = g.cfg.nvar;
+    std::vector<int64_t> soff(L.ngrids);
+    int64_t tot = 0;
+    for (int p = 0; p < L.ngrids; p++) {
+        soff[p] = tot;
+        tot += (int64_t)nv * L.patches[p].plane;
+    }
+    double* h_stage = nullptr;
+    double* d_stage = nullptr;
+    int64_t* d_soff = nullptr;
+    HIPCHK(hipHostMalloc((void**)&h_stage, tot * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipMalloc(&d_stage, tot * sizeof(double)));
+    HIPCHK(hipMalloc(&d_soff, L.ngrids * sizeof(int64_t)));
+    HIPCHK(hipMemcpy(d_soff, soff.data(), L.ngrids * sizeof(int64_t), hipMemcpyHostToDevice));
+    void* lvl = L.buf[which == 0 ? L.cur : 1 - L.cur];
+    int rc = AMRGPU_OK;
+    hipError_t e = hipSuccess;
+    if (to_dev) {
+        for (int p = 0; p < L.ngrids; p++)
+            std::memcpy(h_stage + soff[p], alloc_base + offsets[p],
+                        (size_t)nv * L.patches[p].plane * sizeof(double));
+        e = hipMemcpyAsync(d_stage, h_stage, tot * sizeof(double), hipMemcpyHostToDevice, g.stream);
+        if (e == hipSuccess)
+            e = launch_aos_soa(true, (int)g.esize, d_stage, d_soff, L.d_off, L.d_plane,
+                               L.d_cell_start, L.ngrids, nv, L.ncell_total, lvl, g.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    } else {
+        e = launch_aos_soa(false, (int)g.esize, d_stage, d_soff, L.d_off, L.d_plane,
+                           L.d_cell_start, L.ngrids, nv, L.ncell_total, lvl, g.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h_stage, d_stage, tot * sizeof(double), hipMemcpyDeviceToHost, g.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+        if (e == hipSuccess)
+            for (int p = 0; p < L.ngrids; p++)
+                std::memcpy(alloc_base + offsets[p], h_stage + soff[p],
+                            (size_t)nv * L.patches[p].plane * sizeof(double));
+    }
+    if (e != hipSuccess) rc = fail(AMRGPU_ERR_HIP, "level transfer: %s", hipGetErrorString(e));
+    hipFree(d_soff);
+    hipFree(d_stage);
+    hipHostFree(h_stage);
+    return rc;
 }
 
 int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev) {
@@ -418,6 +495,22 @@ int amrgpu_get_grid(int level, int grid, int which, double* q) {
     if (grid < 0 || grid >= L.ngrids || (which != 0 && which != 1) || !q)
         return fail(AMRGPU_ERR_ARG, "bad get_grid arguments");
     return get_grid(L, grid, which, q);
+}
+
+int amrgpu_put_level(int level, int which, const double* alloc_base, const int64_t* offsets) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if ((which != 0 && which != 1) || !alloc_base || !offsets)
+        return fail(AMRGPU_ERR_ARG, "bad put_level arguments");
+    return xfer_level(g.levels[level], true, which, const_cast<double*>(alloc_base), offsets);
+}
+
+int amrgpu_get_level(int level, int which, double* alloc_base, const int64_t* offsets) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if ((which != 0 && which != 1) || !alloc_base || !offsets)
+        return fail(AMRGPU_ERR_ARG, "bad get_level arguments");
+    return xfer_level(g.levels[level], false, which, alloc_base, offsets);
 }
 
 int amrgpu_sweep_level(int level, double dt, double* cfl_level, double* dtlevnew) {

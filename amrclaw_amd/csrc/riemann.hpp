@@ -45,9 +45,25 @@ template <typename T> __device__ __forceinline__ T dsqrt(T x);
 template <> __device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqrt(x); }
 template <> __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
 
+// Arithmetic policies.  Exact: IEEE division and square root, the
+// reference's FP64 arithmetic.  Fast (FP32 bulk path): hardware reciprocal
+// and square root (about 1 ulp), which removes the refinement sequences of
+// correctly rounded FP32 division from the solvers' inner loops.
+template <typename T> struct ExactMath {
+    using type = T;
+    __device__ static __forceinline__ T div(T a, T b) { return a / b; }
+    __device__ static __forceinline__ T sqrt(T x) { return dsqrt(x); }
+};
+struct FastMathF32 {
+    using type = float;
+    __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+
 // ----------------------------------------------------------------------------
-template <typename T> struct Advection {
+template <typename T, class Mth = ExactMath<T>> struct Advection {
     static constexpr int NV = 1, NW = 1;
+    using Math = Mth;
     struct Cache {};
     __device__ static Cache prep(const Physics&, int, const T*, const T*) { return {}; }
     __device__ static void rpn2(const Physics& ph, int ixy, const T* ql, const T* qr,
@@ -68,8 +84,9 @@ template <typename T> struct Advection {
 };
 
 // ----------------------------------------------------------------------------
-template <typename T> struct Acoustics {
+template <typename T, class Mth = ExactMath<T>> struct Acoustics {
     static constexpr int NV = 3, NW = 2;
+    using Math = Mth;
     struct Cache {};
     __device__ static Cache prep(const Physics&, int, const T*, const T*) { return {}; }
     __device__ static void rpn2(const Physics& ph, int ixy, const T* ql, const T* qr,
@@ -79,8 +96,8 @@ template <typename T> struct Acoustics {
         const T zz = T(ph.zz), cc = T(ph.cc);
         T d1 = qr[0] - ql[0];
         T d2 = qr[mu] - ql[mu];
-        T a1 = (-d1 + zz * d2) / (T(2) * zz);
-        T a2 = (d1 + zz * d2) / (T(2) * zz);
+        T a1 = Mth::div(-d1 + zz * d2, T(2) * zz);
+        T a2 = Mth::div(d1 + zz * d2, T(2) * zz);
         wave[0][0] = -a1 * zz; wave[0][mu] = a1; wave[0][mv] = T(0); s[0] = -cc;
         wave[1][0] = a2 * zz;  wave[1][mu] = a2; wave[1][mv] = T(0); s[1] = cc;
 #pragma unroll
@@ -94,40 +111,41 @@ template <typename T> struct Acoustics {
                                 const T asdq[NV], T bm[NV], T bp[NV]) {
         const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
         const T zz = T(ph.zz), cc = T(ph.cc);
-        T a1 = (-asdq[0] + zz * asdq[mv]) / (T(2) * zz);
-        T a2 = (asdq[0] + zz * asdq[mv]) / (T(2) * zz);
+        T a1 = Mth::div(-asdq[0] + zz * asdq[mv], T(2) * zz);
+        T a2 = Mth::div(asdq[0] + zz * asdq[mv], T(2) * zz);
         bm[0] = cc * a1 * zz; bm[mu] = T(0); bm[mv] = -cc * a1;
         bp[0] = cc * a2 * zz; bp[mu] = T(0); bp[mv] = cc * a2;
     }
 };
 
 // ----------------------------------------------------------------------------
-template <typename T> struct EulerRoe {
+template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
     static constexpr int NV = 4, NW = 4;
+    using Math = Mth;
     struct Cache { T u, v, enth, a, g1a2, euv, u2v2; };
 
     __device__ static Cache prep(const Physics& ph, int ixy, const T* L, const T* R) {
         const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
         const T gamma1 = T(ph.gamma) - T(1);
         Cache c;
-        T rhsqrtl = dsqrt(L[0]);
-        T rhsqrtr = dsqrt(R[0]);
-        T pl = gamma1 * (L[3] - T(0.5) * (L[1] * L[1] + L[2] * L[2]) / L[0]);
-        T pr = gamma1 * (R[3] - T(0.5) * (R[1] * R[1] + R[2] * R[2]) / R[0]);
+        T rhsqrtl = Mth::sqrt(L[0]);
+        T rhsqrtr = Mth::sqrt(R[0]);
+        T pl = gamma1 * (L[3] - Mth::div(T(0.5) * (L[1] * L[1] + L[2] * L[2]), L[0]));
+        T pr = gamma1 * (R[3] - Mth::div(T(0.5) * (R[1] * R[1] + R[2] * R[2]), R[0]));
         T rhsq2 = rhsqrtl + rhsqrtr;
-        c.u = (L[mu] / rhsqrtl + R[mu] / rhsqrtr) / rhsq2;
-        c.v = (L[mv] / rhsqrtl + R[mv] / rhsqrtr) / rhsq2;
-        c.enth = ((L[3] + pl) / rhsqrtl + (R[3] + pr) / rhsqrtr) / rhsq2;
+        c.u = Mth::div(Mth::div(L[mu], rhsqrtl) + Mth::div(R[mu], rhsqrtr), rhsq2);
+        c.v = Mth::div(Mth::div(L[mv], rhsqrtl) + Mth::div(R[mv], rhsqrtr), rhsq2);
+        c.enth = Mth::div(Mth::div(L[3] + pl, rhsqrtl) + Mth::div(R[3] + pr, rhsqrtr), rhsq2);
         c.u2v2 = c.u * c.u + c.v * c.v;
         T a2 = gamma1 * (c.enth - T(0.5) * c.u2v2);
-        c.a = dsqrt(a2);
-        c.g1a2 = gamma1 / a2;
+        c.a = Mth::sqrt(a2);
+        c.g1a2 = Mth::div(gamma1, a2);
         c.euv = c.enth - c.u2v2;
         return c;
     }
 
     __device__ static T pressure(T gamma1, T rho, T mx, T my, T en) {
-        return gamma1 * (en - T(0.5) * (mx * mx + my * my) / rho);
+        return gamma1 * (en - Mth::div(T(0.5) * (mx * mx + my * my), rho));
     }
 
     __device__ static void rpn2(const Physics& ph, int ixy, const T* L, const T* R,
@@ -142,7 +160,7 @@ template <typename T> struct EulerRoe {
         T d4 = R[3] - L[3];
         T a3 = c.g1a2 * (c.euv * d1 + u * d2 + v * d3 - d4);
         T a2 = d3 - v * d1;
-        T a4 = (d2 + (a - u) * d1 - a * a3) / (T(2) * a);
+        T a4 = Mth::div(d2 + (a - u) * d1 - a * a3, T(2) * a);
         T a1 = d1 - a3 - a4;
         wave[0][0] = a1; wave[0][mu] = a1 * (u - a); wave[0][mv] = a1 * v;
         wave[0][3] = a1 * (enth - u * a); s[0] = u - a;
@@ -162,19 +180,19 @@ template <typename T> struct EulerRoe {
         } else {
             T rhoim1 = L[0];
             T pim1 = pressure(gamma1, L[0], L[1], L[2], L[3]);
-            T cim1 = dsqrt(gamma * pim1 / rhoim1);
-            T s0 = L[mu] / rhoim1 - cim1;
+            T cim1 = Mth::sqrt(Mth::div(gamma * pim1, rhoim1));
+            T s0 = Mth::div(L[mu], rhoim1) - cim1;
             if (!(s0 >= T(0) && s[0] > T(0))) {
                 T rho1 = L[0] + wave[0][0];
                 T rhou1 = L[mu] + wave[0][mu];
                 T rhov1 = L[mv] + wave[0][mv];
                 T en1 = L[3] + wave[0][3];
                 T p1 = pressure(gamma1, rho1, rhou1, rhov1, en1);
-                T c1 = dsqrt(gamma * p1 / rho1);
-                T s1 = rhou1 / rho1 - c1;
+                T c1 = Mth::sqrt(Mth::div(gamma * p1, rho1));
+                T s1 = Mth::div(rhou1, rho1) - c1;
                 T sfract;
                 if (s0 < T(0) && s1 > T(0))
-                    sfract = s0 * (s1 - s[0]) / (s1 - s0);
+                    sfract = Mth::div(s0 * (s1 - s[0]), s1 - s0);
                 else if (s[0] < T(0))
                     sfract = s[0];
                 else
@@ -187,19 +205,19 @@ template <typename T> struct EulerRoe {
                         am[m] = am[m] + s[1] * wave[1][m] + s[2] * wave[2][m];
                     T rhoi = R[0];
                     T pi_ = pressure(gamma1, R[0], R[1], R[2], R[3]);
-                    T ci = dsqrt(gamma * pi_ / rhoi);
-                    T s3 = R[mu] / rhoi + ci;
+                    T ci = Mth::sqrt(Mth::div(gamma * pi_, rhoi));
+                    T s3 = Mth::div(R[mu], rhoi) + ci;
                     T rho2 = R[0] - wave[3][0];
                     T rhou2 = R[mu] - wave[3][mu];
                     T rhov2 = R[mv] - wave[3][mv];
                     T en2 = R[3] - wave[3][3];
                     T p2 = pressure(gamma1, rho2, rhou2, rhov2, en2);
-                    T c2 = dsqrt(gamma * p2 / rho2);
-                    T s2 = rhou2 / rho2 + c2;
+                    T c2 = Mth::sqrt(Mth::div(gamma * p2, rho2));
+                    T s2 = Mth::div(rhou2, rho2) + c2;
                     T sf4 = T(0);
                     bool add4 = true;
                     if (s2 < T(0) && s3 > T(0))
-                        sf4 = s2 * (s3 - s[3]) / (s3 - s2);
+                        sf4 = Mth::div(s2 * (s3 - s[3]), s3 - s2);
                     else if (s[3] < T(0))
                         sf4 = s[3];
                     else
@@ -236,7 +254,7 @@ template <typename T> struct EulerRoe {
         T A1 = asdq[0], Amu = asdq[mu], Amv = asdq[mv], A4 = asdq[3];
         T a3 = c.g1a2 * (c.euv * A1 + u * Amu + v * Amv - A4);
         T a2 = Amu - u * A1;
-        T a4 = (Amv + (a - v) * A1 - a * a3) / (T(2) * a);
+        T a4 = Mth::div(Amv + (a - v) * A1 - a * a3, T(2) * a);
         T a1 = A1 - a3 - a4;
         T wb[4][4], sb[4];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);

@@ -46,12 +46,12 @@ template <typename T>
 __device__ __forceinline__ T lane_up(T v, int w) { return __shfl_up(v, 1, w); }
 
 // limiter function of inlinelimiter.f:58-94
-template <typename T>
+template <class Mth, typename T>
 __device__ __forceinline__ T phi_limiter(int mth, T r) {
     switch (mth) {
     case 1: return dmax1(T(0), dmin1(T(1), r));
     case 2: return dmax1(dmax1(T(0), dmin1(T(1), T(2) * r)), dmin1(T(2), r));
-    case 3: return (r + fabs(r)) / (T(1) + fabs(r));
+    case 3: return Mth::div(r + fabs(r), T(1) + fabs(r));
     case 4: {
         T c = (T(1) + r) / T(2);
         return dmax1(T(0), dmin1(dmin1(c, T(2)), T(2) * r));
@@ -108,8 +108,9 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
             T dotl = lane_up(dotr, w);
             const int mth = M.mthlim[mw];
             if (mth != 0 && inner && wn2 != T(0)) {
-                T r = (s[mw] > T(0)) ? dotl / wn2 : dotr / wn2;
-                T wl = phi_limiter(mth, r);
+                using Mth = typename Sol::Math;
+                T r = Mth::div((s[mw] > T(0)) ? dotl : dotr, wn2);
+                T wl = phi_limiter<Mth>(mth, r);
 #pragma unroll
                 for (int m = 0; m < NV; m++) wave[mw][m] = wl * wave[mw][m];
             }

@@ -1,4 +1,5 @@
-// FP32 instantiations of the level sweep: wave and flux arithmetic in FP32.
+// FP32 instantiations of the level sweep: wave and flux arithmetic in FP32
+// with hardware reciprocal and square root.
 #include "sweep.hpp"
 #include "kernels.hpp"
 
@@ -11,23 +12,23 @@ hipError_t launch_sweep_f32(int solver, const SweepArgs& a, size_t lds, hipStrea
     if (!attr_set) {
         // allow tiles whose LDS footprint exceeds the 64 KiB default
         const int max_lds = 160 * 1024;
-        hipFuncSetAttribute((const void*)sweep_kernel<Advection<float>, float, float>,
+        hipFuncSetAttribute((const void*)sweep_kernel<Advection<float, FastMathF32>, float, float>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        hipFuncSetAttribute((const void*)sweep_kernel<Acoustics<float>, float, float>,
+        hipFuncSetAttribute((const void*)sweep_kernel<Acoustics<float, FastMathF32>, float, float>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        hipFuncSetAttribute((const void*)sweep_kernel<EulerRoe<float>, float, float>,
+        hipFuncSetAttribute((const void*)sweep_kernel<EulerRoe<float, FastMathF32>, float, float>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         attr_set = true;
     }
     switch (solver) {
     case SOLVER_ADVECTION:
-        hipLaunchKernelGGL((sweep_kernel<Advection<float>, float, float>), grid, block, lds, st, a);
+        hipLaunchKernelGGL((sweep_kernel<Advection<float, FastMathF32>, float, float>), grid, block, lds, st, a);
         break;
     case SOLVER_ACOUSTICS:
-        hipLaunchKernelGGL((sweep_kernel<Acoustics<float>, float, float>), grid, block, lds, st, a);
+        hipLaunchKernelGGL((sweep_kernel<Acoustics<float, FastMathF32>, float, float>), grid, block, lds, st, a);
         break;
     case SOLVER_EULER:
-        hipLaunchKernelGGL((sweep_kernel<EulerRoe<float>, float, float>), grid, block, lds, st, a);
+        hipLaunchKernelGGL((sweep_kernel<EulerRoe<float, FastMathF32>, float, float>), grid, block, lds, st, a);
         break;
     default:
         return hipErrorInvalidValue;

@@ -67,15 +67,19 @@ def setup_level(capi, workload, precision, npatch, txmax, tymax):
                            cfl=0.9, cflv1=1.0, gamma=1.4, efix=True,
                            precision=precision)
     capi.init(cfg)
+    if not (txmax or tymax):
+        from amrclaw_amd import autotune
+        tuned = autotune.load(capi.EULER, "fp32" if precision == capi.FP32 else "fp64")
+        if tuned:
+            txmax, tymax = tuned
     if txmax or tymax:
         capi.set_tiling(txmax, tymax)
     capi.set_level(1, boxes, h, h)
-    for k in range(len(boxes)):
-        q = workload.patch_initial_state(boxes[k], h, 1000 + k)
-        # both time levels hold the initial data so every step reads valid
-        # ghost cells
-        capi.put_grid(1, k, q, which=0)
-        capi.put_grid(1, k, q, which=1)
+    qs = [workload.patch_initial_state(boxes[k], h, 1000 + k) for k in range(len(boxes))]
+    # both time levels hold the initial data so every step reads valid ghost
+    # cells
+    capi.put_level(1, qs, which=0)
+    capi.put_level(1, qs, which=1)
     return boxes, h, mx, my
 
 
@@ -94,11 +98,8 @@ def run(capi, steps, warmup, dt, sync):
 
 
 def download(capi, boxes):
-    out = []
-    for k, b in enumerate(boxes):
-        shape = (int(b[3] - b[2]) + 5, int(b[1] - b[0]) + 5, 4)
-        out.append(capi.get_grid(1, k, shape)[2:-2, 2:-2])
-    return out
+    shapes = [(int(b[3] - b[2]) + 5, int(b[1] - b[0]) + 5, 4) for b in boxes]
+    return [q[2:-2, 2:-2] for q in capi.get_level(1, shapes)]
 
 
 def cpu_baseline(workload, boxes, h, dt, budget):

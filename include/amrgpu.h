@@ -95,6 +95,16 @@ int amrgpu_set_level(int level, int ngrids, const int* ilo, const int* ihi,
 int amrgpu_put_grid(int level, int grid, int which, const double* q);
 int amrgpu_get_grid(int level, int grid, int which, double* q);
 
+/* Upload / download every grid of a level at once from the reference's heap
+ * layout: grid p's AoS FP64 block (nvar, mx+4, my+4) starts at
+ * alloc_base[offsets[p]] (the shim passes alloc and node(store1|store2,
+ * mptr)-1 for the level's grids in amrgpu_set_level order; amr_module.f90
+ * alloc, store1/store2).  which: 0 = new time, 1 = old time. */
+int amrgpu_put_level(int level, int which, const double* alloc_base,
+                     const int64_t* offsets);
+int amrgpu_get_level(int level, int which, double* alloc_base,
+                     const int64_t* offsets);
+
 /* Advance every grid of a level by dt from its current ghost-filled
  * state: the stepgrid part of par_advanc (advanc.f:170-292) for all grids in
  * one launch.  new -> old, updated interior -> new.  Returns the level's

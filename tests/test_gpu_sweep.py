@@ -110,3 +110,21 @@ def test_level_sweep_many_patches(gpu, precision, tiling):
     else:
         assert abs(cfl - cmax) <= 1e-5 * cmax
     capi.finalize()
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32])
+def test_bulk_level_transfer_round_trip(gpu, precision):
+    """put_level/get_level (alloc + per-grid offsets) equal per-grid puts."""
+    _, c = both_params(orc.EULER, precision=precision)
+    capi.init(c)
+    rng = np.random.default_rng(9)
+    boxes = [(0, 9, 0, 5), (10, 41, 0, 17), (42, 49, 3, 30)]
+    capi.set_level(2, boxes, 0.01, 0.01)
+    qs = [random_patch(rng, orc.EULER, b[1] - b[0] + 1, b[3] - b[2] + 1) for b in boxes]
+    capi.put_level(2, qs, which=0)
+    back = capi.get_level(2, [q.shape for q in qs], which=0)
+    for k, q in enumerate(qs):
+        one = capi.get_grid(2, k, q.shape)
+        ref = q if precision == capi.FP64 else q.astype(np.float32).astype(np.float64)
+        assert np.array_equal(back[k], ref) and np.array_equal(one, ref)
+    capi.finalize()
