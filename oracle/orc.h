@@ -94,6 +94,41 @@ void orc_stepgrid(const orc_params* p, int mx, int my, double* q,
                   double* fm, double* fp, double* gm, double* gp,
                   double* cflgrid, double* dtnew);
 
+/* ---- hierarchy description (the parts of amr_module the hot path reads) --
+ * Levels are numbered 1..nlev like the reference.  Grid k of level l has the
+ * global index box boxes[l][4k..4k+3] = (ilo, ihi, jlo, jhi) and AoS FP64
+ * arrays (nvar, mx+2ng, my+2ng) at qnew[l] + qoff[l][k] (store1) and
+ * qold[l] + qoff[l][k] (store2).  All grids of a level share the time
+ * time[l] (rnode(timemult)) and step possk[l]; hx/hy = hxposs/hyposs;
+ * iregsz/jregsz = level size in cells; intratx/intraty[l] = ratio from
+ * level l to level l+1. */
+#define ORC_MAXLV 12
+typedef struct {
+    int nlev, mxnest;
+    int ngrids[ORC_MAXLV];
+    const int* boxes[ORC_MAXLV];
+    double* qnew[ORC_MAXLV];
+    double* qold[ORC_MAXLV];
+    const int64_t* qoff[ORC_MAXLV];
+    double hx[ORC_MAXLV], hy[ORC_MAXLV], possk[ORC_MAXLV], time[ORC_MAXLV];
+    int iregsz[ORC_MAXLV], jregsz[ORC_MAXLV];
+    int intratx[ORC_MAXLV], intraty[ORC_MAXLV];
+    double xlower, xupper, ylower, yupper;
+    int xperdom, yperdom, spheredom;
+    int mthbc[4];
+} orc_hier;
+
+/* ghost filling, bound.f90:51-158 (with filrecur, intfil, trimbd,
+ * prefilrecur, bc2amr).  Fills the ghost cells of grid `grid` of `level`
+ * in valbig (the grid's AoS array) at time `time`.  Returns 0, or a nonzero
+ * code where the reference would stop (errors listed in orc_amr.c). */
+int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
+              double time, double* valbig);
+/* bc2amr.f90:84-296 on an enlarged patch val(meqn, nrow, ncol) */
+int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
+               int ncol, double hx, double hy, double xlo_patch,
+               double xhi_patch, double ylo_patch, double yhi_patch);
+
 /* amr_module.f90 declares these real(kind=8) parameters from default-real
  * (single precision) literals 10.e32 and 10.e33, so their values are the
  * float roundings widened to double. */

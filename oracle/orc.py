@@ -233,3 +233,102 @@ def flux(p, ixy, q):
     f = np.zeros(p.meqn)
     lib().orc_flux(C.byref(p), ixy, q, f)
     return f
+
+
+# ------------------------------------------------------------ hierarchy ---
+MAXLV = 12
+
+
+class OrcHier(C.Structure):
+    _fields_ = [
+        ("nlev", C.c_int), ("mxnest", C.c_int),
+        ("ngrids", C.c_int * MAXLV),
+        ("boxes", C.POINTER(C.c_int) * MAXLV),
+        ("qnew", C.POINTER(C.c_double) * MAXLV),
+        ("qold", C.POINTER(C.c_double) * MAXLV),
+        ("qoff", C.POINTER(C.c_int64) * MAXLV),
+        ("hx", C.c_double * MAXLV), ("hy", C.c_double * MAXLV),
+        ("possk", C.c_double * MAXLV), ("time", C.c_double * MAXLV),
+        ("iregsz", C.c_int * MAXLV), ("jregsz", C.c_int * MAXLV),
+        ("intratx", C.c_int * MAXLV), ("intraty", C.c_int * MAXLV),
+        ("xlower", C.c_double), ("xupper", C.c_double),
+        ("ylower", C.c_double), ("yupper", C.c_double),
+        ("xperdom", C.c_int), ("yperdom", C.c_int), ("spheredom", C.c_int),
+        ("mthbc", C.c_int * 4),
+    ]
+
+
+def hier_struct(H):
+    """OrcHier view of a tests.common.Hier (shares its arrays)."""
+    h = OrcHier()
+    h.nlev = h.mxnest = H.nlev
+    for l, L in enumerate(H.levels, start=1):
+        h.ngrids[l] = len(L["boxes"])
+        h.boxes[l] = L["boxes"].ctypes.data_as(C.POINTER(C.c_int))
+        h.qnew[l] = L["qnew"].ctypes.data_as(C.POINTER(C.c_double))
+        h.qold[l] = L["qold"].ctypes.data_as(C.POINTER(C.c_double))
+        h.qoff[l] = L["off"].ctypes.data_as(C.POINTER(C.c_int64))
+        h.hx[l], h.hy[l] = L["hx"], L["hy"]
+        h.possk[l], h.time[l] = L["possk"], L["time"]
+        h.iregsz[l], h.jregsz[l] = L["iregsz"], L["jregsz"]
+        h.intratx[l], h.intraty[l] = L["ratx"], L["raty"]
+    h.xlower, h.xupper, h.ylower, h.yupper = H.domain
+    h.xperdom, h.yperdom, h.spheredom = int(H.xper), int(H.yper), 0
+    for k in range(4):
+        h.mthbc[k] = H.mthbc[k]
+    return h
+
+
+def bound(p, H, level, grid, time):
+    """orc_bound on grid `grid` (0-based) of `level`, in place on H."""
+    L = lib()
+    L.orc_bound.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcHier), C.c_int,
+                            C.c_int, C.c_double, C.c_void_p]
+    h = hier_struct(H)
+    val = H.grid(level, grid, 0)
+    rc = L.orc_bound(C.byref(p), C.byref(h), level, grid, time,
+                     val.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError("orc_bound error %d" % rc)
+
+
+def ref_load_hier(p, H):
+    """Install the hierarchy in the reference's amr_module (harness)."""
+    R = _ref_setup(p)
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    R.ref_hier_setup.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, dp, dp,
+                                 dp, dp, ip, ip, ip, ip, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_int, ip]
+    R.ref_hier_xfer.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
+    lv = H.levels
+    f64 = lambda k: np.array([L[k] for L in lv], dtype=np.float64)
+    i32 = lambda k: np.array([L[k] for L in lv], dtype=np.int32)
+    boxes = np.ascontiguousarray(np.concatenate([L["boxes"].ravel() for L in lv]).astype(np.int32))
+    ngr = np.array([len(L["boxes"]) for L in lv], dtype=np.int32)
+    R.ref_hier_setup(H.nlev, H.nlev, H.nvar, H.ng, ngr, boxes, f64("hx"), f64("hy"),
+                     f64("possk"), f64("time"), i32("iregsz"), i32("jregsz"),
+                     i32("ratx"), i32("raty"), *H.domain, int(H.xper), int(H.yper),
+                     np.array(H.mthbc, dtype=np.int32))
+    for l in range(1, H.nlev + 1):
+        for k in range(len(lv[l - 1]["boxes"])):
+            for which in (0, 1):
+                g = np.ascontiguousarray(H.grid(l, k, which))
+                R.ref_hier_xfer(l, k, which, H.nvar, g.ctypes.data_as(C.c_void_p), 1)
+    return R
+
+
+def ref_get_grid(H, level, grid, which=0):
+    R = ref()
+    out = np.zeros(H.levels[level - 1]["shapes"][grid])
+    R.ref_hier_xfer(level, grid, which, H.nvar, out.ctypes.data_as(C.c_void_p), 0)
+    return out
+
+
+def ref_bound(p, H, level, grid, time):
+    """The reference's bound.f90 on a freshly installed copy of H; returns
+    the grid's new array after the call."""
+    R = ref_load_hier(p, H)
+    R.ref_bound.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
+    R.ref_bound(level, grid, time, H.nvar)
+    return ref_get_grid(H, level, grid)

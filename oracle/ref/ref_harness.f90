@@ -129,4 +129,131 @@ contains
     cflmax = max(cmax, cfl_level)
   end subroutine ref_stepgrid_batch
 
+  ! ---------------------------------------------------------------------
+  ! Hierarchy set-up: fills node/rnode/alloc/level lists the way regrid and
+  ! makeGridList leave them (amr_module.f90:30-151), then makeBndryList
+  ! (init_bndryList.f:50).  Grid k of level l gets node number
+  ! gbase(l)+k; its new/old solution blocks are consecutive in alloc.
+  subroutine ref_hier_setup(nlev, mxnest_in, nvar, ng, ngr, boxes, hx, hy, pk, tm, &
+       isz, jsz, ratx, raty, xlo, xhi, ylo, yhi, xper, yper, bc) bind(C, name="ref_hier_setup")
+    integer(c_int), value :: nlev, mxnest_in, nvar, ng
+    integer(c_int), intent(in) :: ngr(nlev), boxes(*), isz(nlev), jsz(nlev), ratx(nlev), raty(nlev), bc(4)
+    real(c_double), intent(in) :: hx(nlev), hy(nlev), pk(nlev), tm(nlev)
+    real(c_double), value :: xlo, xhi, ylo, yhi
+    integer(c_int), value :: xper, yper
+    integer :: l, k, mptr, nb, ntot, loc, mitot, mjtot, mprev
+    ntot = sum(ngr(1:nlev))
+    maxgr = max(ntot + 10, 16)
+    if (allocated(node)) deallocate(node)
+    if (allocated(rnode)) deallocate(rnode)
+    if (allocated(listOfGrids)) deallocate(listOfGrids)
+    allocate(node(nsize, maxgr), rnode(rsize, maxgr), listOfGrids(maxgr))
+    node = 0
+    rnode = 0.d0
+    ! solution storage
+    memsize = 1
+    nb = 0
+    do l = 1, nlev
+       do k = 1, ngr(l)
+          mitot = boxes(4*nb+2) - boxes(4*nb+1) + 1 + 2*ng
+          mjtot = boxes(4*nb+4) - boxes(4*nb+3) + 1 + 2*ng
+          memsize = memsize + 2*nvar*mitot*mjtot
+          nb = nb + 1
+       end do
+    end do
+    memsize = memsize + 16
+    if (allocated(alloc)) deallocate(alloc)
+    allocate(alloc(memsize))
+    alloc = 0.d0
+    mxnest = mxnest_in
+    nghost = ng
+    xlower = xlo
+    xupper = xhi
+    ylower = ylo
+    yupper = yhi
+    xperdom = (xper /= 0)
+    yperdom = (yper /= 0)
+    spheredom = .false.
+    mthbc(1:4) = bc(1:4)
+    lstart = 0
+    numgrids = 0
+    listStart = 0
+    nb = 0
+    loc = 2
+    mptr = 0
+    do l = 1, nlev
+       hxposs(l) = hx(l)
+       hyposs(l) = hy(l)
+       possk(l) = pk(l)
+       iregsz(l) = isz(l)
+       jregsz(l) = jsz(l)
+       intratx(l) = ratx(l)
+       intraty(l) = raty(l)
+       numgrids(l) = ngr(l)
+       listStart(l) = mptr + 1
+       mprev = 0
+       do k = 1, ngr(l)
+          mptr = mptr + 1
+          listOfGrids(mptr) = mptr
+          if (k == 1) lstart(l) = mptr
+          if (mprev /= 0) node(levelptr, mprev) = mptr
+          node(levelptr, mptr) = null
+          mprev = mptr
+          node(nestlevel, mptr) = l
+          node(ndilo, mptr) = boxes(4*nb+1)
+          node(ndihi, mptr) = boxes(4*nb+2)
+          node(ndjlo, mptr) = boxes(4*nb+3)
+          node(ndjhi, mptr) = boxes(4*nb+4)
+          rnode(cornxlo, mptr) = node(ndilo, mptr)*hx(l) + xlower
+          rnode(cornxhi, mptr) = (node(ndihi, mptr)+1)*hx(l) + xlower
+          rnode(cornylo, mptr) = node(ndjlo, mptr)*hy(l) + ylower
+          rnode(cornyhi, mptr) = (node(ndjhi, mptr)+1)*hy(l) + ylower
+          rnode(timemult, mptr) = tm(l)
+          mitot = node(ndihi, mptr) - node(ndilo, mptr) + 1 + 2*ng
+          mjtot = node(ndjhi, mptr) - node(ndjlo, mptr) + 1 + 2*ng
+          node(store1, mptr) = loc
+          node(store2, mptr) = loc + nvar*mitot*mjtot
+          node(storeaux, mptr) = 1
+          node(bndListSt, mptr) = null
+          node(bndListNum, mptr) = 0
+          loc = loc + 2*nvar*mitot*mjtot
+          nb = nb + 1
+       end do
+    end do
+    listStart(nlev+1) = mptr + 1
+    if (allocated(bndList)) deallocate(bndList)
+    call initBndryList()
+    do l = 1, nlev
+       call makeBndryList(l)
+    end do
+  end subroutine ref_hier_setup
+
+  ! copy grid (level l, index k from 0) new (which=0) / old (1) to or from q
+  subroutine ref_hier_xfer(l, k, which, nvar, q, to_ref) bind(C, name="ref_hier_xfer")
+    integer(c_int), value :: l, k, which, nvar, to_ref
+    real(c_double) :: q(*)
+    integer :: mptr, loc, n
+    mptr = listOfGrids(listStart(l) + k)
+    loc = node(store1, mptr)
+    if (which == 1) loc = node(store2, mptr)
+    n = nvar*(node(ndihi,mptr)-node(ndilo,mptr)+1+2*nghost)*(node(ndjhi,mptr)-node(ndjlo,mptr)+1+2*nghost)
+    if (to_ref /= 0) then
+       alloc(loc:loc+n-1) = q(1:n)
+    else
+       q(1:n) = alloc(loc:loc+n-1)
+    end if
+  end subroutine ref_hier_xfer
+
+  ! bound.f90:51 on grid k of level l at time t, in place on its new array
+  subroutine ref_bound(l, k, t, nvar) bind(C, name="ref_bound")
+    integer(c_int), value :: l, k, nvar
+    real(c_double), value :: t
+    integer :: mptr, mitot, mjtot
+    mptr = listOfGrids(listStart(l) + k)
+    mitot = node(ndihi,mptr)-node(ndilo,mptr)+1+2*nghost
+    mjtot = node(ndjhi,mptr)-node(ndjlo,mptr)+1+2*nghost
+    call bound(t, nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, mptr, &
+         alloc(node(storeaux,mptr)), 0)
+  end subroutine ref_bound
+
 end module ref_harness

@@ -54,3 +54,101 @@ def both_params(solver, **kw):
 
 def rel_l1(a, b):
     return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
+
+
+class Hier:
+    """A grid hierarchy in the reference's terms: per level the global index
+    boxes, mesh widths, step, time, index-space size and refinement ratio
+    to the next level, plus packed AoS FP64 new/old solution arrays (one
+    (my+4, mx+4, nvar) block per grid, like the reference's alloc)."""
+
+    def __init__(self, nvar, domain=(0.0, 1.0, 0.0, 1.0), mthbc=(1, 1, 1, 1),
+                 xper=False, yper=False, ng=2):
+        self.nvar, self.ng = nvar, ng
+        self.domain = domain
+        self.mthbc = tuple(mthbc)
+        self.xper, self.yper = xper, yper
+        self.levels = []
+
+    def add_level(self, boxes, hx, hy, possk, time, iregsz, jregsz, ratx=2, raty=2):
+        boxes = np.asarray(boxes, dtype=np.int32).reshape(-1, 4)
+        shapes = [(int(b[3] - b[2]) + 1 + 2 * self.ng, int(b[1] - b[0]) + 1 + 2 * self.ng, self.nvar)
+                  for b in boxes]
+        sizes = np.array([int(np.prod(s)) for s in shapes], dtype=np.int64)
+        off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        tot = max(int(sizes.sum()), 1)
+        self.levels.append(dict(boxes=np.ascontiguousarray(boxes), shapes=shapes, off=off,
+                                qnew=np.zeros(tot), qold=np.zeros(tot), hx=hx, hy=hy,
+                                possk=possk, time=time, iregsz=iregsz, jregsz=jregsz,
+                                ratx=ratx, raty=raty))
+
+    @property
+    def nlev(self):
+        return len(self.levels)
+
+    def grid(self, l, k, which=0):
+        """View of grid k of level l (1-based) as (my+4, mx+4, nvar)."""
+        L = self.levels[l - 1]
+        a = L["qnew"] if which == 0 else L["qold"]
+        n = int(np.prod(L["shapes"][k]))
+        return a[L["off"][k]:L["off"][k] + n].reshape(L["shapes"][k])
+
+    def copy(self):
+        import copy
+        H = copy.copy(self)
+        H.levels = [dict(L, qnew=L["qnew"].copy(), qold=L["qold"].copy()) for L in self.levels]
+        return H
+
+
+def disjoint_boxes(rng, isz, jsz, nbox, lo=4, hi=20, align=2, within=None, tries=400):
+    """Random disjoint boxes (ilo, ihi, jlo, jhi) inside [0,isz) x [0,jsz)
+    (or inside the union `within` of boxes), corners aligned to `align`."""
+    out = []
+    for _ in range(tries):
+        if len(out) >= nbox:
+            break
+        w = int(rng.integers(lo // align, hi // align + 1)) * align
+        h = int(rng.integers(lo // align, hi // align + 1)) * align
+        if w > isz or h > jsz:
+            continue
+        i0 = int(rng.integers(0, (isz - w) // align + 1)) * align
+        j0 = int(rng.integers(0, (jsz - h) // align + 1)) * align
+        b = (i0, i0 + w - 1, j0, j0 + h - 1)
+        if any(not (b[1] < c[0] or c[1] < b[0] or b[3] < c[2] or c[3] < b[2]) for c in out):
+            continue
+        if within is not None and not any(c[0] <= b[0] and b[1] <= c[1] and c[2] <= b[2]
+                                          and b[3] <= c[3] for c in within):
+            continue
+        out.append(b)
+    return out
+
+
+def random_hierarchy(rng, nvar=3, nx=24, ny=20, nlev=3, ratio=2, mthbc=(1, 1, 1, 1),
+                     xper=False, yper=False, fine_phase=(1, 1), level1_split=(2, 2)):
+    """Level 1 tiles the domain; finer levels hold random disjoint boxes.
+    Times follow Berger-Oliger: every level l < nlev has just been advanced
+    to T = 1.0 (new) from T - possk (old); level nlev sits inside the last
+    coarse step at fine_phase (k, r): time = T - possk(l-1) + k possk(l)."""
+    H = Hier(nvar, mthbc=mthbc, xper=xper, yper=yper)
+    hx, hy = 1.0 / nx, 1.0 / ny
+    dt1 = 0.08
+    T = 1.0
+    isz, jsz = nx, ny
+    cuts_i = np.linspace(0, nx, level1_split[0] + 1).astype(int)
+    cuts_j = np.linspace(0, ny, level1_split[1] + 1).astype(int)
+    boxes = [(cuts_i[a], cuts_i[a + 1] - 1, cuts_j[b], cuts_j[b + 1] - 1)
+             for b in range(level1_split[1]) for a in range(level1_split[0])]
+    possk = dt1
+    for l in range(1, nlev + 1):
+        if l > 1:
+            isz, jsz = isz * ratio, jsz * ratio
+            hx, hy = hx / ratio, hy / ratio
+            possk = possk / ratio
+            boxes = disjoint_boxes(rng, isz, jsz, 6, lo=4, hi=max(6, isz // 3), align=ratio)
+        t = T if l < nlev else T - possk * ratio + fine_phase[0] * possk
+        H.add_level(boxes, hx, hy, possk, t, isz, jsz, ratio, ratio)
+    for l in range(1, nlev + 1):
+        L = H.levels[l - 1]
+        L["qnew"][:] = rng.normal(size=L["qnew"].shape)
+        L["qold"][:] = rng.normal(size=L["qold"].shape)
+    return H
