@@ -25,6 +25,8 @@ UNITS = [
     ("sweep_f64.hip", ["-O3", "-ffp-contract=off"]),
     ("sweep_f32.hip", ["-O3"]),
     ("util.hip", ["-O2"]),
+    ("fill.hip", ["-O3", "-ffp-contract=off"]),
+    ("fill_plan.cpp", ["-O2", "-x", "hip"]),
     ("capi.cpp", ["-O2", "-x", "hip"]),
 ]
 

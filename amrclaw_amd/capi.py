@@ -34,6 +34,18 @@ class AmrgpuConfig(C.Structure):
     ]
 
 
+class FillOp(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("type", "dst", "src", "src_level", "di0", "dj0", "ni",
+                                       "nj", "si0", "sj0", "mask_off", "ilo", "jlo", "iplo",
+                                       "jplo", "ratx", "raty", "level")] + \
+               [("bc", C.c_int * 4), ("stage", C.c_int), ("pad", C.c_int * 2)]
+
+
+class FillBuf(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("kind", "level", "patch", "nrow", "ncol", "plane", "pad")] + \
+               [("off", C.c_int64)]
+
+
 class AmrgpuError(RuntimeError):
     pass
 
@@ -69,6 +81,9 @@ def lib():
         L.amrgpu_get_level.argtypes = [C.c_int, C.c_int, _dp, _lp]
         L.amrgpu_sweep_level.argtypes = [C.c_int, C.c_double, C.c_void_p,
                                          C.c_void_p]
+        L.amrgpu_set_geometry.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
+        L.amrgpu_set_time.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.amrgpu_bound_level.argtypes = [C.c_int, C.c_double]
         L.amrgpu_event_record.argtypes = [C.c_int]
         L.amrgpu_event_elapsed.argtypes = [C.c_int, C.c_int,
                                            C.POINTER(C.c_float)]
@@ -182,6 +197,74 @@ def sweep_level(level, dt, want_cfl=True):
         return cfl.value, dtn.value
     _check(lib().amrgpu_sweep_level(level, dt, None, None))
     return None
+
+
+def set_geometry(iregsz, jregsz, intratx, intraty, hxposs, hyposs):
+    """Per-level geometry for levels 1..len(iregsz) (mxnest = len)."""
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    _check(lib().amrgpu_set_geometry(len(iregsz), i32(iregsz), i32(jregsz), i32(intratx),
+                                     i32(intraty), f64(hxposs), f64(hyposs)))
+
+
+def set_time(level, time, possk):
+    _check(lib().amrgpu_set_time(level, time, possk))
+
+
+def bound_level(level, time):
+    _check(lib().amrgpu_bound_level(level, time))
+
+
+def load_hierarchy(H):
+    """Install a tests.common.Hier-like hierarchy: geometry, layouts, both
+    time levels of every grid and the level times."""
+    lv = H.levels
+    set_geometry([L["iregsz"] for L in lv], [L["jregsz"] for L in lv],
+                 [L["ratx"] for L in lv], [L["raty"] for L in lv],
+                 [L["hx"] for L in lv], [L["hy"] for L in lv])
+    for l, L in enumerate(lv, start=1):
+        set_level(l, L["boxes"], L["hx"], L["hy"])
+        if len(L["boxes"]):
+            _check(lib().amrgpu_put_level(l, 0, L["qnew"], L["off"]))
+            _check(lib().amrgpu_put_level(l, 1, L["qold"], L["off"]))
+        set_time(l, L["time"], L["possk"])
+
+
+def read_level(H, l, which=0):
+    """Download level l into a packed array laid out like H.levels[l-1]."""
+    L = H.levels[l - 1]
+    out = np.zeros_like(L["qnew"])
+    _check(lib().amrgpu_get_level(l, which, out, L["off"]))
+    return out
+
+
+def plan_fill(H, level, domain=None, mthbc=None):
+    """Plan the ghost fill of `level` of a Hier-like hierarchy on the host
+    (no device).  Returns (ops, bufs, masks) as ctypes arrays / bytes."""
+    lv = H.levels
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    boxes = i32(np.concatenate([np.asarray(L["boxes"]).ravel() for L in lv]))
+    nops, nbufs = C.c_int(), C.c_int()
+    nmask, nscr = C.c_int64(), C.c_int64()
+    L = lib()
+    L.amrgpu_plan_fill.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip,
+                                   C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    _check(L.amrgpu_plan_fill(len(lv), i32([len(x["boxes"]) for x in lv]), boxes,
+                              i32([x["iregsz"] for x in lv]), i32([x["jregsz"] for x in lv]),
+                              i32([x["ratx"] for x in lv]), i32([x["raty"] for x in lv]),
+                              f64([x["hx"] for x in lv]), f64([x["hy"] for x in lv]),
+                              f64(domain if domain is not None else H.domain),
+                              i32(mthbc if mthbc is not None else H.mthbc), H.nvar, level,
+                              C.byref(nops), C.byref(nbufs), C.byref(nmask), C.byref(nscr)))
+    ops = (FillOp * max(nops.value, 1))()
+    bufs = (FillBuf * max(nbufs.value, 1))()
+    masks = (C.c_ubyte * max(nmask.value, 1))()
+    L.amrgpu_plan_fill_get.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    _check(L.amrgpu_plan_fill_get(C.cast(ops, C.c_void_p), C.cast(bufs, C.c_void_p),
+                                  C.cast(masks, C.c_void_p)))
+    return list(ops)[:nops.value], list(bufs)[:nbufs.value], bytes(masks)[:nmask.value]
 
 
 def event_record(slot):

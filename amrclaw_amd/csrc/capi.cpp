@@ -8,11 +8,13 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/amrgpu.h"
+#include "fill_plan.hpp"
 #include "kernels.hpp"
 
 namespace amrgpu {
@@ -21,6 +23,7 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t 
 hipError_t launch_aos_soa(bool to_soa, int esize, double* stage, const int64_t* stage_off,
                           const int64_t* off, const int32_t* plane, const int64_t* cell_start,
                           int npatch, int nv, int64_t ncell, void* level, hipStream_t st);
+hipError_t launch_fill_stage(int esize, const FillArgs& a, hipStream_t st);
 }
 
 using namespace amrgpu;
@@ -49,6 +52,15 @@ struct Level {
     int32_t* d_plane = nullptr;
     int64_t* d_cell_start = nullptr;
     int64_t ncell_total = 0;       // cells incl. ghosts over all patches
+    // ghost-fill program (bound) of this level
+    bool plan_valid = false;
+    FillPlan plan;
+    FillOp* d_ops = nullptr;
+    FillBuf* d_bufs = nullptr;
+    unsigned char* d_masks = nullptr;
+    double* d_scratch = nullptr;
+    // time of the level's grids (rnode(timemult)) and its step (possk)
+    double time = 0.0, possk = 0.0;
 };
 
 struct Context {
@@ -60,6 +72,9 @@ struct Context {
     hipEvent_t ev[NEVENTS] = {};
     int txmax = 28, tymax = 28;
     Level levels[MAXLV];
+    int mxnest = 0;
+    int iregsz[MAXLV] = {}, jregsz[MAXLV] = {}, ratx[MAXLV] = {}, raty[MAXLV] = {};
+    double hxposs[MAXLV] = {}, hyposs[MAXLV] = {};
     std::vector<unsigned long long> h_cfl;
 };
 
@@ -83,7 +98,21 @@ int fail(int code, const char* fmt, ...) {
             return fail(AMRGPU_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+void free_plan(Level& L) {
+    if (L.d_ops) hipFree(L.d_ops);
+    if (L.d_bufs) hipFree(L.d_bufs);
+    if (L.d_masks) hipFree(L.d_masks);
+    if (L.d_scratch) hipFree(L.d_scratch);
+    L.d_ops = nullptr;
+    L.d_bufs = nullptr;
+    L.d_masks = nullptr;
+    L.d_scratch = nullptr;
+    L.plan = FillPlan();
+    L.plan_valid = false;
+}
+
 void free_level(Level& L) {
+    free_plan(L);
     if (L.d_patches) hipFree(L.d_patches);
     if (L.d_tiles) hipFree(L.d_tiles);
     for (auto& b : L.buf)
@@ -92,7 +121,10 @@ void free_level(Level& L) {
     if (L.d_off) hipFree(L.d_off);
     if (L.d_plane) hipFree(L.d_plane);
     if (L.d_cell_start) hipFree(L.d_cell_start);
+    double t = L.time, pk = L.possk;
     L = Level();
+    L.time = t;
+    L.possk = pk;
 }
 
 int seg_width(int n) { return n + 3 <= 16 ? 16 : (n + 3 <= 32 ? 32 : 64); }
@@ -158,6 +190,8 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
                  const int* jhi, double hx, double hy) {
     Level& L = g.levels[level];
     free_level(L);
+    // fill programs of this and finer levels read this level's layout
+    for (int l = level; l < MAXLV; l++) free_plan(g.levels[l]);
     L.ngrids = ngrids;
     L.hx = hx;
     L.hy = hy;
@@ -299,6 +333,116 @@ int xfer_level(Level& L, bool to_dev, int which, double* alloc_base, const int64
     return rc;
 }
 
+// hierarchy geometry for the planner
+HierGeom make_geom() {
+    HierGeom H;
+    H.nlev = 0;
+    for (int l = 1; l < MAXLV; l++)
+        if (g.levels[l].ngrids > 0 || l <= g.mxnest) H.nlev = l;
+    for (int l = 1; l <= H.nlev; l++) {
+        LevelGeom& G = H.lev[l];
+        for (const auto& pd : g.levels[l].patches) G.boxes.push_back({pd.ilo, pd.ihi, pd.jlo, pd.jhi});
+        G.iregsz = g.iregsz[l];
+        G.jregsz = g.jregsz[l];
+        G.ratx = g.ratx[l];
+        G.raty = g.raty[l];
+        G.hx = g.hxposs[l];
+        G.hy = g.hyposs[l];
+    }
+    H.xlower = g.cfg.xlower;
+    H.xupper = g.cfg.xupper;
+    H.ylower = g.cfg.ylower;
+    H.yupper = g.cfg.yupper;
+    H.xperdom = g.cfg.mthbc[0] == 2 && g.cfg.mthbc[1] == 2;    // amr2.f90:279-280
+    H.yperdom = g.cfg.mthbc[2] == 2 && g.cfg.mthbc[3] == 2;
+    H.spheredom = false;
+    for (int k = 0; k < 4; k++) H.mthbc[k] = g.cfg.mthbc[k];
+    return H;
+}
+
+int ensure_plan(int level) {
+    Level& L = g.levels[level];
+    if (L.plan_valid) return AMRGPU_OK;
+    if (g.mxnest < level) return fail(AMRGPU_ERR_STATE, "amrgpu_set_geometry must cover level %d", level);
+    free_plan(L);
+    HierGeom H = make_geom();
+    std::string err;
+    if (!plan_fill_level(H, level, g.cfg.nvar, L.plan, err))
+        return fail(AMRGPU_ERR_ARG, "level %d ghost fill: %s", level, err.c_str());
+    const FillPlan& P = L.plan;
+    if (!P.ops.empty()) {
+        HIPCHK(hipMalloc(&L.d_ops, P.ops.size() * sizeof(FillOp)));
+        HIPCHK(hipMemcpy(L.d_ops, P.ops.data(), P.ops.size() * sizeof(FillOp), hipMemcpyHostToDevice));
+    }
+    if (!P.bufs.empty()) {
+        HIPCHK(hipMalloc(&L.d_bufs, P.bufs.size() * sizeof(FillBuf)));
+        HIPCHK(hipMemcpy(L.d_bufs, P.bufs.data(), P.bufs.size() * sizeof(FillBuf), hipMemcpyHostToDevice));
+    }
+    if (!P.masks.empty()) {
+        HIPCHK(hipMalloc(&L.d_masks, P.masks.size()));
+        HIPCHK(hipMemcpy(L.d_masks, P.masks.data(), P.masks.size(), hipMemcpyHostToDevice));
+    }
+    if (P.scratch_elems > 0) HIPCHK(hipMalloc(&L.d_scratch, P.scratch_elems * sizeof(double)));
+    L.plan_valid = true;
+    return AMRGPU_OK;
+}
+
+// intfil's choice of new / old / blended source values (intfil.f90:194-234)
+int time_weights(double t, FillArgs& a, uint32_t mask) {
+    const double t_epsilon = 1.0e-4;
+    for (int l = 1; l < MAXLV; l++) {
+        if (!(mask & (1u << l))) continue;
+        const Level& S = g.levels[l];
+        if (!(S.possk > 0)) return fail(AMRGPU_ERR_STATE, "level %d has no time step (amrgpu_set_time)", l);
+        double alphac = (S.time - t) / S.possk;
+        double alphai = 1.0 - alphac;
+        if (alphai < -t_epsilon || alphai > 1.0 + t_epsilon)
+            return fail(AMRGPU_ERR_ARG, "time %.16e not available from level %d (time %.16e)", t, l, S.time);
+        a.alphai[l] = alphai;
+        a.alphac[l] = alphac;
+        if (std::fabs(alphai - 1.0) < t_epsilon) {
+            a.tmode[l] = TIME_NEW;
+        } else {
+            if (l == g.mxnest)
+                return fail(AMRGPU_ERR_ARG, "finest level %d keeps no old-time values", l);
+            a.tmode[l] = std::fabs(alphai) < t_epsilon ? TIME_OLD : TIME_BLEND;
+        }
+    }
+    return AMRGPU_OK;
+}
+
+int bound_level(int level, double t) {
+    Level& L = g.levels[level];
+    if (L.ngrids == 0) return AMRGPU_OK;
+    int rc = ensure_plan(level);
+    if (rc) return rc;
+    FillArgs a{};
+    rc = time_weights(t, a, L.plan.src_levels_mask);
+    if (rc) return rc;
+    a.bufs = L.d_bufs;
+    a.scratch = L.d_scratch;
+    a.masks = L.d_masks;
+    for (int l = 1; l < MAXLV && l < FILL_MAXLV; l++) {
+        Level& S = g.levels[l];
+        a.lvl_new[l] = S.buf[S.cur];
+        a.lvl_old[l] = S.buf[1 - S.cur];
+        a.lvl_patches[l] = S.d_patches;
+        a.hx[l] = g.hxposs[l];
+        a.hy[l] = g.hyposs[l];
+    }
+    a.xlower = g.cfg.xlower;
+    a.ylower = g.cfg.ylower;
+    for (int k = 0; k < 4; k++) a.mthbc[k] = g.cfg.mthbc[k];
+    a.nv = g.cfg.nvar;
+    for (int s = 0; s < L.plan.nstages; s++) {
+        a.ops = L.d_ops + L.plan.stage_start[s];
+        a.nops = L.plan.stage_start[s + 1] - L.plan.stage_start[s];
+        hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
+        if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "fill stage %d: %s", s, hipGetErrorString(e));
+    }
+    return AMRGPU_OK;
+}
+
 int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev) {
     if (L.ngrids == 0) return AMRGPU_OK;
     HIPCHK(launch_fill_u64(L.d_cfl, 0ull, L.ngrids, g.stream));
@@ -360,6 +504,12 @@ int amrgpu_init(const amrgpu_config* cfg) {
     if (cfg->nghost != NGHOST) return fail(AMRGPU_ERR_UNSUPPORTED, "nghost must be 2");
     if (cfg->naux != 0 || cfg->mcapa != 0)
         return fail(AMRGPU_ERR_UNSUPPORTED, "aux arrays / capacity are not supported");
+    if ((cfg->mthbc[0] == 2) != (cfg->mthbc[1] == 2) || (cfg->mthbc[2] == 2) != (cfg->mthbc[3] == 2))
+        return fail(AMRGPU_ERR_ARG, "periodic boundaries must be periodic on both sides");
+    for (int k = 0; k < 4; k++)
+        if (cfg->mthbc[k] < 1 || cfg->mthbc[k] > 3)
+            return fail(AMRGPU_ERR_UNSUPPORTED, "mthbc(%d)=%d: supported are 1 extrap, 2 periodic, 3 wall",
+                        k + 1, cfg->mthbc[k]);
     if (cfg->precision != AMRGPU_FP64 && cfg->precision != AMRGPU_FP32)
         return fail(AMRGPU_ERR_ARG, "precision must be AMRGPU_FP64 or AMRGPU_FP32");
     if (cfg->method[1] < 1 || cfg->method[1] > 2 || cfg->method[2] < 0 || cfg->method[2] > 2)
@@ -520,6 +670,98 @@ int amrgpu_sweep_level(int level, double dt, double* cfl_level, double* dtlevnew
     rc = sweep(L, dt, nullptr, nullptr);
     if (rc) return rc;
     if (cfl_level || dtlevnew) return level_cfl(L, dt, cfl_level, dtlevnew);
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_geometry(int mxnest, const int* iregsz, const int* jregsz, const int* intratx,
+                        const int* intraty, const double* hxposs, const double* hyposs) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (mxnest < 1 || mxnest >= MAXLV || !iregsz || !jregsz || !intratx || !intraty || !hxposs ||
+        !hyposs)
+        return fail(AMRGPU_ERR_ARG, "bad geometry arguments");
+    for (int l = 1; l <= mxnest; l++) {
+        if (iregsz[l - 1] < 1 || jregsz[l - 1] < 1 || intratx[l - 1] < 1 || intraty[l - 1] < 1 ||
+            !(hxposs[l - 1] > 0) || !(hyposs[l - 1] > 0))
+            return fail(AMRGPU_ERR_ARG, "bad geometry of level %d", l);
+        g.iregsz[l] = iregsz[l - 1];
+        g.jregsz[l] = jregsz[l - 1];
+        g.ratx[l] = intratx[l - 1];
+        g.raty[l] = intraty[l - 1];
+        g.hxposs[l] = hxposs[l - 1];
+        g.hyposs[l] = hyposs[l - 1];
+    }
+    g.mxnest = mxnest;
+    for (auto& L : g.levels) free_plan(L);
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_time(int level, double time, double possk) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    g.levels[level].time = time;
+    g.levels[level].possk = possk;
+    return AMRGPU_OK;
+}
+
+int amrgpu_bound_level(int level, double time) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (level < 1 || level > g.mxnest) return fail(AMRGPU_ERR_ARG, "level %d outside 1..mxnest", level);
+    return bound_level(level, time);
+}
+
+static_assert(sizeof(amrgpu_fill_op) == sizeof(FillOp), "fill op layout");
+static_assert(sizeof(amrgpu_fill_buf) == sizeof(FillBuf), "fill buffer layout");
+static_assert(offsetof(amrgpu_fill_op, stage) == offsetof(FillOp, stage), "fill op layout");
+static_assert(offsetof(amrgpu_fill_buf, off) == offsetof(FillBuf, off), "fill buffer layout");
+
+static FillPlan g_inspect;
+
+int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes, const int* iregsz,
+                     const int* jregsz, const int* intratx, const int* intraty,
+                     const double* hxposs, const double* hyposs, const double* domain,
+                     const int* mthbc, int nvar, int level, int* nops, int* nbufs,
+                     int64_t* nmask, int64_t* nscratch) {
+    if (nlev < 1 || nlev >= FILL_MAXLV || level < 1 || level > nlev || !ngrids || !boxes ||
+        !iregsz || !jregsz || !intratx || !intraty || !hxposs || !hyposs || !domain || !mthbc ||
+        nvar < 1 || !nops || !nbufs || !nmask || !nscratch)
+        return fail(AMRGPU_ERR_ARG, "bad plan_fill arguments");
+    HierGeom H;
+    H.nlev = nlev;
+    int nb = 0;
+    for (int l = 1; l <= nlev; l++) {
+        LevelGeom& G = H.lev[l];
+        for (int k = 0; k < ngrids[l - 1]; k++, nb++)
+            G.boxes.push_back({boxes[4 * nb], boxes[4 * nb + 1], boxes[4 * nb + 2], boxes[4 * nb + 3]});
+        G.iregsz = iregsz[l - 1];
+        G.jregsz = jregsz[l - 1];
+        G.ratx = intratx[l - 1];
+        G.raty = intraty[l - 1];
+        G.hx = hxposs[l - 1];
+        G.hy = hyposs[l - 1];
+    }
+    H.xlower = domain[0];
+    H.xupper = domain[1];
+    H.ylower = domain[2];
+    H.yupper = domain[3];
+    H.xperdom = mthbc[0] == 2 && mthbc[1] == 2;
+    H.yperdom = mthbc[2] == 2 && mthbc[3] == 2;
+    for (int k = 0; k < 4; k++) H.mthbc[k] = mthbc[k];
+    std::string err;
+    if (!plan_fill_level(H, level, nvar, g_inspect, err))
+        return fail(AMRGPU_ERR_ARG, "ghost fill: %s", err.c_str());
+    *nops = (int)g_inspect.ops.size();
+    *nbufs = (int)g_inspect.bufs.size();
+    *nmask = (int64_t)g_inspect.masks.size();
+    *nscratch = g_inspect.scratch_elems;
+    return AMRGPU_OK;
+}
+
+int amrgpu_plan_fill_get(amrgpu_fill_op* ops, amrgpu_fill_buf* bufs, unsigned char* masks) {
+    if (!ops || !bufs || !masks) return fail(AMRGPU_ERR_ARG, "bad plan_fill_get arguments");
+    std::memcpy(ops, g_inspect.ops.data(), g_inspect.ops.size() * sizeof(FillOp));
+    std::memcpy(bufs, g_inspect.bufs.data(), g_inspect.bufs.size() * sizeof(FillBuf));
+    std::memcpy(masks, g_inspect.masks.data(), g_inspect.masks.size());
     return AMRGPU_OK;
 }
 

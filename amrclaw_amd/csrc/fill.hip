@@ -1,0 +1,182 @@
+// fill.hip -- executes one stage of a ghost-cell fill program.
+// Built with -ffp-contract=off: the time blend and the coarse-fine
+// interpolation are evaluated exactly as the reference's FP64 code.
+#include <hip/hip_runtime.h>
+#include "fill.hpp"
+
+namespace amrgpu {
+
+template <typename S>
+struct BufRef {
+    S* lvl;          // kind 0
+    double* scr;     // kind 1
+    int64_t off;
+    int32_t nrow, plane;
+    __device__ double load(int m, int i, int j) const {
+        int64_t k = off + (int64_t)m * plane + (int64_t)j * nrow + i;
+        return lvl ? (double)lvl[k] : scr[k];
+    }
+    __device__ void store(int m, int i, int j, double v) const {
+        int64_t k = off + (int64_t)m * plane + (int64_t)j * nrow + i;
+        if (lvl) lvl[k] = (S)v; else scr[k] = v;
+    }
+};
+
+template <typename S>
+__device__ BufRef<S> bufref(const FillArgs& a, int b) {
+    const FillBuf fb = a.bufs[b];
+    BufRef<S> r;
+    if (fb.kind == 0) {
+        r.lvl = static_cast<S*>(a.lvl_new[fb.level]);
+        r.scr = nullptr;
+        r.off = a.lvl_patches[fb.level][fb.patch].off;
+    } else {
+        r.lvl = nullptr;
+        r.scr = a.scratch;
+        r.off = fb.off;
+    }
+    r.nrow = fb.nrow;
+    r.plane = fb.plane;
+    return r;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
+    const FillOp op = a.ops[blockIdx.x];
+    const int tid = threadIdx.x, nv = a.nv;
+    const BufRef<S> dst = bufref<S>(a, op.dst);
+    const int ncell = op.ni * op.nj;
+
+    if (op.type == FOP_COPY) {
+        const PatchDesc sp = a.lvl_patches[op.src_level][op.src];
+        const S* qn = static_cast<const S*>(a.lvl_new[op.src_level]) + sp.off;
+        const S* qo = static_cast<const S*>(a.lvl_old[op.src_level]) + sp.off;
+        const int mode = a.tmode[op.src_level];
+        const double ai = a.alphai[op.src_level], ac = a.alphac[op.src_level];
+        for (int t = tid; t < ncell * nv; t += 256) {
+            const int m = t / ncell, c = t - m * ncell;
+            const int dj = c / op.ni, di = c - dj * op.ni;
+            const int64_t s = (int64_t)m * sp.plane + (int64_t)(op.sj0 + dj) * sp.mitot + (op.si0 + di);
+            double v;
+            if (mode == TIME_NEW) v = (double)qn[s];
+            else if (mode == TIME_OLD) v = (double)qo[s];
+            else v = (double)qn[s] * ai + (double)qo[s] * ac;
+            dst.store(m, op.di0 + di, op.dj0 + dj, v);
+        }
+    } else if (op.type == FOP_INTERP) {
+        const BufRef<S> crs = bufref<S>(a, op.src);
+        const double dxf = a.hx[op.level], dyf = a.hy[op.level];
+        const double dxc = a.hx[op.level - 1], dyc = a.hy[op.level - 1];
+        const double xlow_coarse = a.xlower + op.iplo * dxc;
+        const double ylow_coarse = a.ylower + op.jplo * dyc;
+        const double ratiox = (double)op.ratx, ratioy = (double)op.raty;
+        const unsigned char* mask = a.masks + op.mask_off;
+        for (int c = tid; c < ncell; c += 256) {
+            if (mask[c]) continue;
+            const int jf = c / op.ni + 1, if_ = c - (jf - 1) * op.ni + 1;   // 1-based
+            const int j_coarse = (int)floor((double)(jf + op.jlo - 1) / ratioy) - op.jplo + 1;
+            const double ycent_coarse = ylow_coarse + ((double)j_coarse - .5) * dyc;
+            const double ycent_fine = a.ylower + ((double)(jf - 1 + op.jlo) + .5) * dyf;
+            const double eta2 = (ycent_fine - ycent_coarse) / dyc;
+            const int i_coarse = (int)floor((double)(if_ + op.ilo - 1) / ratiox) - op.iplo + 1;
+            const double xcent_coarse = xlow_coarse + ((double)i_coarse - .5) * dxc;
+            const double xcent_fine = a.xlower + ((double)(if_ - 1 + op.ilo) + .5) * dxf;
+            const double eta1 = (xcent_fine - xcent_coarse) / dxc;
+            const int ic = i_coarse - 1, jc = j_coarse - 1;                  // 0-based
+            for (int n = 0; n < nv; n++) {
+                const double valp10 = crs.load(n, ic + 1, jc);
+                const double valm10 = crs.load(n, ic - 1, jc);
+                const double valc = crs.load(n, ic, jc);
+                const double valp01 = crs.load(n, ic, jc + 1);
+                const double valm01 = crs.load(n, ic, jc - 1);
+                const double dupc = valp10 - valc, dumc = valc - valm10, ducc = valp10 - valm10;
+                double du = fmin(fabs(dupc), fabs(dumc));
+                du = fmin(2.0 * du, 0.5 * fabs(ducc));
+                const double fu = fmax(0.0, copysign(1.0, dupc * dumc));
+                const double uslope = du * copysign(1.0, ducc) * fu;
+                const double dvpc = valp01 - valc, dvmc = valc - valm01, dvcc = valp01 - valm01;
+                double dv = fmin(fabs(dvpc), fabs(dvmc));
+                dv = fmin(2.0 * dv, 0.5 * fabs(dvcc));
+                const double fv = fmax(0.0, copysign(1.0, dvpc * dvmc));
+                const double vslope = dv * copysign(1.0, dvcc) * fv;
+                const double valint = valc + eta1 * uslope + eta2 * vslope;
+                dst.store(n, op.di0 + if_ - 1, op.dj0 + jf - 1, valint);
+            }
+        }
+    } else if (op.type == FOP_COPYBACK) {
+        const BufRef<S> src = bufref<S>(a, op.src);
+        for (int t = tid; t < ncell * nv; t += 256) {
+            const int m = t / ncell, c = t - m * ncell;
+            const int dj = c / op.ni, di = c - dj * op.ni;
+            dst.store(m, op.di0 + di, op.dj0 + dj, src.load(m, di, dj));
+        }
+    } else if (op.type == FOP_BC) {
+        // sides in the reference's order, each complete before the next
+        const int nrow = op.ni, ncol = op.nj;
+        // left: columns 0..nxl-1 from column nxl (extrapolation) or mirrored (wall)
+        const int nxl = op.bc[0];
+        if (nxl > 0 && (a.mthbc[0] == 1 || a.mthbc[0] == 3)) {
+            for (int t = tid; t < nxl * ncol; t += 256) {
+                const int j = t / nxl, i = t - j * nxl;
+                const int isrc = a.mthbc[0] == 1 ? nxl : 2 * nxl - 1 - i;
+                for (int m = 0; m < nv; m++) {
+                    double v = dst.load(m, isrc, j);
+                    if (a.mthbc[0] == 3 && m == 1) v = -v;
+                    dst.store(m, i, j, v);
+                }
+            }
+        }
+        __syncthreads();
+        const int ibeg = op.bc[1];              // 1-based first column set
+        if (ibeg > 0 && (a.mthbc[1] == 1 || a.mthbc[1] == 3)) {
+            const int nr = nrow - ibeg + 1;
+            for (int t = tid; t < nr * ncol; t += 256) {
+                const int j = t / nr, i = ibeg - 1 + (t - j * nr);
+                const int isrc = a.mthbc[1] == 1 ? ibeg - 2 : 2 * ibeg - 1 - (i + 1) - 1;
+                for (int m = 0; m < nv; m++) {
+                    double v = dst.load(m, isrc, j);
+                    if (a.mthbc[1] == 3 && m == 1) v = -v;
+                    dst.store(m, i, j, v);
+                }
+            }
+        }
+        __syncthreads();
+        const int nyb = op.bc[2];
+        if (nyb > 0 && (a.mthbc[2] == 1 || a.mthbc[2] == 3)) {
+            for (int t = tid; t < nyb * nrow; t += 256) {
+                const int j = t / nrow, i = t - j * nrow;
+                const int jsrc = a.mthbc[2] == 1 ? nyb : 2 * nyb - 1 - j;
+                for (int m = 0; m < nv; m++) {
+                    double v = dst.load(m, i, jsrc);
+                    if (a.mthbc[2] == 3 && m == 2) v = -v;
+                    dst.store(m, i, j, v);
+                }
+            }
+        }
+        __syncthreads();
+        const int jbeg = op.bc[3];
+        if (jbeg > 0 && (a.mthbc[3] == 1 || a.mthbc[3] == 3)) {
+            const int nc = ncol - jbeg + 1;
+            for (int t = tid; t < nc * nrow; t += 256) {
+                const int j = jbeg - 1 + t / nrow, i = t - (t / nrow) * nrow;
+                const int jsrc = a.mthbc[3] == 1 ? jbeg - 2 : 2 * jbeg - 1 - (j + 1) - 1;
+                for (int m = 0; m < nv; m++) {
+                    double v = dst.load(m, i, jsrc);
+                    if (a.mthbc[3] == 3 && m == 2) v = -v;
+                    dst.store(m, i, j, v);
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_fill_stage(int esize, const FillArgs& a, hipStream_t st) {
+    if (a.nops <= 0) return hipSuccess;
+    if (esize == 8)
+        hipLaunchKernelGGL(fill_stage_kernel<double>, dim3(a.nops), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(fill_stage_kernel<float>, dim3(a.nops), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace amrgpu

@@ -1,0 +1,73 @@
+// fill.hpp -- ghost-cell fill programs (bound.f90 and its recursion) as data.
+//
+// The host planner replays bound -> filrecur -> intfil/trimbd ->
+// (recursive coarse patch) -> interpolation -> bc2amr, and prefilrecur's
+// periodic split, on the index boxes of the hierarchy once per regrid.  It
+// emits a list of operations on "buffers" (the level's patches and FP64
+// scratch arrays for the coarse patches valcrse and periodic pieces
+// valPatch), each tagged with a dependency stage.  At every level step one
+// launch per stage executes all operations of that stage, one workgroup per
+// operation.  Values are computed in FP64 in the reference's order.
+#pragma once
+#include <cstdint>
+#include "types.hpp"
+
+namespace amrgpu {
+
+constexpr int FILL_MAXLV = 12;
+
+enum FillOpType : int32_t {
+    FOP_COPY = 0,      // intfil: copy / time-blend from a grid (intfil.f90:237-259)
+    FOP_INTERP = 1,    // filrecur: limited linear interpolation (filpatch.f90:260-327)
+    FOP_BC = 2,        // bc2amr on a whole buffer (bc2amr.f90:84-296)
+    FOP_COPYBACK = 3   // prefilrecur: patchCopyOut (prefilp.f90:218-245)
+};
+
+// A destination/scratch array, SoA over components: cell (i, j), 0-based
+// with ghosts, component m at off + m*plane + j*nrow + i.
+struct FillBuf {
+    int32_t kind;      // 0: patch `patch` of level `level` (level storage); 1: FP64 scratch
+    int32_t level;
+    int32_t patch;
+    int32_t nrow, ncol, plane;
+    int32_t pad;
+    int64_t off;       // kind 1: element offset in the scratch arena
+};
+
+struct FillOp {
+    int32_t type;
+    int32_t dst;              // destination buffer
+    int32_t src;              // COPY: source grid of src_level; INTERP/COPYBACK: source buffer
+    int32_t src_level;        // COPY: level of the source grid
+    int32_t di0, dj0, ni, nj; // destination rectangle (0-based cells of dst)
+    int32_t si0, sj0;         // COPY: matching cell of the source grid (0-based, with ghosts)
+    int32_t mask_off;         // INTERP: offset of the ni*nj flaguse bytes (row-major)
+    int32_t ilo, jlo;         // INTERP: global index of rectangle cell (0,0), fine level
+    int32_t iplo, jplo;       // INTERP: global index of coarse cell (0,0) of src buffer
+    int32_t ratx, raty;       // INTERP: refinement ratio coarse -> fine
+    int32_t level;            // INTERP: fine level
+    int32_t bc[4];            // BC: nxl, ibeg, nyb, jbeg (0 = side not applied)
+    int32_t stage;
+    int32_t pad[2];
+};
+
+enum TimeMode : int32_t { TIME_NEW = 0, TIME_OLD = 1, TIME_BLEND = 2 };
+
+struct FillArgs {
+    const FillOp* ops;
+    const FillBuf* bufs;
+    double* scratch;
+    const unsigned char* masks;
+    void* lvl_new[FILL_MAXLV];           // level storage, new time (store1)
+    void* lvl_old[FILL_MAXLV];           // old time (store2)
+    const PatchDesc* lvl_patches[FILL_MAXLV];
+    double alphai[FILL_MAXLV], alphac[FILL_MAXLV];
+    int32_t tmode[FILL_MAXLV];
+    double hx[FILL_MAXLV], hy[FILL_MAXLV];
+    double xlower, ylower;
+    int32_t mthbc[4];
+    int32_t nv;
+    int32_t nops;
+};
+
+}  // namespace amrgpu

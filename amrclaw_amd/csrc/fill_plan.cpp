@@ -1,0 +1,381 @@
+// fill_plan.cpp -- host planner of the ghost-cell fill programs.
+//
+// Replays bound.f90:51-158 for every grid of a level on index boxes only:
+// the four ghost strips, filrecur (filpatch.f90:71-355) with intfil's
+// coverage marks (intfil.f90:98-311), trimbd's unset rectangle
+// (trimbd.f90:32-97), the coarse patch formula (filpatch.f90:205-210), the
+// recursion to coarser levels, prefilrecur's periodic split
+// (prefilp.f90:18-214) and bc2amr's side extents (bc2amr.f90:104-257).
+// Every array index a generated operation will touch on the device is
+// checked here, so a program that would read or write outside a buffer is
+// rejected on the host instead of launched.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fill_plan.hpp"
+
+namespace amrgpu {
+
+namespace {
+
+struct Planner {
+    const HierGeom& H;
+    FillPlan& P;
+    int nv;
+    std::string err;
+    std::vector<std::vector<int>> patch_buf;   // [level][grid] -> buffer id
+
+    Planner(const HierGeom& h, FillPlan& p, int nvar) : H(h), P(p), nv(nvar) {
+        patch_buf.resize(H.nlev + 1);
+        for (int l = 1; l <= H.nlev; l++) patch_buf[l].assign(H.lev[l].boxes.size(), -1);
+    }
+
+    int pbuf(int level, int grid) {
+        int& b = patch_buf[level][grid];
+        if (b < 0) {
+            const auto& bx = H.lev[level].boxes[grid];
+            FillBuf fb{};
+            fb.kind = 0;
+            fb.level = level;
+            fb.patch = grid;
+            fb.nrow = bx[1] - bx[0] + 1 + 2 * NGHOST;
+            fb.ncol = bx[3] - bx[2] + 1 + 2 * NGHOST;
+            fb.plane = fb.nrow * fb.ncol;
+            b = (int)P.bufs.size();
+            P.bufs.push_back(fb);
+        }
+        return b;
+    }
+
+    int scratch(int nrow, int ncol) {
+        FillBuf fb{};
+        fb.kind = 1;
+        fb.level = 0;
+        fb.patch = -1;
+        fb.nrow = nrow;
+        fb.ncol = ncol;
+        fb.plane = nrow * ncol;
+        fb.off = P.scratch_elems;
+        P.scratch_elems += ((int64_t)nv * fb.plane + 31) / 32 * 32;
+        P.bufs.push_back(fb);
+        return (int)P.bufs.size() - 1;
+    }
+
+    bool fail(const char* msg) {
+        if (err.empty()) err = msg;
+        return false;
+    }
+
+    void push(FillOp op) { P.ops.push_back(op); }
+
+    // bc2amr on a whole buffer with the given physical extent
+    bool bc_op(int buf, double hx, double hy, double xlo, double xhi, double ylo, double yhi,
+               int stage) {
+        if (H.xperdom && (H.yperdom || H.spheredom)) return true;   // bc2amr.f90:109
+        const FillBuf& fb = P.bufs[buf];
+        const int nrow = fb.nrow, ncol = fb.ncol;
+        FillOp op{};
+        op.type = FOP_BC;
+        op.dst = buf;
+        op.ni = nrow;
+        op.nj = ncol;
+        op.stage = stage;
+        const double hxmarg = hx * .01, hymarg = hy * .01;
+        bool any = false;
+        if (xlo < H.xlower - hxmarg) {
+            int nxl = (int)((H.xlower + hxmarg - xlo) / hx);
+            int mb = H.mthbc[0];
+            if (mb != 1 && mb != 2 && mb != 3 && mb != 4) return fail("bc2amr: user/unknown BC on the left");
+            // a buffer lying wholly outside the domain on this side has no
+            // source column; the reference reads past the array there and the
+            // values are overwritten by the grid's own bc2amr, so skip it
+            bool fits = (mb == 1 && nxl < nrow) || (mb == 3 && 2 * nxl <= nrow);
+            if (nxl > 0 && (mb == 1 || mb == 3) && fits) {
+                op.bc[0] = nxl;
+                any = true;
+            }
+        }
+        if (xhi > H.xupper + hxmarg) {
+            int nxr = (int)((xhi - H.xupper + hxmarg) / hx);
+            int ibeg = std::max(nrow - nxr + 1, 1);
+            int mb = H.mthbc[1];
+            if (mb != 1 && mb != 2 && mb != 3 && mb != 4) return fail("bc2amr: user/unknown BC on the right");
+            bool fits = (mb == 1 && ibeg >= 2) || (mb == 3 && 2 * ibeg - 1 - nrow >= 1);
+            if (ibeg <= nrow && (mb == 1 || mb == 3) && fits) {
+                op.bc[1] = ibeg;
+                any = true;
+            }
+        }
+        if (ylo < H.ylower - hymarg) {
+            int nyb = (int)((H.ylower + hymarg - ylo) / hy);
+            int mb = H.mthbc[2];
+            if (mb != 1 && mb != 2 && mb != 3 && mb != 4) return fail("bc2amr: user/unknown BC at the bottom");
+            bool fits = (mb == 1 && nyb < ncol) || (mb == 3 && 2 * nyb <= ncol);
+            if (nyb > 0 && (mb == 1 || mb == 3) && fits) {
+                op.bc[2] = nyb;
+                any = true;
+            }
+        }
+        if (yhi > H.yupper + hymarg) {
+            int nyt = (int)((yhi - H.yupper + hymarg) / hy);
+            int jbeg = std::max(ncol - nyt + 1, 1);
+            int mb = H.mthbc[3];
+            if (mb != 1 && mb != 2 && mb != 3 && mb != 4) return fail("bc2amr: user/unknown BC at the top");
+            bool fits = (mb == 1 && jbeg >= 2) || (mb == 3 && 2 * jbeg - 1 - ncol >= 1);
+            if (jbeg <= ncol && (mb == 1 || mb == 3) && fits) {
+                op.bc[3] = jbeg;
+                any = true;
+            }
+        }
+        if (any) push(op);
+        return true;
+    }
+
+    // filrecur: fill rectangle [ilo..ihi]x[jlo..jhi] of `level` into buffer
+    // `buf` starting at its 1-based cell (nrowst, ncolst).  Returns the stage
+    // after which the rectangle (and, with patchOnly, the buffer) is final.
+    int filrecur(int level, int buf, int nrowst, int ncolst, int ilo, int ihi, int jlo, int jhi,
+                 bool patchOnly) {
+        if (!err.empty()) return -1;
+        const LevelGeom& G = H.lev[level];
+        const int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
+        const FillBuf fb = P.bufs[buf];
+        if (nrowst - 1 < 0 || ncolst - 1 < 0 || nrowst - 1 + ni > fb.nrow || ncolst - 1 + nj > fb.ncol) {
+            fail("filrecur: rectangle outside its buffer");
+            return -1;
+        }
+        std::vector<unsigned char> flag((size_t)ni * nj, 0);
+        // intfil: same-level grids intersecting the rectangle
+        for (int k = 0; k < (int)G.boxes.size(); k++) {
+            const auto& b = G.boxes[k];
+            int ixlo = std::max(b[0], ilo), ixhi = std::min(b[1], ihi);
+            int jxlo = std::max(b[2], jlo), jxhi = std::min(b[3], jhi);
+            if (!(ixlo <= ixhi && jxlo <= jxhi)) continue;
+            FillOp op{};
+            op.type = FOP_COPY;
+            op.dst = buf;
+            op.src = k;
+            op.src_level = level;
+            op.di0 = ixlo - ilo + nrowst - 1;
+            op.dj0 = jxlo - jlo + ncolst - 1;
+            op.ni = ixhi - ixlo + 1;
+            op.nj = jxhi - jxlo + 1;
+            op.si0 = ixlo - b[0] + NGHOST;
+            op.sj0 = jxlo - b[2] + NGHOST;
+            op.stage = 0;
+            push(op);
+            P.src_levels_mask |= 1u << level;
+            for (int j = jxlo; j <= jxhi; j++)
+                for (int i = ixlo; i <= ixhi; i++) flag[(size_t)(j - jlo) * ni + (i - ilo)] = 1;
+        }
+        // cells outside the domain are left to the boundary conditions
+        auto mark = [&](int i0, int i1, int j0, int j1) {
+            i0 = std::max(i0, ilo); i1 = std::min(i1, ihi);
+            j0 = std::max(j0, jlo); j1 = std::min(j1, jhi);
+            for (int j = j0; j <= j1; j++)
+                for (int i = i0; i <= i1; i++) flag[(size_t)(j - jlo) * ni + (i - ilo)] = 1;
+        };
+        if (jhi >= G.jregsz) mark(ilo, ihi, std::max(G.jregsz, jlo), jhi);
+        if (jlo < 0) mark(ilo, ihi, jlo, std::min(-1, ncolst + jhi - jlo));
+        if (ilo < 0) mark(ilo, std::min(-1, nrowst + ihi - ilo), jlo, jhi);
+        if (ihi >= G.iregsz) mark(std::max(G.iregsz, ilo), ihi, jlo, jhi);
+        // trimbd
+        int un[4] = {0, 0, 0, 0};
+        bool set = true;
+        {
+            int imn = ni + 1, imx = 0, jmn = nj + 1, jmx = 0;
+            for (int j = 1; j <= nj; j++)
+                for (int i = 1; i <= ni; i++)
+                    if (!flag[(size_t)(j - 1) * ni + (i - 1)]) {
+                        set = false;
+                        imn = std::min(imn, i); imx = std::max(imx, i);
+                        jmn = std::min(jmn, j); jmx = std::max(jmx, j);
+                    }
+            un[0] = imn; un[1] = imx; un[2] = jmn; un[3] = jmx;
+        }
+        int ready = 0;
+        if (!set) {
+            if (level == 1) {
+                fail("filrecur: level 1 cells left unset");
+                return -1;
+            }
+            const LevelGeom& C = H.lev[level - 1];
+            un[0] += ilo - 1; un[1] += ilo - 1; un[2] += jlo - 1; un[3] += jlo - 1;
+            const int rx = C.ratx, ry = C.raty, ng = NGHOST;
+            int iplo = (un[0] - rx + ng * rx) / rx - ng;
+            int iphi = (un[1] + rx) / rx;
+            int jplo = (un[2] - ry + ng * ry) / ry - ng;
+            int jphi = (un[3] + ry) / ry;
+            int mic = iphi - iplo + 1, mjc = jphi - jplo + 1;
+            if (mic > ihi - ilo + 3 || mjc > jhi - jlo + 3) {
+                fail("filrecur: coarse scratch too small");
+                return -1;
+            }
+            int cbuf = scratch(mic, mjc);
+            bool sticks = iplo < 0 || jplo < 0 || iphi >= C.iregsz || jphi >= C.jregsz;
+            int rc;
+            if ((H.xperdom || H.yperdom || H.spheredom) && sticks)
+                rc = prefil(level - 1, cbuf, iplo, iphi, jplo, jphi, iplo, jplo);
+            else
+                rc = filrecur(level - 1, cbuf, 1, 1, iplo, iphi, jplo, jphi, true);
+            if (rc < 0) return -1;
+            // interpolation reads coarse cells ic-1..ic+1, jc-1..jc+1 of the
+            // scratch for every unset fine cell: check them all
+            for (int j = 1; j <= nj; j++)
+                for (int i = 1; i <= ni; i++) {
+                    if (flag[(size_t)(j - 1) * ni + (i - 1)]) continue;
+                    int jc = (int)std::floor((double)(j + jlo - 1) / (double)ry) - jplo + 1;
+                    int ic = (int)std::floor((double)(i + ilo - 1) / (double)rx) - iplo + 1;
+                    if (ic - 1 < 1 || ic + 1 > mic || jc - 1 < 1 || jc + 1 > mjc) {
+                        fail("filrecur: interpolation stencil outside the coarse patch");
+                        return -1;
+                    }
+                }
+            FillOp op{};
+            op.type = FOP_INTERP;
+            op.dst = buf;
+            op.src = cbuf;
+            op.di0 = nrowst - 1;
+            op.dj0 = ncolst - 1;
+            op.ni = ni;
+            op.nj = nj;
+            op.mask_off = (int32_t)P.masks.size();
+            P.masks.insert(P.masks.end(), flag.begin(), flag.end());
+            op.ilo = ilo;
+            op.jlo = jlo;
+            op.iplo = iplo;
+            op.jplo = jplo;
+            op.ratx = rx;
+            op.raty = ry;
+            op.level = level;
+            op.stage = rc + 1;
+            push(op);
+            ready = rc + 1;
+        }
+        if (patchOnly) {
+            const double dxf = G.hx, dyf = G.hy;
+            double xlo = H.xlower + ilo * dxf, ylo = H.ylower + jlo * dyf;
+            double xhi = H.xlower + (ihi + 1) * dxf, yhi = H.ylower + (jhi + 1) * dyf;
+            if (!bc_op(buf, dxf, dyf, xlo, xhi, ylo, yhi, ready + 1)) return -1;
+            ready = ready + 1;
+        }
+        return ready;
+    }
+
+    // prefilrecur (periodic split); buffer cell (1,1) is global (iglo, jglo)
+    int prefil(int level, int buf, int ilo, int ihi, int jlo, int jhi, int iglo, int jglo) {
+        if (!err.empty()) return -1;
+        if (H.spheredom) { fail("spherical domains are not supported"); return -1; }
+        const LevelGeom& G = H.lev[level];
+        int ist[3], iend[3], ishift[3], jst[3], jend[3], jshift[3];
+        if (H.xperdom) {
+            ist[0] = ilo; ist[1] = 0; ist[2] = G.iregsz;
+            iend[0] = -1; iend[1] = G.iregsz - 1; iend[2] = ihi;
+            ishift[0] = G.iregsz; ishift[1] = 0; ishift[2] = -G.iregsz;
+        } else {
+            ist[0] = G.iregsz; ist[1] = ilo; ist[2] = G.iregsz;
+            iend[0] = -1; iend[1] = ihi; iend[2] = -1;
+            ishift[0] = ishift[1] = ishift[2] = 0;
+        }
+        if (H.yperdom) {
+            jst[0] = jlo; jst[1] = 0; jst[2] = G.jregsz;
+            jend[0] = -1; jend[1] = G.jregsz - 1; jend[2] = jhi;
+            jshift[0] = G.jregsz; jshift[1] = 0; jshift[2] = -G.jregsz;
+        } else {
+            jst[0] = G.jregsz; jst[1] = jlo; jst[2] = G.jregsz;
+            jend[0] = -1; jend[1] = jhi; jend[2] = -1;
+            jshift[0] = jshift[1] = jshift[2] = 0;
+        }
+        int ready = 0;
+        for (int a = 0; a < 3; a++) {
+            int i1 = std::max(ilo, ist[a]), i2 = std::min(ihi, iend[a]);
+            if (i1 > i2) continue;
+            for (int b = 0; b < 3; b++) {
+                int j1 = std::max(jlo, jst[b]), j2 = std::min(jhi, jend[b]);
+                if (j1 > j2) continue;
+                int mi = i2 - i1 + 1, mj = j2 - j1 + 1;
+                int tbuf = scratch(mi, mj);
+                int rc = filrecur(level, tbuf, 1, 1, i1 + ishift[a], i2 + ishift[a],
+                                  j1 + jshift[b], j2 + jshift[b], true);
+                if (rc < 0) return -1;
+                FillOp op{};
+                op.type = FOP_COPYBACK;
+                op.dst = buf;
+                op.src = tbuf;
+                op.di0 = i1 - iglo;
+                op.dj0 = j1 - jglo;
+                op.ni = mi;
+                op.nj = mj;
+                op.stage = rc + 1;
+                const FillBuf& db = P.bufs[buf];
+                if (op.di0 < 0 || op.dj0 < 0 || op.di0 + mi > db.nrow || op.dj0 + mj > db.ncol) {
+                    fail("prefilrecur: piece outside its buffer");
+                    return -1;
+                }
+                push(op);
+                ready = std::max(ready, rc + 1);
+            }
+        }
+        return ready;
+    }
+
+    // bound.f90:51-158 for grid `grid` of `level`
+    bool bound(int level, int grid) {
+        const LevelGeom& G = H.lev[level];
+        const auto& b = G.boxes[grid];
+        const int ng = NGHOST;
+        const int ilo = b[0], ihi = b[1], jlo = b[2], jhi = b[3];
+        const int mitot = ihi - ilo + 1 + 2 * ng, mjtot = jhi - jlo + 1 + 2 * ng;
+        const double hx = G.hx, hy = G.hy;
+        const double xleft = ilo * hx + H.xlower, xright = (ihi + 1) * hx + H.xlower;
+        const double ybot = jlo * hy + H.ylower, ytop = (jhi + 1) * hy + H.ylower;
+        const int buf = pbuf(level, grid);
+        int r[4];
+        double xl = xleft - ng * hx, xr = xright + ng * hx;
+        if (xl < H.xlower && H.xperdom)
+            r[0] = prefil(level, buf, ilo - ng, ilo - 1, jlo, jhi, ilo - ng, jlo - ng);
+        else
+            r[0] = filrecur(level, buf, 1, ng + 1, ilo - ng, ilo - 1, jlo, jhi, false);
+        if (xr > H.xupper && H.xperdom)
+            r[1] = prefil(level, buf, ihi + 1, ihi + ng, jlo, jhi, ilo - ng, jlo - ng);
+        else
+            r[1] = filrecur(level, buf, mitot - ng + 1, ng + 1, ihi + 1, ihi + ng, jlo, jhi, false);
+        double yb = ybot - ng * hy, yt = ytop + ng * hy;
+        bool xsticks = (xl < H.xlower || xr > H.xupper) && H.xperdom;
+        if ((yb < H.ylower && (H.yperdom || H.spheredom)) || xsticks)
+            r[2] = prefil(level, buf, ilo - ng, ihi + ng, jlo - ng, jlo - 1, ilo - ng, jlo - ng);
+        else
+            r[2] = filrecur(level, buf, 1, 1, ilo - ng, ihi + ng, jlo - ng, jlo - 1, false);
+        if ((yt > H.yupper && (H.yperdom || H.spheredom)) || xsticks)
+            r[3] = prefil(level, buf, ilo - ng, ihi + ng, jhi + 1, jhi + ng, ilo - ng, jlo - ng);
+        else
+            r[3] = filrecur(level, buf, 1, mjtot - ng + 1, ilo - ng, ihi + ng, jhi + 1, jhi + ng, false);
+        if (!err.empty()) return false;
+        int ready = std::max(std::max(r[0], r[1]), std::max(r[2], r[3]));
+        return bc_op(buf, hx, hy, xl, xr, yb, yt, ready + 1);
+    }
+};
+
+}  // namespace
+
+bool plan_fill_level(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err) {
+    P = FillPlan();
+    Planner pl(H, P, nvar);
+    for (int k = 0; k < (int)H.lev[level].boxes.size(); k++)
+        if (!pl.bound(level, k)) {
+            err = pl.err.empty() ? "ghost-fill planning failed" : pl.err;
+            return false;
+        }
+    std::stable_sort(P.ops.begin(), P.ops.end(),
+                     [](const FillOp& a, const FillOp& b) { return a.stage < b.stage; });
+    P.nstages = P.ops.empty() ? 0 : P.ops.back().stage + 1;
+    P.stage_start.assign(P.nstages + 1, 0);
+    for (const auto& op : P.ops) P.stage_start[op.stage + 1]++;
+    for (int s = 0; s < P.nstages; s++) P.stage_start[s + 1] += P.stage_start[s];
+    return true;
+}
+
+}  // namespace amrgpu

@@ -1,0 +1,41 @@
+// fill_plan.hpp -- hierarchy geometry and the ghost-fill planner interface.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fill.hpp"
+
+namespace amrgpu {
+
+// Index-space description of one level (the parts of amr_module's
+// node/hxposs/iregsz/intratx tables the fill needs).
+struct LevelGeom {
+    std::vector<std::array<int, 4>> boxes;   // (ilo, ihi, jlo, jhi) per grid
+    int iregsz = 0, jregsz = 0;              // level size in cells
+    int ratx = 2, raty = 2;                  // ratio to the next finer level
+    double hx = 0, hy = 0;
+};
+
+struct HierGeom {
+    int nlev = 0;
+    LevelGeom lev[FILL_MAXLV];
+    double xlower = 0, xupper = 1, ylower = 0, yupper = 1;
+    bool xperdom = false, yperdom = false, spheredom = false;
+    int mthbc[4] = {1, 1, 1, 1};
+};
+
+struct FillPlan {
+    std::vector<FillBuf> bufs;
+    std::vector<FillOp> ops;                 // sorted by stage
+    std::vector<unsigned char> masks;
+    std::vector<int> stage_start;            // ops of stage s: [stage_start[s], stage_start[s+1])
+    int nstages = 0;
+    int64_t scratch_elems = 0;
+    uint32_t src_levels_mask = 0;            // levels read as time sources
+};
+
+bool plan_fill_level(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
+
+}  // namespace amrgpu

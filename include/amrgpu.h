@@ -83,6 +83,60 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out,
                     double dx, double dy, double dt, double* fm, double* fp,
                     double* gm, double* gp, double* cflgrid, double* dtnew);
 
+/* Level geometry of the whole run (domain.f:92-106, amr2.f90:455): for
+ * levels 1..mxnest the index-space size iregsz/jregsz, the refinement ratio
+ * to the next level intratx/intraty and the mesh widths hxposs/hyposs
+ * (arrays indexed from 0 for level 1).  The domain and boundary conditions
+ * come from amrgpu_config (periodic directions: mthbc 2 on both sides,
+ * amr2.f90:279-280). */
+int amrgpu_set_geometry(int mxnest, const int* iregsz, const int* jregsz,
+                        const int* intratx, const int* intraty,
+                        const double* hxposs, const double* hyposs);
+
+/* Time of every grid of a level (rnode(timemult)) and its step possk. */
+int amrgpu_set_time(int level, double time, double possk);
+
+/* Fill the ghost cells of every grid of a level at time `time`:
+ * bound.f90:51-158 for all grids (advanc.f:60-84), including same-level
+ * copies, time-interpolated coarse values and limited linear coarse-fine
+ * interpolation through coarser levels (filpatch.f90:71-355), periodic
+ * wrap (prefilp.f90:18) and physical boundary conditions (bc2amr.f90:84).
+ * The fill program is planned on the host at the first call after a
+ * layout change and replayed on the device afterwards. */
+int amrgpu_bound_level(int level, double time);
+
+/* Ghost-fill program inspection (host only; needs no device and no init).
+ * Plans bound for every grid of `level` of the hierarchy described by the
+ * arguments (levels 1..nlev; grid boxes concatenated level by level as
+ * (ilo, ihi, jlo, jhi); per-level arrays indexed from 0 for level 1;
+ * domain = xlower, xupper, ylower, yupper) and reports its size; the
+ * records are then copied out with amrgpu_plan_fill_get.  Operation types:
+ * 0 copy/time-blend from a grid, 1 coarse-fine interpolation, 2 boundary
+ * conditions on a whole buffer, 3 copy of a periodic piece.  Buffers of
+ * kind 0 are grids of the level store, kind 1 FP64 scratch arrays. */
+typedef struct amrgpu_fill_op {
+    int type, dst, src, src_level;
+    int di0, dj0, ni, nj;
+    int si0, sj0;
+    int mask_off;
+    int ilo, jlo, iplo, jplo, ratx, raty, level;
+    int bc[4];
+    int stage;
+    int pad[2];
+} amrgpu_fill_op;
+typedef struct amrgpu_fill_buf {
+    int kind, level, patch, nrow, ncol, plane, pad;
+    int64_t off;
+} amrgpu_fill_buf;
+int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes,
+                     const int* iregsz, const int* jregsz, const int* intratx,
+                     const int* intraty, const double* hxposs,
+                     const double* hyposs, const double* domain,
+                     const int* mthbc, int nvar, int level, int* nops,
+                     int* nbufs, int64_t* nmask, int64_t* nscratch);
+int amrgpu_plan_fill_get(amrgpu_fill_op* ops, amrgpu_fill_buf* bufs,
+                         unsigned char* masks);
+
 /* Define the grids of one level (after regrid): global index boxes
  * [ilo..ihi] x [jlo..jhi] and mesh widths hx, hy (hxposs/hyposs).
  * Replaces the layout bookkeeping of regrid.f:72-118 on the device side;

@@ -10,6 +10,12 @@
  *   ORC_E_TIME    intfil: time not available from a source grid (intfil.f90:197-233)
  *   ORC_E_BC      bc2amr: user (0) or unknown boundary condition (bc2amr.f90:122-154)
  *   ORC_E_SPHERE  spherical domains are not restated
+ *
+ * One guard departs from the literal Fortran: bc2amr on an array lying wholly
+ * outside the domain on one side (a periodic piece of a ghost strip below or
+ * above a non-periodic boundary) has no source column/row inside the array;
+ * the reference reads past the array there (undefined) and bound's own
+ * bc2amr overwrites those cells afterwards.  The oracle skips such a side.
  */
 #include <math.h>
 #include <stdint.h>
@@ -35,7 +41,9 @@ int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
     if (h->xperdom && (h->yperdom || h->spheredom)) return 0;    /* :109 */
     if (xlo_patch < h->xlower - hxmarg) {                         /* :118 */
         int nxl = (int)((h->xlower + hxmarg - xlo_patch) / hx);
-        switch (h->mthbc[0]) {
+        int mb = h->mthbc[0];
+        if ((mb == 1 && nxl >= nrow) || (mb == 3 && 2 * nxl > nrow)) mb = 2;
+        switch (mb) {
         case 1:
             for (int j = 1; j <= ncol; j++)
                 for (int i = 1; i <= nxl; i++)
@@ -58,7 +66,9 @@ int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
     if (xhi_patch > h->xupper + hxmarg) {                         /* :161 */
         int nxr = (int)((xhi_patch - h->xupper + hxmarg) / hx);
         int ibeg = imax(nrow - nxr + 1, 1);
-        switch (h->mthbc[1]) {
+        int mb = h->mthbc[1];
+        if ((mb == 1 && ibeg < 2) || (mb == 3 && 2 * ibeg - 1 - nrow < 1)) mb = 2;
+        switch (mb) {
         case 1:
             for (int i = ibeg; i <= nrow; i++)
                 for (int j = 1; j <= ncol; j++)
@@ -80,7 +90,9 @@ int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
     }
     if (ylo_patch < h->ylower - hymarg) {                         /* :207 */
         int nyb = (int)((h->ylower + hymarg - ylo_patch) / hy);
-        switch (h->mthbc[2]) {
+        int mb = h->mthbc[2];
+        if ((mb == 1 && nyb >= ncol) || (mb == 3 && 2 * nyb > ncol)) mb = 2;
+        switch (mb) {
         case 1:
             for (int j = 1; j <= nyb; j++)
                 for (int i = 1; i <= nrow; i++)
@@ -103,7 +115,9 @@ int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
     if (yhi_patch > h->yupper + hymarg) {                         /* :253 */
         int nyt = (int)((yhi_patch - h->yupper + hymarg) / hy);
         int jbeg = imax(ncol - nyt + 1, 1);
-        switch (h->mthbc[3]) {
+        int mb = h->mthbc[3];
+        if ((mb == 1 && jbeg < 2) || (mb == 3 && 2 * jbeg - 1 - ncol < 1)) mb = 2;
+        switch (mb) {
         case 1:
             for (int j = jbeg; j <= ncol; j++)
                 for (int i = 1; i <= nrow; i++)
