@@ -129,6 +129,42 @@ int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
                int ncol, double hx, double hy, double xlo_patch,
                double xhi_patch, double ylo_patch, double yhi_patch);
 
+/* ---- coarse-fine flux registers (prepf.f, prepc.f, setuse.f) ------------
+ * Fine side, levels 2..nlev: grid k's block at reg[l] + roff[l][k] holds
+ * svdflx(nvar, lenbc) then qc1d(nvar, lenbc), lenbc = 2(nx/r + ny/r)
+ * (prepf.f:35-48).  Coarse side, levels 1..nlev-1: grid k's listbc entries
+ * (i, j, side, kid, lkid) are listbc[l][5e..5e+4] for e in
+ * [lst_start[l][k], lst_start[l][k] + lst_count[l][k]) (setuse.f:61-213);
+ * kid is the 0-based grid index on level l+1, i/j local 1-based with ghosts. */
+typedef struct {
+    int nlev;
+    int* lenbc[ORC_MAXLV];
+    int64_t* roff[ORC_MAXLV];
+    double* reg[ORC_MAXLV];
+    int64_t nreg[ORC_MAXLV];
+    int* lst_start[ORC_MAXLV];
+    int* lst_count[ORC_MAXLV];
+    int* listbc[ORC_MAXLV];
+    int nlist[ORC_MAXLV];
+} orc_regs;
+
+int orc_prep_regs(const orc_params* p, const orc_hier* h, orc_regs* r);
+void orc_free_regs(orc_regs* r);
+/* saveqc.f:13-109 for the grids of fine level `level` */
+int orc_saveqc(const orc_params* p, const orc_hier* h, orc_regs* r, int level);
+/* qad.f:9-360 for grid `grid` of fine level `level` (uses its new array) */
+int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid);
+/* advanc.f:10-142 with par_advanc :170-292 (no gauges): bound every grid,
+ * saveqc for the next level, then per grid new->old copy (level < mxnest),
+ * qad, stepgrid, fluxsv, fluxad, time += possk.  Returns dtlevnew and the
+ * level's max Courant number. */
+int orc_advanc(const orc_params* p, orc_hier* h, orc_regs* r, int level,
+               double* dtlevnew, double* cfl_level);
+/* update.f:16-189 with upbnd.f:18-127 */
+int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level);
+/* conck.f:6-73: hx*hy*sum of q(1) over the interiors of the level's grids */
+double orc_conck(const orc_params* p, const orc_hier* h, int level);
+
 /* amr_module.f90 declares these real(kind=8) parameters from default-real
  * (single precision) literals 10.e32 and 10.e33, so their values are the
  * float roundings widened to double. */

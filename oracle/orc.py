@@ -332,3 +332,127 @@ def ref_bound(p, H, level, grid, time):
     R.ref_bound.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
     R.ref_bound(level, grid, time, H.nvar)
     return ref_get_grid(H, level, grid)
+
+
+# ------------------------------------------------------------ registers ---
+class OrcRegs(C.Structure):
+    _fields_ = [
+        ("nlev", C.c_int),
+        ("lenbc", C.POINTER(C.c_int) * MAXLV),
+        ("roff", C.POINTER(C.c_int64) * MAXLV),
+        ("reg", C.POINTER(C.c_double) * MAXLV),
+        ("nreg", C.c_int64 * MAXLV),
+        ("lst_start", C.POINTER(C.c_int) * MAXLV),
+        ("lst_count", C.POINTER(C.c_int) * MAXLV),
+        ("listbc", C.POINTER(C.c_int) * MAXLV),
+        ("nlist", C.c_int * MAXLV),
+    ]
+
+
+class OracleAMR:
+    """The oracle's level update over a hierarchy (advanc / update / conck),
+    working in place on the arrays of a tests.common.Hier."""
+
+    def __init__(self, p, H):
+        L = lib()
+        P, Hh, Rr = C.POINTER(OrcParams), C.POINTER(OrcHier), C.POINTER(OrcRegs)
+        L.orc_prep_regs.argtypes = [P, Hh, Rr]
+        L.orc_free_regs.argtypes = [Rr]
+        L.orc_advanc.argtypes = [P, Hh, Rr, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_update.argtypes = [P, Hh, Rr, C.c_int]
+        L.orc_conck.argtypes = [P, Hh, C.c_int]
+        L.orc_conck.restype = C.c_double
+        self.p, self.H = p, H
+        self.h = hier_struct(H)
+        self.r = OrcRegs()
+        rc = L.orc_prep_regs(C.byref(p), C.byref(self.h), C.byref(self.r))
+        if rc:
+            raise RuntimeError("orc_prep_regs error %d" % rc)
+
+    def __del__(self):
+        try:
+            lib().orc_free_regs(C.byref(self.r))
+        except Exception:
+            pass
+
+    def advanc(self, level):
+        dtn, cfl = C.c_double(), C.c_double()
+        rc = lib().orc_advanc(C.byref(self.p), C.byref(self.h), C.byref(self.r), level,
+                              C.byref(dtn), C.byref(cfl))
+        if rc:
+            raise RuntimeError("orc_advanc error %d" % rc)
+        self.H.levels[level - 1]["time"] = self.h.time[level]
+        return dtn.value, cfl.value
+
+    def update(self, level):
+        rc = lib().orc_update(C.byref(self.p), C.byref(self.h), C.byref(self.r), level)
+        if rc:
+            raise RuntimeError("orc_update error %d" % rc)
+
+    def conck(self, level):
+        return lib().orc_conck(C.byref(self.p), C.byref(self.h), level)
+
+    def reg(self, level, k):
+        n = 2 * self.p.meqn * self.r.lenbc[level][k]
+        off = self.r.roff[level][k]
+        return np.ctypeslib.as_array(self.r.reg[level], shape=(self.r.nreg[level],))[off:off + n].copy()
+
+    def listbc(self, level, k):
+        if not self.r.listbc[level]:
+            return np.zeros((0, 5), dtype=np.int32)
+        a = np.ctypeslib.as_array(self.r.listbc[level], shape=(5 * self.r.nlist[level],))
+        s, c = self.r.lst_start[level][k], self.r.lst_count[level][k]
+        return a[5 * s:5 * (s + c)].reshape(-1, 5).copy()
+
+
+class RefAMR:
+    """The reference's own routines over a copy of a hierarchy installed in
+    amr_module (prepf/prepc, bound, saveqc, qad, stepgrid, fluxsv, fluxad,
+    update)."""
+
+    def __init__(self, p, H):
+        self.p, self.H = p, H
+        R = ref_load_hier(p, H)
+        R.ref_prep_regs.argtypes = [C.c_int, C.c_int]
+        R.ref_advanc.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        R.ref_update.argtypes = [C.c_int, C.c_int]
+        R.ref_get_reg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
+        R.ref_get_listbc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
+        R.ref_get_time.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double)]
+        R.ref_prep_regs(H.nlev, H.nvar)
+        self.R = R
+
+    def advanc(self, level):
+        dtn, cfl = C.c_double(), C.c_double()
+        self.R.ref_advanc(level, self.H.nvar, C.byref(dtn), C.byref(cfl))
+        return dtn.value, cfl.value
+
+    def update(self, level):
+        self.R.ref_update(level, self.H.nvar)
+
+    def grid(self, level, k, which=0):
+        return ref_get_grid(self.H, level, k, which)
+
+    def reg(self, level, k):
+        b = self.H.levels[level - 1]["boxes"][k]
+        r = self.H.levels[level - 2]["ratx"]
+        lenbc = 2 * ((b[1] - b[0] + 1) // r + (b[3] - b[2] + 1) // r)
+        out = np.zeros(2 * self.H.nvar * lenbc)
+        self.R.ref_get_reg(level, k, self.H.nvar, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def listbc(self, level, k):
+        n = C.c_int()
+        buf = np.zeros(5 * 100000, dtype=np.int32)
+        self.R.ref_get_listbc(level, k, 100000, buf.ctypes.data_as(C.c_void_p), C.byref(n))
+        out = buf[:5 * n.value].reshape(-1, 5).copy()
+        # kid is a node number; the harness numbers the grids of level l+1
+        # consecutively after those of levels 1..l
+        first = 1 + sum(len(L["boxes"]) for L in self.H.levels[:level])
+        out[:, 3] -= first
+        return out
+
+    def time(self, level, k=0):
+        t = C.c_double()
+        self.R.ref_get_time(level, k, C.byref(t))
+        return t.value

@@ -445,3 +445,502 @@ int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
     return orc_bc2amr(p, h, valbig, mitot, mjtot, hx, hy, xloWithGhost, xhiWithGhost,
                       yloWithGhost, yhiWithGhost);
 }
+
+/* =================================================== flux registers ===== */
+
+static int orc_list_push(int** a, int* n, int* cap, const int e[5]) {
+    if (*n + 1 > *cap) {
+        int nc = *cap ? 2 * *cap : 64;
+        int* na = realloc(*a, (size_t)nc * 5 * sizeof(int));
+        if (!na) return ORC_E_ALLOC;
+        *a = na;
+        *cap = nc;
+    }
+    memcpy(*a + 5 * (size_t)(*n), e, 5 * sizeof(int));
+    (*n)++;
+    return 0;
+}
+
+/* setuse.f:61-213, regular (kflag = 1) case */
+static int setuse(int** lst, int* n, int* cap, int mkid, int ng, int ilo, int ihi,
+                  int jlo, int jhi, int iclo, int ichi, int jclo, int jchi) {
+    int ist = iclo - 1, iend = ichi + 1, jst = jclo - 1, jend = jchi + 1;
+    int e[5], rc;
+    if (!(ist < ilo)) {                                   /* :106-116 */
+        int lkid = imax(jlo, jclo) - jclo + 1;
+        for (int j = imax(jlo, jclo); j <= imin(jhi, jchi); j++) {
+            e[0] = ist - ilo + ng + 1; e[1] = j - jlo + ng + 1; e[2] = 3; e[3] = mkid; e[4] = lkid++;
+            if ((rc = orc_list_push(lst, n, cap, e))) return rc;
+        }
+    }
+    if (!(jend > jhi)) {                                  /* :120-133 */
+        int lkid = (jchi - jclo + 1) + imax(ilo, iclo) - iclo + 1;
+        for (int i = imax(ilo, iclo); i <= imin(ihi, ichi); i++) {
+            e[0] = i - ilo + ng + 1; e[1] = jend - jlo + ng + 1; e[2] = 4; e[3] = mkid; e[4] = lkid++;
+            if ((rc = orc_list_push(lst, n, cap, e))) return rc;
+        }
+    }
+    if (!(iend > ihi)) {                                  /* :159-170 */
+        int lkid = (ichi - iclo + 1) + (jchi - jclo + 1) + imax(jlo, jclo) - jclo + 1;
+        for (int j = imax(jlo, jclo); j <= imin(jhi, jchi); j++) {
+            e[0] = iend - ilo + ng + 1; e[1] = j - jlo + ng + 1; e[2] = 1; e[3] = mkid; e[4] = lkid++;
+            if ((rc = orc_list_push(lst, n, cap, e))) return rc;
+        }
+    }
+    if (!(jst < jlo)) {                                   /* :175-186 */
+        int lkid = 2 * (jchi - jclo + 1) + (ichi - iclo + 1) + imax(ilo, iclo) - iclo + 1;
+        for (int i = imax(ilo, iclo); i <= imin(ihi, ichi); i++) {
+            e[0] = i - ilo + ng + 1; e[1] = jst - jlo + ng + 1; e[2] = 2; e[3] = mkid; e[4] = lkid++;
+            if ((rc = orc_list_push(lst, n, cap, e))) return rc;
+        }
+    }
+    return 0;
+}
+
+void orc_free_regs(orc_regs* r) {
+    for (int l = 0; l < ORC_MAXLV; l++) {
+        free(r->lenbc[l]); free(r->roff[l]); free(r->reg[l]);
+        free(r->lst_start[l]); free(r->lst_count[l]); free(r->listbc[l]);
+    }
+    memset(r, 0, sizeof *r);
+}
+
+/* prepf.f:18-53 (fine levels) and prepc.f:11-157 (coarse levels) */
+int orc_prep_regs(const orc_params* p, const orc_hier* h, orc_regs* r) {
+    const int nv = p->meqn, ng = p->mbc;
+    memset(r, 0, sizeof *r);
+    r->nlev = h->nlev;
+    for (int l = 2; l <= h->nlev; l++) {
+        int n = h->ngrids[l];
+        r->lenbc[l] = calloc((size_t)n + 1, sizeof(int));
+        r->roff[l] = calloc((size_t)n + 1, sizeof(int64_t));
+        int64_t tot = 0;
+        for (int k = 0; k < n; k++) {
+            const int* b = h->boxes[l] + 4 * k;
+            int nx = b[1] - b[0] + 1, ny = b[3] - b[2] + 1;
+            r->lenbc[l][k] = 2 * (nx / h->intratx[l - 1] + ny / h->intraty[l - 1]);
+            r->roff[l][k] = tot;
+            tot += 2 * (int64_t)nv * r->lenbc[l][k];
+        }
+        r->nreg[l] = tot;
+        r->reg[l] = calloc((size_t)tot + 1, sizeof(double));
+        if (!r->reg[l]) return ORC_E_ALLOC;
+    }
+    for (int l = 1; l < h->nlev; l++) {
+        if (h->ngrids[l + 1] == 0) continue;
+        int nc = h->ngrids[l];
+        int cap = 0, n = 0;
+        int* lst = NULL;
+        r->lst_start[l] = calloc((size_t)nc + 1, sizeof(int));
+        r->lst_count[l] = calloc((size_t)nc + 1, sizeof(int));
+        int imax_ = h->iregsz[l] - 1, jmax_ = h->jregsz[l] - 1;
+        int rx = h->intratx[l], ry = h->intraty[l];
+        for (int m = 0; m < nc; m++) {
+            const int* cb = h->boxes[l] + 4 * m;
+            int ilo = cb[0], ihi = cb[1], jlo = cb[2], jhi = cb[3];
+            r->lst_start[l][m] = n;
+            for (int kid = 0; kid < h->ngrids[l + 1]; kid++) {
+                const int* fb = h->boxes[l + 1] + 4 * kid;
+                int iclo = fb[0] / rx, jclo = fb[2] / ry, ichi = fb[1] / rx, jchi = fb[3] / ry;
+                int iplo = imax(ilo, iclo), jplo = imax(jlo, jclo);
+                int iphi = imin(ihi, ichi), jphi = imin(jhi, jchi);
+                int rc = 0;
+                if (iplo <= iphi + 1 && jplo <= jphi + 1)
+                    rc = setuse(&lst, &n, &cap, kid, ng, ilo, ihi, jlo, jhi, iclo, ichi, jclo, jchi);
+                if (!rc && h->xperdom && ilo == 0 && ichi == imax_)
+                    rc = setuse(&lst, &n, &cap, kid, ng, ilo, ihi, jlo, jhi, iclo - h->iregsz[l],
+                                ichi - h->iregsz[l], jclo, jchi);
+                if (!rc && h->xperdom && iclo == 0 && ihi == imax_)
+                    rc = setuse(&lst, &n, &cap, kid, ng, ilo, ihi, jlo, jhi, iclo + h->iregsz[l],
+                                ichi + h->iregsz[l], jclo, jchi);
+                if (!rc && h->yperdom && jlo == 0 && jchi == jmax_)
+                    rc = setuse(&lst, &n, &cap, kid, ng, ilo, ihi, jlo, jhi, iclo, ichi,
+                                jclo - h->jregsz[l], jchi - h->jregsz[l]);
+                if (!rc && h->yperdom && jclo == 0 && jhi == jmax_)
+                    rc = setuse(&lst, &n, &cap, kid, ng, ilo, ihi, jlo, jhi, iclo, ichi,
+                                jclo + h->jregsz[l], jchi + h->jregsz[l]);
+                if (rc) { free(lst); return rc; }
+            }
+            r->lst_count[l][m] = n - r->lst_start[l][m];
+        }
+        r->listbc[l] = lst;
+        r->nlist[l] = n;
+    }
+    return 0;
+}
+
+/* icall.f (copy from the level's grids, new time) into val at (iputst,jputst) */
+static void icall(const orc_params* p, const orc_hier* h, double* val, int nrow, int ilo,
+                  int ihi, int jlo, int jhi, int level, int iputst, int jputst) {
+    const int nv = p->meqn, ng = p->mbc;
+    for (int k = 0; k < h->ngrids[level]; k++) {
+        const int* b = h->boxes[level] + 4 * k;
+        int ixlo = imax(b[0], ilo), ixhi = imin(b[1], ihi);
+        int jxlo = imax(b[2], jlo), jxhi = imin(b[3], jhi);
+        if (!(ixlo <= ixhi && jxlo <= jxhi)) continue;
+        int mitot = b[1] - b[0] + 1 + 2 * ng;
+        const double* q = h->qnew[level] + h->qoff[level][k];
+        for (int j = jxlo; j <= jxhi; j++)
+            for (int i = ixlo; i <= ixhi; i++)
+                for (int m = 1; m <= nv; m++)
+                    VAL(val, nv, nrow, m, i - ilo + iputst, j - jlo + jputst) =
+                        VAL(q, nv, mitot, m, i - b[0] + ng + 1, j - b[2] + ng + 1);
+    }
+}
+
+/* preicall.f (periodic wrap of icall) */
+static void preicall(const orc_params* p, const orc_hier* h, double* val, int nrow, int ilo,
+                     int ihi, int jlo, int jhi, int level) {
+    const int ng = p->mbc;
+    int ist[3], iend[3], ishift[3], jst[3], jend[3], jshift[3];
+    if (h->xperdom) {
+        ist[0] = ilo; ist[1] = 0; ist[2] = h->iregsz[level];
+        iend[0] = -1; iend[1] = h->iregsz[level] - 1; iend[2] = ihi;
+        ishift[0] = h->iregsz[level]; ishift[1] = 0; ishift[2] = -h->iregsz[level];
+    } else {
+        ist[0] = h->iregsz[level]; ist[1] = ilo; ist[2] = h->iregsz[level];
+        iend[0] = -ng; iend[1] = ihi; iend[2] = -ng;
+        ishift[0] = ishift[1] = ishift[2] = 0;
+    }
+    if (h->yperdom) {
+        jst[0] = jlo; jst[1] = 0; jst[2] = h->jregsz[level];
+        jend[0] = -1; jend[1] = h->jregsz[level] - 1; jend[2] = jhi;
+        jshift[0] = h->jregsz[level]; jshift[1] = 0; jshift[2] = -h->jregsz[level];
+    } else {
+        jst[0] = h->jregsz[level]; jst[1] = jlo; jst[2] = h->jregsz[level];
+        jend[0] = -ng; jend[1] = jhi; jend[2] = -ng;
+        jshift[0] = jshift[1] = jshift[2] = 0;
+    }
+    for (int a = 0; a < 3; a++) {
+        int i1 = imax(ilo, ist[a]), i2 = imin(ihi, iend[a]);
+        if (i1 > i2) continue;
+        for (int b = 0; b < 3; b++) {
+            int j1 = imax(jlo, jst[b]), j2 = imin(jhi, jend[b]);
+            if (j1 > j2) continue;
+            icall(p, h, val, nrow, i1 + ishift[a], i2 + ishift[a], j1 + jshift[b], j2 + jshift[b],
+                  level, i1 - ilo + 1, j1 - jlo + 1);
+        }
+    }
+}
+
+/* saveqc.f:13-109 and cstore.f:4-63 */
+int orc_saveqc(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
+    const int nv = p->meqn;
+    const int levc = level - 1;
+    const double hxc = h->hx[levc], hyc = h->hy[levc];
+    const int rx = h->intratx[levc], ry = h->intraty[levc];
+    for (int k = 0; k < h->ngrids[level]; k++) {
+        const int* b = h->boxes[level] + 4 * k;
+        int ilo = b[0], ihi = b[1], jlo = b[2], jhi = b[3];
+        int nx = ihi - ilo + 1, ny = jhi - jlo + 1;
+        int lenbc = 2 * (nx / rx + ny / ry);
+        int iclo = ilo / rx - 1, jclo = jlo / ry - 1, ichi = (ihi + 1) / rx, jchi = (jhi + 1) / ry;
+        int nrow = ichi - iclo + 1, ncol = jchi - jclo + 1;
+        double xl = (ilo * h->hx[level] + h->xlower) - hxc;
+        double yb = (jlo * h->hy[level] + h->ylower) - hyc;
+        double xr = ((ihi + 1) * h->hx[level] + h->xlower) + hxc;
+        double yt = ((jhi + 1) * h->hy[level] + h->ylower) + hyc;
+        double* val = malloc((size_t)nrow * ncol * nv * sizeof(double));
+        if (!val) return ORC_E_ALLOC;
+        for (size_t t = 0; t < (size_t)nrow * ncol * nv; t++) val[t] = NAN;   /* igetsp: unset */
+        int sticks = iclo < 0 || ichi == h->iregsz[levc] || jclo < 0 || jchi == h->jregsz[levc];
+        if (sticks && (h->xperdom || h->yperdom || h->spheredom))
+            preicall(p, h, val, nrow, iclo, ichi, jclo, jchi, levc);
+        else
+            icall(p, h, val, nrow, iclo, ichi, jclo, jchi, levc, 1, 1);
+        int rc = orc_bc2amr(p, h, val, nrow, ncol, hxc, hyc, xl, xr, yb, yt);
+        if (rc) { free(val); return rc; }
+        double* qc1d = r->reg[level] + r->roff[level][k] + (size_t)nv * lenbc;
+        int index = 0;
+#define QC(m, ix) qc1d[((m) - 1) + (size_t)nv * ((ix) - 1)]
+        for (int j = 2; j <= ncol - 1; j++) { index++; for (int m = 1; m <= nv; m++) QC(m, index) = VAL(val, nv, nrow, m, 1, j); }
+        for (int i = 2; i <= nrow - 1; i++) { index++; for (int m = 1; m <= nv; m++) QC(m, index) = VAL(val, nv, nrow, m, i, ncol); }
+        for (int j = 2; j <= ncol - 1; j++) { index++; for (int m = 1; m <= nv; m++) QC(m, index) = VAL(val, nv, nrow, m, nrow, j); }
+        for (int i = 2; i <= nrow - 1; i++) { index++; for (int m = 1; m <= nv; m++) QC(m, index) = VAL(val, nv, nrow, m, i, 1); }
+#undef QC
+        free(val);
+    }
+    return 0;
+}
+
+/* qad.f:9-360 (no aux, no source terms) */
+int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid) {
+    const int nv = p->meqn, ng = p->mbc, mw = p->mwaves;
+    const int* b = h->boxes[level] + 4 * grid;
+    const int nr = b[1] - b[0] + 1, nc = b[3] - b[2] + 1;
+    const int mitot = nr + 2 * ng, mjtot = nc + 2 * ng;
+    const int lrx = h->intratx[level - 1], lry = h->intraty[level - 1];
+    const double hx = h->hx[level], hy = h->hy[level], delt = h->possk[level];
+    const int lenbc = r->lenbc[level][grid];
+    double* svdflx = r->reg[level] + r->roff[level][grid];
+    const double* qc1d = svdflx + (size_t)nv * lenbc;
+    const double* valbig = h->qnew[level] + h->qoff[level][grid];
+    const int npos = (nr > nc ? nr : nc) + 2;          /* positions 1..npos */
+    double* ql = calloc((size_t)nv * npos, sizeof(double));
+    double* qr = calloc((size_t)nv * npos, sizeof(double));
+    double* wave = calloc((size_t)nv * mw * npos, sizeof(double));
+    double* s = calloc((size_t)mw * npos, sizeof(double));
+    double* amdq = calloc((size_t)nv * npos, sizeof(double));
+    double* apdq = calloc((size_t)nv * npos, sizeof(double));
+    double aux1[1] = {0};
+    if (!ql || !qr || !wave || !s || !amdq || !apdq) return ORC_E_ALLOC;
+#define P1(a, m, pos) (a)[((m) - 1) + (size_t)nv * ((pos) - 1)]
+#define QB(m, i, j) VAL(valbig, nv, mitot, m, i, j)
+#define QC(m, ix) qc1d[((m) - 1) + (size_t)nv * ((ix) - 1)]
+#define SV(m, ix) svdflx[((m) - 1) + (size_t)nv * ((ix) - 1)]
+    /* rpn2 is called with mbc = nghost and mx = n+1-2*nghost, arrays indexed
+     * from position 1 (= Fortran index 1-mbc): interfaces at positions 2..n+1 */
+    int index = 0, influx = 0;
+    /* side 1 */
+    for (int j = ng + 1; j <= mjtot - ng; j++)
+        for (int m = 1; m <= nv; m++) P1(ql, m, j - ng + 1) = QB(m, ng, j);
+    {
+        int lind = 0;
+        for (int jc = 1; jc <= nc / lry; jc++) {
+            index++;
+            for (int l = 1; l <= lry; l++) { lind++; for (int m = 1; m <= nv; m++) P1(qr, m, lind) = QC(m, index); }
+        }
+    }
+    orc_rpn2(p, 1, npos - 2 * ng, nv, mw, 0, ng, nc + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+    for (int j = 1; j <= nc / lry; j++) {
+        influx++;
+        int jfine = (j - 1) * lry;
+        for (int m = 1; m <= nv; m++)
+            for (int l = 1; l <= lry; l++)
+                SV(m, influx) = SV(m, influx) + P1(amdq, m, jfine + l + 1) * hy * delt
+                                + P1(apdq, m, jfine + l + 1) * hy * delt;
+    }
+    if (mjtot != 2 * ng + 1) {
+        /* side 2 */
+        for (int i = ng + 1; i <= mitot - ng; i++)
+            for (int m = 1; m <= nv; m++) P1(qr, m, i - ng) = QB(m, i, mjtot - ng + 1);
+        int lind = 0;
+        for (int ic = 1; ic <= nr / lrx; ic++) {
+            index++;
+            for (int l = 1; l <= lrx; l++) { lind++; for (int m = 1; m <= nv; m++) P1(ql, m, lind + 1) = QC(m, index); }
+        }
+        orc_rpn2(p, 2, npos - 2 * ng, nv, mw, 0, ng, nr + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+        for (int i = 1; i <= nr / lrx; i++) {
+            influx++;
+            int ifine = (i - 1) * lrx;
+            for (int m = 1; m <= nv; m++)
+                for (int l = 1; l <= lrx; l++)
+                    SV(m, influx) = SV(m, influx) - P1(amdq, m, ifine + l + 1) * hx * delt
+                                    - P1(apdq, m, ifine + l + 1) * hx * delt;
+        }
+    }
+    /* side 3 */
+    for (int j = ng + 1; j <= mjtot - ng; j++)
+        for (int m = 1; m <= nv; m++) P1(qr, m, j - ng) = QB(m, mitot - ng + 1, j);
+    {
+        int lind = 0;
+        for (int jc = 1; jc <= nc / lry; jc++) {
+            index++;
+            for (int l = 1; l <= lry; l++) { lind++; for (int m = 1; m <= nv; m++) P1(ql, m, lind + 1) = QC(m, index); }
+        }
+    }
+    orc_rpn2(p, 1, npos - 2 * ng, nv, mw, 0, ng, nc + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+    for (int j = 1; j <= nc / lry; j++) {
+        influx++;
+        int jfine = (j - 1) * lry;
+        for (int m = 1; m <= nv; m++)
+            for (int l = 1; l <= lry; l++)
+                SV(m, influx) = SV(m, influx) - P1(amdq, m, jfine + l + 1) * hy * delt
+                                - P1(apdq, m, jfine + l + 1) * hy * delt;
+    }
+    if (mjtot != 2 * ng + 1) {
+        /* side 4 */
+        for (int i = ng + 1; i <= mitot - ng; i++)
+            for (int m = 1; m <= nv; m++) P1(ql, m, i - ng + 1) = QB(m, i, ng);
+        int lind = 0;
+        for (int ic = 1; ic <= nr / lrx; ic++) {
+            index++;
+            for (int l = 1; l <= lrx; l++) { lind++; for (int m = 1; m <= nv; m++) P1(qr, m, lind) = QC(m, index); }
+        }
+        orc_rpn2(p, 2, npos - 2 * ng, nv, mw, 0, ng, nr + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+        for (int i = 1; i <= nr / lrx; i++) {
+            influx++;
+            int ifine = (i - 1) * lrx;
+            for (int m = 1; m <= nv; m++)
+                for (int l = 1; l <= lrx; l++)
+                    SV(m, influx) = SV(m, influx) + P1(amdq, m, ifine + l + 1) * hx * delt
+                                    + P1(apdq, m, ifine + l + 1) * hx * delt;
+        }
+    }
+#undef P1
+#undef QB
+#undef QC
+#undef SV
+    free(ql); free(qr); free(wave); free(s); free(amdq); free(apdq);
+    return 0;
+}
+
+/* fluxsv.f:10-109 for coarse grid `grid` of `level` */
+static void fluxsv(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid,
+                   const double* fm, const double* fp, const double* gm, const double* gp) {
+    const int nv = p->meqn, ng = p->mbc;
+    if (!r->listbc[level]) return;
+    const int* b = h->boxes[level] + 4 * grid;
+    const int mitot = b[1] - b[0] + 1 + 2 * ng;
+    const double dtc = h->possk[level], hx = h->hx[level], hy = h->hy[level];
+    for (int e = r->lst_start[level][grid]; e < r->lst_start[level][grid] + r->lst_count[level][grid]; e++) {
+        const int* L = r->listbc[level] + 5 * (size_t)e;
+        int i = L[0], j = L[1], side = L[2], kid = L[3], lkid = L[4];
+        double* slot = r->reg[level + 1] + r->roff[level + 1][kid] + (size_t)nv * (lkid - 1);
+        for (int m = 1; m <= nv; m++) {
+            double v = 0.0;
+            if (side == 1) v = -VAL(fp, nv, mitot, m, i, j) * dtc * hy;
+            if (side == 2) v = -VAL(gm, nv, mitot, m, i, j + 1) * dtc * hx;
+            if (side == 3) v = -VAL(fm, nv, mitot, m, i + 1, j) * dtc * hy;
+            if (side == 4) v = -VAL(gp, nv, mitot, m, i, j) * dtc * hx;
+            slot[m - 1] = v;
+        }
+    }
+}
+
+/* fluxad.f:7-96 for fine grid `grid` of `level` */
+static void fluxad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid,
+                   const double* fm, const double* fp, const double* gm, const double* gp) {
+    const int nv = p->meqn, ng = p->mbc;
+    const int* b = h->boxes[level] + 4 * grid;
+    const int nx = b[1] - b[0] + 1, ny = b[3] - b[2] + 1;
+    const int mitot = nx + 2 * ng, mjtot = ny + 2 * ng;
+    const int lrx = h->intratx[level - 1], lry = h->intraty[level - 1];
+    const int nyc = ny / lry, nxc = nx / lrx;
+    const double dtf = h->possk[level], dx = h->hx[level], dy = h->hy[level];
+    double* svdflx = r->reg[level] + r->roff[level][grid];
+#define SV(m, ix) svdflx[((m) - 1) + (size_t)nv * ((ix) - 1)]
+    int lind = 0;
+    for (int j = 1; j <= nyc; j++) {
+        lind++;
+        int jfine = (j - 1) * lry + ng;
+        for (int m = 1; m <= nv; m++)
+            for (int l = 1; l <= lry; l++)
+                SV(m, lind) = SV(m, lind) + VAL(fm, nv, mitot, m, ng + 1, jfine + l) * dtf * dy;
+    }
+    for (int i = 1; i <= nxc; i++) {
+        lind++;
+        int ifine = (i - 1) * lrx + ng;
+        for (int m = 1; m <= nv; m++)
+            for (int l = 1; l <= lrx; l++)
+                SV(m, lind) = SV(m, lind) + VAL(gp, nv, mitot, m, ifine + l, mjtot - ng + 1) * dtf * dx;
+    }
+    for (int j = 1; j <= nyc; j++) {
+        lind++;
+        int jfine = (j - 1) * lry + ng;
+        for (int m = 1; m <= nv; m++)
+            for (int l = 1; l <= lry; l++)
+                SV(m, lind) = SV(m, lind) + VAL(fp, nv, mitot, m, mitot - ng + 1, jfine + l) * dtf * dy;
+    }
+    for (int i = 1; i <= nxc; i++) {
+        lind++;
+        int ifine = (i - 1) * lrx + ng;
+        for (int m = 1; m <= nv; m++)
+            for (int l = 1; l <= lrx; l++)
+                SV(m, lind) = SV(m, lind) + VAL(gm, nv, mitot, m, ifine + l, ng + 1) * dtf * dx;
+    }
+#undef SV
+}
+
+int orc_advanc(const orc_params* p, orc_hier* h, orc_regs* r, int level, double* dtlevnew,
+               double* cfl_level) {
+    const int nv = p->meqn, ng = p->mbc;
+    int rc;
+    for (int k = 0; k < h->ngrids[level]; k++) {                    /* advanc.f:60-84 */
+        rc = orc_bound(p, h, level, k, h->time[level], h->qnew[level] + h->qoff[level][k]);
+        if (rc) return rc;
+    }
+    int has_fine = level + 1 <= h->mxnest && level + 1 <= h->nlev && h->ngrids[level + 1] > 0;
+    if (has_fine && (rc = orc_saveqc(p, h, r, level + 1))) return rc; /* :92-96 */
+    double dmin = ORC_RINFINITY, cmax = 0.0;
+    for (int k = 0; k < h->ngrids[level]; k++) {                    /* :106-129 */
+        const int* b = h->boxes[level] + 4 * k;
+        int mx = b[1] - b[0] + 1, my = b[3] - b[2] + 1;
+        size_t ntot = (size_t)nv * (mx + 2 * ng) * (my + 2 * ng);
+        double* q = h->qnew[level] + h->qoff[level][k];
+        if (level < h->mxnest) memcpy(h->qold[level] + h->qoff[level][k], q, ntot * sizeof(double));
+        if (level > 1 && (rc = orc_qad(p, h, r, level, k))) return rc;
+        double* f = malloc(4 * ntot * sizeof(double));
+        if (!f) return ORC_E_ALLOC;
+        double cflgrid, dtnew;
+        orc_stepgrid(p, mx, my, q, NULL, h->hx[level], h->hy[level], h->possk[level], f, f + ntot,
+                     f + 2 * ntot, f + 3 * ntot, &cflgrid, &dtnew);
+        if (has_fine) fluxsv(p, h, r, level, k, f, f + ntot, f + 2 * ntot, f + 3 * ntot);
+        if (level > 1) fluxad(p, h, r, level, k, f, f + ntot, f + 2 * ntot, f + 3 * ntot);
+        free(f);
+        if (dtnew < dmin) dmin = dtnew;
+        if (cflgrid > cmax) cmax = cflgrid;
+    }
+    h->time[level] = h->time[level] + h->possk[level];              /* :289 */
+    if (dtlevnew) *dtlevnew = dmin;
+    if (cfl_level) *cfl_level = cmax;
+    return 0;
+}
+
+/* update.f:16-189 with upbnd.f:18-127 (no capacity) */
+int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
+    const int nv = p->meqn, ng = p->mbc;
+    const int rx = h->intratx[level], ry = h->intraty[level];
+    const double area = h->hx[level] * h->hy[level];
+    for (int m0 = 0; m0 < h->ngrids[level]; m0++) {
+        const int* cb = h->boxes[level] + 4 * m0;
+        int ilo = cb[0], ihi = cb[1], jlo = cb[2], jhi = cb[3];
+        int mitot = ihi - ilo + 1 + 2 * ng, mjtot = jhi - jlo + 1 + 2 * ng;
+        double* val = h->qnew[level] + h->qoff[level][m0];
+        if (r->listbc[level]) {                                    /* upbnd */
+            int* iused = calloc((size_t)mitot * mjtot, sizeof(int));
+            if (!iused) return ORC_E_ALLOC;
+            for (int e = r->lst_start[level][m0]; e < r->lst_start[level][m0] + r->lst_count[level][m0]; e++) {
+                const int* L = r->listbc[level] + 5 * (size_t)e;
+                int icrse = L[0], jcrse = L[1], iside = L[2], kid = L[3], lkid = L[4];
+                int norm = 1 << (iside - 1);
+                int* u = &iused[(icrse - 1) + (size_t)mitot * (jcrse - 1)];
+                if ((*u / norm) % 2 == 1) continue;                 /* :79-83 */
+                double sgnm = (iside == 2 || iside == 3 || iside == 6) ? -1.0 : 1.0;
+                const double* svd = r->reg[level + 1] + r->roff[level + 1][kid] + (size_t)nv * (lkid - 1);
+                for (int m = 1; m <= nv; m++)
+                    VAL(val, nv, mitot, m, icrse, jcrse) = VAL(val, nv, mitot, m, icrse, jcrse) + sgnm * svd[m - 1] / area;
+                *u += norm;
+            }
+            free(iused);
+        }
+        if (level + 1 > h->nlev) continue;
+        for (int kid = 0; kid < h->ngrids[level + 1]; kid++) {     /* :95-177 */
+            const int* fb = h->boxes[level + 1] + 4 * kid;
+            int iclo = fb[0] / rx, jclo = fb[2] / ry, ichi = fb[1] / rx, jchi = fb[3] / ry;
+            int mi = fb[1] - fb[0] + 1 + 2 * ng;
+            const double* qf = h->qnew[level + 1] + h->qoff[level + 1][kid];
+            int iplo = imax(ilo, iclo), jplo = imax(jlo, jclo), iphi = imin(ihi, ichi), jphi = imin(jhi, jchi);
+            if (iplo > iphi || jplo > jphi) continue;
+            double totrat = (double)(rx * ry);
+            for (int i = iplo - ilo + ng + 1; i <= iphi - ilo + ng + 1; i++) {
+                int iff = iplo * rx - fb[0] + ng + 1 + (i - (iplo - ilo + ng + 1)) * rx;
+                for (int j = jplo - jlo + ng + 1; j <= jphi - jlo + ng + 1; j++) {
+                    int jff = jplo * ry - fb[2] + ng + 1 + (j - (jplo - jlo + ng + 1)) * ry;
+                    for (int m = 1; m <= nv; m++) VAL(val, nv, mitot, m, i, j) = 0.0;
+                    for (int jco = 1; jco <= ry; jco++)
+                        for (int ico = 1; ico <= rx; ico++)
+                            for (int m = 1; m <= nv; m++)
+                                VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j)
+                                    + VAL(qf, nv, mi, m, iff + ico - 1, jff + jco - 1);
+                    for (int m = 1; m <= nv; m++)
+                        VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j) / totrat;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+double orc_conck(const orc_params* p, const orc_hier* h, int level) {
+    const int nv = p->meqn, ng = p->mbc;
+    double totmass = 0.0;
+    for (int k = 0; k < h->ngrids[level]; k++) {
+        const int* b = h->boxes[level] + 4 * k;
+        int mitot = b[1] - b[0] + 1 + 2 * ng, mjtot = b[3] - b[2] + 1 + 2 * ng;
+        const double* q = h->qnew[level] + h->qoff[level][k];
+        for (int j = ng + 1; j <= mjtot - ng; j++)
+            for (int i = ng + 1; i <= mitot - ng; i++) totmass = totmass + VAL(q, nv, mitot, 1, i, j);
+    }
+    return totmass * h->hx[level] * h->hy[level];
+}

@@ -161,7 +161,7 @@ contains
           nb = nb + 1
        end do
     end do
-    memsize = memsize + 16
+    memsize = memsize + 4000000    ! room for registers and saveqc scratch
     if (allocated(alloc)) deallocate(alloc)
     allocate(alloc(memsize))
     alloc = 0.d0
@@ -221,6 +221,11 @@ contains
        end do
     end do
     listStart(nlev+1) = mptr + 1
+    ! free list of the heap (igetsp.f) after the solution blocks
+    lenf = 1
+    lfree(1,1) = loc
+    lfree(1,2) = memsize - loc + 1
+    lendim = loc
     if (allocated(bndList)) deallocate(bndList)
     call initBndryList()
     do l = 1, nlev
@@ -256,4 +261,143 @@ contains
          alloc(node(storeaux,mptr)), 0)
   end subroutine ref_bound
 
+  ! prepf (fine levels) and prepc (coarse levels), as regrid.f:81-82 does
+  subroutine ref_prep_regs(nlev, nvar) bind(C, name="ref_prep_regs")
+    integer(c_int), value :: nlev, nvar
+    integer :: l
+    do l = 2, nlev
+       call prepf(l, nvar, 0)
+    end do
+    do l = 1, nlev - 1
+       call prepc(l, nvar)
+    end do
+  end subroutine ref_prep_regs
+
+  ! advanc.f:10-142 (the loop is restated here because advanc.f needs
+  ! gauges_module, which needs the absent clawpack utility_module); every
+  ! routine it calls is the reference's own
+  subroutine ref_advanc(level, nvar, dtlevnew, cflout) bind(C, name="ref_advanc")
+    integer(c_int), value :: level, nvar
+    real(c_double) :: dtlevnew, cflout
+    integer :: j, mptr, mitot, mjtot
+    real(kind=8) :: dtnew
+    do j = 1, numgrids(level)
+       mptr = listOfGrids(listStart(level)+j-1)
+       mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
+       mjtot = node(ndjhi,mptr) - node(ndjlo,mptr) + 1 + 2*nghost
+       call bound(rnode(timemult,mptr), nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, &
+            mptr, alloc(node(storeaux,mptr)), 0)
+    end do
+    if (level+1 .le. mxnest) then
+       if (lstart(level+1) .ne. null) call saveqc(level+1, nvar, 0)
+    endif
+    dtlevnew = rinfinity
+    cfl_level = 0.d0
+    do j = 1, numgrids(level)
+       mptr = listOfGrids(listStart(level)+j-1)
+       mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
+       mjtot = node(ndjhi,mptr) - node(ndjlo,mptr) + 1 + 2*nghost
+       call harness_par_advanc(mptr, mitot, mjtot, nvar, dtnew)
+       dtlevnew = dmin1(dtlevnew, dtnew)
+    end do
+    cflout = cfl_level
+  end subroutine ref_advanc
+
+  ! advanc.f:170-292 without gauges, timers and dimensional splitting
+  subroutine harness_par_advanc(mptr, mitot, mjtot, nvar, dtnew)
+    integer :: mptr, mitot, mjtot, nvar
+    real(kind=8) :: dtnew
+    real(kind=8) :: fp(nvar,mitot,mjtot), fm(nvar,mitot,mjtot)
+    real(kind=8) :: gp(nvar,mitot,mjtot), gm(nvar,mitot,mjtot)
+    integer :: level, nx, ny, locold, locnew, locaux, ntot, i, lenbc, locsvf, locsvq, locx1d
+    real(kind=8) :: hx, hy, delt, time, xlow, ylow
+    level = node(nestlevel,mptr)
+    hx = hxposs(level)
+    hy = hyposs(level)
+    delt = possk(level)
+    nx = node(ndihi,mptr) - node(ndilo,mptr) + 1
+    ny = node(ndjhi,mptr) - node(ndjlo,mptr) + 1
+    time = rnode(timemult,mptr)
+    locold = node(store2, mptr)
+    locnew = node(store1, mptr)
+    if (level .lt. mxnest) then
+       ntot = mitot * mjtot * nvar
+       do i = 1, ntot
+          alloc(locold + i - 1) = alloc(locnew + i - 1)
+       end do
+    endif
+    xlow = rnode(cornxlo,mptr) - nghost*hx
+    ylow = rnode(cornylo,mptr) - nghost*hy
+    locaux = node(storeaux,mptr)
+    if (node(ffluxptr,mptr) .ne. 0) then
+       lenbc = 2*(nx/intratx(level-1)+ny/intraty(level-1))
+       locsvf = node(ffluxptr,mptr)
+       locsvq = locsvf + nvar*lenbc
+       locx1d = locsvq + nvar*lenbc
+       call qad(alloc(locnew),mitot,mjtot,nvar,alloc(locsvf),alloc(locsvq),lenbc, &
+            intratx(level-1),intraty(level-1),hx,hy,0,alloc(locaux),alloc(locx1d),delt,mptr)
+    endif
+    call stepgrid(alloc(locnew),fm,fp,gm,gp,mitot,mjtot,nghost,delt,dtnew,hx,hy,nvar, &
+         xlow,ylow,time,mptr,0,alloc(locaux))
+    if (node(cfluxptr,mptr) .ne. 0) call fluxsv(mptr,fm,fp,gm,gp,alloc(node(cfluxptr,mptr)), &
+         mitot,mjtot,nvar,listsp(level),delt,hx,hy)
+    if (node(ffluxptr,mptr) .ne. 0) then
+       lenbc = 2*(nx/intratx(level-1)+ny/intraty(level-1))
+       locsvf = node(ffluxptr,mptr)
+       call fluxad(fm,fp,gm,gp,alloc(locsvf),mptr,mitot,mjtot,nvar,lenbc, &
+            intratx(level-1),intraty(level-1),nghost,delt,hx,hy)
+    endif
+    rnode(timemult,mptr) = rnode(timemult,mptr)+delt
+  end subroutine harness_par_advanc
+
+  subroutine ref_update(level, nvar) bind(C, name="ref_update")
+    integer(c_int), value :: level, nvar
+    call update(level, nvar, 0)
+  end subroutine ref_update
+
+  ! register block (svdflx then qc1d) of grid k (0-based) of fine level l
+  subroutine ref_get_reg(l, k, nvar, out) bind(C, name="ref_get_reg")
+    integer(c_int), value :: l, k, nvar
+    real(c_double) :: out(*)
+    integer :: mptr, lenbc, nx, ny
+    mptr = listOfGrids(listStart(l) + k)
+    nx = node(ndihi,mptr) - node(ndilo,mptr) + 1
+    ny = node(ndjhi,mptr) - node(ndjlo,mptr) + 1
+    lenbc = 2*(nx/intratx(l-1)+ny/intraty(l-1))
+    out(1:2*nvar*lenbc) = alloc(node(ffluxptr,mptr):node(ffluxptr,mptr)+2*nvar*lenbc-1)
+  end subroutine ref_get_reg
+
+  ! listbc of grid k (0-based) of coarse level l: returns the entry count
+  ! and copies up to nmax entries (5 integers each)
+  subroutine ref_get_listbc(l, k, nmax, out, n) bind(C, name="ref_get_listbc")
+    integer(c_int), value :: l, k, nmax
+    integer(c_int) :: out(*), n
+    integer :: mptr
+    external copy_listbc
+    mptr = listOfGrids(listStart(l) + k)
+    n = 0
+    if (node(cfluxptr,mptr) .eq. 0) return
+    call copy_listbc(alloc(node(cfluxptr,mptr)), listsp(l), nmax, out, n)
+  end subroutine ref_get_listbc
+
+  subroutine ref_get_time(l, k, t) bind(C, name="ref_get_time")
+    integer(c_int), value :: l, k
+    real(c_double) :: t
+    t = rnode(timemult, listOfGrids(listStart(l) + k))
+  end subroutine ref_get_time
+
 end module ref_harness
+
+! listbc lives in the double heap as integers (prepc.f:55-59, setuse.f:95);
+! this external routine reads it through an integer dummy like fluxsv does
+subroutine copy_listbc(listbc, maxsp, nmax, out, n)
+  implicit none
+  integer :: maxsp, nmax, n, listbc(5, maxsp), out(5, *)
+  integer :: e
+  n = 0
+  do e = 1, maxsp
+     if (listbc(1, e) .eq. 0) exit
+     n = n + 1
+     if (n .le. nmax) out(:, n) = listbc(:, e)
+  end do
+end subroutine copy_listbc

@@ -152,3 +152,69 @@ def random_hierarchy(rng, nvar=3, nx=24, ny=20, nlev=3, ratio=2, mthbc=(1, 1, 1,
         L["qnew"][:] = rng.normal(size=L["qnew"].shape)
         L["qold"][:] = rng.normal(size=L["qold"].shape)
     return H
+
+
+def nested_hierarchy(rng, solver_states, nvar, nx=24, ny=20, nlev=3, ratio=2, mthbc=(1, 1, 1, 1),
+                     xper=False, yper=False, dt1=None, nfine=5, level1_split=(2, 2)):
+    """Properly nested hierarchy at one time (t = 0, old = new) for time
+    stepping: every level-(l+1) grid, coarsened and grown by one coarse
+    cell, lies inside one level-l grid or outside the domain (the nesting
+    saveqc and the coarse-fine registers assume).  solver_states(rng, shape)
+    returns physical states of shape shape + (nvar,)."""
+    H = Hier(nvar, mthbc=mthbc, xper=xper, yper=yper)
+    hx, hy = 1.0 / nx, 1.0 / ny
+    isz, jsz = nx, ny
+    cuts_i = np.linspace(0, nx, level1_split[0] + 1).astype(int)
+    cuts_j = np.linspace(0, ny, level1_split[1] + 1).astype(int)
+    boxes = [(cuts_i[a], cuts_i[a + 1] - 1, cuts_j[b], cuts_j[b + 1] - 1)
+             for b in range(level1_split[1]) for a in range(level1_split[0])]
+    possk = dt1 if dt1 is not None else 0.2 * min(hx, hy)
+    for l in range(1, nlev + 1):
+        if l > 1:
+            within = []
+            for (a, b, c, d) in boxes:
+                within.append((0 if a == 0 else (a + 1) * ratio,
+                               isz * ratio - 1 if b == isz - 1 else b * ratio - 1,
+                               0 if c == 0 else (c + 1) * ratio,
+                               jsz * ratio - 1 if d == jsz - 1 else d * ratio - 1))
+            isz, jsz = isz * ratio, jsz * ratio
+            hx, hy = hx / ratio, hy / ratio
+            possk = possk / ratio
+            boxes = disjoint_boxes(rng, isz, jsz, nfine, lo=4, hi=max(8, isz // 3),
+                                   align=ratio, within=within, tries=4000)
+        H.add_level(boxes, hx, hy, possk, 0.0, isz, jsz, ratio, ratio)
+    for l in range(1, nlev + 1):
+        L = H.levels[l - 1]
+        for k in range(len(L["boxes"])):
+            g = H.grid(l, k, 0)
+            g[:] = solver_states(rng, g.shape[:2])
+            H.grid(l, k, 1)[:] = g
+    return H
+
+
+def smooth_euler(rng, shape):
+    """Positive, smoothly varying Euler states (for hierarchies)."""
+    ny, nx = shape
+    y, x = np.meshgrid(np.linspace(0, 1, ny), np.linspace(0, 1, nx), indexing="ij")
+    ph = rng.random(4) * 6.28
+    rho = 1.0 + 0.3 * np.sin(3 * x + ph[0]) * np.cos(2 * y + ph[1]) + 0.05 * rng.random(shape)
+    u = 0.4 * np.sin(2 * y + ph[2]) + 0.05 * rng.normal(size=shape)
+    v = 0.4 * np.cos(2 * x + ph[3]) + 0.05 * rng.normal(size=shape)
+    p = 1.0 + 0.2 * rng.random(shape)
+    return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)], -1)
+
+
+def acoustic_states(rng, shape):
+    return rng.normal(size=shape + (3,))
+
+
+def berger_oliger(impl, level, nlev, ratio, log=None):
+    """One step of `level` and the subcycled steps of the finer levels
+    (tick.f:230-300 with fixed time steps)."""
+    r = impl.advanc(level)
+    if log is not None:
+        log.append((level, r))
+    if level < nlev:
+        for _ in range(ratio):
+            berger_oliger(impl, level + 1, nlev, ratio, log)
+        impl.update(level)
