@@ -26,7 +26,10 @@ UNITS = [
     ("sweep_f32.hip", ["-O3"]),
     ("util.hip", ["-O2"]),
     ("fill.hip", ["-O3", "-ffp-contract=off"]),
+    ("regs.hip", ["-O3", "-ffp-contract=off"]),
+    ("flag.hip", ["-O3", "-ffp-contract=off"]),
     ("fill_plan.cpp", ["-O2", "-x", "hip"]),
+    ("regs_plan.cpp", ["-O2", "-x", "hip"]),
     ("capi.cpp", ["-O2", "-x", "hip"]),
 ]
 

@@ -38,7 +38,7 @@ class FillOp(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("type", "dst", "src", "src_level", "di0", "dj0", "ni",
                                        "nj", "si0", "sj0", "mask_off", "ilo", "jlo", "iplo",
                                        "jplo", "ratx", "raty", "level")] + \
-               [("bc", C.c_int * 4), ("stage", C.c_int), ("pad", C.c_int * 2)]
+               [("bc", C.c_int * 4), ("stage", C.c_int), ("flags", C.c_int), ("pad", C.c_int)]
 
 
 class FillBuf(C.Structure):
@@ -84,6 +84,13 @@ def lib():
         L.amrgpu_set_geometry.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _dp, _dp]
         L.amrgpu_set_time.argtypes = [C.c_int, C.c_double, C.c_double]
         L.amrgpu_bound_level.argtypes = [C.c_int, C.c_double]
+        L.amrgpu_advance_level.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.amrgpu_update_level.argtypes = [C.c_int]
+        L.amrgpu_conck.argtypes = [C.c_int, C.POINTER(C.c_double)]
+        L.amrgpu_get_registers.argtypes = [C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]
+        L.amrgpu_set_corners.argtypes = [C.c_int, _dp, _dp]
+        L.amrgpu_flag_level.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p,
+                                        C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
         L.amrgpu_event_record.argtypes = [C.c_int]
         L.amrgpu_event_elapsed.argtypes = [C.c_int, C.c_int,
                                            C.POINTER(C.c_float)]
@@ -215,6 +222,59 @@ def bound_level(level, time):
     _check(lib().amrgpu_bound_level(level, time))
 
 
+def advance_level(level):
+    """advanc for one level; returns (dtlevnew, cfl_level)."""
+    dt, cfl = C.c_double(), C.c_double()
+    _check(lib().amrgpu_advance_level(level, C.byref(dt), C.byref(cfl)))
+    return dt.value, cfl.value
+
+
+def update_level(level):
+    _check(lib().amrgpu_update_level(level))
+
+
+def conck(level):
+    t = C.c_double()
+    _check(lib().amrgpu_conck(level, C.byref(t)))
+    return t.value
+
+
+def get_registers(level, grid, nvar, maxlen):
+    """(svdflx, qc1d) of one fine grid, each (lenbc, nvar)."""
+    buf = np.zeros(2 * nvar * maxlen)
+    n = C.c_int()
+    _check(lib().amrgpu_get_registers(level, grid, buf, C.byref(n)))
+    lb = n.value
+    return buf[:nvar * lb].reshape(lb, nvar), buf[nvar * lb:2 * nvar * lb].reshape(lb, nvar)
+
+
+def set_corners(level, xlo, ylo):
+    _check(lib().amrgpu_set_corners(level, np.ascontiguousarray(xlo, dtype=np.float64),
+                                    np.ascontiguousarray(ylo, dtype=np.float64)))
+
+
+def flag_level(level, tolsp, start_time, regions=(), mbuff=4, shapes=None):
+    """Flag the cells of every grid of a level.  With shapes (list of (mx,
+    my)) the flags are returned as a list of (my+2 mbuff, mx+2 mbuff)
+    arrays; returns (flags or None, number of DOFLAG cells)."""
+    reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1))
+    n = C.c_int64()
+    rp = reg.ctypes.data_as(C.c_void_p) if reg.size else None
+    if shapes is None:
+        _check(lib().amrgpu_flag_level(level, tolsp, start_time, len(regions), rp, mbuff,
+                                       None, None, C.byref(n)))
+        return None, n.value
+    sizes = [(mx + 2 * mbuff) * (my + 2 * mbuff) for mx, my in shapes]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    out = np.zeros(int(sum(sizes)))
+    _check(lib().amrgpu_flag_level(level, tolsp, start_time, len(regions), rp, mbuff,
+                                   out.ctypes.data_as(C.c_void_p), offs.ctypes.data_as(C.c_void_p),
+                                   C.byref(n)))
+    flags = [out[o:o + sz].reshape(my + 2 * mbuff, mx + 2 * mbuff)
+             for o, sz, (mx, my) in zip(offs, sizes, shapes)]
+    return flags, n.value
+
+
 def load_hierarchy(H):
     """Install a tests.common.Hier-like hierarchy: geometry, layouts, both
     time levels of every grid and the level times."""
@@ -265,6 +325,23 @@ def plan_fill(H, level, domain=None, mthbc=None):
     _check(L.amrgpu_plan_fill_get(C.cast(ops, C.c_void_p), C.cast(bufs, C.c_void_p),
                                   C.cast(masks, C.c_void_p)))
     return list(ops)[:nops.value], list(bufs)[:nbufs.value], bytes(masks)[:nmask.value]
+
+
+def plan_listbc(H, level, grid):
+    """Host-planned listbc entries of one coarse grid (no device)."""
+    lv = H.levels
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    boxes = i32(np.concatenate([np.asarray(L["boxes"]).ravel() for L in lv]))
+    L = lib()
+    L.amrgpu_plan_listbc.argtypes = [C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _ip, C.c_int, C.c_int,
+                                     C.c_int, _ip, C.c_int, C.POINTER(C.c_int)]
+    out = np.zeros(5 * 4096, dtype=np.int32)
+    n = C.c_int()
+    _check(L.amrgpu_plan_listbc(len(lv), i32([len(x["boxes"]) for x in lv]), boxes,
+                                i32([x["iregsz"] for x in lv]), i32([x["jregsz"] for x in lv]),
+                                i32([x["ratx"] for x in lv]), i32([x["raty"] for x in lv]),
+                                i32(H.mthbc), H.nvar, level, grid, out, 4096, C.byref(n)))
+    return out[:5 * n.value].reshape(-1, 5)
 
 
 def event_record(slot):

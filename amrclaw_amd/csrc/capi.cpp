@@ -16,6 +16,7 @@
 #include "../../include/amrgpu.h"
 #include "fill_plan.hpp"
 #include "kernels.hpp"
+#include "regs_plan.hpp"
 
 namespace amrgpu {
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, int64_t n,
@@ -61,6 +62,36 @@ struct Level {
     double* d_scratch = nullptr;
     // time of the level's grids (rnode(timemult)) and its step (possk)
     double time = 0.0, possk = 0.0;
+    // coarse-fine registers: this level's arena (fine side) and the device
+    // programs of fluxsv / upbnd / averaging (coarse side) and qad (fine side)
+    double* d_reg = nullptr;
+    SvEntry* d_sv = nullptr;
+    QadItem* d_qad = nullptr;
+    int nqad = 0;
+    UpbndCell* d_ucells = nullptr;
+    UpbndEntry* d_uents = nullptr;
+    int nucells = 0;
+    AvgBlock* d_avg = nullptr;
+    int64_t* d_avg_start = nullptr;
+    int navg = 0;
+    int64_t navg_cells = 0;
+    // saveqc program filling this level's qc1d from the next coarser level
+    FillPlan sq;
+    FillOp* d_sq_ops = nullptr;
+    FillBuf* d_sq_bufs = nullptr;
+    double* d_sq_scratch = nullptr;
+    double* d_partial = nullptr;        // conck partial sums, one per grid
+    // grid corners rnode(cornxlo/cornylo) (default ilo*hx + xlower)
+    std::vector<double> cxlo, cylo;
+    // flagging: FP64 flags of every grid, region rectangles, counters
+    double* d_flags = nullptr;
+    int64_t flags_cap = 0;
+    int64_t* d_foff = nullptr;
+    int foff_mbuff = -1;
+    std::vector<int64_t> foff;
+    RegionRect* d_rects = nullptr;
+    int64_t rects_cap = 0;
+    unsigned long long* d_counters = nullptr;
 };
 
 struct Context {
@@ -76,6 +107,7 @@ struct Context {
     int iregsz[MAXLV] = {}, jregsz[MAXLV] = {}, ratx[MAXLV] = {}, raty[MAXLV] = {};
     double hxposs[MAXLV] = {}, hyposs[MAXLV] = {};
     std::vector<unsigned long long> h_cfl;
+    bool regs_valid = false;
 };
 
 Context g;
@@ -98,7 +130,27 @@ int fail(int code, const char* fmt, ...) {
             return fail(AMRGPU_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+// Every host <-> device transfer and clear goes through the library's
+// stream, so it is ordered after the kernels already queued there (the
+// stream does not synchronize with the null stream).
+hipError_t h2d(void* dst, const void* src, size_t n) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, g.stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(g.stream);
+}
+hipError_t d2h(void* dst, const void* src, size_t n) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, g.stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(g.stream);
+}
+hipError_t dzero(void* p, size_t n) { return n ? hipMemsetAsync(p, 0, n, g.stream) : hipSuccess; }
+// wait for the queued work before freeing or overwriting what it may use
+void quiesce() {
+    if (g.stream) hipStreamSynchronize(g.stream);
+}
+
 void free_plan(Level& L) {
+    quiesce();
     if (L.d_ops) hipFree(L.d_ops);
     if (L.d_bufs) hipFree(L.d_bufs);
     if (L.d_masks) hipFree(L.d_masks);
@@ -111,8 +163,37 @@ void free_plan(Level& L) {
     L.plan_valid = false;
 }
 
+void free_regs(Level& L) {
+    quiesce();
+    void* ps[] = {L.d_reg, L.d_sv, L.d_qad, L.d_ucells, L.d_uents, L.d_avg, L.d_avg_start,
+                  L.d_sq_ops, L.d_sq_bufs, L.d_sq_scratch};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    L.d_reg = nullptr;
+    L.d_sv = nullptr;
+    L.d_qad = nullptr;
+    L.d_ucells = nullptr;
+    L.d_uents = nullptr;
+    L.d_avg = nullptr;
+    L.d_avg_start = nullptr;
+    L.d_sq_ops = nullptr;
+    L.d_sq_bufs = nullptr;
+    L.d_sq_scratch = nullptr;
+    L.nqad = L.nucells = L.navg = 0;
+    L.navg_cells = 0;
+    L.sq = FillPlan();
+    for (auto& t : L.tiles) t.reg_lo = t.reg_hi = 0;
+    for (auto& p : L.patches) p.reg_off = 0, p.lenbc = 0;
+}
+
 void free_level(Level& L) {
+    quiesce();
     free_plan(L);
+    free_regs(L);
+    if (L.d_partial) hipFree(L.d_partial);
+    void* fl[] = {L.d_flags, L.d_foff, L.d_rects, L.d_counters};
+    for (void* p : fl)
+        if (p) hipFree(p);
     if (L.d_patches) hipFree(L.d_patches);
     if (L.d_tiles) hipFree(L.d_tiles);
     for (auto& b : L.buf)
@@ -170,12 +251,13 @@ int plan_level(Level& L) {
     if (lds > 160 * 1024)
         return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", g.txmax, g.tymax, lds);
     L.lds = lds;
+    g.regs_valid = false;
+    quiesce();
     if (L.d_tiles) hipFree(L.d_tiles);
     L.d_tiles = nullptr;
     if (!L.tiles.empty()) {
         HIPCHK(hipMalloc(&L.d_tiles, L.tiles.size() * sizeof(TileDesc)));
-        HIPCHK(hipMemcpy(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(TileDesc),
-                         hipMemcpyHostToDevice));
+        HIPCHK(h2d(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(TileDesc)));
     }
     return AMRGPU_OK;
 }
@@ -192,6 +274,7 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
     free_level(L);
     // fill programs of this and finer levels read this level's layout
     for (int l = level; l < MAXLV; l++) free_plan(g.levels[l]);
+    g.regs_valid = false;
     L.ngrids = ngrids;
     L.hx = hx;
     L.hy = hy;
@@ -214,15 +297,19 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
         L.patches.push_back(pd);
     }
     L.nelem = off;
+    for (int p = 0; p < ngrids; p++) {
+        L.cxlo.push_back(L.patches[p].ilo * hx + g.cfg.xlower);    // grdfit2.f:98-101
+        L.cylo.push_back(L.patches[p].jlo * hy + g.cfg.ylower);
+    }
     if (ngrids > 0) {
         HIPCHK(hipMalloc(&L.d_patches, ngrids * sizeof(PatchDesc)));
-        HIPCHK(hipMemcpy(L.d_patches, L.patches.data(), ngrids * sizeof(PatchDesc),
-                         hipMemcpyHostToDevice));
+        HIPCHK(h2d(L.d_patches, L.patches.data(), ngrids * sizeof(PatchDesc)));
         for (auto& b : L.buf) {
             HIPCHK(hipMalloc(&b, (size_t)L.nelem * g.esize));
-            HIPCHK(hipMemset(b, 0, (size_t)L.nelem * g.esize));
+            HIPCHK(dzero(b, (size_t)L.nelem * g.esize));
         }
         HIPCHK(hipMalloc(&L.d_cfl, ngrids * sizeof(unsigned long long)));
+        HIPCHK(hipMalloc(&L.d_partial, ngrids * sizeof(double)));
         std::vector<int64_t> off(ngrids), cs(ngrids);
         std::vector<int32_t> pl(ngrids);
         int64_t c = 0;
@@ -236,9 +323,9 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
         HIPCHK(hipMalloc(&L.d_off, ngrids * sizeof(int64_t)));
         HIPCHK(hipMalloc(&L.d_plane, ngrids * sizeof(int32_t)));
         HIPCHK(hipMalloc(&L.d_cell_start, ngrids * sizeof(int64_t)));
-        HIPCHK(hipMemcpy(L.d_off, off.data(), ngrids * sizeof(int64_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(L.d_plane, pl.data(), ngrids * sizeof(int32_t), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(L.d_cell_start, cs.data(), ngrids * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(h2d(L.d_off, off.data(), ngrids * sizeof(int64_t)));
+        HIPCHK(h2d(L.d_plane, pl.data(), ngrids * sizeof(int32_t)));
+        HIPCHK(h2d(L.d_cell_start, cs.data(), ngrids * sizeof(int64_t)));
     }
     return plan_level(L);
 }
@@ -253,12 +340,12 @@ int put_grid(Level& L, int grid, int which, const double* q) {
         std::vector<double> h(n);
         for (int64_t c = 0; c < pd.plane; c++)
             for (int m = 0; m < nv; m++) h[m * pd.plane + c] = q[c * nv + m];
-        HIPCHK(hipMemcpy(dst, h.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(h2d(dst, h.data(), n * 8));
     } else {
         std::vector<float> h(n);
         for (int64_t c = 0; c < pd.plane; c++)
             for (int m = 0; m < nv; m++) h[m * pd.plane + c] = (float)q[c * nv + m];
-        HIPCHK(hipMemcpy(dst, h.data(), n * 4, hipMemcpyHostToDevice));
+        HIPCHK(h2d(dst, h.data(), n * 4));
     }
     return AMRGPU_OK;
 }
@@ -271,12 +358,12 @@ int get_grid(Level& L, int grid, int which, double* q) {
     HIPCHK(hipStreamSynchronize(g.stream));
     if (g.esize == 8) {
         std::vector<double> h(n);
-        HIPCHK(hipMemcpy(h.data(), src, n * 8, hipMemcpyDeviceToHost));
+        HIPCHK(d2h(h.data(), src, n * 8));
         for (int64_t c = 0; c < pd.plane; c++)
             for (int m = 0; m < nv; m++) q[c * nv + m] = h[m * pd.plane + c];
     } else {
         std::vector<float> h(n);
-        HIPCHK(hipMemcpy(h.data(), src, n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(d2h(h.data(), src, n * 4));
         for (int64_t c = 0; c < pd.plane; c++)
             for (int m = 0; m < nv; m++) q[c * nv + m] = (double)h[m * pd.plane + c];
     }
@@ -302,7 +389,7 @@ int xfer_level(Level& L, bool to_dev, int which, double* alloc_base, const int64
     HIPCHK(hipHostMalloc((void**)&h_stage, tot * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipMalloc(&d_stage, tot * sizeof(double)));
     HIPCHK(hipMalloc(&d_soff, L.ngrids * sizeof(int64_t)));
-    HIPCHK(hipMemcpy(d_soff, soff.data(), L.ngrids * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(h2d(d_soff, soff.data(), L.ngrids * sizeof(int64_t)));
     void* lvl = L.buf[which == 0 ? L.cur : 1 - L.cur];
     int rc = AMRGPU_OK;
     hipError_t e = hipSuccess;
@@ -372,15 +459,15 @@ int ensure_plan(int level) {
     const FillPlan& P = L.plan;
     if (!P.ops.empty()) {
         HIPCHK(hipMalloc(&L.d_ops, P.ops.size() * sizeof(FillOp)));
-        HIPCHK(hipMemcpy(L.d_ops, P.ops.data(), P.ops.size() * sizeof(FillOp), hipMemcpyHostToDevice));
+        HIPCHK(h2d(L.d_ops, P.ops.data(), P.ops.size() * sizeof(FillOp)));
     }
     if (!P.bufs.empty()) {
         HIPCHK(hipMalloc(&L.d_bufs, P.bufs.size() * sizeof(FillBuf)));
-        HIPCHK(hipMemcpy(L.d_bufs, P.bufs.data(), P.bufs.size() * sizeof(FillBuf), hipMemcpyHostToDevice));
+        HIPCHK(h2d(L.d_bufs, P.bufs.data(), P.bufs.size() * sizeof(FillBuf)));
     }
     if (!P.masks.empty()) {
         HIPCHK(hipMalloc(&L.d_masks, P.masks.size()));
-        HIPCHK(hipMemcpy(L.d_masks, P.masks.data(), P.masks.size(), hipMemcpyHostToDevice));
+        HIPCHK(h2d(L.d_masks, P.masks.data(), P.masks.size()));
     }
     if (P.scratch_elems > 0) HIPCHK(hipMalloc(&L.d_scratch, P.scratch_elems * sizeof(double)));
     L.plan_valid = true;
@@ -411,22 +498,21 @@ int time_weights(double t, FillArgs& a, uint32_t mask) {
     return AMRGPU_OK;
 }
 
-int bound_level(int level, double t) {
-    Level& L = g.levels[level];
-    if (L.ngrids == 0) return AMRGPU_OK;
-    int rc = ensure_plan(level);
-    if (rc) return rc;
+// launch the stages of a fill program (ghost fill or saveqc) at time t
+int run_fill(const FillPlan& P, const FillOp* d_ops, const FillBuf* d_bufs,
+             const unsigned char* d_masks, double* d_scratch, double t) {
     FillArgs a{};
-    rc = time_weights(t, a, L.plan.src_levels_mask);
+    int rc = time_weights(t, a, P.src_levels_mask);
     if (rc) return rc;
-    a.bufs = L.d_bufs;
-    a.scratch = L.d_scratch;
-    a.masks = L.d_masks;
+    a.bufs = d_bufs;
+    a.scratch = d_scratch;
+    a.masks = d_masks;
     for (int l = 1; l < MAXLV && l < FILL_MAXLV; l++) {
         Level& S = g.levels[l];
         a.lvl_new[l] = S.buf[S.cur];
         a.lvl_old[l] = S.buf[1 - S.cur];
         a.lvl_patches[l] = S.d_patches;
+        a.lvl_reg[l] = S.d_reg;
         a.hx[l] = g.hxposs[l];
         a.hy[l] = g.hyposs[l];
     }
@@ -434,16 +520,120 @@ int bound_level(int level, double t) {
     a.ylower = g.cfg.ylower;
     for (int k = 0; k < 4; k++) a.mthbc[k] = g.cfg.mthbc[k];
     a.nv = g.cfg.nvar;
-    for (int s = 0; s < L.plan.nstages; s++) {
-        a.ops = L.d_ops + L.plan.stage_start[s];
-        a.nops = L.plan.stage_start[s + 1] - L.plan.stage_start[s];
+    for (int s = 0; s < P.nstages; s++) {
+        a.ops = d_ops + P.stage_start[s];
+        a.nops = P.stage_start[s + 1] - P.stage_start[s];
         hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
         if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "fill stage %d: %s", s, hipGetErrorString(e));
     }
     return AMRGPU_OK;
 }
 
-int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev) {
+int bound_level(int level, double t) {
+    Level& L = g.levels[level];
+    if (L.ngrids == 0) return AMRGPU_OK;
+    int rc = ensure_plan(level);
+    if (rc) return rc;
+    return run_fill(L.plan, L.d_ops, L.d_bufs, L.d_masks, L.d_scratch, t);
+}
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& v) {
+    quiesce();
+    if (*dst) hipFree(*dst);
+    *dst = nullptr;
+    if (v.empty()) return AMRGPU_OK;
+    HIPCHK(hipMalloc(dst, v.size() * sizeof(T)));
+    HIPCHK(h2d(*dst, v.data(), v.size() * sizeof(T)));
+    return AMRGPU_OK;
+}
+
+// Register layout and programs of the whole hierarchy (prepf.f / prepc.f
+// at every regrid).  Arenas start at zero, as prepf.f:44-48 leaves them.
+int ensure_regs() {
+    if (g.regs_valid) return AMRGPU_OK;
+    if (g.mxnest < 1) return fail(AMRGPU_ERR_STATE, "amrgpu_set_geometry has not been called");
+    HierGeom H = make_geom();
+    const int nv = g.cfg.nvar;
+    std::vector<RegLevel> R(FILL_MAXLV);
+    std::string err;
+    if (!plan_registers(H, nv, R.data(), err)) return fail(AMRGPU_ERR_ARG, "registers: %s", err.c_str());
+    for (int l = 1; l <= H.nlev && l < MAXLV; l++) {
+        Level& L = g.levels[l];
+        free_regs(L);
+        if (L.ngrids == 0) continue;
+        const RegLevel& Rl = R[l];
+        int rc;
+        if (l >= 2 && !Rl.reg_off.empty()) {
+            for (int p = 0; p < L.ngrids; p++) {
+                L.patches[p].reg_off = Rl.reg_off[p];
+                L.patches[p].lenbc = Rl.lenbc[p];
+            }
+            HIPCHK(hipMalloc(&L.d_reg, Rl.reg_elems * sizeof(double)));
+            HIPCHK(dzero(L.d_reg, Rl.reg_elems * sizeof(double)));
+            if ((rc = upload(&L.d_qad, Rl.qad))) return rc;
+            L.nqad = (int)Rl.qad.size();
+            if (!plan_saveqc(H, l, nv, L.sq, err))
+                return fail(AMRGPU_ERR_ARG, "level %d saveqc: %s", l, err.c_str());
+            if ((rc = upload(&L.d_sq_ops, L.sq.ops))) return rc;
+            if ((rc = upload(&L.d_sq_bufs, L.sq.bufs))) return rc;
+            if (L.sq.scratch_elems > 0)
+                HIPCHK(hipMalloc(&L.d_sq_scratch, L.sq.scratch_elems * sizeof(double)));
+        }
+        if (!Rl.sv_grid_start.empty()) {
+            // fluxsv entries that write their slot, grouped by the tile that
+            // owns their coarse cell
+            std::vector<std::vector<int>> tiles_of(L.ngrids);
+            for (int t = 0; t < (int)L.tiles.size(); t++) tiles_of[L.tiles[t].patch].push_back(t);
+            std::vector<std::vector<SvEntry>> per_tile(L.tiles.size());
+            for (int m = 0; m < L.ngrids; m++)
+                for (int e = Rl.sv_grid_start[m]; e < Rl.sv_grid_start[m + 1]; e++) {
+                    if (!Rl.sv_write[e]) continue;
+                    const SvEntry& s = Rl.sv[e];
+                    int owner = -1;
+                    for (int t : tiles_of[m]) {
+                        const TileDesc& td = L.tiles[t];
+                        if (s.I > td.i0 && s.I <= td.i0 + td.tx && s.J > td.j0 && s.J <= td.j0 + td.ty) {
+                            owner = t;
+                            break;
+                        }
+                    }
+                    if (owner < 0)
+                        return fail(AMRGPU_ERR_ARG, "register entry (%d,%d) outside grid %d of level %d",
+                                    s.I, s.J, m, l);
+                    per_tile[owner].push_back(s);
+                }
+            std::vector<SvEntry> flat;
+            for (size_t t = 0; t < L.tiles.size(); t++) {
+                L.tiles[t].reg_lo = (int32_t)flat.size();
+                flat.insert(flat.end(), per_tile[t].begin(), per_tile[t].end());
+                L.tiles[t].reg_hi = (int32_t)flat.size();
+            }
+            if ((rc = upload(&L.d_sv, flat))) return rc;
+            if ((rc = upload(&L.d_ucells, Rl.ucells))) return rc;
+            if ((rc = upload(&L.d_uents, Rl.uents))) return rc;
+            L.nucells = (int)Rl.ucells.size();
+            if ((rc = upload(&L.d_avg, Rl.avg))) return rc;
+            if ((rc = upload(&L.d_avg_start, Rl.avg_start))) return rc;
+            L.navg = (int)Rl.avg.size();
+            L.navg_cells = Rl.avg_start.empty() ? 0 : Rl.avg_start.back();
+        }
+        HIPCHK(h2d(L.d_patches, L.patches.data(), L.ngrids * sizeof(PatchDesc)));
+        HIPCHK(h2d(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(TileDesc)));
+    }
+    g.regs_valid = true;
+    return AMRGPU_OK;
+}
+
+struct RegLink {
+    double* reg = nullptr;
+    double* reg_fine = nullptr;
+    const PatchDesc* fine_patches = nullptr;
+    const SvEntry* sv = nullptr;
+};
+
+int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev,
+          const RegLink& rl = RegLink()) {
     if (L.ngrids == 0) return AMRGPU_OK;
     HIPCHK(launch_fill_u64(L.d_cfl, 0ull, L.ngrids, g.stream));
     SweepArgs a{};
@@ -459,6 +649,10 @@ int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev
     a.cfl_patch = L.d_cfl;
     a.flux_out = flux_out_dev;
     a.flux_off = flux_off_dev;
+    a.reg = rl.reg;
+    a.reg_fine = rl.reg_fine;
+    a.fine_patches = rl.fine_patches;
+    a.sv = rl.sv;
     hipError_t e = g.esize == 8 ? launch_sweep_f64(g.cfg.solver, a, L.lds, g.stream)
                                 : launch_sweep_f32(g.cfg.solver, a, L.lds, g.stream);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "sweep launch: %s", hipGetErrorString(e));
@@ -484,6 +678,173 @@ int level_cfl(Level& L, double dt, double* cfl_level, double* dtlevnew) {
     }
     if (cfl_level) *cfl_level = cmax;
     if (dtlevnew) *dtlevnew = dmin;
+    return AMRGPU_OK;
+}
+
+// advanc.f:16-305 for one level: ghost fill, saveqc of the finer level,
+// the sweep with fluxsv / fluxad's perimeter fluxes, qad + fluxad, time
+int advance_level(int level, double* dtlevnew, double* cfl_level) {
+    Level& L = g.levels[level];
+    int rc = ensure_regs();
+    if (rc) return rc;
+    if (L.ngrids == 0) {
+        if (dtlevnew) *dtlevnew = RINFINITY;
+        if (cfl_level) *cfl_level = 0.0;
+        return AMRGPU_OK;
+    }
+    if (!(L.possk > 0)) return fail(AMRGPU_ERR_STATE, "level %d has no time step (amrgpu_set_time)", level);
+    rc = bound_level(level, L.time);
+    if (rc) return rc;
+    const bool has_fine = level + 1 <= g.mxnest && level + 1 < MAXLV && g.levels[level + 1].ngrids > 0;
+    RegLink rl;
+    if (has_fine) {
+        Level& F = g.levels[level + 1];
+        rc = run_fill(F.sq, F.d_sq_ops, F.d_sq_bufs, nullptr, F.d_sq_scratch, L.time);
+        if (rc) return rc;
+        rl.reg_fine = F.d_reg;
+        rl.fine_patches = F.d_patches;
+        rl.sv = L.d_sv;
+    }
+    if (level > 1) rl.reg = L.d_reg;
+    rc = sweep(L, L.possk, nullptr, nullptr, rl);
+    if (rc) return rc;
+    if (level > 1 && L.nqad > 0) {
+        QadArgs a{};
+        a.items = L.d_qad;
+        a.nitems = L.nqad;
+        a.nv = g.cfg.nvar;
+        a.patches = L.d_patches;
+        a.q = L.buf[1 - L.cur];          // time t with its ghost cells
+        a.reg = L.d_reg;
+        a.ratx = g.ratx[level - 1];
+        a.raty = g.raty[level - 1];
+        a.hx = L.hx;
+        a.hy = L.hy;
+        a.delt = L.possk;
+        hipError_t e = launch_finereg(g.cfg.solver, (int)g.esize, a, g.meth.ph, g.stream);
+        if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "register fix-up: %s", hipGetErrorString(e));
+    }
+    L.time = L.time + L.possk;                                   // advanc.f:289
+    if (cfl_level || dtlevnew) return level_cfl(L, L.possk, cfl_level, dtlevnew);
+    return AMRGPU_OK;
+}
+
+// update.f:16-189: reflux (upbnd) and average the finer level onto `level`
+int update_level(int level) {
+    Level& C = g.levels[level];
+    if (level + 1 >= MAXLV || level + 1 > g.mxnest) return AMRGPU_OK;
+    Level& F = g.levels[level + 1];
+    if (C.ngrids == 0 || F.ngrids == 0) return AMRGPU_OK;
+    int rc = ensure_regs();
+    if (rc) return rc;
+    UpdateArgs a{};
+    a.cells = C.d_ucells;
+    a.ncells = C.nucells;
+    a.nv = g.cfg.nvar;
+    a.entries = C.d_uents;
+    a.blocks = C.d_avg;
+    a.nblocks = C.navg;
+    a.ratx = g.ratx[level];
+    a.raty = g.raty[level];
+    a.patches = C.d_patches;
+    a.fine_patches = F.d_patches;
+    a.q = C.buf[C.cur];
+    a.qf = F.buf[F.cur];
+    a.reg = F.d_reg;
+    a.area = C.hx * C.hy;
+    hipError_t e = launch_update((int)g.esize, a, C.d_avg_start, C.navg_cells, g.stream);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "update: %s", hipGetErrorString(e));
+    return AMRGPU_OK;
+}
+
+// flagger.f:56-164 for every grid of `level` (no Richardson estimate)
+int flag_level(int level, double tolsp, double start_time, int nreg, const double* regions,
+               int mbuff, double* flags_base, const int64_t* offsets, int64_t* nflagged) {
+    Level& L = g.levels[level];
+    if (nflagged) *nflagged = 0;
+    if (L.ngrids == 0) return AMRGPU_OK;
+    const double t = L.time;
+    // flagger.f:98: the exact test chooses the new values (after bound) or,
+    // at the initial time, the old ones whose ghost cells are already set
+    const bool use_old = !(start_time + L.possk != t);
+    int rc;
+    if (!use_old && (rc = bound_level(level, t))) return rc;
+    if (L.foff_mbuff != mbuff) {
+        quiesce();
+        L.foff.assign(L.ngrids + 1, 0);
+        for (int p = 0; p < L.ngrids; p++)
+            L.foff[p + 1] = L.foff[p] + (int64_t)(L.patches[p].mx + 2 * mbuff) * (L.patches[p].my + 2 * mbuff);
+        if (L.foff.back() > L.flags_cap) {
+            if (L.d_flags) hipFree(L.d_flags);
+            L.d_flags = nullptr;
+            HIPCHK(hipMalloc(&L.d_flags, L.foff.back() * sizeof(double)));
+            L.flags_cap = L.foff.back();
+        }
+        if (!L.d_foff) HIPCHK(hipMalloc(&L.d_foff, (L.ngrids + 1) * sizeof(int64_t)));
+        HIPCHK(h2d(L.d_foff, L.foff.data(), (L.ngrids + 1) * sizeof(int64_t)));
+        L.foff_mbuff = mbuff;
+    }
+    // flagregions2.f90:61-92: each region clipped to each grid, on the host
+    // with the reference's arithmetic
+    std::vector<RegionRect> rects((size_t)L.ngrids * std::max(nreg, 1));
+    for (int p = 0; p < L.ngrids; p++) {
+        const PatchDesc& pd = L.patches[p];
+        const double dx = L.hx, dy = L.hy, xlower = L.cxlo[p], ylower = L.cylo[p];
+        const double xupper = xlower + pd.mx * dx, yupper = ylower + pd.my * dy;
+        for (int m = 0; m < nreg; m++) {
+            const double* r = regions + 8 * m;
+            RegionRect& R = rects[(size_t)p * nreg + m];
+            R = RegionRect{1, 0, 1, 0, 0, 0};
+            if (t < r[2] || t > r[3]) continue;
+            if (xlower >= r[5] || xupper <= r[4]) continue;
+            if (ylower >= r[7] || yupper <= r[6]) continue;
+            R.i1 = std::max((int)std::floor((r[4] - xlower) / dx) + 1, 1);
+            R.i2 = std::min((int)std::floor((r[5] - xlower) / dx) + 1, pd.mx);
+            R.j1 = std::max((int)std::floor((r[6] - ylower) / dy) + 1, 1);
+            R.j2 = std::min((int)std::floor((r[7] - ylower) / dy) + 1, pd.my);
+            R.min_level = (int)r[0];
+            R.max_level = (int)r[1];
+        }
+    }
+    quiesce();   // the previous call's rectangles may still be in use
+    if ((int64_t)rects.size() > L.rects_cap) {
+        if (L.d_rects) hipFree(L.d_rects);
+        L.d_rects = nullptr;
+        HIPCHK(hipMalloc(&L.d_rects, rects.size() * sizeof(RegionRect)));
+        L.rects_cap = (int64_t)rects.size();
+    }
+    if (nreg > 0)
+        HIPCHK(hipMemcpyAsync(L.d_rects, rects.data(), rects.size() * sizeof(RegionRect),
+                              hipMemcpyHostToDevice, g.stream));
+    if (!L.d_counters) HIPCHK(hipMalloc(&L.d_counters, 2 * sizeof(unsigned long long)));
+    HIPCHK(launch_fill_u64(L.d_counters, 0ull, 2, g.stream));
+    FlagArgs a{};
+    a.patches = L.d_patches;
+    a.q = use_old ? L.buf[1 - L.cur] : L.buf[L.cur];
+    a.flags = L.d_flags;
+    a.foff = L.d_foff;
+    a.rects = L.d_rects;
+    a.counters = L.d_counters;
+    a.nrect = nreg;
+    a.nv = g.cfg.nvar;
+    a.mbuff = mbuff;
+    a.level = level;
+    a.tolsp = tolsp;
+    hipError_t e = launch_flag((int)g.esize, a, L.ngrids, g.stream);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "flag: %s", hipGetErrorString(e));
+    unsigned long long cnt[2];
+    HIPCHK(hipMemcpyAsync(cnt, L.d_counters, sizeof cnt, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (cnt[1] & 1ull)
+        return fail(AMRGPU_ERR_ARG, "regions: minlevel > maxlevel in a cell of level %d (flagregions2.f90:96-100)", level);
+    if (nflagged) *nflagged = (int64_t)cnt[0];
+    if (flags_base) {
+        std::vector<double> h(L.foff.back());
+        HIPCHK(d2h(h.data(), L.d_flags, h.size() * sizeof(double)));
+        for (int p = 0; p < L.ngrids; p++)
+            std::memcpy(flags_base + offsets[p], h.data() + L.foff[p],
+                        (size_t)(L.foff[p + 1] - L.foff[p]) * sizeof(double));
+    }
     return AMRGPU_OK;
 }
 
@@ -591,9 +952,9 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out, double dx
     int64_t* d_off = nullptr;
     if (want_flux) {
         HIPCHK(hipMalloc(&d_flux, 4 * arr * sizeof(double)));
-        HIPCHK(hipMemset(d_flux, 0, 4 * arr * sizeof(double)));
+        HIPCHK(dzero(d_flux, 4 * arr * sizeof(double)));
         HIPCHK(hipMalloc(&d_off, sizeof(int64_t)));
-        HIPCHK(hipMemset(d_off, 0, sizeof(int64_t)));
+        HIPCHK(dzero(d_off, sizeof(int64_t)));
     }
     // ghost cells of the output keep the input values, as in stepgrid.f
     rc = put_grid(L, 0, 1, q_in);
@@ -606,7 +967,7 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out, double dx
     if (rc) return rc;
     if (want_flux) {
         std::vector<double> h(4 * arr);
-        HIPCHK(hipMemcpy(h.data(), d_flux, 4 * arr * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(d2h(h.data(), d_flux, 4 * arr * sizeof(double)));
         double* outs[4] = {fm, fp, gm, gp};
         for (int k = 0; k < 4; k++)
             for (int64_t c = 0; c < plane; c++)
@@ -692,6 +1053,7 @@ int amrgpu_set_geometry(int mxnest, const int* iregsz, const int* jregsz, const 
     }
     g.mxnest = mxnest;
     for (auto& L : g.levels) free_plan(L);
+    g.regs_valid = false;
     return AMRGPU_OK;
 }
 
@@ -708,6 +1070,76 @@ int amrgpu_bound_level(int level, double time) {
     if (rc) return rc;
     if (level < 1 || level > g.mxnest) return fail(AMRGPU_ERR_ARG, "level %d outside 1..mxnest", level);
     return bound_level(level, time);
+}
+
+int amrgpu_advance_level(int level, double* dtlevnew, double* cfl_level) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (level < 1 || level > g.mxnest) return fail(AMRGPU_ERR_ARG, "level %d outside 1..mxnest", level);
+    return advance_level(level, dtlevnew, cfl_level);
+}
+
+int amrgpu_update_level(int level) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (level < 1 || level > g.mxnest) return fail(AMRGPU_ERR_ARG, "level %d outside 1..mxnest", level);
+    return update_level(level);
+}
+
+int amrgpu_conck(int level, double* totmass) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (!totmass) return fail(AMRGPU_ERR_ARG, "totmass is NULL");
+    Level& L = g.levels[level];
+    *totmass = 0.0;
+    if (L.ngrids == 0) return AMRGPU_OK;
+    hipError_t e = launch_conck((int)g.esize, L.d_patches, L.ngrids, L.buf[L.cur], L.d_partial, g.stream);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "conck: %s", hipGetErrorString(e));
+    std::vector<double> h(L.ngrids);
+    HIPCHK(hipMemcpyAsync(h.data(), L.d_partial, L.ngrids * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    double tot = 0.0;
+    for (double v : h) tot = tot + v;
+    *totmass = tot * L.hx * L.hy;                                   // conck.f:63
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_corners(int level, const double* xlo, const double* ylo) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Level& L = g.levels[level];
+    if ((L.ngrids > 0 && (!xlo || !ylo))) return fail(AMRGPU_ERR_ARG, "bad corner arrays");
+    L.cxlo.assign(xlo, xlo + L.ngrids);
+    L.cylo.assign(ylo, ylo + L.ngrids);
+    return AMRGPU_OK;
+}
+
+int amrgpu_flag_level(int level, double tolsp, double start_time, int nregions,
+                      const double* regions, int mbuff, double* flags_base,
+                      const int64_t* offsets, int64_t* nflagged) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (level < 1 || level > g.mxnest) return fail(AMRGPU_ERR_ARG, "level %d outside 1..mxnest", level);
+    if (nregions < 0 || (nregions > 0 && !regions) || mbuff < NGHOST || mbuff > 64 ||
+        (flags_base && !offsets))
+        return fail(AMRGPU_ERR_ARG, "bad flag_level arguments");
+    return flag_level(level, tolsp, start_time, nregions, regions, mbuff, flags_base, offsets, nflagged);
+}
+
+int amrgpu_get_registers(int level, int grid, double* svdflx_qc1d, int* lenbc) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Level& L = g.levels[level];
+    if (grid < 0 || grid >= L.ngrids || !svdflx_qc1d || !lenbc)
+        return fail(AMRGPU_ERR_ARG, "bad get_registers arguments");
+    rc = ensure_regs();
+    if (rc) return rc;
+    if (!L.d_reg) return fail(AMRGPU_ERR_STATE, "level %d has no coarse-fine registers", level);
+    const PatchDesc& pd = L.patches[grid];
+    *lenbc = pd.lenbc;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(d2h(svdflx_qc1d, L.d_reg + pd.reg_off, 2 * (size_t)g.cfg.nvar * pd.lenbc * sizeof(double)));
+    return AMRGPU_OK;
 }
 
 static_assert(sizeof(amrgpu_fill_op) == sizeof(FillOp), "fill op layout");
@@ -754,6 +1186,51 @@ int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes, const int* i
     *nbufs = (int)g_inspect.bufs.size();
     *nmask = (int64_t)g_inspect.masks.size();
     *nscratch = g_inspect.scratch_elems;
+    return AMRGPU_OK;
+}
+
+static std::vector<SvEntry> g_inspect_sv;
+
+int amrgpu_plan_listbc(int nlev, const int* ngrids, const int* boxes, const int* iregsz,
+                       const int* jregsz, const int* intratx, const int* intraty,
+                       const int* mthbc, int nvar, int level, int grid, int* list5,
+                       int maxn, int* n) {
+    if (nlev < 1 || nlev >= FILL_MAXLV || level < 1 || level >= nlev || !ngrids || !boxes ||
+        !iregsz || !jregsz || !intratx || !intraty || !mthbc || nvar < 1 || !n || (maxn > 0 && !list5))
+        return fail(AMRGPU_ERR_ARG, "bad plan_listbc arguments");
+    HierGeom H;
+    H.nlev = nlev;
+    int nb = 0;
+    for (int l = 1; l <= nlev; l++) {
+        LevelGeom& G = H.lev[l];
+        for (int k = 0; k < ngrids[l - 1]; k++, nb++)
+            G.boxes.push_back({boxes[4 * nb], boxes[4 * nb + 1], boxes[4 * nb + 2], boxes[4 * nb + 3]});
+        G.iregsz = iregsz[l - 1];
+        G.jregsz = jregsz[l - 1];
+        G.ratx = intratx[l - 1];
+        G.raty = intraty[l - 1];
+        G.hx = G.hy = 1.0;
+    }
+    H.xperdom = mthbc[0] == 2 && mthbc[1] == 2;
+    H.yperdom = mthbc[2] == 2 && mthbc[3] == 2;
+    if (grid < 0 || grid >= (int)H.lev[level].boxes.size()) return fail(AMRGPU_ERR_ARG, "grid out of range");
+    std::vector<RegLevel> R(FILL_MAXLV);
+    std::string err;
+    if (!plan_registers(H, nvar, R.data(), err)) return fail(AMRGPU_ERR_ARG, "registers: %s", err.c_str());
+    const RegLevel& Rl = R[level];
+    *n = 0;
+    if (Rl.sv_grid_start.empty()) return AMRGPU_OK;
+    for (int e = Rl.sv_grid_start[grid]; e < Rl.sv_grid_start[grid + 1]; e++, (*n)++) {
+        if (*n >= maxn) continue;
+        const SvEntry& s = Rl.sv[e];
+        // the reference's listbc(5, .) with i, j counted from the first ghost cell
+        int* o = list5 + 5 * (*n);
+        o[0] = s.I + NGHOST;
+        o[1] = s.J + NGHOST;
+        o[2] = s.side;
+        o[3] = s.kid;
+        o[4] = s.lkid;
+    }
     return AMRGPU_OK;
 }
 

@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
         const PatchDesc sp = a.lvl_patches[op.src_level][op.src];
         const S* qn = static_cast<const S*>(a.lvl_new[op.src_level]) + sp.off;
         const S* qo = static_cast<const S*>(a.lvl_old[op.src_level]) + sp.off;
-        const int mode = a.tmode[op.src_level];
+        const int mode = (op.flags & FOPF_FORCE_NEW) ? (int)TIME_NEW : a.tmode[op.src_level];
         const double ai = a.alphai[op.src_level], ac = a.alphac[op.src_level];
         for (int t = tid; t < ncell * nv; t += 256) {
             const int m = t / ncell, c = t - m * ncell;
@@ -109,6 +109,24 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
             const int m = t / ncell, c = t - m * ncell;
             const int dj = c / op.ni, di = c - dj * op.ni;
             dst.store(m, op.di0 + di, op.dj0 + dj, src.load(m, di, dj));
+        }
+    } else if (op.type == FOP_RING) {
+        // cstore.f: sides 1 (left column), 2 (top row), 3 (right column),
+        // 4 (bottom row) of the coarsened patch, corners excluded
+        const BufRef<S> src = bufref<S>(a, op.src);
+        const PatchDesc pf = a.lvl_patches[op.level][op.si0];
+        double* qc1d = a.lvl_reg[op.level] + pf.reg_off + (int64_t)nv * pf.lenbc;
+        const int nrow = op.ni, ncol = op.nj;
+        const int n1 = ncol - 2, n2 = nrow - 2;
+        const int len = 2 * (n1 + n2);
+        for (int t = tid; t < len * nv; t += 256) {
+            const int idx = t / nv, m = t - idx * nv;
+            int i, j;
+            if (idx < n1) { i = 0; j = 1 + idx; }
+            else if (idx < n1 + n2) { i = 1 + idx - n1; j = ncol - 1; }
+            else if (idx < 2 * n1 + n2) { i = nrow - 1; j = 1 + idx - n1 - n2; }
+            else { i = 1 + idx - 2 * n1 - n2; j = 0; }
+            qc1d[(int64_t)idx * nv + m] = src.load(m, i, j);
         }
     } else if (op.type == FOP_BC) {
         // sides in the reference's order, each complete before the next

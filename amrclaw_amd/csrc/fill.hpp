@@ -20,8 +20,11 @@ enum FillOpType : int32_t {
     FOP_COPY = 0,      // intfil: copy / time-blend from a grid (intfil.f90:237-259)
     FOP_INTERP = 1,    // filrecur: limited linear interpolation (filpatch.f90:260-327)
     FOP_BC = 2,        // bc2amr on a whole buffer (bc2amr.f90:84-296)
-    FOP_COPYBACK = 3   // prefilrecur: patchCopyOut (prefilp.f90:218-245)
+    FOP_COPYBACK = 3,  // prefilrecur: patchCopyOut (prefilp.f90:218-245)
+    FOP_RING = 4       // cstore: coarse perimeter ring of a buffer -> qc1d (cstore.f:4-63)
 };
+// FillOp.flags bits
+enum : int32_t { FOPF_FORCE_NEW = 1 };   // icall copies store1, no time blend (icall.f)
 
 // A destination/scratch array, SoA over components: cell (i, j), 0-based
 // with ghosts, component m at off + m*plane + j*nrow + i.
@@ -48,7 +51,8 @@ struct FillOp {
     int32_t level;            // INTERP: fine level
     int32_t bc[4];            // BC: nxl, ibeg, nyb, jbeg (0 = side not applied)
     int32_t stage;
-    int32_t pad[2];
+    int32_t flags;            // FOPF_* bits
+    int32_t pad;
 };
 
 enum TimeMode : int32_t { TIME_NEW = 0, TIME_OLD = 1, TIME_BLEND = 2 };
@@ -61,6 +65,7 @@ struct FillArgs {
     void* lvl_new[FILL_MAXLV];           // level storage, new time (store1)
     void* lvl_old[FILL_MAXLV];           // old time (store2)
     const PatchDesc* lvl_patches[FILL_MAXLV];
+    double* lvl_reg[FILL_MAXLV];          // register arenas (FOP_RING)
     double alphai[FILL_MAXLV], alphac[FILL_MAXLV];
     int32_t tmode[FILL_MAXLV];
     double hx[FILL_MAXLV], hy[FILL_MAXLV];

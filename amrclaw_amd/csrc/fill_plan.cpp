@@ -322,6 +322,129 @@ struct Planner {
         return ready;
     }
 
+    // icall.f: copy the level's grids (store1) over [ilo..ihi]x[jlo..jhi]
+    // into `buf` with (ilo, jlo) at 1-based cell (iputst, jputst)
+    void icall(int level, int buf, int ilo, int ihi, int jlo, int jhi, int iputst, int jputst,
+               std::vector<unsigned char>& covered) {
+        const LevelGeom& G = H.lev[level];
+        const FillBuf fb = P.bufs[buf];
+        for (int k = 0; k < (int)G.boxes.size(); k++) {
+            const auto& b = G.boxes[k];
+            int ixlo = std::max(b[0], ilo), ixhi = std::min(b[1], ihi);
+            int jxlo = std::max(b[2], jlo), jxhi = std::min(b[3], jhi);
+            if (!(ixlo <= ixhi && jxlo <= jxhi)) continue;
+            FillOp op{};
+            op.type = FOP_COPY;
+            op.flags = FOPF_FORCE_NEW;
+            op.dst = buf;
+            op.src = k;
+            op.src_level = level;
+            op.di0 = ixlo - ilo + iputst - 1;
+            op.dj0 = jxlo - jlo + jputst - 1;
+            op.ni = ixhi - ixlo + 1;
+            op.nj = jxhi - jxlo + 1;
+            op.si0 = ixlo - b[0] + NGHOST;
+            op.sj0 = jxlo - b[2] + NGHOST;
+            op.stage = 0;
+            if (op.di0 < 0 || op.dj0 < 0 || op.di0 + op.ni > fb.nrow || op.dj0 + op.nj > fb.ncol) {
+                fail("icall: copy outside its buffer");
+                return;
+            }
+            push(op);
+            for (int j = 0; j < op.nj; j++)
+                for (int i = 0; i < op.ni; i++)
+                    covered[(size_t)(op.dj0 + j) * fb.nrow + (op.di0 + i)] = 1;
+        }
+    }
+
+    // saveqc.f:13-109 for grid k of fine level `level`: coarsened patch
+    // grown by one coarse cell, copied from the coarse grids (preicall.f for
+    // periodic domains), bc2amr, then cstore's ring into the register
+    bool saveqc(int level, int k) {
+        const LevelGeom& F = H.lev[level];
+        const LevelGeom& C = H.lev[level - 1];
+        const int levc = level - 1;
+        const auto& b = F.boxes[k];
+        const int rx = C.ratx, ry = C.raty, ng = NGHOST;
+        const int iclo = b[0] / rx - 1, jclo = b[2] / ry - 1;
+        const int ichi = (b[1] + 1) / rx, jchi = (b[3] + 1) / ry;
+        const int nrow = ichi - iclo + 1, ncol = jchi - jclo + 1;
+        const int S = scratch(nrow, ncol);
+        std::vector<unsigned char> covered((size_t)nrow * ncol, 0);
+        bool sticks = iclo < 0 || ichi == C.iregsz || jclo < 0 || jchi == C.jregsz;
+        if (sticks && (H.xperdom || H.yperdom || H.spheredom)) {
+            int ist[3], iend[3], ishift[3], jst[3], jend[3], jshift[3];
+            if (H.xperdom) {
+                ist[0] = iclo; ist[1] = 0; ist[2] = C.iregsz;
+                iend[0] = -1; iend[1] = C.iregsz - 1; iend[2] = ichi;
+                ishift[0] = C.iregsz; ishift[1] = 0; ishift[2] = -C.iregsz;
+            } else {
+                ist[0] = C.iregsz; ist[1] = iclo; ist[2] = C.iregsz;
+                iend[0] = -ng; iend[1] = ichi; iend[2] = -ng;
+                ishift[0] = ishift[1] = ishift[2] = 0;
+            }
+            if (H.yperdom) {
+                jst[0] = jclo; jst[1] = 0; jst[2] = C.jregsz;
+                jend[0] = -1; jend[1] = C.jregsz - 1; jend[2] = jchi;
+                jshift[0] = C.jregsz; jshift[1] = 0; jshift[2] = -C.jregsz;
+            } else {
+                jst[0] = C.jregsz; jst[1] = jclo; jst[2] = C.jregsz;
+                jend[0] = -ng; jend[1] = jchi; jend[2] = -ng;
+                jshift[0] = jshift[1] = jshift[2] = 0;
+            }
+            for (int a = 0; a < 3; a++) {
+                int i1 = std::max(iclo, ist[a]), i2 = std::min(ichi, iend[a]);
+                if (i1 > i2) continue;
+                for (int c = 0; c < 3; c++) {
+                    int j1 = std::max(jclo, jst[c]), j2 = std::min(jchi, jend[c]);
+                    if (j1 > j2) continue;
+                    icall(levc, S, i1 + ishift[a], i2 + ishift[a], j1 + jshift[c], j2 + jshift[c],
+                          i1 - iclo + 1, j1 - jclo + 1, covered);
+                }
+            }
+        } else {
+            icall(levc, S, iclo, ichi, jclo, jchi, 1, 1, covered);
+        }
+        if (!err.empty()) return false;
+        const double hxc = C.hx, hyc = C.hy;
+        const double xl = (b[0] * F.hx + H.xlower) - hxc, yb = (b[2] * F.hy + H.ylower) - hyc;
+        const double xr = ((b[1] + 1) * F.hx + H.xlower) + hxc, yt = ((b[3] + 1) * F.hy + H.ylower) + hyc;
+        size_t nops0 = P.ops.size();
+        if (!bc_op(S, hxc, hyc, xl, xr, yb, yt, 1)) return false;
+        // every ring cell (the cells cstore reads) must come from a coarse
+        // grid or from the boundary conditions; one left unset means the fine
+        // grid is not properly nested in the coarse level
+        std::vector<unsigned char> set = covered;
+        for (int j = 1; j < ncol - 1; j++)
+            for (int i = 1; i < nrow - 1; i++) set[(size_t)j * nrow + i] = 1;
+        set[0] = set[nrow - 1] = set[(size_t)(ncol - 1) * nrow] = set[(size_t)ncol * nrow - 1] = 1;
+        if (P.ops.size() > nops0) {
+            const FillOp& bc = P.ops.back();
+            auto mark = [&](int i0, int i1, int j0, int j1) {
+                for (int j = j0; j < j1; j++)
+                    for (int i = i0; i < i1; i++) set[(size_t)j * nrow + i] = 1;
+            };
+            if (bc.bc[0]) mark(0, bc.bc[0], 0, ncol);
+            if (bc.bc[1]) mark(bc.bc[1] - 1, nrow, 0, ncol);
+            if (bc.bc[2]) mark(0, nrow, 0, bc.bc[2]);
+            if (bc.bc[3]) mark(0, nrow, bc.bc[3] - 1, ncol);
+        }
+        for (int j = 0; j < ncol; j++)
+            for (int i = 0; i < nrow; i++)
+                if (!set[(size_t)j * nrow + i]) return fail("saveqc: fine grid not properly nested in the coarse level");
+        FillOp ring{};
+        ring.type = FOP_RING;
+        ring.dst = S;
+        ring.src = S;
+        ring.level = level;
+        ring.si0 = k;
+        ring.ni = nrow;
+        ring.nj = ncol;
+        ring.stage = 2;
+        push(ring);
+        return true;
+    }
+
     // bound.f90:51-158 for grid `grid` of `level`
     bool bound(int level, int grid) {
         const LevelGeom& G = H.lev[level];
@@ -361,6 +484,27 @@ struct Planner {
 
 }  // namespace
 
+static void finish_plan(FillPlan& P) {
+    std::stable_sort(P.ops.begin(), P.ops.end(),
+                     [](const FillOp& a, const FillOp& b) { return a.stage < b.stage; });
+    P.nstages = P.ops.empty() ? 0 : P.ops.back().stage + 1;
+    P.stage_start.assign(P.nstages + 1, 0);
+    for (const auto& op : P.ops) P.stage_start[op.stage + 1]++;
+    for (int s = 0; s < P.nstages; s++) P.stage_start[s + 1] += P.stage_start[s];
+}
+
+bool plan_saveqc(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err) {
+    P = FillPlan();
+    Planner pl(H, P, nvar);
+    for (int k = 0; k < (int)H.lev[level].boxes.size(); k++)
+        if (!pl.saveqc(level, k)) {
+            err = pl.err.empty() ? "saveqc planning failed" : pl.err;
+            return false;
+        }
+    finish_plan(P);
+    return true;
+}
+
 bool plan_fill_level(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err) {
     P = FillPlan();
     Planner pl(H, P, nvar);
@@ -369,12 +513,7 @@ bool plan_fill_level(const HierGeom& H, int level, int nvar, FillPlan& P, std::s
             err = pl.err.empty() ? "ghost-fill planning failed" : pl.err;
             return false;
         }
-    std::stable_sort(P.ops.begin(), P.ops.end(),
-                     [](const FillOp& a, const FillOp& b) { return a.stage < b.stage; });
-    P.nstages = P.ops.empty() ? 0 : P.ops.back().stage + 1;
-    P.stage_start.assign(P.nstages + 1, 0);
-    for (const auto& op : P.ops) P.stage_start[op.stage + 1]++;
-    for (int s = 0; s < P.nstages; s++) P.stage_start[s + 1] += P.stage_start[s];
+    finish_plan(P);
     return true;
 }
 

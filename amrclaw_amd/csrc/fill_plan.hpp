@@ -37,5 +37,7 @@ struct FillPlan {
 };
 
 bool plan_fill_level(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
+// saveqc.f for the grids of fine level `level` (coarse level `level`-1)
+bool plan_saveqc(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
 
 }  // namespace amrgpu

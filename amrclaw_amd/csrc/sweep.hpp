@@ -38,7 +38,25 @@ struct SweepArgs {
     unsigned long long* cfl_patch;  // per-patch max Courant number (double bits)
     double* flux_out;               // optional: fm,fp,gm,gp per patch (tests)
     const int64_t* flux_off;        // element offset of each patch in flux_out
+    // coarse-fine registers (FP64).  reg: this level's arena when the level
+    // is a fine level; each tile on a grid edge stores that edge's fluxes in
+    // the grid's perimeter block (after svdflx and qc1d).  sv / reg_fine /
+    // fine_patches: when a finer level exists, the fluxsv entries of each
+    // tile (TileDesc.reg_lo..reg_hi) write -flux*dt*h into the finer level's
+    // register slots.
+    double* reg;
+    double* reg_fine;
+    const PatchDesc* fine_patches;
+    const SvEntry* sv;
 };
+
+// Perimeter flux block of a fine grid (FP64, after svdflx and qc1d):
+// side 0 left fm at x-edge 1 (rows 1..my), side 1 top gp at y-edge my+1
+// (columns 1..mx), side 2 right fp at x-edge mx+1, side 3 bottom gm at
+// y-edge 1; component fastest.  These are the fluxes fluxad.f:36-92 adds.
+__host__ __device__ inline int64_t perim_offset(const PatchDesc& pd, int nv) {
+    return pd.reg_off + 2 * (int64_t)nv * pd.lenbc;
+}
 
 template <typename T>
 __device__ __forceinline__ T lane_down(T v, int w) { return __shfl_down(v, 1, w); }
@@ -333,6 +351,25 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
                 qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
             }
         }
+        // ghost cells on the patch edges keep their time-t values in the new
+        // array, as stepgrid.f:154-165 updates q in place over the interior
+        // only; each ghost cell is written by the one tile whose range,
+        // extended by the ghost width on its patch-edge sides, holds it
+        {
+            const int xlo = td.i0 == 0 ? -2 : 0, xhi = td.i0 + tx == pd.mx ? tx + 2 : tx;
+            const int ylo = td.j0 == 0 ? -2 : 0, yhi = td.j0 + ty == pd.my ? ty + 2 : ty;
+            if (xlo != 0 || xhi != tx || ylo != 0 || yhi != ty) {
+                const int w = xhi - xlo, hgt = yhi - ylo;
+                for (int idx = tid; idx < w * hgt; idx += 256) {
+                    const int j = ylo + idx / w, k = xlo + idx % w;
+                    if (k >= 0 && k < tx && j >= 0 && j < ty) continue;
+                    const int li = (td.j0 + j + 2) * pd.mitot + (td.i0 + k + 2);
+#pragma unroll
+                    for (int m = 0; m < NV; m++)
+                        qout[(int64_t)m * pd.plane + li] = S(qs[(m * qsy + j + 2) * qsx + k + 2]);
+                }
+            }
+        }
         if (a.flux_out) {
             // fm,fp,gm,gp of the patch, each (NV, my+4, mx+4) like the
             // reference's flux arrays: x-edge e of row j -> fm(e, j)
@@ -351,6 +388,51 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
                 fo[2 * arr + li] = double(gm[idx]);
                 fo[3 * arr + li] = double(gp[idx]);
             }
+        }
+    }
+    // 5. coarse-fine registers (reads the final fluxes in LDS only)
+    if (a.reg) {
+        double* per = a.reg + perim_offset(pd, NV);
+        if (td.i0 == 0)
+            for (int idx = tid; idx < NV * ty; idx += 256) {
+                const int j = idx / NV, m = idx - j * NV;
+                per[(int64_t)(td.j0 + j) * NV + m] = double(fm[m * fpl + j * fstr]);
+            }
+        if (td.i0 + tx == pd.mx)
+            for (int idx = tid; idx < NV * ty; idx += 256) {
+                const int j = idx / NV, m = idx - j * NV;
+                per[(int64_t)NV * (pd.my + pd.mx) + (int64_t)(td.j0 + j) * NV + m] =
+                    double(fp[m * fpl + j * fstr + tx]);
+            }
+        if (td.j0 + ty == pd.my)
+            for (int idx = tid; idx < NV * tx; idx += 256) {
+                const int k = idx / NV, m = idx - k * NV;
+                per[(int64_t)NV * pd.my + (int64_t)(td.i0 + k) * NV + m] = double(gp[m * gpl + ty * tx + k]);
+            }
+        if (td.j0 == 0)
+            for (int idx = tid; idx < NV * tx; idx += 256) {
+                const int k = idx / NV, m = idx - k * NV;
+                per[(int64_t)NV * (2 * pd.my + pd.mx) + (int64_t)(td.i0 + k) * NV + m] =
+                    double(gm[m * gpl + k]);
+            }
+    }
+    if (a.sv) {
+        // fluxsv.f:43-75: slot = -flux * dtc * h of the coarse cell's side
+        const double dtc = a.dt;
+        for (int e = td.reg_lo + tid; e < td.reg_hi; e += 256) {
+            const SvEntry s = a.sv[e];
+            const int k = s.I - 1 - td.i0, j = s.J - 1 - td.j0;
+            double* slot = a.reg_fine + a.fine_patches[s.kid].reg_off + (int64_t)NV * (s.lkid - 1);
+            // the flux through the side of the coarse cell that faces the
+            // fine grid; side chosen once, then one strided read per component
+            const T* src;
+            int stride;
+            double h;
+            if (s.side == 1) { src = fp + j * fstr + k; stride = fpl; h = a.dy; }
+            else if (s.side == 2) { src = gm + (j + 1) * tx + k; stride = gpl; h = a.dx; }
+            else if (s.side == 3) { src = fm + j * fstr + k + 1; stride = fpl; h = a.dy; }
+            else { src = gp + j * tx + k; stride = gpl; h = a.dx; }
+            for (int m = 0; m < NV; m++) slot[m] = -double(src[m * stride]) * dtc * h;
         }
     }
     cfl = wave_max(cfl);

@@ -19,6 +19,17 @@ struct PatchDesc {
     int32_t plane;      // mitot * mjtot (component stride)
     int32_t ilo, jlo;   // global index of the first interior cell (level space)
     int32_t ihi, jhi;
+    int64_t reg_off;    // fine grids: offset of svdflx(nvar, lenbc) in the level's
+                        // register arena; qc1d(nvar, lenbc) follows it
+    int32_t lenbc;      // 2 (mx/r + my/r) coarse segments around the grid (prepf.f:35-39)
+    int32_t pad;
+};
+
+// fluxsv.f entry of a coarse grid: its interior cell (I, J) (1-based),
+// side 1..4 of that cell, fine grid `kid` of the next level and the slot
+// `lkid` (1-based) of the kid's register (setuse.f:61-213)
+struct SvEntry {
+    int32_t I, J, side, kid, lkid, pad;
 };
 
 // One workgroup's piece of a patch for the sweep.  Interior cells

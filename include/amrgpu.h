@@ -105,6 +105,56 @@ int amrgpu_set_time(int level, double time, double possk);
  * layout change and replayed on the device afterwards. */
 int amrgpu_bound_level(int level, double time);
 
+/* One time step of every grid of a level, the body of advanc.f:16-305:
+ * bound at the level's time; saveqc.f:13-109 for the next finer level (its
+ * registers receive the coarse values around each fine grid); the sweep of
+ * stepgrid.f for all grids with fluxsv.f (the coarse fluxes through the
+ * edges of the finer grids into their registers) and, for a fine level,
+ * qad.f + fluxad.f (the Riemann fix-up and the fine fluxes along the grid
+ * perimeters); the level's time advances by possk.  Returns the smallest
+ * new time step over the level's grids (stepgrid.f:206-212,
+ * advanc.f:124-126) and the largest Courant number.  The register layout
+ * (prepf.f/prepc.f) is rebuilt after any change of the hierarchy. */
+int amrgpu_advance_level(int level, double* dtlevnew, double* cfl_level);
+
+/* update.f:16-189 for coarse level `level`: upbnd.f's conservative reflux
+ * from the registers of level+1 into the coarse cells next to the fine
+ * grids, then the coarse cells under level+1 set to the mean of the fine
+ * cells they cover. */
+int amrgpu_update_level(int level);
+
+/* conck.f:36-63: sum of component 1 over the interior of all grids of
+ * `level`, times hx*hy.  Summed on the device in a fixed tree order, so it
+ * agrees with the reference's sequential sum to rounding. */
+int amrgpu_conck(int level, double* totmass);
+
+/* Lower-left corners of the grids of a level, rnode(cornxlo/cornylo)
+ * (grdfit2.f:98-101, birect.f:67), used where the reference computes
+ * positions from them (flagregions2.f90:61-92).  Default after
+ * amrgpu_set_level: ilo*hx + xlower, jlo*hy + ylower. */
+int amrgpu_set_corners(int level, const double* xlo, const double* ylo);
+
+/* Flag the cells of every grid of `level` as flagger.f:56-164 does without
+ * Richardson estimation: the flags (mx+2*mbuff) x (my+2*mbuff) of each grid
+ * (FP64: -1 UNSET, 0 DONTFLAG, 2 DOFLAG; amr_module.f90:163-168) are set
+ * UNSET, then flagregions2.f90:40-120 applies the regions (nregions
+ * records of 8 doubles in regions.data order: min_level, max_level, t_low,
+ * t_hi, x_low, x_hi, y_low, y_hi), then flag2refine2.f90:96-122 flags
+ * UNSET interior cells whose undivided difference exceeds tolsp.  Like
+ * flagger.f:98, the new values are used after a ghost fill at the level's
+ * time unless start_time + possk equals that time exactly (then the old
+ * values).  With flags_base non-NULL each grid's flags are copied to
+ * flags_base + offsets[k] (node(storeflags)); *nflagged (optional) gets the
+ * number of DOFLAG cells. */
+int amrgpu_flag_level(int level, double tolsp, double start_time, int nregions,
+                      const double* regions, int mbuff, double* flags_base,
+                      const int64_t* offsets, int64_t* nflagged);
+
+/* The registers of grid `grid` of fine level `level`: svdflx(nvar, lenbc)
+ * followed by qc1d(nvar, lenbc), component fastest, as the reference's
+ * node(ffluxptr) block (prepf.f:33-48).  For tests and debugging. */
+int amrgpu_get_registers(int level, int grid, double* svdflx_qc1d, int* lenbc);
+
 /* Ghost-fill program inspection (host only; needs no device and no init).
  * Plans bound for every grid of `level` of the hierarchy described by the
  * arguments (levels 1..nlev; grid boxes concatenated level by level as
@@ -112,7 +162,8 @@ int amrgpu_bound_level(int level, double time);
  * domain = xlower, xupper, ylower, yupper) and reports its size; the
  * records are then copied out with amrgpu_plan_fill_get.  Operation types:
  * 0 copy/time-blend from a grid, 1 coarse-fine interpolation, 2 boundary
- * conditions on a whole buffer, 3 copy of a periodic piece.  Buffers of
+ * conditions on a whole buffer, 3 copy of a periodic piece (4, the saved
+ * coarse ring of the flux registers, only appears in saveqc programs).  Buffers of
  * kind 0 are grids of the level store, kind 1 FP64 scratch arrays. */
 typedef struct amrgpu_fill_op {
     int type, dst, src, src_level;
@@ -122,7 +173,8 @@ typedef struct amrgpu_fill_op {
     int ilo, jlo, iplo, jplo, ratx, raty, level;
     int bc[4];
     int stage;
-    int pad[2];
+    int flags;      /* 1: copy the new-time values, no time blend (icall.f) */
+    int pad;
 } amrgpu_fill_op;
 typedef struct amrgpu_fill_buf {
     int kind, level, patch, nrow, ncol, plane, pad;
@@ -136,6 +188,17 @@ int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes,
                      int* nbufs, int64_t* nmask, int64_t* nscratch);
 int amrgpu_plan_fill_get(amrgpu_fill_op* ops, amrgpu_fill_buf* bufs,
                          unsigned char* masks);
+
+/* Register-list inspection (host only; needs no device and no init): the
+ * listbc entries (prepc.f:47-154 with setuse.f:61-213) of grid `grid` of
+ * coarse level `level` for the hierarchy described as in amrgpu_plan_fill,
+ * five ints each (i, j counted from the first ghost cell, side, kid
+ * numbered from 0 within level+1, slot), at most maxn written; *n is the
+ * full count. */
+int amrgpu_plan_listbc(int nlev, const int* ngrids, const int* boxes,
+                       const int* iregsz, const int* jregsz, const int* intratx,
+                       const int* intraty, const int* mthbc, int nvar, int level,
+                       int grid, int* list5, int maxn, int* n);
 
 /* Define the grids of one level (after regrid): global index boxes
  * [ilo..ihi] x [jlo..jhi] and mesh widths hx, hy (hxposs/hyposs).

@@ -165,6 +165,19 @@ int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level);
 /* conck.f:6-73: hx*hy*sum of q(1) over the interiors of the level's grids */
 double orc_conck(const orc_params* p, const orc_hier* h, int level);
 
+/* cell flagging (orc_flag.c) */
+#define ORC_UNSET (-1.0)
+#define ORC_DONTFLAG 0.0
+#define ORC_DOFLAG 2.0
+int orc_flagregions2(int mx, int my, int mbuff, double xlower, double ylower, double dx,
+                     double dy, int level, double t, const double* regions, int nreg,
+                     double* amrflags);
+void orc_flag2refine2(int mx, int my, int mbc, int mbuff, int meqn, double tolsp, const double* q,
+                      double* amrflags);
+int orc_flag_level(const orc_params* p, const orc_hier* h, int level, double tolsp,
+                   const double* regions, int nreg, int use_old, int mbuff, double* flags,
+                   const long* foff);
+
 /* amr_module.f90 declares these real(kind=8) parameters from default-real
  * (single precision) literals 10.e32 and 10.e33, so their values are the
  * float roundings widened to double. */

@@ -456,3 +456,46 @@ class RefAMR:
         t = C.c_double()
         self.R.ref_get_time(level, k, C.byref(t))
         return t.value
+
+
+# ---------------------------------------------------------------- flagging
+UNSET, DONTFLAG, DOFLAG = -1.0, 0.0, 2.0
+
+
+def flag_grid(q, mx, my, mbuff, xlower, ylower, dx, dy, t, level, tolsp, regions=()):
+    """Oracle flags of one grid: UNSET, flagregions2, flag2refine2.  q is the
+    grid's AoS array (my+4, mx+4, meqn); regions rows are (min_level,
+    max_level, t_low, t_hi, x_low, x_hi, y_low, y_hi).  Returns flags of
+    shape (my + 2 mbuff, mx + 2 mbuff)."""
+    L = lib()
+    L.orc_flagregions2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, C.c_double, _dp, C.c_int, _dp]
+    L.orc_flag2refine2.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp]
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    meqn = q.shape[-1]
+    f = np.full((my + 2 * mbuff) * (mx + 2 * mbuff), UNSET)
+    reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1), dtype=np.float64)
+    if reg.size == 0:
+        reg = np.zeros(8)
+    rc = L.orc_flagregions2(mx, my, mbuff, xlower, ylower, dx, dy, level, t, reg, len(regions), f)
+    if rc:
+        raise RuntimeError("orc_flagregions2 error %d" % rc)
+    L.orc_flag2refine2(mx, my, 2, mbuff, meqn, tolsp, q.reshape(-1), f)
+    return f.reshape(my + 2 * mbuff, mx + 2 * mbuff)
+
+
+def ref_flag_grid(q, mx, my, mbuff, xlower, ylower, dx, dy, t, level, tolsp, regions=()):
+    """The reference's flagregions2 + flag2refine2 on the same inputs."""
+    R = ref()
+    R.ref_flag_grid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _dp,
+                                C.c_int, _dp, _dp]
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    meqn = q.shape[-1]
+    f = np.zeros((my + 2 * mbuff) * (mx + 2 * mbuff))
+    reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1), dtype=np.float64)
+    if reg.size == 0:
+        reg = np.zeros(8)
+    R.ref_flag_grid(mx, my, mbuff, meqn, xlower, ylower, dx, dy, t, level, tolsp,
+                    q.reshape(-1), len(regions), reg, f)
+    return f.reshape(my + 2 * mbuff, mx + 2 * mbuff)

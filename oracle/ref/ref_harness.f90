@@ -386,6 +386,37 @@ contains
     t = rnode(timemult, listOfGrids(listStart(l) + k))
   end subroutine ref_get_time
 
+  ! flagger.f:131-164 for one grid without Richardson estimation:
+  ! flags = UNSET, flagregions2, flag2refine2 on q (ghost cells set)
+  subroutine ref_flag_grid(mx, my, mbuff, meqn, xlower, ylower, dx, dy, t, level, tolsp, &
+                           q, nreg, regs, flags) bind(C, name="ref_flag_grid")
+    use regions_module, only: num_regions, regions
+    integer(c_int), value :: mx, my, mbuff, meqn, level, nreg
+    real(c_double), value :: xlower, ylower, dx, dy, t, tolsp
+    real(c_double) :: q(meqn, 1-2:mx+2, 1-2:my+2), regs(8, *)
+    real(c_double) :: flags(1-mbuff:mx+mbuff, 1-mbuff:my+mbuff)
+    real(c_double) :: aux(1, 1-2:mx+2, 1-2:my+2)
+    integer :: m
+    if (allocated(regions)) deallocate(regions)
+    num_regions = nreg
+    allocate(regions(max(nreg, 1)))
+    do m = 1, nreg
+       regions(m)%min_level = int(regs(1, m))
+       regions(m)%max_level = int(regs(2, m))
+       regions(m)%t_low = regs(3, m)
+       regions(m)%t_hi = regs(4, m)
+       regions(m)%x_low = regs(5, m)
+       regions(m)%x_hi = regs(6, m)
+       regions(m)%y_low = regs(7, m)
+       regions(m)%y_hi = regs(8, m)
+    end do
+    aux = 0.d0
+    flags = UNSET
+    call flagregions2(mx, my, mbuff, xlower, ylower, dx, dy, level, t, flags)
+    call flag2refine2(mx, my, 2, mbuff, meqn, 0, xlower, ylower, dx, dy, t, level, tolsp, &
+                      q, aux, flags)
+  end subroutine ref_flag_grid
+
 end module ref_harness
 
 ! listbc lives in the double heap as integers (prepc.f:55-59, setuse.f:95);

@@ -1,0 +1,85 @@
+"""CPU pin of the oracle's cell flagging (flagregions2 + flag2refine2, as
+flagger.f calls them) against the reference's own Fortran, plus closed-form
+checks of the undivided-difference test and of the region rules."""
+import numpy as np
+import pytest
+
+from oracle import orc
+
+needs_ref = pytest.mark.skipif(not orc.ref_available(), reason="reference build absent")
+
+
+def random_regions(rng, n, t):
+    out = []
+    for _ in range(n):
+        x0, x1 = np.sort(rng.uniform(-0.2, 1.2, 2))
+        y0, y1 = np.sort(rng.uniform(-0.2, 1.2, 2))
+        lo = int(rng.integers(0, 4))
+        hi = lo + int(rng.integers(0, 3))
+        t0 = t - rng.uniform(-0.1, 0.5)
+        out.append((lo, hi, t0, t0 + rng.uniform(0, 1), x0, x1, y0, y1))
+    return out
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(12))
+def test_flags_match_reference(seed):
+    rng = np.random.default_rng(900 + seed)
+    mx, my = (int(v) for v in rng.integers(2, 40, 2))
+    meqn = int(rng.choice([1, 3, 4]))
+    mbuff = int(rng.choice([2, 3, 4]))
+    q = rng.normal(size=(my + 4, mx + 4, meqn))
+    if seed % 3 == 0:   # piecewise constant data: exact zeros and exact jumps
+        q = np.round(q)
+    dx, dy = 1.0 / (mx * 3), 1.0 / (my * 2)
+    xlo, ylo = rng.uniform(0, 0.5), rng.uniform(0, 0.5)
+    t = rng.uniform(0, 1)
+    level = int(rng.integers(1, 4))
+    tol = float(rng.choice([0.5, 1.0, 2.0]))
+    regs = random_regions(rng, int(rng.integers(0, 5)), t)
+    a = orc.flag_grid(q, mx, my, mbuff, xlo, ylo, dx, dy, t, level, tol, regs)
+    b = orc.ref_flag_grid(q, mx, my, mbuff, xlo, ylo, dx, dy, t, level, tol, regs)
+    assert np.array_equal(a, b)
+
+
+def test_step_is_flagged_on_both_sides():
+    """A jump of 1 between columns 5 and 6 flags exactly columns 5 and 6
+    (centred difference |q(i+1)-q(i-1)| = 1 > tol) and nothing else; the
+    buffer zone stays UNSET."""
+    mx, my, mbuff = 12, 6, 3
+    q = np.zeros((my + 4, mx + 4, 1))
+    q[:, 2 + 5:, 0] = 1.0          # cells i >= 6 (1-based) hold 1
+    f = orc.flag_grid(q, mx, my, mbuff, 0.0, 0.0, 0.1, 0.1, 0.0, 1, 0.5)
+    inner = f[mbuff:-mbuff, mbuff:-mbuff]
+    want = np.zeros_like(inner) + orc.UNSET
+    want[:, 4:6] = orc.DOFLAG      # 1-based i = 5, 6
+    assert np.array_equal(inner, want)
+    assert np.all(f[:mbuff] == orc.UNSET) and np.all(f[:, :mbuff] == orc.UNSET)
+
+
+def test_tolerance_is_strict():
+    """dq == tolsp does not flag (flag2refine2.f90:113 uses >)."""
+    mx, my = 6, 6
+    q = np.zeros((my + 4, mx + 4, 1))
+    q[:, 2 + 3:, 0] = 0.25
+    f = orc.flag_grid(q, mx, my, 2, 0.0, 0.0, 0.1, 0.1, 0.0, 1, 0.25)
+    assert np.all(f[2:-2, 2:-2] == orc.UNSET)
+
+
+def test_region_rules():
+    """Inside a region with min_level 3: level 2 cells are DOFLAG whatever
+    the data; a region with max_level 2 forbids refinement of level 2
+    (DONTFLAG) even across a jump; outside regions the gradient decides."""
+    mx, my, mb = 10, 10, 2
+    q = np.zeros((my + 4, mx + 4, 1))
+    q[:, 2 + 5:, 0] = 5.0
+    h = 0.1
+    # left half: force refinement; right half: forbid it
+    regs = [(3, 4, 0.0, 1.0, 0.0, 0.45, 0.0, 1.0), (0, 2, 0.0, 1.0, 0.55, 1.0, 0.0, 1.0)]
+    f = orc.flag_grid(q, mx, my, mb, 0.0, 0.0, h, h, 0.5, 2, 1.0, regs)[mb:-mb, mb:-mb]
+    assert np.all(f[:, :5] == orc.DOFLAG)         # i1..i2 = 1..5 from floor(0.45/0.1)+1
+    assert np.all(f[:, 6:] == orc.DONTFLAG)       # i = 7..10 (floor(0.55/0.1)+1 = 6 -> 6..10)
+    # a region outside its time window does nothing
+    f2 = orc.flag_grid(q, mx, my, mb, 0.0, 0.0, h, h, 2.0, 2, 1.0, regs)[mb:-mb, mb:-mb]
+    assert np.array_equal(f2[:, 4:6], np.full((my, 2), orc.DOFLAG))
+    assert np.all(f2[:, :4] == orc.UNSET) and np.all(f2[:, 6:] == orc.UNSET)
