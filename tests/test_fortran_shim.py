@@ -1,0 +1,34 @@
+"""The Fortran side of the drop-in boundary compiles against the
+reference's amr_module, defines advanc/update with the reference's names,
+and every amrgpu_* routine it binds is exported by libamrgpu.so."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SHIM = os.path.join(ROOT, "amrclaw_amd", "fortran", "amrgpu_shim.f90")
+MODS = os.path.join(ROOT, "oracle", "_ref", "obj")
+LIB = os.path.join(ROOT, "amrclaw_amd", "libamrgpu.so")
+FC = shutil.which("amdflang") or "/opt/rocm/llvm/bin/amdflang"
+
+
+@pytest.mark.skipif(not os.path.exists(FC) or not os.path.exists(os.path.join(MODS, "amr_module.mod")),
+                    reason="amdflang or the reference module build absent")
+def test_shim_compiles_and_binds_exported_symbols(tmp_path):
+    obj = tmp_path / "shim.o"
+    r = subprocess.run([FC, "-c", "-I" + MODS, SHIM, "-o", str(obj), "-module-dir", str(tmp_path)],
+                       capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    syms = subprocess.run(["nm", str(obj)], capture_output=True, text=True).stdout.split("\n")
+    defined = {l.split()[-1] for l in syms if " T " in l}
+    undefined = {l.split()[-1] for l in syms if " U " in l}
+    assert "advanc_" in defined and "update_" in defined
+    bound = {s for s in undefined if s.startswith("amrgpu_")}
+    assert {"amrgpu_advance_level", "amrgpu_update_level", "amrgpu_put_level",
+            "amrgpu_set_level"} <= bound
+    exported = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
+                              text=True).stdout
+    for s in bound:
+        assert (" " + s + "\n") in exported, s
