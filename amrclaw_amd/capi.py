@@ -89,6 +89,10 @@ def lib():
         L.amrgpu_conck.argtypes = [C.c_int, C.POINTER(C.c_double)]
         L.amrgpu_get_registers.argtypes = [C.c_int, C.c_int, _dp, C.POINTER(C.c_int)]
         L.amrgpu_set_corners.argtypes = [C.c_int, _dp, _dp]
+        L.amrgpu_set_launch_mode.argtypes = [C.c_int]
+        L.amrgpu_set_graphs.argtypes = [C.c_int]
+        L.amrgpu_set_timing.argtypes = [C.c_int]
+        L.amrgpu_kernel_time.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.amrgpu_flag_level.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p,
                                         C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
         L.amrgpu_event_record.argtypes = [C.c_int]
@@ -222,11 +226,38 @@ def bound_level(level, time):
     _check(lib().amrgpu_bound_level(level, time))
 
 
-def advance_level(level):
-    """advanc for one level; returns (dtlevnew, cfl_level)."""
+def advance_level(level, want_cfl=True):
+    """advanc for one level; returns (dtlevnew, cfl_level), or None without
+    waiting for the device when want_cfl is False."""
+    if not want_cfl:
+        _check(lib().amrgpu_advance_level(level, None, None))
+        return None
     dt, cfl = C.c_double(), C.c_double()
     _check(lib().amrgpu_advance_level(level, C.byref(dt), C.byref(cfl)))
     return dt.value, cfl.value
+
+
+LAUNCH_BATCHED, LAUNCH_PER_PATCH = 0, 1
+KT_SWEEP, KT_FILL, KT_REGISTERS, KT_UPDATE, KT_FLAG = range(5)
+
+
+def set_launch_mode(mode):
+    _check(lib().amrgpu_set_launch_mode(mode))
+
+
+def set_graphs(on):
+    _check(lib().amrgpu_set_graphs(int(bool(on))))
+
+
+def set_timing(on):
+    _check(lib().amrgpu_set_timing(int(bool(on))))
+
+
+def kernel_time(kind):
+    """(total ms, launches) of one kernel class since set_timing(True)."""
+    ms, n = C.c_double(), C.c_int64()
+    _check(lib().amrgpu_kernel_time(kind, C.byref(ms), C.byref(n)))
+    return ms.value, n.value
 
 
 def update_level(level):

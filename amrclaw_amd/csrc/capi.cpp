@@ -92,6 +92,15 @@ struct Level {
     RegionRect* d_rects = nullptr;
     int64_t rects_cap = 0;
     unsigned long long* d_counters = nullptr;
+    // per-patch launch mode: tiles grouped by patch
+    std::vector<int> patch_tile_start;
+    // captured level steps, keyed by everything their launch arguments hold
+    struct Graph {
+        std::vector<double> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t last_use = 0;
+    };
+    std::vector<Graph> graphs;
 };
 
 struct Context {
@@ -108,6 +117,12 @@ struct Context {
     double hxposs[MAXLV] = {}, hyposs[MAXLV] = {};
     std::vector<unsigned long long> h_cfl;
     bool regs_valid = false;
+    int launch_mode = AMRGPU_LAUNCH_BATCHED;   // or one sweep launch per patch
+    bool timing = false;                        // per-kernel-class HIP events
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[AMRGPU_KT_COUNT];
+    size_t tev_used[AMRGPU_KT_COUNT] = {};
+    bool graphs = false;                        // replay advance_level as hipGraphs
+    uint64_t layout_version = 1;                // bumped by every layout change
 };
 
 Context g;
@@ -149,6 +164,28 @@ void quiesce() {
     if (g.stream) hipStreamSynchronize(g.stream);
 }
 
+// per-kernel-class timing: an event pair around each launch on the stream
+void tbegin(int kind) {
+    if (!g.timing) return;
+    auto& v = g.tev[kind];
+    size_t& u = g.tev_used[kind];
+    if (u == v.size()) {
+        std::pair<hipEvent_t, hipEvent_t> p{};
+        if (hipEventCreate(&p.first) != hipSuccess || hipEventCreate(&p.second) != hipSuccess) return;
+        v.push_back(p);
+    }
+    hipEventRecord(v[u].first, g.stream);
+}
+void tend(int kind) {
+    if (!g.timing) return;
+    auto& v = g.tev[kind];
+    size_t& u = g.tev_used[kind];
+    if (u < v.size()) {
+        hipEventRecord(v[u].second, g.stream);
+        u++;
+    }
+}
+
 void free_plan(Level& L) {
     quiesce();
     if (L.d_ops) hipFree(L.d_ops);
@@ -186,8 +223,16 @@ void free_regs(Level& L) {
     for (auto& p : L.patches) p.reg_off = 0, p.lenbc = 0;
 }
 
+void free_graphs(Level& L) {
+    quiesce();
+    for (auto& gr : L.graphs)
+        if (gr.exec) hipGraphExecDestroy(gr.exec);
+    L.graphs.clear();
+}
+
 void free_level(Level& L) {
     quiesce();
+    free_graphs(L);
     free_plan(L);
     free_regs(L);
     if (L.d_partial) hipFree(L.d_partial);
@@ -244,10 +289,20 @@ int plan_level(Level& L) {
                 lds = std::max(lds, b);
             }
     }
-    // largest tiles first (the reference orders grids by cost, regrid.f:132-169)
-    std::stable_sort(L.tiles.begin(), L.tiles.end(), [](const TileDesc& a, const TileDesc& b) {
-        return a.tx * a.ty > b.tx * b.ty;
-    });
+    if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH) {
+        // one launch per patch: the patch's tiles are contiguous
+        L.patch_tile_start.assign(L.ngrids + 1, 0);
+        for (const auto& t : L.tiles) L.patch_tile_start[t.patch + 1]++;
+        for (int p = 0; p < L.ngrids; p++) L.patch_tile_start[p + 1] += L.patch_tile_start[p];
+    } else {
+        // largest tiles first (the reference orders grids by cost, regrid.f:132-169)
+        std::stable_sort(L.tiles.begin(), L.tiles.end(), [](const TileDesc& a, const TileDesc& b) {
+            return a.tx * a.ty > b.tx * b.ty;
+        });
+        L.patch_tile_start.clear();
+    }
+    g.layout_version++;
+    free_graphs(L);
     if (lds > 160 * 1024)
         return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", g.txmax, g.tymax, lds);
     L.lds = lds;
@@ -274,7 +329,9 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
     free_level(L);
     // fill programs of this and finer levels read this level's layout
     for (int l = level; l < MAXLV; l++) free_plan(g.levels[l]);
+    for (auto& Lv : g.levels) free_graphs(Lv);
     g.regs_valid = false;
+    g.layout_version++;
     L.ngrids = ngrids;
     L.hx = hx;
     L.hy = hy;
@@ -520,12 +577,14 @@ int run_fill(const FillPlan& P, const FillOp* d_ops, const FillBuf* d_bufs,
     a.ylower = g.cfg.ylower;
     for (int k = 0; k < 4; k++) a.mthbc[k] = g.cfg.mthbc[k];
     a.nv = g.cfg.nvar;
+    tbegin(AMRGPU_KT_FILL);
     for (int s = 0; s < P.nstages; s++) {
         a.ops = d_ops + P.stage_start[s];
         a.nops = P.stage_start[s + 1] - P.stage_start[s];
         hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
         if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "fill stage %d: %s", s, hipGetErrorString(e));
     }
+    tend(AMRGPU_KT_FILL);
     return AMRGPU_OK;
 }
 
@@ -632,8 +691,10 @@ struct RegLink {
     const SvEntry* sv = nullptr;
 };
 
-int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev,
-          const RegLink& rl = RegLink()) {
+// the sweep of every grid of a level; buf[1-cur] receives the new values
+// (the caller swaps the roles afterwards)
+int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev,
+                 const RegLink& rl = RegLink()) {
     if (L.ngrids == 0) return AMRGPU_OK;
     HIPCHK(launch_fill_u64(L.d_cfl, 0ull, L.ngrids, g.stream));
     SweepArgs a{};
@@ -653,13 +714,34 @@ int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev
     a.reg_fine = rl.reg_fine;
     a.fine_patches = rl.fine_patches;
     a.sv = rl.sv;
-    hipError_t e = g.esize == 8 ? launch_sweep_f64(g.cfg.solver, a, L.lds, g.stream)
-                                : launch_sweep_f32(g.cfg.solver, a, L.lds, g.stream);
+    auto launch = [&](const SweepArgs& x) {
+        return g.esize == 8 ? launch_sweep_f64(g.cfg.solver, x, L.lds, g.stream)
+                            : launch_sweep_f32(g.cfg.solver, x, L.lds, g.stream);
+    };
+    hipError_t e = hipSuccess;
+    tbegin(AMRGPU_KT_SWEEP);
+    if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH && !L.patch_tile_start.empty()) {
+        for (int p = 0; p < L.ngrids && e == hipSuccess; p++) {
+            SweepArgs x = a;
+            x.tiles = L.d_tiles + L.patch_tile_start[p];
+            x.ntiles = L.patch_tile_start[p + 1] - L.patch_tile_start[p];
+            if (x.ntiles > 0) e = launch(x);
+        }
+    } else {
+        e = launch(a);
+    }
+    tend(AMRGPU_KT_SWEEP);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "sweep launch: %s", hipGetErrorString(e));
-    // the updated interior is now in buf[1-cur]; buf[cur] keeps time t
-    // (with its ghost cells) and becomes the old-time array (store2)
-    L.cur = 1 - L.cur;
     return AMRGPU_OK;
+}
+
+// the updated interior is now in buf[1-cur]; buf[cur] keeps time t (with
+// its ghost cells) and becomes the old-time array (store2)
+int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev,
+          const RegLink& rl = RegLink()) {
+    int rc = launch_sweep(L, dt, flux_out_dev, flux_off_dev, rl);
+    if (rc == AMRGPU_OK && L.ngrids > 0) L.cur = 1 - L.cur;
+    return rc;
 }
 
 int level_cfl(Level& L, double dt, double* cfl_level, double* dtlevnew) {
@@ -683,17 +765,13 @@ int level_cfl(Level& L, double dt, double* cfl_level, double* dtlevnew) {
 
 // advanc.f:16-305 for one level: ghost fill, saveqc of the finer level,
 // the sweep with fluxsv / fluxad's perimeter fluxes, qad + fluxad, time
-int advance_level(int level, double* dtlevnew, double* cfl_level) {
+// the launches of one level step (bound, saveqc of the finer level, sweep,
+// qad + fluxad), issued on the stream with the current state; no host sync
+int launch_level_step(int level) {
     Level& L = g.levels[level];
-    int rc = ensure_regs();
+    int rc = ensure_plan(level);
     if (rc) return rc;
-    if (L.ngrids == 0) {
-        if (dtlevnew) *dtlevnew = RINFINITY;
-        if (cfl_level) *cfl_level = 0.0;
-        return AMRGPU_OK;
-    }
-    if (!(L.possk > 0)) return fail(AMRGPU_ERR_STATE, "level %d has no time step (amrgpu_set_time)", level);
-    rc = bound_level(level, L.time);
+    rc = run_fill(L.plan, L.d_ops, L.d_bufs, L.d_masks, L.d_scratch, L.time);
     if (rc) return rc;
     const bool has_fine = level + 1 <= g.mxnest && level + 1 < MAXLV && g.levels[level + 1].ngrids > 0;
     RegLink rl;
@@ -706,7 +784,7 @@ int advance_level(int level, double* dtlevnew, double* cfl_level) {
         rl.sv = L.d_sv;
     }
     if (level > 1) rl.reg = L.d_reg;
-    rc = sweep(L, L.possk, nullptr, nullptr, rl);
+    rc = launch_sweep(L, L.possk, nullptr, nullptr, rl);
     if (rc) return rc;
     if (level > 1 && L.nqad > 0) {
         QadArgs a{};
@@ -714,16 +792,111 @@ int advance_level(int level, double* dtlevnew, double* cfl_level) {
         a.nitems = L.nqad;
         a.nv = g.cfg.nvar;
         a.patches = L.d_patches;
-        a.q = L.buf[1 - L.cur];          // time t with its ghost cells
+        a.q = L.buf[L.cur];              // time t with its ghost cells
         a.reg = L.d_reg;
         a.ratx = g.ratx[level - 1];
         a.raty = g.raty[level - 1];
         a.hx = L.hx;
         a.hy = L.hy;
         a.delt = L.possk;
+        tbegin(AMRGPU_KT_REGISTERS);
         hipError_t e = launch_finereg(g.cfg.solver, (int)g.esize, a, g.meth.ph, g.stream);
+        tend(AMRGPU_KT_REGISTERS);
         if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "register fix-up: %s", hipGetErrorString(e));
     }
+    return AMRGPU_OK;
+}
+
+// Everything the launch arguments of a level step depend on besides the
+// layout: the step, the buffer roles of every level and the time weights
+// of the source levels of the ghost fill.
+std::vector<double> step_key(int level) {
+    Level& L = g.levels[level];
+    std::vector<double> k;
+    k.push_back((double)g.layout_version);
+    k.push_back(L.possk);
+    double mask = 0;
+    for (int l = 1; l < MAXLV; l++) mask += g.levels[l].cur ? (double)(1 << l) : 0.0;
+    k.push_back(mask);
+    FillArgs a{};
+    if (time_weights(L.time, a, L.plan.src_levels_mask) == AMRGPU_OK)
+        for (int l = 1; l < MAXLV && l < FILL_MAXLV; l++)
+            if (L.plan.src_levels_mask & (1u << l)) {
+                k.push_back(a.alphai[l]);
+                k.push_back(a.alphac[l]);
+                k.push_back((double)a.tmode[l]);
+            }
+    return k;
+}
+
+// replay (or capture, on first use of a key) the level step as a hipGraph
+int graph_level_step(int level) {
+    Level& L = g.levels[level];
+    int rc = ensure_plan(level);
+    if (rc) return rc;
+    for (int l = 1; l < MAXLV; l++)
+        if (L.plan.src_levels_mask & (1u << l)) {
+            FillArgs a{};
+            rc = time_weights(L.time, a, L.plan.src_levels_mask);   // report unavailable times now
+            if (rc) return rc;
+            break;
+        }
+    static uint64_t clock = 0;
+    std::vector<double> key = step_key(level);
+    for (auto& gr : L.graphs)
+        if (gr.key == key) {
+            gr.last_use = ++clock;
+            HIPCHK(hipGraphLaunch(gr.exec, g.stream));
+            return AMRGPU_OK;
+        }
+    const bool timing = g.timing;
+    g.timing = false;          // no event records inside a capture
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_level_step(level);
+    hipError_t e = hipStreamEndCapture(g.stream, &graph);
+    g.timing = timing;
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    Level::Graph gr;
+    gr.key = key;
+    gr.last_use = ++clock;
+    e = hipGraphInstantiate(&gr.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    // keep the 8 most recently used steps of the level
+    if (L.graphs.size() >= 8) {
+        auto old = std::min_element(L.graphs.begin(), L.graphs.end(),
+                                    [](const Level::Graph& a, const Level::Graph& b) {
+                                        return a.last_use < b.last_use;
+                                    });
+        quiesce();
+        hipGraphExecDestroy(old->exec);
+        L.graphs.erase(old);
+    }
+    L.graphs.push_back(gr);
+    HIPCHK(hipGraphLaunch(L.graphs.back().exec, g.stream));
+    return AMRGPU_OK;
+}
+
+// advanc.f:16-305 for one level: ghost fill, saveqc of the finer level,
+// the sweep with fluxsv / fluxad's perimeter fluxes, qad + fluxad, time
+int advance_level(int level, double* dtlevnew, double* cfl_level) {
+    Level& L = g.levels[level];
+    int rc = ensure_regs();
+    if (rc) return rc;
+    if (L.ngrids == 0) {
+        if (dtlevnew) *dtlevnew = RINFINITY;
+        if (cfl_level) *cfl_level = 0.0;
+        return AMRGPU_OK;
+    }
+    if (!(L.possk > 0)) return fail(AMRGPU_ERR_STATE, "level %d has no time step (amrgpu_set_time)", level);
+    rc = g.graphs ? graph_level_step(level) : launch_level_step(level);
+    if (rc) return rc;
+    L.cur = 1 - L.cur;                                           // new <-> old roles
     L.time = L.time + L.possk;                                   // advanc.f:289
     if (cfl_level || dtlevnew) return level_cfl(L, L.possk, cfl_level, dtlevnew);
     return AMRGPU_OK;
@@ -752,7 +925,9 @@ int update_level(int level) {
     a.qf = F.buf[F.cur];
     a.reg = F.d_reg;
     a.area = C.hx * C.hy;
+    tbegin(AMRGPU_KT_UPDATE);
     hipError_t e = launch_update((int)g.esize, a, C.d_avg_start, C.navg_cells, g.stream);
+    tend(AMRGPU_KT_UPDATE);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "update: %s", hipGetErrorString(e));
     return AMRGPU_OK;
 }
@@ -830,7 +1005,9 @@ int flag_level(int level, double tolsp, double start_time, int nreg, const doubl
     a.mbuff = mbuff;
     a.level = level;
     a.tolsp = tolsp;
+    tbegin(AMRGPU_KT_FLAG);
     hipError_t e = launch_flag((int)g.esize, a, L.ngrids, g.stream);
+    tend(AMRGPU_KT_FLAG);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "flag: %s", hipGetErrorString(e));
     unsigned long long cnt[2];
     HIPCHK(hipMemcpyAsync(cnt, L.d_counters, sizeof cnt, hipMemcpyDeviceToHost, g.stream));
@@ -897,6 +1074,12 @@ int amrgpu_init(const amrgpu_config* cfg) {
     for (int mw = 0; mw < NWMAX; mw++) M.mthlim[mw] = mw < cfg->mwaves ? cfg->mthlim[mw] : 0;
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     for (auto& e : g.ev) HIPCHK(hipEventCreate(&e));
+    {
+        // set the kernels' LDS limits now, outside any stream capture
+        SweepArgs none{};
+        HIPCHK(launch_sweep_f64(g.cfg.solver, none, 0, g.stream));
+        HIPCHK(launch_sweep_f32(g.cfg.solver, none, 0, g.stream));
+    }
     g.txmax = g.esize == 8 ? 28 : 28;
     g.tymax = g.esize == 8 ? 12 : 28;
     g.inited = true;
@@ -907,6 +1090,11 @@ int amrgpu_finalize(void) {
     if (!g.inited) return AMRGPU_OK;
     hipStreamSynchronize(g.stream);
     for (auto& L : g.levels) free_level(L);
+    for (auto& v : g.tev)
+        for (auto& p : v) {
+            hipEventDestroy(p.first);
+            hipEventDestroy(p.second);
+        }
     for (auto& e : g.ev)
         if (e) hipEventDestroy(e);
     hipStreamDestroy(g.stream);
@@ -1124,6 +1312,51 @@ int amrgpu_flag_level(int level, double tolsp, double start_time, int nregions,
         (flags_base && !offsets))
         return fail(AMRGPU_ERR_ARG, "bad flag_level arguments");
     return flag_level(level, tolsp, start_time, nregions, regions, mbuff, flags_base, offsets, nflagged);
+}
+
+int amrgpu_set_launch_mode(int mode) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (mode != AMRGPU_LAUNCH_BATCHED && mode != AMRGPU_LAUNCH_PER_PATCH)
+        return fail(AMRGPU_ERR_ARG, "unknown launch mode %d", mode);
+    if (mode == g.launch_mode) return AMRGPU_OK;
+    g.launch_mode = mode;
+    for (auto& L : g.levels)
+        if (L.ngrids) {
+            int rc = plan_level(L);
+            if (rc) return rc;
+        }
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_graphs(int on) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    g.graphs = on != 0;
+    if (!g.graphs)
+        for (auto& L : g.levels) free_graphs(L);
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_timing(int on) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    quiesce();
+    g.timing = on != 0;
+    for (auto& u : g.tev_used) u = 0;
+    return AMRGPU_OK;
+}
+
+int amrgpu_kernel_time(int kind, double* ms, int64_t* launches) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (kind < 0 || kind >= AMRGPU_KT_COUNT || !ms) return fail(AMRGPU_ERR_ARG, "bad kernel_time arguments");
+    quiesce();
+    double tot = 0.0;
+    for (size_t k = 0; k < g.tev_used[kind]; k++) {
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, g.tev[kind][k].first, g.tev[kind][k].second));
+        tot += t;
+    }
+    *ms = tot;
+    if (launches) *launches = (int64_t)g.tev_used[kind];
+    return AMRGPU_OK;
 }
 
 int amrgpu_get_registers(int level, int grid, double* svdflx_qc1d, int* lenbc) {

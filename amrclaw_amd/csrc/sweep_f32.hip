@@ -7,7 +7,6 @@ namespace amrgpu {
 
 hipError_t launch_sweep_f32(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
     dim3 grid(a.ntiles), block(256);
-    if (a.ntiles == 0) return hipSuccess;
     static bool attr_set = false;
     if (!attr_set) {
         // allow tiles whose LDS footprint exceeds the 64 KiB default
@@ -20,6 +19,7 @@ hipError_t launch_sweep_f32(int solver, const SweepArgs& a, size_t lds, hipStrea
                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         attr_set = true;
     }
+    if (a.ntiles == 0) return hipSuccess;   // (a call with no tiles only sets the attributes)
     switch (solver) {
     case SOLVER_ADVECTION:
         hipLaunchKernelGGL((sweep_kernel<Advection<float, FastMathF32>, float, float>), grid, block, lds, st, a);

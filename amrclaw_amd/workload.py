@@ -78,3 +78,47 @@ def patch_initial_state(box, h, seed, nghost=2, perturb=1e-3):
     rng = np.random.default_rng(seed)
     q *= 1.0 + perturb * (2.0 * rng.random(q.shape[:2] + (1,)) - 1.0)
     return q
+
+
+def max_wave_speed(q, gamma=GAMMA):
+    """max(|u| + c, |v| + c) over the cells of an Euler array (..., 4)."""
+    rho = q[..., 0]
+    u, v = q[..., 1] / rho, q[..., 2] / rho
+    p = (gamma - 1.0) * (q[..., 3] - 0.5 * rho * (u * u + v * v))
+    c = np.sqrt(gamma * p / rho)
+    return float(max((np.abs(u) + c).max(), (np.abs(v) + c).max()))
+
+
+def batched_euler_hierarchy(n=4096, seed=20261202, coarse_block=256, courant=0.8):
+    """Two-level hierarchy for the level-update benchmark: level 2 holds the
+    n batched patches (batched_euler_level), level 1 covers the whole domain
+    at ratio 2 with grids of coarse_block^2 cells, so every level-2 grid is
+    properly nested.  Level 1 has been advanced from t = 0 to dt1 (old and
+    new hold the initial state); level 2 sits half way through that coarse
+    step (time dt2 = dt1 / 2), so its ghost fill blends both coarse times.
+    dt2 gives the level-2 Courant number `courant` on the initial state.
+    Returns the Hier (amrclaw_amd.hierarchy) and the interior cell count of
+    level 2."""
+    from .hierarchy import Hier
+    boxes2, h2, mx, my = batched_euler_level(n, seed)
+    side = int(round(1.0 / h2))
+    csz = side // 2
+    h1 = 2.0 * h2
+    cuts = list(range(0, csz, coarse_block)) + [csz]
+    boxes1 = [(a, b - 1, c, d - 1) for c, d in zip(cuts[:-1], cuts[1:])
+              for a, b in zip(cuts[:-1], cuts[1:])]
+    H = Hier(4)
+    H.add_level(boxes1, h1, h1, 0.0, 0.0, csz, csz, 2, 2)
+    H.add_level(boxes2, h2, h2, 0.0, 0.0, side, side, 2, 2)
+    smax = 0.0
+    for l, (boxes, h, s0) in enumerate(((boxes1, h1, 500000), (boxes2, h2, 1000)), start=1):
+        for k, b in enumerate(boxes):
+            g = H.grid(l, k, 0)
+            g[:] = patch_initial_state(b, h, s0 + k)
+            H.grid(l, k, 1)[:] = g
+            if l == 2:
+                smax = max(smax, max_wave_speed(g))
+    dt2 = courant * h2 / smax
+    H.levels[0].update(possk=2.0 * dt2, time=2.0 * dt2)
+    H.levels[1].update(possk=dt2, time=dt2)
+    return H, int((mx * my).sum())

@@ -1,17 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: cell-updates/s of the batched 2-D Euler level update on one GPU.
 
-Workload (one "step" = one level update of one level): N = 4096 patches,
-mx, my ~ uniform even numbers in [8, 128] (numpy default_rng(20261202)),
-2 ghost cells, shelf-packed on the unit square; 2-D Euler (gamma 1.4) Roe
-solver with entropy fix, MC limiter, transverse corrections (method 2,2);
-four-quadrant initial state with a seeded +-1e-3 perturbation.  Inputs are
-resident in HBM before the timed region.
+Workload: a two-level hierarchy (amrclaw_amd.workload.batched_euler_hierarchy).
+Level 2 holds N = 4096 patches, mx, my uniform even numbers in [8, 128]
+(numpy default_rng(20261202)), 2 ghost cells, shelf-packed on the unit
+square (18,855,660 interior cells); level 1 covers the domain at ratio 2.
+2-D Euler (gamma 1.4), Roe solver with entropy fix, MC limiter, transverse
+corrections (method(2) = method(3) = 2), extrapolation boundaries; the
+four-quadrant initial state with a seeded +-1e-3 perturbation.
 
-The headline is the FP32-batched path; the FP64 batched path runs the same
-steps on the same layout and dt for the speed-up and the L1 error of FP32
-against FP64 (the FP64 path is bit-exact with the CPU oracle; see
-tests/test_gpu_sweep.py).
+One step = one level update of level 2 (amrgpu_advance_level: ghost fill
+with time-interpolated coarse values, the sweep of all 4096 patches with
+the fine-side register work qad + fluxad), always at the same point of the
+coarse step (level-2 time dt2, between the coarse times 0 and 2 dt2).
+Inputs are resident in HBM before the timed region; the CFL readback is
+not part of the step.
+
+Runs: FP32 batched (the headline), FP64 batched and FP64 with one sweep
+launch per patch (the plain per-patch GPU path) on the same layout and dt;
+the FP32 result's relative L1 error and mass difference against FP64 are
+reported.  The CPU baseline is the reference's own advanc (compiled from
+its Fortran by oracle/Makefile) over a sample of the same hierarchy.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--npatch N]
 """
@@ -27,6 +36,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "cell-updates/s, FP32-batched vs FP64 per-patch at fixed L1 err; % HBM peak"
 
 
 def parse():
@@ -38,9 +48,13 @@ def parse():
     ap.add_argument("--txmax", type=int, default=0)
     ap.add_argument("--tymax", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--per-patch", action="store_true",
+                    help="time the one-launch-per-patch sweep instead")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the level steps as hipGraphs in the timed region")
     ap.add_argument("--no-compare", action="store_true",
-                    help="skip the FP64 comparison run")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="skip the FP64 comparison runs")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 disables it)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB "
@@ -61,11 +75,12 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "dispatches": 5}))
 
 
-def setup_level(capi, workload, precision, npatch, txmax, tymax):
-    boxes, h, mx, my = workload.batched_euler_level(npatch)
-    cfg = capi.make_config(capi.EULER, order=2, transverse=2, limiter=4,
-                           cfl=0.9, cflv1=1.0, gamma=1.4, efix=True,
-                           precision=precision)
+def install(capi, H, precision, per_patch, graphs, txmax, tymax):
+    """Load the hierarchy and bring it to the timed state: level 1 advanced
+    once from t = 0 (saveqc fills the level-2 registers' coarse ring),
+    level 2 half way through that coarse step."""
+    cfg = capi.make_config(capi.EULER, order=2, transverse=2, limiter=4, cfl=0.9,
+                           cflv1=1.0, gamma=1.4, efix=True, precision=precision)
     capi.init(cfg)
     if not (txmax or tymax):
         from amrclaw_amd import autotune
@@ -74,62 +89,109 @@ def setup_level(capi, workload, precision, npatch, txmax, tymax):
             txmax, tymax = tuned
     if txmax or tymax:
         capi.set_tiling(txmax, tymax)
-    capi.set_level(1, boxes, h, h)
-    qs = [workload.patch_initial_state(boxes[k], h, 1000 + k) for k in range(len(boxes))]
-    # both time levels hold the initial data so every step reads valid ghost
-    # cells
-    capi.put_level(1, qs, which=0)
-    capi.put_level(1, qs, which=1)
-    return boxes, h, mx, my
+    if per_patch:
+        capi.set_launch_mode(capi.LAUNCH_PER_PATCH)
+    capi.load_hierarchy(H)
+    dt2 = H.levels[1]["possk"]
+    capi.set_time(1, 0.0, 2.0 * dt2)
+    capi.advance_level(1)
+    capi.set_graphs(graphs)
+    return dt2
 
 
-def run(capi, steps, warmup, dt, sync):
+def step(capi, dt2):
+    capi.set_time(2, dt2, dt2)
+    capi.advance_level(2, want_cfl=False)
+
+
+def run(capi, dt2, steps, warmup, sync):
     for _ in range(warmup):
-        capi.sweep_level(1, dt, want_cfl=False)
+        step(capi, dt2)
     sync()
+    capi.set_timing(True)
     t0 = time.perf_counter()
     capi.event_record(0)
     for _ in range(steps):
-        capi.sweep_level(1, dt, want_cfl=False)
+        step(capi, dt2)
     capi.event_record(1)
     sync()
     wall = time.perf_counter() - t0
-    return wall, capi.event_elapsed(0, 1) / 1e3
+    out = {"wall": wall, "events_ms": capi.event_elapsed(0, 1)}
+    for name, kind in (("sweep", capi.KT_SWEEP), ("fill", capi.KT_FILL),
+                       ("registers", capi.KT_REGISTERS)):
+        ms, n = capi.kernel_time(kind)
+        out[name] = (ms, n)
+    capi.set_timing(False)
+    return out
 
 
-def download(capi, boxes):
-    shapes = [(int(b[3] - b[2]) + 5, int(b[1] - b[0]) + 5, 4) for b in boxes]
-    return [q[2:-2, 2:-2] for q in capi.get_level(1, shapes)]
+def interiors(capi, H):
+    L = H.levels[1]
+    got = capi.read_level(H, 2, 0)
+    return [got[o:o + int(np.prod(s))].reshape(s)[2:-2, 2:-2]
+            for o, s in zip(L["off"], L["shapes"])]
 
 
-def cpu_baseline(workload, boxes, h, dt, budget):
-    """The reference's stepgrid (built from its own Fortran by oracle/Makefile)
-    with its OpenMP patch loop, on the first patches of the same workload;
-    falls back to the oracle's C port when the reference build is absent."""
+def _quiet_stdout(fn):
+    """Run fn with the process's stdout (fd 1) sent to stderr: the
+    reference's Fortran prints progress there, and stdout carries only the
+    result line."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return fn()
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def cpu_baseline(H, dt2, budget, nsample=256):
+    """The reference's advanc (its own Fortran, OpenMP over grids as
+    advanc.f:60-129) on level 2 of a sample of the hierarchy: level 1 whole,
+    the first nsample level-2 patches; same state and times as the GPU
+    step.  Falls back to the oracle's C port (serial) without the build."""
     from oracle import orc
+    from amrclaw_amd.hierarchy import Hier
     p = orc.make_params(orc.EULER, order=2, transverse=2, limiter=4)
-    nsamp = min(128, len(boxes))
-    qs = [workload.patch_initial_state(boxes[k], h, 1000 + k) for k in range(nsamp)]
-    cells = sum((q.shape[0] - 4) * (q.shape[1] - 4) for q in qs)
+    S = Hier(4)
+    L1, L2 = H.levels
+    S.add_level(L1["boxes"], L1["hx"], L1["hy"], 2.0 * dt2, 0.0, L1["iregsz"], L1["jregsz"], 2, 2)
+    S.add_level(L2["boxes"][:nsample], L2["hx"], L2["hy"], dt2, dt2, L2["iregsz"], L2["jregsz"], 2, 2)
+    for l in (1, 2):
+        for k in range(len(S.levels[l - 1]["boxes"])):
+            S.grid(l, k, 0)[:] = H.grid(l, k, 0)
+            S.grid(l, k, 1)[:] = H.grid(l, k, 1)
+    b = S.levels[1]["boxes"]
+    cells = int(((b[:, 1] - b[:, 0] + 1) * (b[:, 3] - b[:, 2] + 1)).sum())
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if orc.ref_available():
         kind = "reference"
-        fn = lambda: orc.ref_stepgrid_batch(p, qs, h, h, dt)
+        A = orc.RefAMR(p, S)
+        A.set_parallel(True)
+        A.advanc(1)                           # level 1: 0 -> 2 dt2, saveqc(2)
+        reset = lambda: A.set_time(2, dt2)
     else:
         kind, threads = "port", 1
-        fn = lambda: [orc.stepgrid(p, q, h, h, dt) for q in qs]
+        A = orc.OracleAMR(p, S)
+        A.advanc(1)
+        reset = lambda: A.h.time.__setitem__(2, dt2)
     t0 = time.perf_counter()
     reps = 0
     while True:
-        fn()
+        reset()
+        A.advanc(2)
         reps += 1
         el = time.perf_counter() - t0
         if el >= budget:
             break
-    return {"value": cells * reps / el, "unit": "cell-updates/s",
-            "cores": threads, "kind": kind,
-            "sample": "%d patches (%d cells) of the same workload, %d passes of "
-                      "stepgrid in %.1f s" % (nsamp, cells, reps, el)}
+    if kind == "reference":
+        A.flush()
+    return {"value": cells * reps / el, "unit": "cell-updates/s", "cores": threads,
+            "kind": kind,
+            "sample": "level-2 update (advanc: bound with coarse interpolation, qad, "
+                      "stepgrid, fluxad) of the first %d of the 4096 patches (%d cells) "
+                      "over the whole level 1, %d passes in %.1f s" % (nsample, cells, reps, el)}
 
 
 def main():
@@ -148,57 +210,80 @@ def main():
         capi.synchronize()
         torch.cuda.synchronize()
 
+    print("bench: building the workload", file=sys.stderr, flush=True)
+    H, cells = workload.batched_euler_hierarchy(a.npatch)
+    npatch = len(H.levels[1]["boxes"])
     prec = capi.FP32 if a.precision == "fp32" else capi.FP64
-    boxes, h, mx, my = setup_level(capi, workload, prec, a.npatch, a.txmax, a.tymax)
-    cells = int((mx * my).sum())
-    # dt from the Courant number of the initial state, cfl_desired = 0.9
-    cfl0, _ = capi.sweep_level(1, 0.1 * h)
-    dt = 0.1 * h * 0.9 / cfl0
-    # restore the initial state (the probe advanced it)
-    boxes, h, mx, my = setup_level(capi, workload, prec, a.npatch, a.txmax, a.tymax)
-
-    wall, kern = run(capi, a.steps, a.warmup, dt, sync)
     esize = 4 if prec == capi.FP32 else 8
     bytes_per_cell = 2 * 4 * esize            # (2 nvar + maux) * s
-    value = cells * a.steps / wall
-    achieved = cells * bytes_per_cell / (kern / a.steps) / 1e9
+
+    print("bench: timed run", a.precision, file=sys.stderr, flush=True)
+    dt2 = install(capi, H, prec, a.per_patch, a.graphs, a.txmax, a.tymax)
+    r = run(capi, dt2, a.steps, a.warmup, sync)
+    q_main = interiors(capi, H) if not a.no_compare else None
+    capi.finalize()
+    value = cells * a.steps / r["wall"]
+    sweep_ms, sweep_n = r["sweep"]
     result = {
-        "metric": "cell-updates/s, FP32-batched vs FP64 per-patch at fixed L1 err; % HBM peak",
+        "metric": METRIC,
         "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": wall / a.steps * 1e3,
+        "ms_per_step": r["wall"] / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if prec == capi.FP32 else "f64", "data": "synthetic",
-        "config": {"workload": "batched 2-D Euler level update: %d patches, mx,my "
-                               "even U[8,128] seed 20261202, Roe+efix, MC, "
-                               "transverse 2" % len(boxes),
-                   "patches": len(boxes), "cells_per_step": cells,
-                   "nghost": 2, "precision": a.precision},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "kernel": "sweep_kernel<EulerRoe>",
-                     "kernel_ms": kern / a.steps * 1e3,
-                     "bytes_per_cell_update": bytes_per_cell},
+        "config": {"workload": "batched 2-D Euler level update (level 2 of 2): %d patches, "
+                               "mx,my even U[8,128] seed 20261202, Roe+efix, MC, "
+                               "transverse 2, extrapolation BCs" % npatch,
+                   "patches": npatch, "cells_per_step": cells, "nghost": 2,
+                   "precision": a.precision,
+                   "launch": "per-patch" if a.per_patch else "batched",
+                   "graphs": bool(a.graphs)},
+        "step_ms_events": r["events_ms"] / a.steps,
+        "kernels_ms_per_step": {k: (r[k][0] / a.steps if r[k][1] else None)
+                                for k in ("sweep", "fill", "registers")},
     }
+    if sweep_n:
+        kern_ms = sweep_ms / sweep_n        # one timed record per step's sweep
+        achieved = cells * bytes_per_cell / (kern_ms / 1e3) / 1e9
+        result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                              "traffic": None,
+                              "kernel": "sweep_kernel<EulerRoe>",
+                              "kernel_ms": kern_ms,
+                              "bytes_per_cell_update": bytes_per_cell,
+                              "algorithmic_bytes_per_launch": cells * bytes_per_cell}
     if not a.no_compare and prec == capi.FP32:
-        q32 = download(capi, boxes)
-        setup_level(capi, workload, capi.FP64, a.npatch, a.txmax, a.tymax)
-        wall64, kern64 = run(capi, a.steps, a.warmup, dt, sync)
-        q64 = download(capi, boxes)
-        num = sum(float(np.abs(x - y).sum()) for x, y in zip(q32, q64))
+        print("bench: FP64 batched", file=sys.stderr, flush=True)
+        install(capi, H, capi.FP64, False, False, 0, 0)
+        r64 = run(capi, dt2, a.steps, a.warmup, sync)
+        q64 = interiors(capi, H)
+        capi.finalize()
+        print("bench: FP64 per patch", file=sys.stderr, flush=True)
+        install(capi, H, capi.FP64, True, False, 0, 0)
+        rpp = run(capi, dt2, a.steps, a.warmup, sync)
+        qpp = interiors(capi, H)
+        capi.finalize()
+        same = all(np.array_equal(x, y) for x, y in zip(q64, qpp))
+        num = sum(float(np.abs(x - y).sum()) for x, y in zip(q_main, q64))
         den = sum(float(np.abs(y).sum()) for y in q64)
-        m32 = sum(float(x[..., 0].sum()) for x in q32)
-        m64 = sum(float(y[..., 0].sum()) for y in q64)
-        result["fp64_batched"] = {"value": cells * a.steps / wall64,
-                                  "kernel_ms": kern64 / a.steps * 1e3}
-        result["speedup_vs_fp64_batched"] = wall64 / wall
+        h2 = H.levels[1]["hx"]
+        m32 = sum(float(x[..., 0].sum()) for x in q_main) * h2 * h2
+        m64 = sum(float(y[..., 0].sum()) for y in q64) * h2 * h2
+        result["fp64_batched"] = {"value": cells * a.steps / r64["wall"],
+                                  "ms_per_step": r64["wall"] / a.steps * 1e3,
+                                  "sweep_ms": r64["sweep"][0] / max(r64["sweep"][1], 1)}
+        result["fp64_per_patch"] = {"value": cells * a.steps / rpp["wall"],
+                                    "ms_per_step": rpp["wall"] / a.steps * 1e3,
+                                    "sweep_launches_per_step": npatch,
+                                    "identical_to_fp64_batched": same}
+        result["speedup_vs_fp64_per_patch"] = rpp["wall"] / r["wall"]
+        result["speedup_vs_fp64_batched"] = r64["wall"] / r["wall"]
         result["l1_rel_vs_fp64"] = num / den
         result["mass_rel_diff_vs_fp64"] = abs(m32 - m64) / abs(m64)
         result["steps_compared"] = a.steps + a.warmup
-    capi.finalize()
     if a.cpu_seconds > 0:
-        result["cpu_baseline"] = cpu_baseline(workload, boxes, h, dt, a.cpu_seconds)
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline(H, dt2, a.cpu_seconds))
     print(json.dumps(result))
 
 

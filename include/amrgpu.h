@@ -150,6 +150,29 @@ int amrgpu_flag_level(int level, double tolsp, double start_time, int nregions,
                       const double* regions, int mbuff, double* flags_base,
                       const int64_t* offsets, int64_t* nflagged);
 
+/* Sweep launch shape: AMRGPU_LAUNCH_BATCHED (default) puts all tiles of
+ * all grids of a level into one launch; AMRGPU_LAUNCH_PER_PATCH issues one
+ * launch per grid, the plain per-patch GPU path, kept as the comparison
+ * baseline.  Results are identical.  No reference counterpart. */
+enum { AMRGPU_LAUNCH_BATCHED = 0, AMRGPU_LAUNCH_PER_PATCH = 1 };
+int amrgpu_set_launch_mode(int mode);
+
+/* With on != 0, amrgpu_advance_level captures its launches (ghost fill,
+ * saveqc, sweep, register fix-up) into a hipGraph on first use and replays
+ * it while the layout, the step, the buffer roles and the time-interpolation
+ * weights are the same (between regrids, every step of a fixed-dt run
+ * repeats one of a few such keys).  No reference counterpart. */
+int amrgpu_set_graphs(int on);
+
+/* Per-kernel-class timing with HIP events on the library's stream (not in
+ * graph mode).  amrgpu_set_timing(1) resets and starts it; amrgpu_kernel_time
+ * returns the summed milliseconds and the number of timed launches of one
+ * class.  No reference counterpart. */
+enum { AMRGPU_KT_SWEEP = 0, AMRGPU_KT_FILL = 1, AMRGPU_KT_REGISTERS = 2, AMRGPU_KT_UPDATE = 3,
+       AMRGPU_KT_FLAG = 4, AMRGPU_KT_COUNT = 5 };
+int amrgpu_set_timing(int on);
+int amrgpu_kernel_time(int kind, double* ms, int64_t* launches);
+
 /* The registers of grid `grid` of fine level `level`: svdflx(nvar, lenbc)
  * followed by qc1d(nvar, lenbc), component fastest, as the reference's
  * node(ffluxptr) block (prepf.f:33-48).  For tests and debugging. */

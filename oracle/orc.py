@@ -427,6 +427,18 @@ class RefAMR:
         self.R.ref_advanc(level, self.H.nvar, C.byref(dtn), C.byref(cfl))
         return dtn.value, cfl.value
 
+    def set_parallel(self, on):
+        """OpenMP over the grids of a level, as the reference's advanc."""
+        self.R.ref_set_parallel.argtypes = [C.c_int]
+        self.R.ref_set_parallel(int(bool(on)))
+
+    def set_time(self, level, t):
+        self.R.ref_set_time.argtypes = [C.c_int, C.c_double]
+        self.R.ref_set_time(level, t)
+
+    def flush(self):
+        self.R.ref_flush()
+
     def update(self, level):
         self.R.ref_update(level, self.H.nvar)
 

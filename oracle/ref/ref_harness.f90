@@ -44,6 +44,7 @@ module ref_harness
   use iso_c_binding
   use amr_module
   implicit none
+  logical :: harness_parallel = .false.
 contains
 
   ! method(1:7), mthlim(1:mwaves), mcapa, use_fwaves, cfl_desired, cfl_max
@@ -276,11 +277,19 @@ contains
   ! advanc.f:10-142 (the loop is restated here because advanc.f needs
   ! gauges_module, which needs the absent clawpack utility_module); every
   ! routine it calls is the reference's own
+  ! OpenMP over the grids of a level as advanc.f:60-84 and :106-129 do
+  ! (off by default so that parity runs are deterministic)
+  subroutine ref_set_parallel(on) bind(C, name="ref_set_parallel")
+    integer(c_int), value :: on
+    harness_parallel = on .ne. 0
+  end subroutine ref_set_parallel
+
   subroutine ref_advanc(level, nvar, dtlevnew, cflout) bind(C, name="ref_advanc")
     integer(c_int), value :: level, nvar
     real(c_double) :: dtlevnew, cflout
     integer :: j, mptr, mitot, mjtot
     real(kind=8) :: dtnew
+    !$omp parallel do if(harness_parallel) schedule(dynamic,1) private(j,mptr,mitot,mjtot)
     do j = 1, numgrids(level)
        mptr = listOfGrids(listStart(level)+j-1)
        mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
@@ -288,11 +297,14 @@ contains
        call bound(rnode(timemult,mptr), nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, &
             mptr, alloc(node(storeaux,mptr)), 0)
     end do
+    !$omp end parallel do
     if (level+1 .le. mxnest) then
        if (lstart(level+1) .ne. null) call saveqc(level+1, nvar, 0)
     endif
     dtlevnew = rinfinity
     cfl_level = 0.d0
+    !$omp parallel do if(harness_parallel) schedule(dynamic,1) &
+    !$omp& private(j,mptr,mitot,mjtot,dtnew) reduction(min:dtlevnew)
     do j = 1, numgrids(level)
        mptr = listOfGrids(listStart(level)+j-1)
        mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
@@ -300,6 +312,7 @@ contains
        call harness_par_advanc(mptr, mitot, mjtot, nvar, dtnew)
        dtlevnew = dmin1(dtlevnew, dtnew)
     end do
+    !$omp end parallel do
     cflout = cfl_level
   end subroutine ref_advanc
 
@@ -307,10 +320,10 @@ contains
   subroutine harness_par_advanc(mptr, mitot, mjtot, nvar, dtnew)
     integer :: mptr, mitot, mjtot, nvar
     real(kind=8) :: dtnew
-    real(kind=8) :: fp(nvar,mitot,mjtot), fm(nvar,mitot,mjtot)
-    real(kind=8) :: gp(nvar,mitot,mjtot), gm(nvar,mitot,mjtot)
+    real(kind=8), allocatable :: fp(:,:,:), fm(:,:,:), gp(:,:,:), gm(:,:,:)
     integer :: level, nx, ny, locold, locnew, locaux, ntot, i, lenbc, locsvf, locsvq, locx1d
     real(kind=8) :: hx, hy, delt, time, xlow, ylow
+    allocate(fp(nvar,mitot,mjtot), fm(nvar,mitot,mjtot), gp(nvar,mitot,mjtot), gm(nvar,mitot,mjtot))
     level = node(nestlevel,mptr)
     hx = hxposs(level)
     hy = hyposs(level)
@@ -348,6 +361,7 @@ contains
             intratx(level-1),intraty(level-1),nghost,delt,hx,hy)
     endif
     rnode(timemult,mptr) = rnode(timemult,mptr)+delt
+    deallocate(fp, fm, gp, gm)
   end subroutine harness_par_advanc
 
   subroutine ref_update(level, nvar) bind(C, name="ref_update")
@@ -379,6 +393,21 @@ contains
     if (node(cfluxptr,mptr) .eq. 0) return
     call copy_listbc(alloc(node(cfluxptr,mptr)), listsp(l), nmax, out, n)
   end subroutine ref_get_listbc
+
+  ! flush the reference's list-directed output (unit 6) so a caller that
+  ! redirected the process's stdout around a call gets it all there
+  subroutine ref_flush() bind(C, name="ref_flush")
+    flush(6)
+  end subroutine ref_flush
+
+  subroutine ref_set_time(l, t) bind(C, name="ref_set_time")
+    integer(c_int), value :: l
+    real(c_double), value :: t
+    integer :: j
+    do j = 1, numgrids(l)
+       rnode(timemult, listOfGrids(listStart(l) + j - 1)) = t
+    end do
+  end subroutine ref_set_time
 
   subroutine ref_get_time(l, k, t) bind(C, name="ref_get_time")
     integer(c_int), value :: l, k

@@ -3,6 +3,7 @@ import numpy as np
 
 from oracle import orc
 from amrclaw_amd import capi
+from amrclaw_amd.hierarchy import Hier  # noqa: F401  (re-exported for the tests)
 
 SOLVERS = {"advection": orc.ADVECTION, "acoustics": orc.ACOUSTICS,
            "euler": orc.EULER}
@@ -54,50 +55,6 @@ def both_params(solver, **kw):
 
 def rel_l1(a, b):
     return float(np.abs(a - b).sum() / max(np.abs(b).sum(), 1e-300))
-
-
-class Hier:
-    """A grid hierarchy in the reference's terms: per level the global index
-    boxes, mesh widths, step, time, index-space size and refinement ratio
-    to the next level, plus packed AoS FP64 new/old solution arrays (one
-    (my+4, mx+4, nvar) block per grid, like the reference's alloc)."""
-
-    def __init__(self, nvar, domain=(0.0, 1.0, 0.0, 1.0), mthbc=(1, 1, 1, 1),
-                 xper=False, yper=False, ng=2):
-        self.nvar, self.ng = nvar, ng
-        self.domain = domain
-        self.mthbc = tuple(mthbc)
-        self.xper, self.yper = xper, yper
-        self.levels = []
-
-    def add_level(self, boxes, hx, hy, possk, time, iregsz, jregsz, ratx=2, raty=2):
-        boxes = np.asarray(boxes, dtype=np.int32).reshape(-1, 4)
-        shapes = [(int(b[3] - b[2]) + 1 + 2 * self.ng, int(b[1] - b[0]) + 1 + 2 * self.ng, self.nvar)
-                  for b in boxes]
-        sizes = np.array([int(np.prod(s)) for s in shapes], dtype=np.int64)
-        off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-        tot = max(int(sizes.sum()), 1)
-        self.levels.append(dict(boxes=np.ascontiguousarray(boxes), shapes=shapes, off=off,
-                                qnew=np.zeros(tot), qold=np.zeros(tot), hx=hx, hy=hy,
-                                possk=possk, time=time, iregsz=iregsz, jregsz=jregsz,
-                                ratx=ratx, raty=raty))
-
-    @property
-    def nlev(self):
-        return len(self.levels)
-
-    def grid(self, l, k, which=0):
-        """View of grid k of level l (1-based) as (my+4, mx+4, nvar)."""
-        L = self.levels[l - 1]
-        a = L["qnew"] if which == 0 else L["qold"]
-        n = int(np.prod(L["shapes"][k]))
-        return a[L["off"][k]:L["off"][k] + n].reshape(L["shapes"][k])
-
-    def copy(self):
-        import copy
-        H = copy.copy(self)
-        H.levels = [dict(L, qnew=L["qnew"].copy(), qold=L["qold"].copy()) for L in self.levels]
-        return H
 
 
 def disjoint_boxes(rng, isz, jsz, nbox, lo=4, hi=20, align=2, within=None, tries=400):
