@@ -165,6 +165,9 @@ int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level);
 /* conck.f:6-73: hx*hy*sum of q(1) over the interiors of the level's grids */
 double orc_conck(const orc_params* p, const orc_hier* h, int level);
 
+int orc_bound_ng(const orc_params* p, const orc_hier* h, int level, int grid,
+                 double time, double* valbig, int ng);
+
 /* cell flagging (orc_flag.c) */
 #define ORC_UNSET (-1.0)
 #define ORC_DONTFLAG 0.0
@@ -176,7 +179,10 @@ void orc_flag2refine2(int mx, int my, int mbc, int mbuff, int meqn, double tolsp
                       double* amrflags);
 int orc_flag_level(const orc_params* p, const orc_hier* h, int level, double tolsp,
                    const double* regions, int nreg, int use_old, int mbuff, double* flags,
-                   const long* foff);
+                   const long* foff, int flag_gradient, int flag_richardson, double tol,
+                   int iorder);
+int orc_errest(const orc_params* p, const orc_hier* h, int level, int k, const double* big,
+               double tol, int iorder, int mbuff, double* flags);
 
 /* amr_module.f90 declares these real(kind=8) parameters from default-real
  * (single precision) literals 10.e32 and 10.e33, so their values are the

@@ -392,6 +392,45 @@ class OracleAMR:
     def conck(self, level):
         return lib().orc_conck(C.byref(self.p), C.byref(self.h), level)
 
+    def flag_level(self, level, tolsp, start_time, regions=(), ibuff=3, gradient=True,
+                   richardson=False, tol=0.0, iorder=2):
+        """flagger.f for every grid of `level`: ghost fill of the new values
+        (unless start_time + possk equals the level's time: old values),
+        regions, gradient and/or Richardson flags.  Returns one flag array
+        (ny + 2 mbuff, nx + 2 mbuff) per grid."""
+        L = lib()
+        L.orc_flag_level.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcHier), C.c_int, C.c_double,
+                                     _dp, C.c_int, C.c_int, C.c_int, _dp,
+                                     np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                                     C.c_int, C.c_int, C.c_double, C.c_int]
+        Lv = self.H.levels[level - 1]
+        t = self.h.time[level]
+        use_old = not (start_time + Lv["possk"] != t)
+        if not use_old:
+            L.orc_bound.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcHier), C.c_int,
+                                    C.c_int, C.c_double, C.c_void_p]
+            for k in range(len(Lv["boxes"])):
+                val = self.H.grid(level, k, 0)
+                rc = L.orc_bound(C.byref(self.p), C.byref(self.h), level, k, t,
+                                 val.ctypes.data_as(C.c_void_p))
+                if rc:
+                    raise RuntimeError("orc_bound error %d" % rc)
+        mbuff = max(self.p.mbc, ibuff + 1)
+        shapes = [(int(b[3] - b[2]) + 1 + 2 * mbuff, int(b[1] - b[0]) + 1 + 2 * mbuff)
+                  for b in Lv["boxes"]]
+        sizes = [a * b for a, b in shapes]
+        foff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        out = np.zeros(max(int(sum(sizes)), 1))
+        reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1))
+        if reg.size == 0:
+            reg = np.zeros(8)
+        rc = L.orc_flag_level(C.byref(self.p), C.byref(self.h), level, tolsp, reg, len(regions),
+                              int(use_old), mbuff, out, foff, int(gradient), int(richardson),
+                              tol, iorder)
+        if rc:
+            raise RuntimeError("orc_flag_level error %d" % rc)
+        return [out[o:o + n].reshape(sh) for o, n, sh in zip(foff, sizes, shapes)]
+
     def reg(self, level, k):
         n = 2 * self.p.meqn * self.r.lenbc[level][k]
         off = self.r.roff[level][k]
@@ -426,6 +465,26 @@ class RefAMR:
         dtn, cfl = C.c_double(), C.c_double()
         self.R.ref_advanc(level, self.H.nvar, C.byref(dtn), C.byref(cfl))
         return dtn.value, cfl.value
+
+    def flag_level(self, level, tolsp, start_time, regions=(), ibuff=3, gradient=True,
+                   richardson=False, tol=0.0, iorder=2):
+        """The reference's flagger.f on the installed hierarchy."""
+        R = self.R
+        R.ref_flag_level.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        Lv = self.H.levels[level - 1]
+        mbuff = max(2, ibuff + 1)
+        shapes = [(int(b[3] - b[2]) + 1 + 2 * mbuff, int(b[1] - b[0]) + 1 + 2 * mbuff)
+                  for b in Lv["boxes"]]
+        sizes = [a * b for a, b in shapes]
+        out = np.zeros(max(int(sum(sizes)), 1))
+        reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1))
+        if reg.size == 0:
+            reg = np.zeros(8)
+        R.ref_flag_level(level, self.H.nvar, start_time, tolsp, tol, iorder, ibuff, int(gradient),
+                         int(richardson), len(regions), reg, out)
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int)
+        return [out[o:o + n].reshape(sh) for o, n, sh in zip(offs, sizes, shapes)]
 
     def set_parallel(self, on):
         """OpenMP over the grids of a level, as the reference's advanc."""

@@ -181,7 +181,7 @@ static void trimbd(const signed char* used, int nrow, int ncol, int* set, int r[
 /* intfil.f90:98-311.  flaguse(ilo:ihi, jlo:jhi) */
 static int intfil(const orc_params* p, const orc_hier* h, double* val, int mi,
                   int mj, double time, signed char* flaguse, int nrowst,
-                  int ncolst, int ilo, int ihi, int jlo, int jhi, int level) {
+                  int ncolst, int ilo, int ihi, int jlo, int jhi, int level, int msrc) {
     const int nv = p->meqn, ng = p->mbc;
     const int fw = ihi - ilo + 1;
 #define FU(i, j) flaguse[((i) - ilo) + (size_t)fw * ((j) - jlo)]
@@ -192,6 +192,15 @@ static int intfil(const orc_params* p, const orc_hier* h, double* val, int mi,
     for (int k = 0; k < h->ngrids[level]; k++) {
         const int* b = h->boxes[level] + 4 * k;
         int imlo = b[0], imhi = b[1], jmlo = b[2], jmhi = b[3];
+        if (msrc >= 0) {
+            /* only the bndList neighbours of grid msrc: the other grids that
+             * meet msrc grown by nghost (init_bndryList.f:69-97;
+             * intfil.f90:153-169) */
+            const int* s0 = h->boxes[level] + 4 * msrc;
+            if (k == msrc || imax(imlo, s0[0] - ng) > imin(imhi, s0[1] + ng) ||
+                imax(jmlo, s0[2] - ng) > imin(jmhi, s0[3] + ng))
+                continue;
+        }
         int mitot = imhi - imlo + 1 + 2 * ng;
         int ixlo = imax(imlo, ilo), ixhi = imin(imhi, ihi);
         int jxlo = imax(jmlo, jlo), jxhi = imin(jmhi, jhi);
@@ -248,7 +257,7 @@ static int prefilrecur(const orc_params* p, const orc_hier* h, int level,
 static int filrecur(const orc_params* p, const orc_hier* h, int level,
                     double* valbig, int mitot, int mjtot, double t,
                     int nrowst, int ncolst, int ilo, int ihi, int jlo, int jhi,
-                    int patchOnly) {
+                    int patchOnly, int msrc) {
     const int nv = p->meqn, ng = p->mbc;
     int rc = 0;
     int mitot_patch = ihi - ilo + 1, mjtot_patch = jhi - jlo + 1;
@@ -257,7 +266,7 @@ static int filrecur(const orc_params* p, const orc_hier* h, int level,
     double ylow_fine = h->ylower + jlo * dy_fine;
     signed char* flaguse = malloc((size_t)mitot_patch * mjtot_patch);
     if (!flaguse) return ORC_E_ALLOC;
-    rc = intfil(p, h, valbig, mitot, mjtot, t, flaguse, nrowst, ncolst, ilo, ihi, jlo, jhi, level);
+    rc = intfil(p, h, valbig, mitot, mjtot, t, flaguse, nrowst, ncolst, ilo, ihi, jlo, jhi, level, msrc);
     if (rc) { free(flaguse); return rc; }
     int set, un[4] = {0, 0, 0, 0};
     trimbd(flaguse, mitot_patch, mjtot_patch, &set, un);
@@ -285,7 +294,7 @@ static int filrecur(const orc_params* p, const orc_hier* h, int level,
                              iplo, iphi, jplo, jphi, iplo, jplo);
         else
             rc = filrecur(p, h, level - 1, valcrse, mitot_coarse, mjtot_coarse, t, 1, 1,
-                          iplo, iphi, jplo, jphi, 1);
+                          iplo, iphi, jplo, jphi, 1, -1);
         if (rc) { free(valcrse); free(flaguse); return rc; }
         double ratiox = (double)rx, ratioy = (double)ry;
 #define CI(n, i, j) valcrse[((n) - 1) + (size_t)nv * ((i) - 1) + (size_t)nv * mitot_coarse * ((j) - 1)]
@@ -374,7 +383,7 @@ static int prefilrecur(const orc_params* p, const orc_hier* h, int level,
             double* valPatch = calloc((size_t)mi * mj * nv, sizeof(double));
             if (!valPatch) return ORC_E_ALLOC;
             int rc = filrecur(p, h, level, valPatch, mi, mj, time, 1, 1, i1 + ishift[i],
-                              i2 + ishift[i], j1 + jshift[j], j2 + jshift[j], 1);
+                              i2 + ishift[i], j1 + jshift[j], j2 + jshift[j], 1, -1);
             if (rc) { free(valPatch); return rc; }
             int ist0 = i1 - iglo + 1, jst0 = j1 - jglo + 1;
             for (int jj = 1; jj <= mj; jj++)
@@ -392,7 +401,13 @@ static int prefilrecur(const orc_params* p, const orc_hier* h, int level,
 /* bound.f90:51-158 */
 int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
               double time, double* valbig) {
-    const int ng = p->mbc;
+    return orc_bound_ng(p, h, level, grid, time, valbig, p->mbc);
+}
+
+/* bound.f90:51-158 with a ghost ring of width ng around the grid (the
+ * storage keeps p->mbc = nghost; prepbigstep.f:53 calls bound with 2 nghost) */
+int orc_bound_ng(const orc_params* p, const orc_hier* h, int level, int grid,
+                 double time, double* valbig, int ng) {
     const int* b = h->boxes[level] + 4 * grid;
     int ilo = b[0], ihi = b[1], jlo = b[2], jhi = b[3];
     int mitot = ihi - ilo + 1 + 2 * ng, mjtot = jhi - jlo + 1 + 2 * ng;
@@ -410,7 +425,7 @@ int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
                          ilo - ng, jlo - ng);
     else
         rc = filrecur(p, h, level, valbig, mitot, mjtot, time, 1, ng + 1, ilo - ng, ilo - 1,
-                      jlo, jhi, 0);
+                      jlo, jhi, 0, grid);
     if (rc) return rc;
     /* right */
     xr = xright + ng * hx;
@@ -419,7 +434,7 @@ int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
                          ilo - ng, jlo - ng);
     else
         rc = filrecur(p, h, level, valbig, mitot, mjtot, time, mitot - ng + 1, ng + 1, ihi + 1,
-                      ihi + ng, jlo, jhi, 0);
+                      ihi + ng, jlo, jhi, 0, grid);
     if (rc) return rc;
     /* bottom */
     xl = xleft - ng * hx; xr = xright + ng * hx; yb = ybot - ng * hy;
@@ -429,7 +444,7 @@ int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
                          jlo - 1, ilo - ng, jlo - ng);
     else
         rc = filrecur(p, h, level, valbig, mitot, mjtot, time, 1, 1, ilo - ng, ihi + ng,
-                      jlo - ng, jlo - 1, 0);
+                      jlo - ng, jlo - 1, 0, grid);
     if (rc) return rc;
     /* top */
     yt = ytop + ng * hy;
@@ -439,7 +454,7 @@ int orc_bound(const orc_params* p, const orc_hier* h, int level, int grid,
                          jhi + ng, ilo - ng, jlo - ng);
     else
         rc = filrecur(p, h, level, valbig, mitot, mjtot, time, 1, mjtot - ng + 1, ilo - ng,
-                      ihi + ng, jhi + 1, jhi + ng, 0);
+                      ihi + ng, jhi + 1, jhi + ng, 0, grid);
     if (rc) return rc;
     (void)yb;
     return orc_bc2amr(p, h, valbig, mitot, mjtot, hx, hy, xloWithGhost, xhiWithGhost,

@@ -446,6 +446,54 @@ contains
                       q, aux, flags)
   end subroutine ref_flag_grid
 
+  ! flagger.f:31-180 for every grid of `level` (flagregions2, flag2refine2
+  ! when fgrad, errest when frich) with the module's tolerances set from
+  ! the arguments; the flags of the grids, each (nx+2 mbuff, ny+2 mbuff),
+  ! are copied to out one after the other and their storage returned
+  subroutine ref_flag_level(level, nvar, start_time, tolsp_in, tol_in, iorder_in, ibuff_in, &
+                            fgrad, frich, nreg, regs, out) bind(C, name="ref_flag_level")
+    use regions_module, only: num_regions, regions
+    integer(c_int), value :: level, nvar, iorder_in, ibuff_in, fgrad, frich, nreg
+    real(c_double), value :: start_time, tolsp_in, tol_in
+    real(c_double) :: regs(8, *), out(*)
+    integer :: j, m, mptr, nx, ny, mbuff, n, off, loc
+    if (allocated(regions)) deallocate(regions)
+    num_regions = nreg
+    allocate(regions(max(nreg, 1)))
+    do m = 1, nreg
+       regions(m)%min_level = int(regs(1, m))
+       regions(m)%max_level = int(regs(2, m))
+       regions(m)%t_low = regs(3, m)
+       regions(m)%t_hi = regs(4, m)
+       regions(m)%x_low = regs(5, m)
+       regions(m)%x_hi = regs(6, m)
+       regions(m)%y_low = regs(7, m)
+       regions(m)%y_hi = regs(8, m)
+    end do
+    tolsp = tolsp_in
+    tol = tol_in
+    iorder = iorder_in
+    ibuff = ibuff_in
+    flag_gradient = fgrad .ne. 0
+    flag_richardson = frich .ne. 0
+    eprint = .false.
+    edebug = .false.
+    call flagger(nvar, 0, level, start_time)
+    mbuff = max(nghost, ibuff + 1)
+    off = 0
+    do j = 1, numgrids(level)
+       mptr = listOfGrids(listStart(level) + j - 1)
+       nx = node(ndihi, mptr) - node(ndilo, mptr) + 1
+       ny = node(ndjhi, mptr) - node(ndjlo, mptr) + 1
+       n = (nx + 2*mbuff) * (ny + 2*mbuff)
+       loc = node(storeflags, mptr)
+       out(off+1:off+n) = alloc(loc:loc+n-1)
+       call reclam(loc, n)
+       if (flag_richardson) call reclam(node(tempptr, mptr), (nx + 2*nghost)*(ny + 2*nghost)*nvar)
+       off = off + n
+    end do
+  end subroutine ref_flag_level
+
 end module ref_harness
 
 ! listbc lives in the double heap as integers (prepc.f:55-59, setuse.f:95);

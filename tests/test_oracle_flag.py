@@ -83,3 +83,85 @@ def test_region_rules():
     f2 = orc.flag_grid(q, mx, my, mb, 0.0, 0.0, h, h, 2.0, 2, 1.0, regs)[mb:-mb, mb:-mb]
     assert np.array_equal(f2[:, 4:6], np.full((my, 2), orc.DOFLAG))
     assert np.all(f2[:, :4] == orc.UNSET) and np.all(f2[:, 6:] == orc.UNSET)
+
+
+# ------------------------------------------------------------ Richardson
+from tests.common import berger_oliger, nested_hierarchy, smooth_euler, acoustic_states  # noqa: E402
+
+FLAGCASES = [((1, 1, 1, 1), False, False), ((3, 3, 3, 3), False, False),
+             ((2, 2, 2, 2), True, True)]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", FLAGCASES)
+@pytest.mark.parametrize("ibuff", [1, 3])
+@pytest.mark.parametrize("phys", ["acoustics", "euler"])
+def test_flagger_matches_reference(case, ibuff, phys):
+    """flagger.f after one Berger-Oliger step: regions, gradient and
+    Richardson flags (errest: ghost fill of width 4 at the old time,
+    coarsening, the 2 dt step, errf1) on every level; ibuff 3 gives
+    mbuff 4 > nghost, where errf1 writes with its nghost origin."""
+    bc, xper, yper = case
+    solver, nvar, states, dt1 = ((orc.ACOUSTICS, 3, acoustic_states, 0.004) if phys == "acoustics"
+                                 else (orc.EULER, 4, smooth_euler, 0.006))
+    rng = np.random.default_rng(1200 + ibuff)
+    H = nested_hierarchy(rng, states, nvar, mthbc=bc, xper=xper, yper=yper, dt1=dt1)
+    p = orc.make_params(solver)
+    O = orc.OracleAMR(p, H.copy())
+    R = orc.RefAMR(p, H)
+    berger_oliger(O, 1, H.nlev, 2)
+    berger_oliger(R, 1, H.nlev, 2)
+    tol = 1e-5 if phys == "euler" else 1e-3
+    # flagging builds level l+1, so it runs on levels below mxnest (whose
+    # old values the Richardson step needs; intfil.f90:205-233)
+    for l in range(1, H.nlev):
+        regs = random_regions(rng, 2, O.h.time[l])
+        a = O.flag_level(l, 0.4, -1.0, regs, ibuff=ibuff, gradient=True, richardson=True, tol=tol)
+        b = R.flag_level(l, 0.4, -1.0, regs, ibuff=ibuff, gradient=True, richardson=True, tol=tol)
+        assert len(a) == len(b)
+        for k, (x, y) in enumerate(zip(a, b)):
+            assert np.array_equal(x, y), (l, k)
+        n = sum(int((x == orc.DOFLAG).sum()) for x in a)
+        assert n > 0
+
+
+@needs_ref
+def test_flagger_initial_time_uses_old_values():
+    rng = np.random.default_rng(77)
+    H = nested_hierarchy(rng, smooth_euler, 4, dt1=0.006)
+    p = orc.make_params(orc.EULER)
+    O = orc.OracleAMR(p, H.copy())
+    R = orc.RefAMR(p, H)
+    berger_oliger(O, 1, H.nlev, 2)
+    berger_oliger(R, 1, H.nlev, 2)
+    for l in range(1, H.nlev):
+        t0 = O.h.time[l] - H.levels[l - 1]["possk"]       # start_time + possk == time
+        a = O.flag_level(l, 0.2, t0, (), ibuff=2, richardson=True, tol=1e-6)
+        b = R.flag_level(l, 0.2, t0, (), ibuff=2, richardson=True, tol=1e-6)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), l
+
+
+@pytest.mark.parametrize("ibuff", [1, 3])
+def test_richardson_closed_form_on_uniform_data(ibuff):
+    """A uniform state: both steps leave it unchanged, so every estimate is
+    exactly 0 — no cell passes tol > 0, and with tol = 0 (est >= tol) every
+    coarse cell marks its 2x2 block.  The blocks start at flag index
+    nghost + 1 (errf1.f:57-58), which for mbuff = 4 (ibuff 3) lies two
+    cells down and left of the first interior cell."""
+    from amrclaw_amd.hierarchy import Hier
+    H = Hier(3)
+    nx, ny = 16, 12
+    H.add_level([(0, nx - 1, 0, ny - 1)], 1.0 / nx, 1.0 / ny, 0.01, 0.01, nx, ny, 2, 2)
+    H.add_level([], 0.5 / nx, 0.5 / ny, 0.005, 0.01, 2 * nx, 2 * ny, 2, 2)
+    H.grid(1, 0, 0)[:] = (0.7, -0.2, 0.3)
+    H.grid(1, 0, 1)[:] = (0.7, -0.2, 0.3)
+    p = orc.make_params(orc.ACOUSTICS)
+    O = orc.OracleAMR(p, H)
+    mb = max(2, ibuff + 1)
+    f = O.flag_level(1, 1.0, -1.0, (), ibuff=ibuff, gradient=False, richardson=True, tol=1e-300)[0]
+    assert np.all(f == orc.UNSET)
+    f = O.flag_level(1, 1.0, -1.0, (), ibuff=ibuff, gradient=False, richardson=True, tol=0.0)[0]
+    want = np.full((ny + 2 * mb, nx + 2 * mb), orc.UNSET)
+    want[2:2 + ny, 2:2 + nx] = orc.DOFLAG          # 0-based start = nghost
+    assert np.array_equal(f, want)
