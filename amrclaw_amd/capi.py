@@ -93,8 +93,9 @@ def lib():
         L.amrgpu_set_graphs.argtypes = [C.c_int]
         L.amrgpu_set_timing.argtypes = [C.c_int]
         L.amrgpu_kernel_time.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
-        L.amrgpu_flag_level.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p,
-                                        C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
+        L.amrgpu_flag_level.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                        C.c_double, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.POINTER(C.c_int64)]
         L.amrgpu_event_record.argtypes = [C.c_int]
         L.amrgpu_event_elapsed.argtypes = [C.c_int, C.c_int,
                                            C.POINTER(C.c_float)]
@@ -284,23 +285,24 @@ def set_corners(level, xlo, ylo):
                                     np.ascontiguousarray(ylo, dtype=np.float64)))
 
 
-def flag_level(level, tolsp, start_time, regions=(), mbuff=4, shapes=None):
+def flag_level(level, tolsp, start_time, regions=(), mbuff=4, shapes=None, gradient=True,
+               richardson=False, tol=0.0, iorder=2):
     """Flag the cells of every grid of a level.  With shapes (list of (mx,
     my)) the flags are returned as a list of (my+2 mbuff, mx+2 mbuff)
     arrays; returns (flags or None, number of DOFLAG cells)."""
     reg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1))
     n = C.c_int64()
     rp = reg.ctypes.data_as(C.c_void_p) if reg.size else None
+    head = (level, int(bool(gradient)), tolsp, int(bool(richardson)), tol, iorder, start_time,
+            len(regions), rp, mbuff)
     if shapes is None:
-        _check(lib().amrgpu_flag_level(level, tolsp, start_time, len(regions), rp, mbuff,
-                                       None, None, C.byref(n)))
+        _check(lib().amrgpu_flag_level(*head, None, None, C.byref(n)))
         return None, n.value
     sizes = [(mx + 2 * mbuff) * (my + 2 * mbuff) for mx, my in shapes]
     offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     out = np.zeros(int(sum(sizes)))
-    _check(lib().amrgpu_flag_level(level, tolsp, start_time, len(regions), rp, mbuff,
-                                   out.ctypes.data_as(C.c_void_p), offs.ctypes.data_as(C.c_void_p),
-                                   C.byref(n)))
+    _check(lib().amrgpu_flag_level(*head, out.ctypes.data_as(C.c_void_p),
+                                   offs.ctypes.data_as(C.c_void_p), C.byref(n)))
     flags = [out[o:o + sz].reshape(my + 2 * mbuff, mx + 2 * mbuff)
              for o, sz, (mx, my) in zip(offs, sizes, shapes)]
     return flags, n.value

@@ -92,6 +92,23 @@ struct Level {
     RegionRect* d_rects = nullptr;
     int64_t rects_cap = 0;
     unsigned long long* d_counters = nullptr;
+    // Richardson estimate (errest.f): its fill program, the two FP64 arenas
+    // (inputs of the two stepgrid calls and their results) and the tiles
+    bool rich_valid = false;
+    RichPlan rich;
+    FillOp* d_rops = nullptr;
+    FillBuf* d_rbufs = nullptr;
+    unsigned char* d_rmasks = nullptr;
+    double* d_rscratch = nullptr;
+    double* d_rin = nullptr;
+    double* d_rout = nullptr;
+    PatchDesc* d_rcrs = nullptr;
+    PatchDesc* d_rfine = nullptr;
+    std::vector<TileDesc> rtiles_c, rtiles_f;
+    TileDesc* d_rtiles_c = nullptr;
+    TileDesc* d_rtiles_f = nullptr;
+    size_t rlds_c = 0, rlds_f = 0;
+    unsigned long long* d_rcfl = nullptr;
     // per-patch launch mode: tiles grouped by patch
     std::vector<int> patch_tile_start;
     // captured level steps, keyed by everything their launch arguments hold
@@ -186,8 +203,24 @@ void tend(int kind) {
     }
 }
 
+void free_rich(Level& L) {
+    quiesce();
+    void* ps[] = {L.d_rops, L.d_rbufs, L.d_rmasks, L.d_rscratch, L.d_rin, L.d_rout, L.d_rcrs,
+                  L.d_rfine, L.d_rtiles_c, L.d_rtiles_f, L.d_rcfl};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    L.d_rops = nullptr; L.d_rbufs = nullptr; L.d_rmasks = nullptr; L.d_rscratch = nullptr;
+    L.d_rin = nullptr; L.d_rout = nullptr; L.d_rcrs = nullptr; L.d_rfine = nullptr;
+    L.d_rtiles_c = nullptr; L.d_rtiles_f = nullptr; L.d_rcfl = nullptr;
+    L.rich = RichPlan();
+    L.rtiles_c.clear();
+    L.rtiles_f.clear();
+    L.rich_valid = false;
+}
+
 void free_plan(Level& L) {
     quiesce();
+    free_rich(L);
     if (L.d_ops) hipFree(L.d_ops);
     if (L.d_bufs) hipFree(L.d_bufs);
     if (L.d_masks) hipFree(L.d_masks);
@@ -932,9 +965,137 @@ int update_level(int level) {
     return AMRGPU_OK;
 }
 
-// flagger.f:56-164 for every grid of `level` (no Richardson estimate)
-int flag_level(int level, double tolsp, double start_time, int nreg, const double* regions,
-               int mbuff, double* flags_base, const int64_t* offsets, int64_t* nflagged) {
+// FP64 tiles for the stepgrid calls of the Richardson estimate
+int plan_f64_tiles(const std::vector<PatchDesc>& P, std::vector<TileDesc>& tiles, size_t& lds) {
+    tiles.clear();
+    lds = 0;
+    const int nv = g.cfg.nvar, txmax = 28, tymax = 12;
+    for (int p = 0; p < (int)P.size(); p++) {
+        auto xs = split_axis(P[p].mx, txmax), ys = split_axis(P[p].my, tymax);
+        for (auto& y : ys)
+            for (auto& x : xs) {
+                TileDesc t{};
+                t.patch = p;
+                t.i0 = (int16_t)x.first; t.tx = (int16_t)x.second;
+                t.j0 = (int16_t)y.first; t.ty = (int16_t)y.second;
+                t.wx = (int16_t)seg_width(t.tx);
+                t.wy = (int16_t)seg_width(t.ty);
+                tiles.push_back(t);
+                lds = std::max(lds, sweep_lds_bytes<double>(nv, t.tx, t.ty));
+            }
+    }
+    return AMRGPU_OK;
+}
+
+int ensure_rich(int level) {
+    Level& L = g.levels[level];
+    if (L.rich_valid) return AMRGPU_OK;
+    free_rich(L);
+    HierGeom H = make_geom();
+    std::string err;
+    if (!plan_richardson(H, level, g.cfg.nvar, L.rich, err))
+        return fail(AMRGPU_ERR_ARG, "level %d Richardson estimate: %s", level, err.c_str());
+    const FillPlan& P = L.rich.fill;
+    int rc;
+    std::vector<FillOp> ops = P.ops;
+    if ((rc = upload(&L.d_rops, ops))) return rc;
+    if ((rc = upload(&L.d_rbufs, P.bufs))) return rc;
+    if ((rc = upload(&L.d_rmasks, P.masks))) return rc;
+    if (P.scratch_elems > 0) HIPCHK(hipMalloc(&L.d_rscratch, P.scratch_elems * sizeof(double)));
+    HIPCHK(hipMalloc(&L.d_rin, std::max<int64_t>(L.rich.arena_elems, 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&L.d_rout, std::max<int64_t>(L.rich.arena_elems, 1) * sizeof(double)));
+    if ((rc = upload(&L.d_rcrs, L.rich.crs))) return rc;
+    if ((rc = upload(&L.d_rfine, L.rich.fine))) return rc;
+    plan_f64_tiles(L.rich.crs, L.rtiles_c, L.rlds_c);
+    plan_f64_tiles(L.rich.fine, L.rtiles_f, L.rlds_f);
+    if ((rc = upload(&L.d_rtiles_c, L.rtiles_c))) return rc;
+    if ((rc = upload(&L.d_rtiles_f, L.rtiles_f))) return rc;
+    HIPCHK(hipMalloc(&L.d_rcfl, std::max(L.ngrids, 1) * sizeof(unsigned long long)));
+    L.rich_valid = true;
+    return AMRGPU_OK;
+}
+
+// errest.f:4-63 for every grid of `level` after the gradient flags
+int richardson(int level, bool big_old, int mbuff, double tol, int iorder) {
+    Level& L = g.levels[level];
+    int rc = ensure_rich(level);
+    if (rc) return rc;
+    // prepbigstep.f: copysol, bound at tpre = time - dt with 2 nghost,
+    // coarsen; flagger's locbig copy
+    const double tpre = L.time - L.possk;
+    FillArgs a{};
+    rc = time_weights(tpre, a, L.rich.fill.src_levels_mask);
+    if (rc) return rc;
+    a.bufs = L.d_rbufs;
+    a.scratch = L.d_rscratch;
+    a.masks = L.d_rmasks;
+    a.arena = L.d_rin;
+    a.big_old = big_old ? 1 : 0;
+    for (int l = 1; l < MAXLV && l < FILL_MAXLV; l++) {
+        Level& S = g.levels[l];
+        a.lvl_new[l] = S.buf[S.cur];
+        a.lvl_old[l] = S.buf[1 - S.cur];
+        a.lvl_patches[l] = S.d_patches;
+        a.lvl_reg[l] = S.d_reg;
+        a.hx[l] = g.hxposs[l];
+        a.hy[l] = g.hyposs[l];
+    }
+    a.xlower = g.cfg.xlower;
+    a.ylower = g.cfg.ylower;
+    for (int k = 0; k < 4; k++) a.mthbc[k] = g.cfg.mthbc[k];
+    a.nv = g.cfg.nvar;
+    tbegin(AMRGPU_KT_FLAG);
+    for (int s = 0; s < L.rich.fill.nstages; s++) {
+        a.ops = L.d_rops + L.rich.fill.stage_start[s];
+        a.nops = L.rich.fill.stage_start[s + 1] - L.rich.fill.stage_start[s];
+        hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
+        if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "Richardson fill stage %d: %s", s, hipGetErrorString(e));
+    }
+    // the 2 dt step on 2 h (prepbigstep.f:61-64) and the dt step
+    // (prepregstep.f:24-27), FP64, into the output arena
+    SweepArgs sa{};
+    sa.qin = L.d_rin;
+    sa.qout = L.d_rout;
+    sa.meth = g.meth;
+    sa.cfl_patch = L.d_rcfl;
+    sa.patches = L.d_rcrs;
+    sa.tiles = L.d_rtiles_c;
+    sa.ntiles = (int)L.rtiles_c.size();
+    sa.dt = 2.0 * L.possk;
+    sa.dx = 2.0 * L.hx;
+    sa.dy = 2.0 * L.hy;
+    hipError_t e = launch_sweep_f64(g.cfg.solver, sa, L.rlds_c, g.stream);
+    if (e == hipSuccess) {
+        sa.patches = L.d_rfine;
+        sa.tiles = L.d_rtiles_f;
+        sa.ntiles = (int)L.rtiles_f.size();
+        sa.dt = L.possk;
+        sa.dx = L.hx;
+        sa.dy = L.hy;
+        e = launch_sweep_f64(g.cfg.solver, sa, L.rlds_f, g.stream);
+    }
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "Richardson steps: %s", hipGetErrorString(e));
+    ErrfArgs ea{};
+    ea.crs = L.d_rcrs;
+    ea.fine = L.d_rfine;
+    ea.crs_q = L.d_rout;
+    ea.fine_q = L.d_rout;
+    ea.flags = L.d_flags;
+    ea.foff = L.d_foff;
+    ea.counters = L.d_counters;
+    ea.mbuff = mbuff;
+    ea.tol = tol;
+    ea.order = (double)((1 << (iorder + 1)) - 2);                 // errf1.f:35
+    e = launch_errf1(ea, L.ngrids, g.stream);
+    tend(AMRGPU_KT_FLAG);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "errf1: %s", hipGetErrorString(e));
+    return AMRGPU_OK;
+}
+
+// flagger.f:56-164 for every grid of `level`
+int flag_level(int level, bool gradient, double tolsp, bool rich, double tol, int iorder,
+               double start_time, int nreg, const double* regions, int mbuff, double* flags_base,
+               const int64_t* offsets, int64_t* nflagged) {
     Level& L = g.levels[level];
     if (nflagged) *nflagged = 0;
     if (L.ngrids == 0) return AMRGPU_OK;
@@ -1004,11 +1165,15 @@ int flag_level(int level, double tolsp, double start_time, int nreg, const doubl
     a.nv = g.cfg.nvar;
     a.mbuff = mbuff;
     a.level = level;
-    a.tolsp = tolsp;
+    // without gradient flagging no cell passes: flag2refine2 is not called
+    a.tolsp = gradient ? tolsp : HUGE_VAL;
     tbegin(AMRGPU_KT_FLAG);
     hipError_t e = launch_flag((int)g.esize, a, L.ngrids, g.stream);
     tend(AMRGPU_KT_FLAG);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "flag: %s", hipGetErrorString(e));
+    // errest runs while any flag is UNSET (flagger.f:166-176); the mbuff
+    // border always is
+    if (rich && (rc = richardson(level, use_old, mbuff, tol, iorder))) return rc;
     unsigned long long cnt[2];
     HIPCHK(hipMemcpyAsync(cnt, L.d_counters, sizeof cnt, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -1302,16 +1467,21 @@ int amrgpu_set_corners(int level, const double* xlo, const double* ylo) {
     return AMRGPU_OK;
 }
 
-int amrgpu_flag_level(int level, double tolsp, double start_time, int nregions,
+int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richardson,
+                      double tol, int iorder, double start_time, int nregions,
                       const double* regions, int mbuff, double* flags_base,
                       const int64_t* offsets, int64_t* nflagged) {
     int rc = check_level(level);
     if (rc) return rc;
     if (level < 1 || level > g.mxnest) return fail(AMRGPU_ERR_ARG, "level %d outside 1..mxnest", level);
     if (nregions < 0 || (nregions > 0 && !regions) || mbuff < NGHOST || mbuff > 64 ||
-        (flags_base && !offsets))
+        (flags_base && !offsets) || iorder < 1 || iorder > 8)
         return fail(AMRGPU_ERR_ARG, "bad flag_level arguments");
-    return flag_level(level, tolsp, start_time, nregions, regions, mbuff, flags_base, offsets, nflagged);
+    if (flag_richardson && level == g.mxnest)
+        return fail(AMRGPU_ERR_ARG, "Richardson estimation needs old values, which level %d "
+                                    "(mxnest) does not keep", level);
+    return flag_level(level, flag_gradient != 0, tolsp, flag_richardson != 0, tol, iorder, start_time,
+                      nregions, regions, mbuff, flags_base, offsets, nflagged);
 }
 
 int amrgpu_set_launch_mode(int mode) {

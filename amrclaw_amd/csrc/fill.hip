@@ -32,7 +32,7 @@ __device__ BufRef<S> bufref(const FillArgs& a, int b) {
         r.off = a.lvl_patches[fb.level][fb.patch].off;
     } else {
         r.lvl = nullptr;
-        r.scr = a.scratch;
+        r.scr = fb.kind == 1 ? a.scratch : a.arena;
         r.off = fb.off;
     }
     r.nrow = fb.nrow;
@@ -51,7 +51,10 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
         const PatchDesc sp = a.lvl_patches[op.src_level][op.src];
         const S* qn = static_cast<const S*>(a.lvl_new[op.src_level]) + sp.off;
         const S* qo = static_cast<const S*>(a.lvl_old[op.src_level]) + sp.off;
-        const int mode = (op.flags & FOPF_FORCE_NEW) ? (int)TIME_NEW : a.tmode[op.src_level];
+        int mode = a.tmode[op.src_level];
+        if (op.flags & FOPF_FORCE_NEW) mode = TIME_NEW;
+        else if (op.flags & FOPF_FORCE_OLD) mode = TIME_OLD;
+        else if (op.flags & FOPF_BIG) mode = a.big_old ? TIME_OLD : TIME_NEW;
         const double ai = a.alphai[op.src_level], ac = a.alphac[op.src_level];
         for (int t = tid; t < ncell * nv; t += 256) {
             const int m = t / ncell, c = t - m * ncell;
@@ -127,6 +130,16 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
             else if (idx < 2 * n1 + n2) { i = nrow - 1; j = 1 + idx - n1 - n2; }
             else { i = 1 + idx - 2 * n1 - n2; j = 0; }
             qc1d[(int64_t)idx * nv + m] = src.load(m, i, j);
+        }
+    } else if (op.type == FOP_COARSEN) {
+        // coarsen.f:15-30: dst(i, j) = mean of src(2i..2i+1, 2j..2j+1)
+        const BufRef<S> src = bufref<S>(a, op.src);
+        for (int t = tid; t < ncell * nv; t += 256) {
+            const int m = t / ncell, c = t - m * ncell;
+            const int j = c / op.ni, i = c - j * op.ni;
+            const double v = (src.load(m, 2 * i, 2 * j) + src.load(m, 2 * i + 1, 2 * j) +
+                              src.load(m, 2 * i, 2 * j + 1) + src.load(m, 2 * i + 1, 2 * j + 1)) / 4.0;
+            dst.store(m, i, j, v);
         }
     } else if (op.type == FOP_BC) {
         // sides in the reference's order, each complete before the next

@@ -21,15 +21,21 @@ enum FillOpType : int32_t {
     FOP_INTERP = 1,    // filrecur: limited linear interpolation (filpatch.f90:260-327)
     FOP_BC = 2,        // bc2amr on a whole buffer (bc2amr.f90:84-296)
     FOP_COPYBACK = 3,  // prefilrecur: patchCopyOut (prefilp.f90:218-245)
-    FOP_RING = 4       // cstore: coarse perimeter ring of a buffer -> qc1d (cstore.f:4-63)
+    FOP_RING = 4,      // cstore: coarse perimeter ring of a buffer -> qc1d (cstore.f:4-63)
+    FOP_COARSEN = 5    // coarsen.f: 2x2 means of a buffer into another (prepbigstep.f:58)
 };
-// FillOp.flags bits
-enum : int32_t { FOPF_FORCE_NEW = 1 };   // icall copies store1, no time blend (icall.f)
+// FillOp.flags bits (COPY)
+enum : int32_t {
+    FOPF_FORCE_NEW = 1,   // icall copies store1, no time blend (icall.f)
+    FOPF_FORCE_OLD = 2,   // copysol of store2 (prepbigstep.f:35)
+    FOPF_BIG = 4          // flagger's locbig: store1, or store2 when FillArgs.big_old (flagger.f:98-117)
+};
 
 // A destination/scratch array, SoA over components: cell (i, j), 0-based
 // with ghosts, component m at off + m*plane + j*nrow + i.
 struct FillBuf {
-    int32_t kind;      // 0: patch `patch` of level `level` (level storage); 1: FP64 scratch
+    int32_t kind;      // 0: patch `patch` of level `level` (level storage); 1: FP64 scratch;
+                       // 2: FP64 arena (FillArgs.arena) read afterwards by other kernels
     int32_t level;
     int32_t patch;
     int32_t nrow, ncol, plane;
@@ -73,6 +79,9 @@ struct FillArgs {
     int32_t mthbc[4];
     int32_t nv;
     int32_t nops;
+    double* arena;                        // kind-2 buffers
+    int32_t big_old;                      // FOPF_BIG copies read store2
+    int32_t pad;
 };
 
 }  // namespace amrgpu

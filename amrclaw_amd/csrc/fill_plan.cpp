@@ -138,7 +138,7 @@ struct Planner {
     // `buf` starting at its 1-based cell (nrowst, ncolst).  Returns the stage
     // after which the rectangle (and, with patchOnly, the buffer) is final.
     int filrecur(int level, int buf, int nrowst, int ncolst, int ilo, int ihi, int jlo, int jhi,
-                 bool patchOnly) {
+                 bool patchOnly, int msrc = -1) {
         if (!err.empty()) return -1;
         const LevelGeom& G = H.lev[level];
         const int ni = ihi - ilo + 1, nj = jhi - jlo + 1;
@@ -151,6 +151,15 @@ struct Planner {
         // intfil: same-level grids intersecting the rectangle
         for (int k = 0; k < (int)G.boxes.size(); k++) {
             const auto& b = G.boxes[k];
+            if (msrc >= 0) {
+                // a grid's own strips take same-level values only from its
+                // bndList neighbours: the grids meeting it grown by nghost
+                // (init_bndryList.f:69-97, intfil.f90:153-169)
+                const auto& m = G.boxes[msrc];
+                if (k == msrc || std::max(b[0], m[0] - NGHOST) > std::min(b[1], m[1] + NGHOST) ||
+                    std::max(b[2], m[2] - NGHOST) > std::min(b[3], m[3] + NGHOST))
+                    continue;
+            }
             int ixlo = std::max(b[0], ilo), ixhi = std::min(b[1], ihi);
             int jxlo = std::max(b[2], jlo), jxhi = std::min(b[3], jhi);
             if (!(ixlo <= ixhi && jxlo <= jxhi)) continue;
@@ -445,37 +454,54 @@ struct Planner {
         return true;
     }
 
-    // bound.f90:51-158 for grid `grid` of `level`
-    bool bound(int level, int grid) {
+    // an FP64 arena buffer (kind 2) laid out like a patch: nrow x ncol cells
+    int arena(int nrow, int ncol, int64_t& arena_elems) {
+        FillBuf fb{};
+        fb.kind = 2;
+        fb.level = 0;
+        fb.patch = -1;
+        fb.nrow = nrow;
+        fb.ncol = ncol;
+        fb.plane = nrow * ncol;
+        fb.off = arena_elems;
+        arena_elems += ((int64_t)nv * fb.plane + 31) / 32 * 32;
+        P.bufs.push_back(fb);
+        return (int)P.bufs.size() - 1;
+    }
+
+    // bound.f90:51-158 for grid `grid` of `level` into its own storage
+    bool bound(int level, int grid) { return bound_into(level, grid, pbuf(level, grid), NGHOST); }
+
+    // bound.f90:51-158 with a ghost ring of width ng into buffer `buf`
+    // ((mx + 2 ng) x (my + 2 ng), the grid's interior at (ng, ng))
+    bool bound_into(int level, int grid, int buf, int ng) {
         const LevelGeom& G = H.lev[level];
         const auto& b = G.boxes[grid];
-        const int ng = NGHOST;
         const int ilo = b[0], ihi = b[1], jlo = b[2], jhi = b[3];
         const int mitot = ihi - ilo + 1 + 2 * ng, mjtot = jhi - jlo + 1 + 2 * ng;
         const double hx = G.hx, hy = G.hy;
         const double xleft = ilo * hx + H.xlower, xright = (ihi + 1) * hx + H.xlower;
         const double ybot = jlo * hy + H.ylower, ytop = (jhi + 1) * hy + H.ylower;
-        const int buf = pbuf(level, grid);
         int r[4];
         double xl = xleft - ng * hx, xr = xright + ng * hx;
         if (xl < H.xlower && H.xperdom)
             r[0] = prefil(level, buf, ilo - ng, ilo - 1, jlo, jhi, ilo - ng, jlo - ng);
         else
-            r[0] = filrecur(level, buf, 1, ng + 1, ilo - ng, ilo - 1, jlo, jhi, false);
+            r[0] = filrecur(level, buf, 1, ng + 1, ilo - ng, ilo - 1, jlo, jhi, false, grid);
         if (xr > H.xupper && H.xperdom)
             r[1] = prefil(level, buf, ihi + 1, ihi + ng, jlo, jhi, ilo - ng, jlo - ng);
         else
-            r[1] = filrecur(level, buf, mitot - ng + 1, ng + 1, ihi + 1, ihi + ng, jlo, jhi, false);
+            r[1] = filrecur(level, buf, mitot - ng + 1, ng + 1, ihi + 1, ihi + ng, jlo, jhi, false, grid);
         double yb = ybot - ng * hy, yt = ytop + ng * hy;
         bool xsticks = (xl < H.xlower || xr > H.xupper) && H.xperdom;
         if ((yb < H.ylower && (H.yperdom || H.spheredom)) || xsticks)
             r[2] = prefil(level, buf, ilo - ng, ihi + ng, jlo - ng, jlo - 1, ilo - ng, jlo - ng);
         else
-            r[2] = filrecur(level, buf, 1, 1, ilo - ng, ihi + ng, jlo - ng, jlo - 1, false);
+            r[2] = filrecur(level, buf, 1, 1, ilo - ng, ihi + ng, jlo - ng, jlo - 1, false, grid);
         if ((yt > H.yupper && (H.yperdom || H.spheredom)) || xsticks)
             r[3] = prefil(level, buf, ilo - ng, ihi + ng, jhi + 1, jhi + ng, ilo - ng, jlo - ng);
         else
-            r[3] = filrecur(level, buf, 1, mjtot - ng + 1, ilo - ng, ihi + ng, jhi + 1, jhi + ng, false);
+            r[3] = filrecur(level, buf, 1, mjtot - ng + 1, ilo - ng, ihi + ng, jhi + 1, jhi + ng, false, grid);
         if (!err.empty()) return false;
         int ready = std::max(std::max(r[0], r[1]), std::max(r[2], r[3]));
         return bc_op(buf, hx, hy, xl, xr, yb, yt, ready + 1);
@@ -501,6 +527,91 @@ bool plan_saveqc(const HierGeom& H, int level, int nvar, FillPlan& P, std::strin
             err = pl.err.empty() ? "saveqc planning failed" : pl.err;
             return false;
         }
+    finish_plan(P);
+    return true;
+}
+
+bool plan_richardson(const HierGeom& H, int level, int nvar, RichPlan& R, std::string& err) {
+    R = RichPlan();
+    FillPlan& P = R.fill;
+    Planner pl(H, P, nvar);
+    const LevelGeom& G = H.lev[level];
+    const int ng = NGHOST, ng2 = 2 * NGHOST;
+    std::vector<int> dub(G.boxes.size());
+    for (int k = 0; k < (int)G.boxes.size(); k++) {
+        const auto& b = G.boxes[k];
+        const int mx = b[1] - b[0] + 1, my = b[3] - b[2] + 1;
+        if (mx % 2 || my % 2) {
+            err = "Richardson estimation needs grids of even size (errest.f:27-28)";
+            return false;
+        }
+        // copysol.f: the old interior into the centre of the wider array
+        const int d = pl.scratch(mx + 2 * ng2, my + 2 * ng2);
+        dub[k] = d;
+        FillOp op{};
+        op.type = FOP_COPY;
+        op.flags = FOPF_FORCE_OLD;
+        op.dst = d;
+        op.src = k;
+        op.src_level = level;
+        op.di0 = ng2;
+        op.dj0 = ng2;
+        op.ni = mx;
+        op.nj = my;
+        op.si0 = ng;
+        op.sj0 = ng;
+        op.stage = 0;
+        pl.push(op);
+        if (!pl.bound_into(level, k, d, ng2)) {
+            err = pl.err.empty() ? "Richardson ghost-fill planning failed" : pl.err;
+            return false;
+        }
+    }
+    int last = 0;
+    for (const auto& op : P.ops) last = std::max(last, op.stage);
+    for (int k = 0; k < (int)G.boxes.size(); k++) {
+        const auto& b = G.boxes[k];
+        const int mx = b[1] - b[0] + 1, my = b[3] - b[2] + 1;
+        // coarsen.f into the coarse array (mx/2 + 2 nghost) x (my/2 + 2 nghost)
+        const int c = pl.arena(mx / 2 + 2 * ng, my / 2 + 2 * ng, R.arena_elems);
+        FillOp op{};
+        op.type = FOP_COARSEN;
+        op.dst = c;
+        op.src = dub[k];
+        op.ni = mx / 2 + 2 * ng;
+        op.nj = my / 2 + 2 * ng;
+        op.stage = last + 1;
+        pl.push(op);
+        // flagger's locbig: the grid's values with their ghost cells
+        const int f = pl.arena(mx + 2 * ng, my + 2 * ng, R.arena_elems);
+        FillOp cp{};
+        cp.type = FOP_COPY;
+        cp.flags = FOPF_BIG;
+        cp.dst = f;
+        cp.src = k;
+        cp.src_level = level;
+        cp.ni = mx + 2 * ng;
+        cp.nj = my + 2 * ng;
+        cp.stage = 0;
+        pl.push(cp);
+        PatchDesc pc{}, pf{};
+        pc.off = P.bufs[c].off;
+        pc.mx = mx / 2;
+        pc.my = my / 2;
+        pc.mitot = pc.mx + 2 * ng;
+        pc.plane = pc.mitot * (pc.my + 2 * ng);
+        pc.ilo = b[0] / 2; pc.ihi = pc.ilo + pc.mx - 1;
+        pc.jlo = b[2] / 2; pc.jhi = pc.jlo + pc.my - 1;
+        pf.off = P.bufs[f].off;
+        pf.mx = mx;
+        pf.my = my;
+        pf.mitot = mx + 2 * ng;
+        pf.plane = pf.mitot * (my + 2 * ng);
+        pf.ilo = b[0]; pf.ihi = b[1];
+        pf.jlo = b[2]; pf.jhi = b[3];
+        R.crs.push_back(pc);
+        R.fine.push_back(pf);
+    }
     finish_plan(P);
     return true;
 }

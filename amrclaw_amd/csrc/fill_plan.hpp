@@ -37,6 +37,19 @@ struct FillPlan {
 };
 
 bool plan_fill_level(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
+
+// errest.f for every grid of `level`: the fill program builds, in an FP64
+// arena, the coarsened old solution of each grid (copysol of store2,
+// bound with 2 nghost ghost cells at the old time, coarsen.f) and the copy
+// of the grid's new (or old) values with ghost cells; crs / fine describe
+// those arrays as patches (offsets into the arena) for the two stepgrid
+// calls of prepbigstep.f / prepregstep.f.
+struct RichPlan {
+    FillPlan fill;
+    std::vector<PatchDesc> crs, fine;
+    int64_t arena_elems = 0;
+};
+bool plan_richardson(const HierGeom& H, int level, int nvar, RichPlan& R, std::string& err);
 // saveqc.f for the grids of fine level `level` (coarse level `level`-1)
 bool plan_saveqc(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
 

@@ -64,6 +64,53 @@ __global__ __launch_bounds__(256) void flag_kernel(FlagArgs a) {
     }
 }
 
+// errf1.f:33-137, one coarse cell per thread.  Its 2x2 block of flags is
+// read and written by that thread only, so the two passes of errf1 (estimate
+// every cell, then mark) merge into one.  As errf1 does, the block of coarse
+// cell (i, j) is addressed from flag index nghost + 1 of the
+// (mx + 2 mbuff) x (my + 2 mbuff) array.
+__global__ __launch_bounds__(256) void errf1_kernel(ErrfArgs a) {
+    const int g = blockIdx.x;
+    const PatchDesc pc = a.crs[g], pf = a.fine[g];
+    const double* rc = a.crs_q + pc.off;             // component 1 plane
+    const double* rf = a.fine_q + pf.off;
+    double* fl = a.flags + a.foff[g];
+    const int mib = pf.mx + 2 * a.mbuff;
+    const int n = pc.mx * pc.my;
+    int count = 0;
+    for (int c = threadIdx.x; c < n; c += 256) {
+        const int jc = c / pc.mx, ic = c - jc * pc.mx;
+        const int ifine = NGHOST + 1 + 2 * ic, jfine = NGHOST + 1 + 2 * jc;   // 1-based
+        double* f00 = fl + (ifine - 1) + (int64_t)mib * (jfine - 1);
+        double* f01 = f00 + mib;
+        if (!(f00[0] == FLAG_UNSET || f00[1] == FLAG_UNSET || f01[0] == FLAG_UNSET ||
+              f01[1] == FLAG_UNSET))
+            continue;
+        const double* r0 = rf + (int64_t)(jfine - 1) * pf.mitot + (ifine - 1);
+        const double* r1 = r0 + pf.mitot;
+        const double term1 = r0[0], term2 = r0[1], term3 = r1[1], term4 = r1[0];
+        const double aval = (term1 + term2 + term3 + term4) / 4.0;
+        const double cv = rc[(int64_t)(jc + NGHOST) * pc.mitot + (ic + NGHOST)];
+        const double est = fabs((aval - cv) / a.order);
+        if (est >= a.tol) {
+            count += (f00[0] != FLAG_DOFLAG) + (f00[1] != FLAG_DOFLAG) +
+                     (f01[0] != FLAG_DOFLAG) + (f01[1] != FLAG_DOFLAG);
+            f00[0] = FLAG_DOFLAG;
+            f00[1] = FLAG_DOFLAG;
+            f01[0] = FLAG_DOFLAG;
+            f01[1] = FLAG_DOFLAG;
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) count += __shfl_xor(count, off, 64);
+    if ((threadIdx.x & 63) == 0 && count) atomicAdd(a.counters, (unsigned long long)count);
+}
+
+hipError_t launch_errf1(const ErrfArgs& a, int ngrids, hipStream_t st) {
+    if (ngrids <= 0) return hipSuccess;
+    hipLaunchKernelGGL(errf1_kernel, dim3(ngrids), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_flag(int esize, const FlagArgs& a, int ngrids, hipStream_t st) {
     if (ngrids <= 0) return hipSuccess;
     if (esize == 8)

@@ -28,4 +28,21 @@ struct FlagArgs {
     double tolsp;
 };
 
+// errf1.f:33-137 over every grid of a level: the coarse cells of the
+// 2 dt step (crs, in `crs_q`) against the 2x2 means of the dt step (fine,
+// in `fine_q`), component 1, FP64
+struct ErrfArgs {
+    const PatchDesc* crs;
+    const PatchDesc* fine;
+    const double* crs_q;
+    const double* fine_q;
+    double* flags;
+    const int64_t* foff;
+    unsigned long long* counters;   // [0] DOFLAG cells (incremented for new marks)
+    int32_t mbuff;
+    int32_t pad;
+    double tol;
+    double order;                   // 2^(iorder+1) - 2
+};
+
 }  // namespace amrgpu

@@ -12,6 +12,7 @@ hipError_t launch_finereg(int solver, int esize, const QadArgs& a, const Physics
 hipError_t launch_update(int esize, const UpdateArgs& a, const int64_t* d_block_start,
                          int64_t navg_cells, hipStream_t st);
 hipError_t launch_flag(int esize, const FlagArgs& a, int ngrids, hipStream_t st);
+hipError_t launch_errf1(const ErrfArgs& a, int ngrids, hipStream_t st);
 hipError_t launch_conck(int esize, const PatchDesc* patches, int npatch, const void* q,
                         double* partial, hipStream_t st);
 }  // namespace amrgpu

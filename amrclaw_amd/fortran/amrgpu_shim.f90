@@ -82,11 +82,12 @@ module amrgpu_iface
        integer(c_int), value :: level
        real(c_double) :: totmass
      end function
-     integer(c_int) function amrgpu_flag_level(level, tolsp, start_time, nregions, regions, &
-          mbuff, flags_base, offsets, nflagged) bind(C, name="amrgpu_flag_level")
+     integer(c_int) function amrgpu_flag_level(level, flag_gradient, tolsp, flag_richardson, &
+          tol, iorder, start_time, nregions, regions, mbuff, flags_base, offsets, nflagged) &
+          bind(C, name="amrgpu_flag_level")
        import :: c_int, c_double, c_int64_t
-       integer(c_int), value :: level, nregions, mbuff
-       real(c_double), value :: tolsp, start_time
+       integer(c_int), value :: level, flag_gradient, flag_richardson, iorder, nregions, mbuff
+       real(c_double), value :: tolsp, tol, start_time
        real(c_double), intent(in) :: regions(*)
        real(c_double) :: flags_base(*)
        integer(c_int64_t), intent(in) :: offsets(*)

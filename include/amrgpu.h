@@ -134,19 +134,24 @@ int amrgpu_conck(int level, double* totmass);
  * amrgpu_set_level: ilo*hx + xlower, jlo*hy + ylower. */
 int amrgpu_set_corners(int level, const double* xlo, const double* ylo);
 
-/* Flag the cells of every grid of `level` as flagger.f:56-164 does without
- * Richardson estimation: the flags (mx+2*mbuff) x (my+2*mbuff) of each grid
- * (FP64: -1 UNSET, 0 DONTFLAG, 2 DOFLAG; amr_module.f90:163-168) are set
- * UNSET, then flagregions2.f90:40-120 applies the regions (nregions
- * records of 8 doubles in regions.data order: min_level, max_level, t_low,
- * t_hi, x_low, x_hi, y_low, y_hi), then flag2refine2.f90:96-122 flags
- * UNSET interior cells whose undivided difference exceeds tolsp.  Like
- * flagger.f:98, the new values are used after a ghost fill at the level's
- * time unless start_time + possk equals that time exactly (then the old
- * values).  With flags_base non-NULL each grid's flags are copied to
- * flags_base + offsets[k] (node(storeflags)); *nflagged (optional) gets the
- * number of DOFLAG cells. */
-int amrgpu_flag_level(int level, double tolsp, double start_time, int nregions,
+/* Flag the cells of every grid of `level` as flagger.f:56-164 does: the
+ * flags (mx+2*mbuff) x (my+2*mbuff) of each grid (FP64: -1 UNSET,
+ * 0 DONTFLAG, 2 DOFLAG; amr_module.f90:163-168) are set UNSET, then
+ * flagregions2.f90:40-120 applies the regions (nregions records of 8
+ * doubles in regions.data order: min_level, max_level, t_low, t_hi, x_low,
+ * x_hi, y_low, y_hi), then, with flag_gradient, flag2refine2.f90:96-122
+ * flags UNSET interior cells whose undivided difference exceeds tolsp,
+ * then, with flag_richardson, errest.f (prepbigstep/prepregstep/errf1:
+ * the old values coarsened by 2 and advanced 2 dt against the values
+ * advanced dt, component 1, est = |mean - coarse| / (2^(iorder+1) - 2)
+ * >= tol marks the 2x2 block; not on level mxnest, which keeps no old
+ * values).  Like flagger.f:98, the new values are used after a ghost fill
+ * at the level's time unless start_time + possk equals that time exactly
+ * (then the old values).  With flags_base non-NULL each grid's flags are
+ * copied to flags_base + offsets[k] (node(storeflags)); *nflagged
+ * (optional) gets the number of DOFLAG cells. */
+int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richardson,
+                      double tol, int iorder, double start_time, int nregions,
                       const double* regions, int mbuff, double* flags_base,
                       const int64_t* offsets, int64_t* nflagged);
 

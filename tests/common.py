@@ -128,12 +128,17 @@ def nested_hierarchy(rng, solver_states, nvar, nx=24, ny=20, nlev=3, ratio=2, mt
     possk = dt1 if dt1 is not None else 0.2 * min(hx, hy)
     for l in range(1, nlev + 1):
         if l > 1:
+            # a fine grid may reach the domain edge where the coarse grid
+            # does, unless the edge is periodic and the coarse level does not
+            # cover the whole domain (its ring would wrap onto uncovered cells)
+            full = l - 1 == 1
+            xe, ye = full or not xper, full or not yper
             within = []
             for (a, b, c, d) in boxes:
-                within.append((0 if a == 0 else (a + 1) * ratio,
-                               isz * ratio - 1 if b == isz - 1 else b * ratio - 1,
-                               0 if c == 0 else (c + 1) * ratio,
-                               jsz * ratio - 1 if d == jsz - 1 else d * ratio - 1))
+                within.append((0 if (a == 0 and xe) else (a + 1) * ratio,
+                               isz * ratio - 1 if (b == isz - 1 and xe) else b * ratio - 1,
+                               0 if (c == 0 and ye) else (c + 1) * ratio,
+                               jsz * ratio - 1 if (d == jsz - 1 and ye) else d * ratio - 1))
             isz, jsz = isz * ratio, jsz * ratio
             hx, hy = hx / ratio, hy / ratio
             possk = possk / ratio

@@ -91,3 +91,47 @@ def test_flag_level_rejects_inverted_region(gpu):
             capi.flag_level(1, 1.0, t - 5.0, bad, 2)
     finally:
         capi.finalize()
+
+
+@pytest.mark.parametrize("ibuff", [1, 3])
+@pytest.mark.parametrize("case", [((1, 1, 1, 1), False, False), ((3, 3, 3, 3), False, False),
+                                  ((2, 2, 2, 2), True, True)])
+def test_flag_level_richardson_fp64(gpu, case, ibuff):
+    """flagger with regions, gradient and Richardson estimation after one
+    Berger-Oliger step, device against the oracle (pinned against the
+    reference's flagger), every level below mxnest; exact flags."""
+    from tests.common import berger_oliger, both_params, nested_hierarchy, smooth_euler
+    from tests.test_gpu_amr import DeviceAMR
+    print("flag richardson", case, ibuff, flush=True)
+    bc, xper, yper = case
+    rng = np.random.default_rng(5100 + ibuff)
+    H = nested_hierarchy(rng, smooth_euler, 4, mthbc=bc, xper=xper, yper=yper, dt1=0.006)
+    p, cfg = both_params(orc.EULER)
+    for k in range(4):
+        cfg.mthbc[k] = bc[k]
+    O = orc.OracleAMR(p, H.copy())
+    capi.init(cfg)
+    try:
+        D = DeviceAMR(H)
+        berger_oliger(O, 1, H.nlev, 2)
+        berger_oliger(D, 1, H.nlev, 2)
+        mb = max(2, ibuff + 1)
+        for l in range(1, H.nlev):
+            regs = random_regions(rng, 2, O.h.time[l])
+            want = O.flag_level(l, 0.4, -1.0, regs, ibuff=ibuff, gradient=True, richardson=True,
+                                tol=1e-5)
+            got, n = capi.flag_level(l, 0.4, -1.0, regs, mb, _shapes(H, l), gradient=True,
+                                     richardson=True, tol=1e-5)
+            for k, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), (l, k)
+            assert n == sum(int((w == orc.DOFLAG).sum()) for w in want)
+            # Richardson alone, at the initial-time branch (old values)
+            t0 = O.h.time[l] - H.levels[l - 1]["possk"]
+            want = O.flag_level(l, 0.4, t0, (), ibuff=ibuff, gradient=False, richardson=True,
+                                tol=2e-6)
+            got, _ = capi.flag_level(l, 0.4, t0, (), mb, _shapes(H, l), gradient=False,
+                                     richardson=True, tol=2e-6)
+            for k, (a, b) in enumerate(zip(got, want)):
+                assert np.array_equal(a, b), ("old", l, k)
+    finally:
+        capi.finalize()
