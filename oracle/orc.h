@@ -48,6 +48,7 @@ typedef struct {
     double rho, bulk, cc, zz;       /* acoustics            */
     double gamma;                   /* Euler                */
     int efix;                       /* Euler entropy fix    */
+    double wbar;                    /* 3-D advection, z velocity */
 } orc_params;
 
 /* ---- Riemann solvers, Fortran calling convention over a 1-D slice -------
@@ -167,6 +168,27 @@ double orc_conck(const orc_params* p, const orc_hier* h, int level);
 
 int orc_bound_ng(const orc_params* p, const orc_hier* h, int level, int grid,
                  double time, double* valbig, int ng);
+
+/* 3-D sweep (orc_sweep3.c): arrays (meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc,
+ * 1-mbc:mz+mbc), component fastest */
+void orc_rpn3(const orc_params* p, int ixyz, int maxm, int meqn, int mwaves, int mbc, int mx,
+              const double* ql, const double* qr, double* wave, double* s, double* amdq,
+              double* apdq);
+void orc_rpt3(const orc_params* p, int ixyz, int icoor, int imp, int maxm, int meqn, int mwaves,
+              int mbc, int mx, const double* ql, const double* qr, const double* asdq,
+              double* bmasdq, double* bpasdq);
+void orc_rptt3(const orc_params* p, int ixyz, int icoor, int imp, int impt, int maxm, int meqn,
+               int mwaves, int mbc, int mx, const double* ql, const double* qr,
+               const double* bsasdq, double* cmbsasdq, double* cpbsasdq);
+void orc_flux3(const orc_params* p, int ixyz, int maxm, int meqn, int mwaves, int mbc, int mx,
+               const double* q1d, const double* dtdx1d, double dtdy, double dtdz, double* faddm,
+               double* faddp, double* gadd, double* hadd, double* cfl1d);
+void orc_step3(const orc_params* p, int mx, int my, int mz, const double* qold, double dx,
+               double dy, double dz, double dt, double* fm, double* fp, double* gm, double* gp,
+               double* hm, double* hp, double* cflgrid);
+void orc_stepgrid3(const orc_params* p, int mx, int my, int mz, double* q, double dx, double dy,
+                   double dz, double dt, double* fm, double* fp, double* gm, double* gp, double* hm,
+                   double* hp, double* cflgrid, double* dtnew);
 
 /* cell flagging (orc_flag.c) */
 #define ORC_UNSET (-1.0)

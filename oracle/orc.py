@@ -23,6 +23,7 @@ class OrcParams(C.Structure):
         ("ubar", C.c_double), ("vbar", C.c_double),
         ("rho", C.c_double), ("bulk", C.c_double), ("cc", C.c_double),
         ("zz", C.c_double), ("gamma", C.c_double), ("efix", C.c_int),
+        ("wbar", C.c_double),
     ]
 
 
@@ -570,3 +571,71 @@ def ref_flag_grid(q, mx, my, mbuff, xlower, ylower, dx, dy, t, level, tolsp, reg
     R.ref_flag_grid(mx, my, mbuff, meqn, xlower, ylower, dx, dy, t, level, tolsp,
                     q.reshape(-1), len(regions), reg, f)
     return f.reshape(my + 2 * mbuff, mx + 2 * mbuff)
+
+
+# ------------------------------------------------------------------ 3-D
+_ref3 = None
+
+
+def make_params3(solver, *, order=2, transverse=22, limiter=4, cfl=0.9, cflv1=1.0, ubar=0.5,
+                 vbar=1.0, wbar=-0.75, gamma=1.4, efix=True, mbc=2):
+    """Params of a 3-D run: advection (1 equation, 1 wave) or Euler (5
+    equations, 3 waves); transverse is method(3) = 10 m3 + m4."""
+    p = OrcParams()
+    p.solver = solver
+    p.meqn, p.mwaves = {ADVECTION: (1, 1), EULER: (5, 3)}[solver]
+    p.maux, p.mbc = 0, mbc
+    for k in range(8):
+        p.method[k] = 0
+    p.method[2], p.method[3] = order, transverse
+    lim = limiter if isinstance(limiter, (list, tuple)) else [limiter] * p.mwaves
+    for k in range(10):
+        p.mthlim[k] = lim[k] if k < len(lim) else 0
+    p.mcapa, p.use_fwaves = 0, 0
+    p.cfl, p.cflv1 = cfl, cflv1
+    p.ubar, p.vbar, p.wbar = ubar, vbar, wbar
+    p.gamma, p.efix = gamma, int(bool(efix))
+    return p
+
+
+def stepgrid3(p, q, dx, dy, dz, dt):
+    """orc_stepgrid3 on one patch q (mz+4, my+4, mx+4, meqn): returns
+    (qnew, [fm, fp, gm, gp, hm, hp], cflgrid, dtnew)."""
+    qn = np.array(q, dtype=np.float64, order="C", copy=True)
+    mz, my, mx = (n - 2 * p.mbc for n in qn.shape[:3])
+    f = [np.zeros_like(qn) for _ in range(6)]
+    cfl, dtn = C.c_double(), C.c_double()
+    L = lib()
+    L.orc_stepgrid3.argtypes = [C.POINTER(OrcParams), C.c_int, C.c_int, C.c_int, _dp, C.c_double,
+                                C.c_double, C.c_double, C.c_double] + [_dp] * 6 + \
+                               [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.orc_stepgrid3(C.byref(p), mx, my, mz, qn, dx, dy, dz, dt, *f, C.byref(cfl), C.byref(dtn))
+    return qn, f, cfl.value, dtn.value
+
+
+def ref3_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libamrref3.so"))
+
+
+def ref3_stepgrid3(p, q, dx, dy, dz, dt):
+    """The reference's 3-D stepgrid.f on the same inputs."""
+    global _ref3
+    if _ref3 is None:
+        lib()
+        _ref3 = C.CDLL(os.path.join(HERE, "_ref", "libamrref3.so"))
+    R = _ref3
+    lib().orc_set_plugin(C.byref(p))
+    R.ref3_set_method.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                  C.c_int, C.c_double, C.c_double, C.c_int]
+    meth = (C.c_int * 7)(*[p.method[k] for k in range(1, 8)])
+    lim = (C.c_int * 10)(*list(p.mthlim))
+    R.ref3_set_method(meth, lim, p.mwaves, p.mcapa, p.use_fwaves, p.cfl, p.cflv1, p.mbc)
+    qn = np.array(q, dtype=np.float64, order="C", copy=True)
+    mz, my, mx = (n - 2 * p.mbc for n in qn.shape[:3])
+    f = [np.zeros_like(qn) for _ in range(6)]
+    cfl, dtn = C.c_double(), C.c_double()
+    R.ref3_stepgrid.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_double,
+                                C.c_double, C.c_double, C.c_double] + [_dp] * 6 + \
+                               [C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    R.ref3_stepgrid(p.meqn, p.mbc, mx, my, mz, qn, dx, dy, dz, dt, *f, C.byref(cfl), C.byref(dtn))
+    return qn, f, cfl.value, dtn.value

@@ -32,3 +32,31 @@ void orc_rpt2_byref(const int* ixy, const int* imp, const int* maxm,
     orc_rpt2(&g_plugin, *ixy, *imp, *maxm, *meqn, *mwaves, *maux, *mbc, *mx,
              ql, qr, aux1, aux2, aux3, asdq, bmasdq, bpasdq);
 }
+
+void orc_rpn3_byref(const int* ixyz, const int* maxm, const int* meqn, const int* mwaves,
+                    const int* maux, const int* mbc, const int* mx, const double* ql,
+                    const double* qr, const double* auxl, const double* auxr, double* wave,
+                    double* s, double* amdq, double* apdq) {
+    (void)maux; (void)auxl; (void)auxr;
+    orc_rpn3(&g_plugin, *ixyz, *maxm, *meqn, *mwaves, *mbc, *mx, ql, qr, wave, s, amdq, apdq);
+}
+
+void orc_rpt3_byref(const int* ixyz, const int* icoor, const int* imp, const int* maxm,
+                    const int* meqn, const int* mwaves, const int* maux, const int* mbc,
+                    const int* mx, const double* ql, const double* qr, const double* aux1,
+                    const double* aux2, const double* aux3, const double* asdq, double* bmasdq,
+                    double* bpasdq) {
+    (void)maux; (void)aux1; (void)aux2; (void)aux3;
+    orc_rpt3(&g_plugin, *ixyz, *icoor, *imp, *maxm, *meqn, *mwaves, *mbc, *mx, ql, qr, asdq,
+             bmasdq, bpasdq);
+}
+
+void orc_rptt3_byref(const int* ixyz, const int* icoor, const int* imp, const int* impt,
+                     const int* maxm, const int* meqn, const int* mwaves, const int* maux,
+                     const int* mbc, const int* mx, const double* ql, const double* qr,
+                     const double* aux1, const double* aux2, const double* aux3,
+                     const double* bsasdq, double* cmbsasdq, double* cpbsasdq) {
+    (void)maux; (void)aux1; (void)aux2; (void)aux3;
+    orc_rptt3(&g_plugin, *ixyz, *icoor, *imp, *impt, *maxm, *meqn, *mwaves, *mbc, *mx, ql, qr,
+              bsasdq, cmbsasdq, cpbsasdq);
+}
