@@ -31,6 +31,9 @@ UNITS = [
     ("fill_plan.cpp", ["-O2", "-x", "hip"]),
     ("regs_plan.cpp", ["-O2", "-x", "hip"]),
     ("capi.cpp", ["-O2", "-x", "hip"]),
+    ("sweep3_f64.hip", ["-O3", "-ffp-contract=off"]),
+    ("sweep3_f32.hip", ["-O3"]),
+    ("capi3.cpp", ["-O2", "-x", "hip"]),
 ]
 
 

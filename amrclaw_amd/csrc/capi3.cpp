@@ -1,0 +1,504 @@
+// capi3.cpp -- host side of the 3-D patch step: the C ABI of
+// include/amrgpu3.h.
+//
+// Owns the device copies of a batch of 3-D grids (solution in the compute
+// precision, structure of arrays per grid, and the six flux arrays of each
+// grid), the work lists of the four launches of a step and the library's 3-D
+// stream.  Grid layout follows the reference's node(ndilo..ndkhi) boxes with
+// nghost = 2 (src/3d/amr_module.f90).
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amrgpu.h"
+#include "../../include/amrgpu3.h"
+#include "kernels3.hpp"
+
+using namespace amrgpu;
+
+namespace {
+
+const double RINFINITY3 = (double)10.e32f;   // amr_module's rinfinity
+constexpr int NKIND = 4;                      // x, y, z sweeps, update
+
+struct Context3 {
+    bool inited = false;
+    amrgpu3_config cfg{};
+    Method3 meth{};
+    size_t esize = 8;
+    hipStream_t stream = nullptr;
+    int npatch = 0;
+    int plane_block = 8;
+    std::vector<Patch3> patches;
+    Patch3* d_patches = nullptr;
+    std::vector<Block3> blocks[NKIND];
+    Block3* d_blocks[NKIND] = {};
+    void* d_q = nullptr;
+    void* d_face = nullptr;
+    int64_t q_elems = 0, face_elems = 0;
+    unsigned long long* d_cfl = nullptr;
+    std::vector<unsigned long long> h_cfl;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[NKIND];
+    size_t tev_used[NKIND] = {};
+};
+
+Context3 g3;
+std::string g3_err;
+
+int fail3(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g3_err = buf;
+    return code;
+}
+
+#define HIPCHK3(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail3(AMRGPU_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// all transfers go through the 3-D stream (ordered after its kernels)
+hipError_t h2d3(void* dst, const void* src, size_t n) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, g3.stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(g3.stream);
+}
+hipError_t d2h3(void* dst, const void* src, size_t n) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, g3.stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(g3.stream);
+}
+
+void free_batch() {
+    if (g3.stream) hipStreamSynchronize(g3.stream);
+    hipFree(g3.d_patches);
+    hipFree(g3.d_q);
+    hipFree(g3.d_face);
+    hipFree(g3.d_cfl);
+    for (int k = 0; k < NKIND; k++) {
+        hipFree(g3.d_blocks[k]);
+        g3.d_blocks[k] = nullptr;
+        g3.blocks[k].clear();
+    }
+    g3.d_patches = nullptr;
+    g3.d_q = g3.d_face = nullptr;
+    g3.d_cfl = nullptr;
+    g3.patches.clear();
+    g3.npatch = 0;
+    g3.q_elems = g3.face_elems = 0;
+}
+
+void free_events() {
+    for (int k = 0; k < NKIND; k++) {
+        for (auto& pr : g3.tev[k]) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+        g3.tev[k].clear();
+        g3.tev_used[k] = 0;
+    }
+}
+
+int need_init() {
+    if (!g3.inited) return fail3(AMRGPU_ERR_STATE, "amrgpu3_init has not been called");
+    return AMRGPU_OK;
+}
+
+int check_patch(int p) {
+    int rc = need_init();
+    if (rc) return rc;
+    if (p < 0 || p >= g3.npatch) return fail3(AMRGPU_ERR_ARG, "patch %d out of range", p);
+    return AMRGPU_OK;
+}
+
+// event pair for one launch of `kind` (timing on)
+int tbegin(int kind, hipEvent_t* stop) {
+    if (!g3.timing) {
+        *stop = nullptr;
+        return AMRGPU_OK;
+    }
+    auto& v = g3.tev[kind];
+    if (g3.tev_used[kind] == v.size()) {
+        hipEvent_t a, b;
+        HIPCHK3(hipEventCreate(&a));
+        HIPCHK3(hipEventCreate(&b));
+        v.emplace_back(a, b);
+    }
+    auto& pr = v[g3.tev_used[kind]++];
+    HIPCHK3(hipEventRecord(pr.first, g3.stream));
+    *stop = pr.second;
+    return AMRGPU_OK;
+}
+int tend(hipEvent_t stop) {
+    if (stop) HIPCHK3(hipEventRecord(stop, g3.stream));
+    return AMRGPU_OK;
+}
+
+// the batch's work lists: per sweep, (patch, plane block); update, (patch,
+// 256 cells)
+void plan_blocks() {
+    for (int k = 0; k < NKIND; k++) g3.blocks[k].clear();
+    const int pbk = g3.plane_block;
+    for (int p = 0; p < g3.npatch; p++) {
+        const Patch3& pt = g3.patches[p];
+        const int np[3] = {pt.mz, pt.mz, pt.my};
+        for (int d = 0; d < 3; d++)
+            for (int b = 1; b <= np[d]; b += pbk)
+                g3.blocks[d].push_back(Block3{p, b, std::min(b + pbk, np[d] + 1), 0});
+        const int ncell = pt.mx * pt.my * pt.mz;
+        for (int c = 0; c < ncell; c += 256)
+            g3.blocks[3].push_back(Block3{p, c, std::min(c + 256, ncell), 0});
+    }
+}
+
+int define_batch(int npatch, const int* mx, const int* my, const int* mz) {
+    free_batch();
+    const int nv = g3.cfg.nvar;
+    g3.patches.resize(npatch);
+    int64_t qo = 0, fo = 0;
+    for (int p = 0; p < npatch; p++) {
+        Patch3& pt = g3.patches[p];
+        pt.mx = mx[p];
+        pt.my = my[p];
+        pt.mz = mz[p];
+        pt.pad = 0;
+        pt.qoff = qo;
+        pt.foff = fo;
+        // align each grid's arrays to 64 elements
+        qo += (q3_elems(pt, nv) + 63) / 64 * 64;
+        fo += (faces3_elems(pt, nv) + 63) / 64 * 64;
+    }
+    g3.npatch = npatch;
+    g3.q_elems = qo;
+    g3.face_elems = fo;
+    plan_blocks();
+    HIPCHK3(hipMalloc(&g3.d_patches, std::max<size_t>(1, npatch) * sizeof(Patch3)));
+    if (npatch) HIPCHK3(h2d3(g3.d_patches, g3.patches.data(), npatch * sizeof(Patch3)));
+    HIPCHK3(hipMalloc(&g3.d_q, std::max<int64_t>(1, qo) * g3.esize));
+    HIPCHK3(hipMalloc(&g3.d_face, std::max<int64_t>(1, fo) * g3.esize));
+    HIPCHK3(hipMemsetAsync(g3.d_q, 0, std::max<int64_t>(1, qo) * g3.esize, g3.stream));
+    HIPCHK3(hipMemsetAsync(g3.d_face, 0, std::max<int64_t>(1, fo) * g3.esize, g3.stream));
+    HIPCHK3(hipMalloc(&g3.d_cfl, std::max(1, npatch) * sizeof(unsigned long long)));
+    for (int k = 0; k < NKIND; k++) {
+        const auto& b = g3.blocks[k];
+        HIPCHK3(hipMalloc(&g3.d_blocks[k], std::max<size_t>(1, b.size()) * sizeof(Block3)));
+        if (!b.empty()) HIPCHK3(h2d3(g3.d_blocks[k], b.data(), b.size() * sizeof(Block3)));
+    }
+    g3.h_cfl.assign(npatch, 0);
+    HIPCHK3(hipStreamSynchronize(g3.stream));
+    return AMRGPU_OK;
+}
+
+// host AoS q(nvar, mx+4, my+4, mz+4) <-> device SoA [m][k][j][i]
+int put_q(int p, const double* q) {
+    const Patch3& pt = g3.patches[p];
+    const int nv = g3.cfg.nvar;
+    const int64_t cells = (int64_t)(pt.mx + 4) * (pt.my + 4) * (pt.mz + 4);
+    if (g3.esize == 8) {
+        std::vector<double> h(nv * cells);
+        for (int64_t c = 0; c < cells; c++)
+            for (int m = 0; m < nv; m++) h[m * cells + c] = q[c * nv + m];
+        HIPCHK3(h2d3(static_cast<double*>(g3.d_q) + pt.qoff, h.data(), h.size() * 8));
+    } else {
+        std::vector<float> h(nv * cells);
+        for (int64_t c = 0; c < cells; c++)
+            for (int m = 0; m < nv; m++) h[m * cells + c] = (float)q[c * nv + m];
+        HIPCHK3(h2d3(static_cast<float*>(g3.d_q) + pt.qoff, h.data(), h.size() * 4));
+    }
+    return AMRGPU_OK;
+}
+
+int get_q(int p, double* q) {
+    const Patch3& pt = g3.patches[p];
+    const int nv = g3.cfg.nvar;
+    const int64_t cells = (int64_t)(pt.mx + 4) * (pt.my + 4) * (pt.mz + 4);
+    if (g3.esize == 8) {
+        std::vector<double> h(nv * cells);
+        HIPCHK3(d2h3(h.data(), static_cast<double*>(g3.d_q) + pt.qoff, h.size() * 8));
+        for (int64_t c = 0; c < cells; c++)
+            for (int m = 0; m < nv; m++) q[c * nv + m] = h[m * cells + c];
+    } else {
+        std::vector<float> h(nv * cells);
+        HIPCHK3(d2h3(h.data(), static_cast<float*>(g3.d_q) + pt.qoff, h.size() * 4));
+        for (int64_t c = 0; c < cells; c++)
+            for (int m = 0; m < nv; m++) q[c * nv + m] = (double)h[m * cells + c];
+    }
+    return AMRGPU_OK;
+}
+
+int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !(dt >= 0))
+        return fail3(AMRGPU_ERR_ARG, "bad mesh widths or time step");
+    HIPCHK3(hipMemsetAsync(g3.d_cfl, 0, std::max(1, g3.npatch) * sizeof(unsigned long long),
+                           g3.stream));
+    Sweep3Args a{};
+    a.q = g3.d_q;
+    a.face = g3.d_face;
+    a.patches = g3.d_patches;
+    a.dtd[0] = dt / dx;
+    a.dtd[1] = dt / dy;
+    a.dtd[2] = dt / dz;
+    a.meth = g3.meth;
+    a.cfl_patch = g3.d_cfl;
+    const int solver = g3.cfg.solver;
+    for (int d = 1; d <= 3; d++) {
+        a.phase = d;
+        a.blocks = g3.d_blocks[d - 1];
+        a.nblocks = (int)g3.blocks[d - 1].size();
+        hipEvent_t stop;
+        int rc = tbegin(d - 1, &stop);
+        if (rc) return rc;
+        HIPCHK3(g3.esize == 8 ? launch_sweep3_f64(solver, a, g3.stream)
+                              : launch_sweep3_f32(solver, a, g3.stream));
+        rc = tend(stop);
+        if (rc) return rc;
+    }
+    {
+        hipEvent_t stop;
+        int rc = tbegin(3, &stop);
+        if (rc) return rc;
+        const int nb = (int)g3.blocks[3].size();
+        if (g3.esize == 8)
+            HIPCHK3(launch_update3_f64(g3.cfg.nvar, static_cast<double*>(g3.d_q),
+                                       static_cast<const double*>(g3.d_face), g3.d_patches,
+                                       g3.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
+                                       g3.stream));
+        else
+            HIPCHK3(launch_update3_f32(g3.cfg.nvar, static_cast<float*>(g3.d_q),
+                                       static_cast<const float*>(g3.d_face), g3.d_patches,
+                                       g3.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
+                                       g3.stream));
+        rc = tend(stop);
+        if (rc) return rc;
+    }
+    if (cflgrid) {
+        HIPCHK3(d2h3(g3.h_cfl.data(), g3.d_cfl, g3.npatch * sizeof(unsigned long long)));
+        for (int p = 0; p < g3.npatch; p++) {
+            double v;
+            std::memcpy(&v, &g3.h_cfl[p], sizeof v);
+            cflgrid[p] = v;
+        }
+    }
+    return AMRGPU_OK;
+}
+
+int get_fluxes(int p, double* const out[6]) {
+    const Patch3& pt = g3.patches[p];
+    const int nv = g3.cfg.nvar;
+    const Faces3 F = faces3(pt, nv);
+    const int64_t n = faces3_elems(pt, nv);
+    std::vector<double> h(n);
+    if (g3.esize == 8) {
+        HIPCHK3(d2h3(h.data(), static_cast<double*>(g3.d_face) + pt.foff, n * 8));
+    } else {
+        std::vector<float> f(n);
+        HIPCHK3(d2h3(f.data(), static_cast<float*>(g3.d_face) + pt.foff, n * 4));
+        for (int64_t k = 0; k < n; k++) h[k] = f[k];
+    }
+    const int mit = pt.mx + 4, mjt = pt.my + 4;
+    auto at = [&](int i, int j, int k) { return (((int64_t)(k + 1) * mjt + (j + 1)) * mit + (i + 1)) * nv; };
+    const int64_t base[6] = {F.fm, F.fp, F.gm, F.gp, F.hm, F.hp};
+    const int64_t cs[3] = {F.sx, F.sy, F.sz};
+    for (int a = 0; a < 6; a++) {
+        if (!out[a]) continue;
+        const int dir = a / 2;
+        const int ni = pt.mx + (dir == 0), nj = pt.my + (dir == 1), nk = pt.mz + (dir == 2);
+        const double* src = h.data() + (base[a] - pt.foff);
+        for (int k = 1; k <= nk; k++)
+            for (int j = 1; j <= nj; j++)
+                for (int i = 1; i <= ni; i++) {
+                    const int64_t f = ((int64_t)(k - 1) * nj + (j - 1)) * ni + (i - 1);
+                    for (int m = 0; m < nv; m++) out[a][at(i, j, k) + m] = src[m * cs[dir] + f];
+                }
+    }
+    return AMRGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amrgpu3_last_error(char* buf, int len) {
+    if (!buf || len <= 0) return AMRGPU_ERR_ARG;
+    std::snprintf(buf, len, "%s", g3_err.c_str());
+    return AMRGPU_OK;
+}
+
+int amrgpu3_init(const amrgpu3_config* cfg) {
+    if (!cfg) return fail3(AMRGPU_ERR_ARG, "null config");
+    const amrgpu3_config& c = *cfg;
+    if (c.solver == AMRGPU_ADVECTION) {
+        if (c.nvar != 1 || c.mwaves != 1)
+            return fail3(AMRGPU_ERR_ARG, "3-D advection has nvar = mwaves = 1");
+    } else if (c.solver == AMRGPU_EULER) {
+        if (c.nvar != 5 || c.mwaves != 3)
+            return fail3(AMRGPU_ERR_ARG, "3-D Euler has nvar = 5, mwaves = 3");
+        if (!(c.gamma > 1.0)) return fail3(AMRGPU_ERR_ARG, "gamma must exceed 1");
+    } else {
+        return fail3(AMRGPU_ERR_UNSUPPORTED, "3-D solver %d not supported", c.solver);
+    }
+    if (c.naux != 0) return fail3(AMRGPU_ERR_UNSUPPORTED, "aux arrays are not supported");
+    if (c.mcapa != 0) return fail3(AMRGPU_ERR_UNSUPPORTED, "capacity is not supported");
+    if (c.nghost != 2) return fail3(AMRGPU_ERR_UNSUPPORTED, "nghost must be 2");
+    if (c.precision != AMRGPU_FP64 && c.precision != AMRGPU_FP32)
+        return fail3(AMRGPU_ERR_ARG, "bad precision");
+    const int mt = c.method[2];
+    int m3, m4;
+    if (mt < 0) {
+        m3 = -1;
+        m4 = 0;
+    } else {
+        m3 = mt / 10;
+        m4 = mt - 10 * m3;
+    }
+    if (mt >= 0 && (m3 > 2 || m4 > 2))
+        return fail3(AMRGPU_ERR_ARG, "method(3) = %d is not 10 m3 + m4 with m3, m4 <= 2", mt);
+    // flux3.f:390-405 reads the cqxx splits, which only m3 = 2 computes
+    if (m4 == 2 && m3 == 1)
+        return fail3(AMRGPU_ERR_UNSUPPORTED, "method(3) = %d: m4 = 2 needs m3 = 2", mt);
+    for (int w = 0; w < c.mwaves; w++)
+        if (c.mthlim[w] < 0 || c.mthlim[w] > 5)
+            return fail3(AMRGPU_ERR_ARG, "limiter %d not supported", c.mthlim[w]);
+    if (g3.inited) amrgpu3_finalize();
+    g3.cfg = c;
+    g3.esize = c.precision == AMRGPU_FP64 ? 8 : 4;
+    Method3 M{};
+    M.ph.ubar = c.ubar;
+    M.ph.vbar = c.vbar;
+    M.ph.wbar = c.wbar;
+    M.ph.gamma = c.gamma;
+    M.ph.efix = c.efix;
+    M.order = c.method[1];
+    M.m3 = m3;
+    M.m4 = m4;
+    M.use_fwaves = c.use_fwaves;
+    for (int w = 0; w < NWMAX3; w++) M.mthlim[w] = w < c.mwaves ? c.mthlim[w] : 0;
+    g3.meth = M;
+    HIPCHK3(hipStreamCreateWithFlags(&g3.stream, hipStreamNonBlocking));
+    g3.inited = true;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_finalize(void) {
+    if (!g3.inited) return AMRGPU_OK;
+    free_batch();
+    free_events();
+    if (g3.stream) hipStreamDestroy(g3.stream);
+    g3.stream = nullptr;
+    g3.inited = false;
+    g3.timing = false;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_set_patches(int npatch, const int* mx, const int* my, const int* mz, int plane_block) {
+    int rc = need_init();
+    if (rc) return rc;
+    if (npatch < 0 || (npatch > 0 && (!mx || !my || !mz)))
+        return fail3(AMRGPU_ERR_ARG, "bad patch arrays");
+    if (plane_block < 0) return fail3(AMRGPU_ERR_ARG, "plane_block must be >= 0");
+    for (int p = 0; p < npatch; p++) {
+        if (mx[p] < 1 || my[p] < 1 || mz[p] < 1)
+            return fail3(AMRGPU_ERR_ARG, "patch %d: sizes must be positive", p);
+        // one round must hold a whole line: n + 3 interfaces <= 256 lanes
+        if (std::max(mx[p], std::max(my[p], mz[p])) + 3 > SWEEP3_LANES)
+            return fail3(AMRGPU_ERR_UNSUPPORTED, "patch %d: a side exceeds %d cells", p,
+                         SWEEP3_LANES - 3);
+    }
+    g3.plane_block = plane_block ? plane_block : 8;
+    return define_batch(npatch, mx, my, mz);
+}
+
+int amrgpu3_put_patch(int patch, const double* q) {
+    int rc = check_patch(patch);
+    if (rc) return rc;
+    if (!q) return fail3(AMRGPU_ERR_ARG, "null q");
+    return put_q(patch, q);
+}
+
+int amrgpu3_get_patch(int patch, double* q) {
+    int rc = check_patch(patch);
+    if (rc) return rc;
+    if (!q) return fail3(AMRGPU_ERR_ARG, "null q");
+    return get_q(patch, q);
+}
+
+int amrgpu3_step_patches(double dx, double dy, double dz, double dt, double* cflgrid) {
+    int rc = need_init();
+    if (rc) return rc;
+    return step_batch(dx, dy, dz, dt, cflgrid);
+}
+
+int amrgpu3_get_fluxes(int patch, double* fm, double* fp, double* gm, double* gp, double* hm,
+                       double* hp) {
+    int rc = check_patch(patch);
+    if (rc) return rc;
+    double* const out[6] = {fm, fp, gm, gp, hm, hp};
+    return get_fluxes(patch, out);
+}
+
+int amrgpu3_stepgrid(int mx, int my, int mz, const double* q_in, double* q_out, double dx,
+                     double dy, double dz, double dt, double* fm, double* fp, double* gm,
+                     double* gp, double* hm, double* hp, double* cflgrid, double* dtnew) {
+    int rc = need_init();
+    if (rc) return rc;
+    if (!q_in || !q_out) return fail3(AMRGPU_ERR_ARG, "null q");
+    rc = amrgpu3_set_patches(1, &mx, &my, &mz, 0);
+    if (rc) return rc;
+    rc = put_q(0, q_in);
+    if (rc) return rc;
+    double cfl = 0.0;
+    rc = step_batch(dx, dy, dz, dt, &cfl);
+    if (rc) return rc;
+    rc = get_q(0, q_out);
+    if (rc) return rc;
+    if (fm && fp && gm && gp && hm && hp) {
+        double* const out[6] = {fm, fp, gm, gp, hm, hp};
+        rc = get_fluxes(0, out);
+        if (rc) return rc;
+    }
+    if (cflgrid) *cflgrid = cfl;
+    if (dtnew) *dtnew = cfl > 0.0 ? dt * g3.cfg.cfl / cfl : RINFINITY3;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_set_timing(int on) {
+    int rc = need_init();
+    if (rc) return rc;
+    HIPCHK3(hipStreamSynchronize(g3.stream));
+    for (int k = 0; k < NKIND; k++) g3.tev_used[k] = 0;
+    g3.timing = on != 0;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_kernel_time(int kind, double* ms, int64_t* launches) {
+    int rc = need_init();
+    if (rc) return rc;
+    if (kind < 0 || kind >= NKIND || !ms) return fail3(AMRGPU_ERR_ARG, "bad kernel kind");
+    HIPCHK3(hipStreamSynchronize(g3.stream));
+    double t = 0.0;
+    for (size_t k = 0; k < g3.tev_used[kind]; k++) {
+        float x = 0.f;
+        HIPCHK3(hipEventElapsedTime(&x, g3.tev[kind][k].first, g3.tev[kind][k].second));
+        t += x;
+    }
+    *ms = t;
+    if (launches) *launches = (int64_t)g3.tev_used[kind];
+    return AMRGPU_OK;
+}
+
+int amrgpu3_synchronize(void) {
+    int rc = need_init();
+    if (rc) return rc;
+    HIPCHK3(hipStreamSynchronize(g3.stream));
+    return AMRGPU_OK;
+}
+
+}  // extern "C"

@@ -122,3 +122,60 @@ def batched_euler_hierarchy(n=4096, seed=20261202, coarse_block=256, courant=0.8
     H.levels[0].update(possk=2.0 * dt2, time=2.0 * dt2)
     H.levels[1].update(possk=dt2, time=dt2)
     return H, int((mx * my).sum())
+
+
+# ---------------------------------------------------------------------------
+# 3-D: the radial-bump Euler problem (rho = p = 2 inside r < 0.2 about the
+# centre of the unit cube, 1 outside, at rest), batched over grids of uneven
+# sizes.
+
+def bump3_state(x, y, z, gamma=GAMMA, r0=0.2):
+    """Conserved (rho, rho u, rho v, rho w, E) of the radial bump."""
+    r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2)
+    inside = r < r0
+    rho = np.where(inside, 2.0, 1.0)
+    p = np.where(inside, 2.0, 1.0)
+    q = np.zeros(np.shape(x) + (5,))
+    q[..., 0] = rho
+    q[..., 4] = p / (gamma - 1.0)
+    return q
+
+
+def patch_sizes3(n, seed=20261203, lo=8, hi=32):
+    rng = np.random.default_rng(seed)
+    s = 2 * rng.integers(lo // 2, hi // 2 + 1, size=(n, 3))
+    return s.astype(np.int64)
+
+
+def batched_euler3_level(n=512, seed=20261203, fine=256, perturb=1e-3):
+    """A level of n 3-D grids with sizes even U[8, 32]^3 on a fine index
+    space of fine^3 cells over the unit cube (mesh width h = 1/fine).  Each
+    grid sits at a seeded random corner and holds the radial bump sampled at
+    its cell centres (ghost cells included) times a seeded +-perturb relative
+    perturbation.  Returns (sizes (n, 3), corners (n, 3), h, list of q
+    arrays (mz+4, my+4, mx+4, 5))."""
+    sizes = patch_sizes3(n, seed)
+    rng = np.random.default_rng(seed + 1)
+    h = 1.0 / fine
+    corners = np.array([[int(rng.integers(0, fine - int(v) + 1)) for v in s] for s in sizes])
+    qs = []
+    for k in range(n):
+        mx, my, mz = (int(v) for v in sizes[k])
+        i0, j0, k0 = (int(v) for v in corners[k])
+        xc = (np.arange(i0 - 2, i0 + mx + 2) + 0.5) * h
+        yc = (np.arange(j0 - 2, j0 + my + 2) + 0.5) * h
+        zc = (np.arange(k0 - 2, k0 + mz + 2) + 0.5) * h
+        Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+        q = bump3_state(X, Y, Z)
+        prng = np.random.default_rng(seed + 100 + k)
+        q *= 1.0 + perturb * (2.0 * prng.random(q.shape[:3] + (1,)) - 1.0)
+        qs.append(q)
+    return sizes, corners, h, qs
+
+
+def max_wave_speed3(q, gamma=GAMMA):
+    rho = q[..., 0]
+    u, v, w = q[..., 1] / rho, q[..., 2] / rho, q[..., 3] / rho
+    p = (gamma - 1.0) * (q[..., 4] - 0.5 * rho * (u * u + v * v + w * w))
+    c = np.sqrt(gamma * p / rho)
+    return float(max((np.abs(u) + c).max(), (np.abs(v) + c).max(), (np.abs(w) + c).max()))

@@ -56,6 +56,11 @@ def parse():
                     help="skip the FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 disables it)")
+    ap.add_argument("--workload", choices=["euler2d", "euler3d"], default="euler2d",
+                    help="euler3d: the batched 3-D Euler step (512 grids) instead of "
+                         "the headline 2-D level update")
+    ap.add_argument("--plane-block", type=int, default=0,
+                    help="3-D: planes per workgroup of a sweep (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB "
                          "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
@@ -194,6 +199,124 @@ def cpu_baseline(H, dt2, budget, nsample=256):
                       "over the whole level 1, %d passes in %.1f s" % (nsample, cells, reps, el)}
 
 
+def cpu_baseline3(sizes, qs, dx, dt, budget):
+    """The reference's 3-D stepgrid.f (compiled from its Fortran by
+    oracle/Makefile, one thread) over the grids of the batch in turn, for
+    about `budget` seconds."""
+    from oracle import orc
+    p = orc.make_params3(orc.EULER, order=2, transverse=22, limiter=4)
+    kind = "reference" if orc.ref3_available() else "port"
+    fn = orc.ref3_stepgrid3 if kind == "reference" else orc.stepgrid3
+    cells = 0
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget:
+        k = n % len(qs)
+        fn(p, qs[k], dx, dx, dx, dt)
+        cells += int(np.prod(sizes[k]))
+        n += 1
+    wall = time.perf_counter() - t0
+    return {"value": cells / wall, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": "stepgrid.f (3-D) of the batch's grids in turn, %d grid steps "
+                      "(%d cells) in %.1f s" % (n, cells, wall)}
+
+
+def bench3(a):
+    """Batched 3-D Euler step: 512 grids, sizes even U[8,32]^3 (seed
+    20261203) on a 256^3 index space of the unit cube, the radial bump
+    (rho = p = 2 for r < 0.2) with a seeded +-1e-3 perturbation; Roe solver
+    with entropy fix, MC limiter, method(3) = 22 (double transverse).  One
+    step = the x-, y-, z-sweeps and the update of every grid (stepgrid.f
+    for each; ghost cells are those of the initial state)."""
+    import torch
+    from amrclaw_amd import capi3, workload
+
+    sizes, corners, h, qs = workload.batched_euler3_level(512)
+    cells = int(np.prod(sizes, axis=1).sum())
+    smax = max(workload.max_wave_speed3(q) for q in qs)
+    dt = 0.8 * h / smax / 3.0
+    prec = capi3.FP32 if a.precision == "fp32" else capi3.FP64
+    esize = 4 if prec == capi3.FP32 else 8
+
+    def install(precision):
+        capi3.init3(capi3.make_config3(capi3.EULER, precision=precision))
+        capi3.set_patches3(sizes, a.plane_block)
+        for k, q in enumerate(qs):
+            capi3.put_patch3(k, q)
+
+    def run(steps, warmup):
+        for _ in range(warmup):
+            capi3.step_patches3(h, h, h, dt)
+        capi3.synchronize3()
+        torch.cuda.synchronize()
+        capi3.set_timing3(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            capi3.step_patches3(h, h, h, dt)
+        capi3.synchronize3()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kt = [capi3.kernel_time3(k) for k in (capi3.KT_X, capi3.KT_Y, capi3.KT_Z,
+                                               capi3.KT_UPDATE)]
+        capi3.set_timing3(False)
+        return wall, kt
+
+    def interiors():
+        return [capi3.get_patch3(k, q.shape)[2:-2, 2:-2, 2:-2] for k, q in enumerate(qs)]
+
+    print("bench3: timed run", a.precision, file=sys.stderr, flush=True)
+    install(prec)
+    wall, kt = run(a.steps, a.warmup)
+    q_main = interiors() if not a.no_compare else None
+    capi3.finalize3()
+    names = ("x_sweep", "y_sweep", "z_sweep", "update")
+    kms = {nm: t[0] / max(t[1], 1) for nm, t in zip(names, kt)}
+    result = {
+        "metric": METRIC,
+        "value": cells * a.steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == capi3.FP32 else "f64", "data": "synthetic",
+        "config": {"workload": "batched 3-D Euler step: 512 grids, mx,my,mz even U[8,32] "
+                               "seed 20261203, radial bump, Roe+efix, MC, method(3)=22",
+                   "patches": len(qs), "cells_per_step": cells, "nghost": 2,
+                   "precision": a.precision, "launch": "batched",
+                   "plane_block": a.plane_block or 8},
+        "kernels_ms_per_step": kms,
+    }
+    step_kern = sum(kms.values())
+    dom = max(names[:3], key=lambda nm: kms[nm])
+    bpc = 2 * 5 * esize
+    achieved = cells * bpc / (step_kern / 1e3) / 1e9
+    result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                          "kernel": "sweep3_phase_kernel (x+y+z) + update3_kernel",
+                          "kernel_ms": step_kern, "dominant": dom,
+                          "bytes_per_cell_update": bpc,
+                          "algorithmic_bytes_per_launch": cells * bpc}
+    if not a.no_compare and prec == capi3.FP32:
+        print("bench3: FP64 batched", file=sys.stderr, flush=True)
+        install(capi3.FP64)
+        wall64, kt64 = run(a.steps, a.warmup)
+        q64 = interiors()
+        capi3.finalize3()
+        num = sum(float(np.abs(x - y).sum()) for x, y in zip(q_main, q64))
+        den = sum(float(np.abs(y).sum()) for y in q64)
+        m32 = sum(float(x[..., 0].sum()) for x in q_main)
+        m64 = sum(float(y[..., 0].sum()) for y in q64)
+        result["fp64_batched"] = {"value": cells * a.steps / wall64,
+                                  "ms_per_step": wall64 / a.steps * 1e3}
+        result["speedup_vs_fp64_batched"] = wall64 / wall
+        result["l1_rel_vs_fp64"] = num / den
+        result["mass_rel_diff_vs_fp64"] = abs(m32 - m64) / abs(m64)
+        result["steps_compared"] = a.steps + a.warmup
+    if a.cpu_seconds > 0:
+        print("bench3: CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = _quiet_stdout(
+            lambda: cpu_baseline3([tuple(s) for s in sizes], qs, h, dt, a.cpu_seconds))
+    print(json.dumps(result))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -202,6 +325,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
+        return
+    if a.workload == "euler3d":
+        bench3(a)
         return
     import torch
     from amrclaw_amd import capi, workload

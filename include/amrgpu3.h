@@ -1,0 +1,97 @@
+/*
+ * amrgpu3.h -- C ABI of the 3-D patch step (libamrgpu.so).
+ *
+ * Drop-in for the per-grid call of the reference's 3-D advanc
+ * (src/3d/advanc.f calls stepgrid for each grid of a level after bound):
+ * the batch below holds every grid of a level on the device and advances all
+ * of them with one launch per sweep direction, step3.f:3-486 + flux3.f:4-633
+ * + stepgrid.f (3-D):79-230 for each grid.  Arrays cross the boundary in the
+ * reference's layout: FP64, q(nvar, 1-nghost:mx+nghost, 1-nghost:my+nghost,
+ * 1-nghost:mz+nghost), component fastest.  Ghost cells must already be
+ * filled (the 3-D ghost fill stays with the caller).
+ *
+ * Every function returns 0 on success and a nonzero code (the AMRGPU_ERR_*
+ * codes of amrgpu.h) on failure; the message is available from
+ * amrgpu3_last_error().  Not thread-safe.
+ */
+#ifndef AMRGPU3_H
+#define AMRGPU3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Run parameters: the amr_module fields the 3-D step reads (claw.data,
+ * amr.data) and the solver constants of common /cparam/. */
+typedef struct amrgpu3_config {
+    int solver;          /* AMRGPU_ADVECTION (1 eqn, 1 wave) or AMRGPU_EULER
+                            (5 eqns, 3 waves)                                  */
+    int nvar;            /* num_eqn                                           */
+    int mwaves;          /* num_waves                                         */
+    int naux;            /* must be 0: no solver here reads aux               */
+    int nghost;          /* must be 2                                         */
+    int method[7];       /* method(1:7): [1] = order, [2] = 10 m3 + m4
+                            (flux3.f:205-211; m4 = 2 needs m3 = 2)            */
+    int mthlim[8];       /* limiter per wave (inlinelimiter.f)                */
+    int mcapa;           /* must be 0                                         */
+    int use_fwaves;
+    double cfl;          /* cfl_desired                                       */
+    double cflv1;        /* cfl_max                                           */
+    double ubar, vbar, wbar;   /* advection velocities                       */
+    double gamma;        /* Euler                                             */
+    int efix;            /* Euler entropy fix                                 */
+    int precision;       /* AMRGPU_FP64 or AMRGPU_FP32 (bulk arithmetic)     */
+} amrgpu3_config;
+
+/* Initialise (or re-initialise) the 3-D step on the current HIP device. */
+int amrgpu3_init(const amrgpu3_config* cfg);
+
+/* Release the 3-D device state. */
+int amrgpu3_finalize(void);
+
+/* Copy the last error message of an amrgpu3_* call. */
+int amrgpu3_last_error(char* buf, int len);
+
+/* Define the batch: npatch grids of mx[p] x my[p] x mz[p] interior cells
+ * (the grids of a level, node(ndilo..ndkhi)); allocates their solution and
+ * flux arrays on the device.  plane_block: planes per workgroup of a sweep
+ * (0: default 8); a smaller block gives more workgroups, a larger one fewer
+ * recomputed planes. */
+int amrgpu3_set_patches(int npatch, const int* mx, const int* my, const int* mz,
+                        int plane_block);
+
+/* Copy one grid's q (with ghost cells) to / from the device. */
+int amrgpu3_put_patch(int patch, const double* q);
+int amrgpu3_get_patch(int patch, double* q);
+
+/* One step of every grid of the batch: stepgrid.f (3-D) for each, in place.
+ * cflgrid[npatch] receives each grid's Courant number (may be NULL). */
+int amrgpu3_step_patches(double dx, double dy, double dz, double dt, double* cflgrid);
+
+/* The six flux arrays of the last step of one grid, in stepgrid's shape
+ * (nvar, mx+4, my+4, mz+4).  Only faces of interior cells are written:
+ * fm/fp at x-faces 1..mx+1 of rows 1..my, 1..mz; gm/gp, hm/hp likewise. */
+int amrgpu3_get_fluxes(int patch, double* fm, double* fp, double* gm, double* gp,
+                       double* hm, double* hp);
+
+/* Single-grid step, src/3d/stepgrid.f:79-230: q_in -> q_out (ghost cells of
+ * q_out = those of q_in), optional flux outputs (all six or NULL), cflgrid
+ * and dtnew = dt cfl / cflgrid (rinfinity when cflgrid = 0).  Replaces the
+ * batch defined by amrgpu3_set_patches. */
+int amrgpu3_stepgrid(int mx, int my, int mz, const double* q_in, double* q_out, double dx,
+                     double dy, double dz, double dt, double* fm, double* fp, double* gm,
+                     double* gp, double* hm, double* hp, double* cflgrid, double* dtnew);
+
+/* Device timing of the step's kernels: amrgpu3_set_timing(1) resets and
+ * starts it; kind 0, 1, 2 = x-, y-, z-sweep, 3 = update. */
+int amrgpu3_set_timing(int on);
+int amrgpu3_kernel_time(int kind, double* ms, int64_t* launches);
+int amrgpu3_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AMRGPU3_H */
