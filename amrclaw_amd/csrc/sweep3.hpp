@@ -134,36 +134,28 @@ __device__ __forceinline__ T phi_limiter3(int mth, T r) {
     }
 }
 
-// LDS bytes of a phase workgroup: the exchange buffer of one round
+// LDS of a phase workgroup: the exchange buffer of one round (slots x 256
+// lanes) and the staged states of the round's lines (y-, z-sweeps): at most
+// lines * (n + 4) <= 256 + lines <= 320 cells, lines <= 64
+template <int NV, int NW>
+__host__ __device__ constexpr int sweep3_xs_slots() {
+    return NW * NV > 6 * NV ? NW * NV : 6 * NV;
+}
+constexpr int SWEEP3_STAGE_CELLS = 320;
 template <int NV, int NW, typename T>
 __host__ __device__ constexpr size_t sweep3_lds_bytes() {
-    return (size_t)(NW * NV > 6 * NV ? NW * NV : 6 * NV) * SWEEP3_LANES * sizeof(T);
-}
-
-// Add (or, first = true, store 0 + v) an increment into one or two face
-// arrays.  Component stride cs.
-template <int NV, typename T>
-__device__ __forceinline__ void face_add(T* a1, T* a2, int64_t idx, int64_t cs, const T* v,
-                                         bool first) {
-#pragma unroll
-    for (int m = 0; m < NV; m++) {
-        const int64_t k = idx + m * cs;
-        if (first) {
-            a1[k] = T(0) + v[m];
-        } else {
-            a1[k] = a1[k] + v[m];
-            if (a2) a2[k] = a2[k] + v[m];
-        }
-    }
+    return ((size_t)sweep3_xs_slots<NV, NW>() * SWEEP3_LANES + (size_t)NV * SWEEP3_STAGE_CELLS) *
+           sizeof(T);
 }
 
 // D: the phase (1 x, 2 y, 3 z), a template parameter so that every
 // component index of the solvers is a compile-time constant
 template <class Sol, typename T, int D>
-__global__ __launch_bounds__(SWEEP3_LANES) void sweep3_phase_kernel(Sweep3Args a) {
+__global__ __launch_bounds__(SWEEP3_LANES, sizeof(T) == 4 ? 3 : 1) void sweep3_phase_kernel(Sweep3Args a) {
     constexpr int NV = Sol::NV, NW = Sol::NW;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* xs = reinterpret_cast<T*>(smem_raw);   // [slot][256]
+    T* qst = xs + sweep3_xs_slots<NV, NW>() * SWEEP3_LANES;   // [NV][n+4][lines]
     const Method3& M = a.meth;
 
     const Block3 blk = a.blocks[blockIdx.x];
@@ -226,28 +218,56 @@ __global__ __launch_bounds__(SWEEP3_LANES) void sweep3_phase_kernel(Sweep3Args a
     const int qi = tid / L, r = tid - qi * L;
     const int nrounds = (NQ + 2 + lines_per - 1) / lines_per;
     const int pe_face = blk.pe == NP + 1 ? NP + 2 : blk.pe;   // owned type-B faces
-    // the reference's first increment of an x-sweep face: from the line and
-    // plane below it (o = +1), or o = 0 when m4 = 0 leaves o = +-1 at zero
-    const int o_first = M.m4 > 0 ? 1 : 0;
     T cfl = T(0);
+    if (D == 1 && trans) {
+        // G and H start at zero (step3.f:134-139): clear the faces this
+        // workgroup owns (k in [pb, pe) of gm; K in [pb, pe_face) of hm)
+        const int64_t g0 = (int64_t)(blk.pb - 1) * (my + 1) * mx, g1 = (int64_t)(blk.pe - 1) * (my + 1) * mx;
+        const int64_t h0 = (int64_t)(blk.pb - 1) * my * mx, h1 = (int64_t)(pe_face - 1) * my * mx;
+        for (int m = 0; m < NV; m++) {
+            for (int64_t e = g0 + tid; e < g1; e += SWEEP3_LANES) a1[m * acs + e] = T(0);
+            for (int64_t e = h0 + tid; e < h1; e += SWEEP3_LANES) b1[m * bcs + e] = T(0);
+        }
+        __syncthreads();
+    }
 
     for (int P = blk.pb - 1; P <= blk.pe; P++) {
         for (int rd = 0; rd < nrounds; rd++) {
             const int Q = rd * lines_per + qi;
             const bool live = qi < lines_per && Q <= NQ + 1;
             const bool inner = live && r >= 1 && r <= n + 1;
-            const bool cellok = inner && r >= 2;       // owns gadd/hadd of cell r-1 in [1, n]
             const int Qc = live ? Q : 0, rc = live ? r : 1;
+            // lines of this round (the last round may hold fewer)
+            const int nl = min(lines_per, NQ + 2 - rd * lines_per);
+            const int64_t plane = (int64_t)(pt.mx + 4) * (pt.my + 4) * (pt.mz + 4);
             T ql[NV], qrr[NV];   // states left (cell r-1) and right (cell r) of interface r
-            {
-                // 1-based cell x sits at array index x + 1 (two ghost cells)
+            if (D == 1) {
+                // lanes run along x: direct loads coalesce.  1-based cell x
+                // sits at array index x + 1 (two ghost cells)
                 const int64_t c0 = (int64_t)rc * qr_s + (int64_t)(Qc + 1) * qq_s +
                                    (int64_t)(P + 1) * qp_s;
-                const int64_t plane = (int64_t)(pt.mx + 4) * (pt.my + 4) * (pt.mz + 4);
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
                     ql[m] = q[m * plane + c0];
                     qrr[m] = q[m * plane + c0 + qr_s];
+                }
+            } else {
+                // lines run along y or z: stage the round's cells through LDS
+                // with consecutive lanes on consecutive lines (= consecutive x)
+                const int rows = n + 4, ncell = nl * rows;
+                const int64_t base = (int64_t)(rd * lines_per + 1) * qq_s + (int64_t)(P + 1) * qp_s;
+                __syncthreads();   // previous round's readers are done
+                for (int idx = tid; idx < NV * ncell; idx += SWEEP3_LANES) {
+                    const int m = idx / ncell, rem = idx - m * ncell;
+                    const int row = rem / nl, qq = rem - row * nl;
+                    qst[idx] = q[m * plane + base + (int64_t)row * qr_s + (int64_t)qq * qq_s];
+                }
+                __syncthreads();
+                const int qq = live ? qi : 0;
+#pragma unroll
+                for (int m = 0; m < NV; m++) {
+                    ql[m] = qst[(m * rows + rc) * nl + qq];
+                    qrr[m] = qst[(m * rows + rc + 1) * nl + qq];
                 }
             }
             const typename Sol::Cache cache = Sol::prep(M.ph, ql, qrr);
@@ -426,41 +446,90 @@ __global__ __launch_bounds__(SWEEP3_LANES) void sweep3_phase_kernel(Sweep3Args a
                     }
                 }
                 // ---- apply to the faces (step3.f:221-250 and analogues) ----
-                const int c = r - 1;
+                // The round's finished cells go to LDS; then one lane per
+                // target face of the round (targets one line beyond the
+                // round's on each side) loads the face once per plane offset,
+                // adds this round's increments in ascending line order and
+                // stores it: the reference's order with a single access per
+                // face.  Consecutive lanes take consecutive x (cells c for the
+                // x-sweep, lines for the y-, z-sweeps), so the accesses coalesce.
+                __syncthreads();   // the partial sums have been read
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    if (!tt && k != 1 && k != 4) continue;
+#pragma unroll
+                    for (int m = 0; m < NV; m++) xs[(k * NV + m) * SWEEP3_LANES + tid] = v[k][m];
+                }
+                __syncthreads();
+                const int Q0 = rd * lines_per;
                 const bool to_a = (fam == 0) == g_to_a;
-                if (to_a) {
-                    // face Q+s-1 normal to Q, cell P+o; Q-offset s-1: s=2 first
+                const int ntq = nl + 2, ntarget = ntq * n;
+                // increment slot k of the cell cc of round line `line`
+                auto inc = [&](int line, int cc, int k, int m) -> T {
+                    return xs[(k * NV + m) * SWEEP3_LANES + (line - Q0) * L + cc + 1];
+                };
+                for (int t = tid; t < ntarget; t += SWEEP3_LANES) {
+                    int qq, cc;
+                    if (D == 1) { cc = t % n + 1; qq = t / n; }
+                    else { qq = t % ntq; cc = t / ntq + 1; }
+                    const int Qt = Q0 - 1 + qq;
+                    if (to_a) {
+                        // face Qt normal to Q: line Qt-1 (s = 2), then line Qt (s = 1)
+                        const bool lo = Qt - 1 >= Q0 && Qt - 1 < Q0 + nl;
+                        const bool hi = Qt >= Q0 && Qt < Q0 + nl;
+                        if (Qt < 1 || Qt > NQ + 1 || !(lo || hi)) continue;
 #pragma unroll
-                    for (int s2 = 2; s2 >= 1; s2--) {
-                        __syncthreads();
-                        const int Qf = Q + s2 - 1;
-                        if (cellok && Qf >= 1 && Qf <= NQ + 1) {
+                        for (int o = -1; o <= 1; o++) {
+                            if (o != 0 && !tt) continue;
+                            const int Pc = P + o;
+                            if (Pc < blk.pb || Pc >= blk.pe) continue;
+                            const int64_t idx = A.at(cc, Qt, Pc);
 #pragma unroll
-                            for (int o = -1; o <= 1; o++) {
-                                if (o != 0 && !tt) continue;
-                                const int Pc = P + o;
-                                if (Pc < blk.pb || Pc >= blk.pe) continue;
-                                const bool first = d == 1 && s2 == 2 && o == o_first;
-                                face_add<NV>(a1, a2, A.at(c, Qf, Pc), acs, v[(s2 - 1) * 3 + o + 1],
-                                             first);
+                            for (int m = 0; m < NV; m++) {
+                                const int64_t e = idx + m * acs;
+                                T x1 = a1[e], x2 = a2 ? a2[e] : T(0);
+                                if (lo) {
+                                    const T w = inc(Qt - 1, cc, 3 + o + 1, m);
+                                    x1 = x1 + w;
+                                    x2 = x2 + w;
+                                }
+                                if (hi) {
+                                    const T w = inc(Qt, cc, o + 1, m);
+                                    x1 = x1 + w;
+                                    x2 = x2 + w;
+                                }
+                                a1[e] = x1;
+                                if (a2) a2[e] = x2;
                             }
                         }
-                    }
-                    __syncthreads();
-                } else {
-                    // cell Q+o, face P+s-1 normal to P; Q-offset o: +1 first
+                    } else {
+                        // face P+s-1 normal to P, cell Qt: lines Qt-1 (o = +1),
+                        // Qt (o = 0), Qt+1 (o = -1)
+                        if (Qt < 1 || Qt > NQ) continue;
+                        bool any = false;
 #pragma unroll
-                    for (int o = 1; o >= -1; o--) {
-                        __syncthreads();
-                        const int Qc2 = Q + o;
-                        if (cellok && (o == 0 || tt) && Qc2 >= 1 && Qc2 <= NQ) {
+                        for (int o = 1; o >= -1; o--)
+                            any = any || ((o == 0 || tt) && Qt - o >= Q0 && Qt - o < Q0 + nl);
+                        if (!any) continue;
 #pragma unroll
-                            for (int s2 = 1; s2 <= 2; s2++) {
-                                const int Pf = P + s2 - 1;
-                                if (Pf < blk.pb || Pf >= pe_face) continue;
-                                const bool first = d == 1 && s2 == 2 && o == o_first;
-                                face_add<NV>(b1, b2, B.at(c, Qc2, Pf), bcs, v[(s2 - 1) * 3 + o + 1],
-                                             first);
+                        for (int s2 = 1; s2 <= 2; s2++) {
+                            const int Pf = P + s2 - 1;
+                            if (Pf < blk.pb || Pf >= pe_face) continue;
+                            const int64_t idx = B.at(cc, Qt, Pf);
+#pragma unroll
+                            for (int m = 0; m < NV; m++) {
+                                const int64_t e = idx + m * bcs;
+                                T x1 = b1[e], x2 = b2 ? b2[e] : T(0);
+#pragma unroll
+                                for (int o = 1; o >= -1; o--) {
+                                    const int line = Qt - o;
+                                    if (!((o == 0 || tt) && line >= Q0 && line < Q0 + nl)) continue;
+                                    const T w = inc(line, cc, (s2 - 1) * 3 + o + 1, m);
+                                    x1 = x1 + w;
+                                    x2 = x2 + w;
+                                }
+                                b1[e] = x1;
+                                if (b2) b2[e] = x2;
                             }
                         }
                     }

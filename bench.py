@@ -234,7 +234,7 @@ def bench3(a):
     sizes, corners, h, qs = workload.batched_euler3_level(512)
     cells = int(np.prod(sizes, axis=1).sum())
     smax = max(workload.max_wave_speed3(q) for q in qs)
-    dt = 0.8 * h / smax / 3.0
+    dt = 0.5 * h / smax
     prec = capi3.FP32 if a.precision == "fp32" else capi3.FP64
     esize = 4 if prec == capi3.FP32 else 8
 
