@@ -32,7 +32,7 @@ struct Context3 {
     size_t esize = 8;
     hipStream_t stream = nullptr;
     int npatch = 0;
-    int plane_block = 8;
+    int plane_block = 4;
     std::vector<Patch3> patches;
     Patch3* d_patches = nullptr;
     std::vector<Block3> blocks[NKIND];
@@ -142,8 +142,18 @@ int tend(hipEvent_t stop) {
     return AMRGPU_OK;
 }
 
-// the batch's work lists: per sweep, (patch, plane block); update, (patch,
-// 256 cells)
+// cost of a sweep work item: planes x rounds of 256 lanes per plane
+int64_t block_cost(const Patch3& pt, int d, const Block3& b) {
+    const int n = d == 0 ? pt.mx : (d == 1 ? pt.my : pt.mz);
+    const int nq = d == 0 ? pt.my : pt.mx;
+    const int lines_per = SWEEP3_LANES / (n + 3);
+    const int rounds = (nq + 2 + lines_per - 1) / lines_per;
+    return (int64_t)(b.pe - b.pb + 2) * rounds;
+}
+
+// the batch's work lists: per sweep, (patch, plane block) in decreasing
+// cost, so that the longest workgroups start first; update, (patch, 256
+// cells)
 void plan_blocks() {
     for (int k = 0; k < NKIND; k++) g3.blocks[k].clear();
     const int pbk = g3.plane_block;
@@ -156,6 +166,12 @@ void plan_blocks() {
         const int ncell = pt.mx * pt.my * pt.mz;
         for (int c = 0; c < ncell; c += 256)
             g3.blocks[3].push_back(Block3{p, c, std::min(c + 256, ncell), 0});
+    }
+    for (int d = 0; d < 3; d++) {
+        auto& v = g3.blocks[d];
+        std::stable_sort(v.begin(), v.end(), [d](const Block3& a, const Block3& b) {
+            return block_cost(g3.patches[a.patch], d, a) > block_cost(g3.patches[b.patch], d, b);
+        });
     }
 }
 
@@ -302,6 +318,12 @@ int get_fluxes(int p, double* const out[6]) {
         std::vector<float> f(n);
         HIPCHK3(d2h3(f.data(), static_cast<float*>(g3.d_face) + pt.foff, n * 4));
         for (int64_t k = 0; k < n; k++) h[k] = f[k];
+        // the FP32 step keeps fp = fm + (faddp - faddm), gp likewise
+        // (sweep3.hpp, merged form): rebuild fp and gp as the update uses them
+        for (int64_t k = 0; k < nv * F.sx; k++)
+            h[F.fp - pt.foff + k] = (double)(f[F.fm - pt.foff + k] + f[F.fp - pt.foff + k]);
+        for (int64_t k = 0; k < nv * F.sy; k++)
+            h[F.gp - pt.foff + k] = (double)(f[F.gm - pt.foff + k] + f[F.gp - pt.foff + k]);
     }
     const int mit = pt.mx + 4, mjt = pt.my + 4;
     auto at = [&](int i, int j, int k) { return (((int64_t)(k + 1) * mjt + (j + 1)) * mit + (i + 1)) * nv; };
@@ -412,7 +434,7 @@ int amrgpu3_set_patches(int npatch, const int* mx, const int* my, const int* mz,
             return fail3(AMRGPU_ERR_UNSUPPORTED, "patch %d: a side exceeds %d cells", p,
                          SWEEP3_LANES - 3);
     }
-    g3.plane_block = plane_block ? plane_block : 8;
+    g3.plane_block = plane_block ? plane_block : 4;
     return define_batch(npatch, mx, my, mz);
 }
 

@@ -161,6 +161,11 @@ __global__ __launch_bounds__(SWEEP3_LANES, sizeof(T) == 4 ? 3 : 1) void sweep3_p
     const Block3 blk = a.blocks[blockIdx.x];
     const Patch3 pt = a.patches[blk.patch];
     constexpr int d = D;
+    // SPLIT (FP64): fm and fp (gm and gp) accumulate the same transverse
+    // increments separately, as the reference does.  Merged (FP32 bulk): the
+    // increments go to fm (gm) only and fp (gp) holds faddp - faddm, so
+    // fp = fm + (faddp - faddm) up to rounding: half the face traffic.
+    constexpr bool SPLIT = sizeof(T) == 8;
     const int mx = pt.mx, my = pt.my, mz = pt.mz;
     const int64_t mit = mx + 4, mij = mit * (my + 4);
     const Faces3 F = faces3(pt, NV);
@@ -188,15 +193,15 @@ __global__ __launch_bounds__(SWEEP3_LANES, sizeof(T) == 4 ? 3 : 1) void sweep3_p
         n = my; NQ = mx; NP = mz;
         qr_s = mit; qq_s = 1; qp_s = mij;
         N = lin3(ys_J, 1, ys_k); n1 = face + F.gm; n2 = face + F.gp; ncs = F.sy;
-        A = lin3(xs_j, 1, xs_k); a1 = face + F.fm; a2 = face + F.fp; acs = F.sx;
+        A = lin3(xs_j, 1, xs_k); a1 = face + F.fm; a2 = SPLIT ? face + F.fp : nullptr; acs = F.sx;
         B = lin3(zs_j, 1, zs_K); b1 = face + F.hm; b2 = nullptr; bcs = F.sz;
         g_to_a = false;
     } else {
         n = mz; NQ = mx; NP = my;
         qr_s = mij; qq_s = 1; qp_s = mit;
         N = lin3(zs_K, 1, zs_j); n1 = face + F.hm; n2 = face + F.hp; ncs = F.sz;
-        A = lin3(xs_k, 1, xs_j); a1 = face + F.fm; a2 = face + F.fp; acs = F.sx;
-        B = lin3(ys_k, 1, ys_J); b1 = face + F.gm; b2 = face + F.gp; bcs = F.sy;
+        A = lin3(xs_k, 1, xs_j); a1 = face + F.fm; a2 = SPLIT ? face + F.fp : nullptr; acs = F.sx;
+        B = lin3(ys_k, 1, ys_J); b1 = face + F.gm; b2 = SPLIT ? face + F.gp : nullptr; bcs = F.sy;
         g_to_a = true;
     }
     const T dtdx = T(a.dtd[d - 1]);
@@ -328,19 +333,42 @@ __global__ __launch_bounds__(SWEEP3_LANES, sizeof(T) == 4 ? 3 : 1) void sweep3_p
                     fadp[m] = fadp[m] + c;
                 }
             }
-            // normal fluxes of the owned faces (plane P in [pb, pe), line Q in [1, NQ])
-            if (inner && Q >= 1 && Q <= NQ && P >= blk.pb && P < blk.pe) {
-                const int64_t idx = N.at(r, Q, P);
+            // normal fluxes of the owned faces (plane P in [pb, pe), line Q in
+            // [1, NQ]).  y-, z-sweeps: through LDS to lanes across the lines,
+            // so that the face accesses run along x
+            if (D == 1) {
+                if (inner && Q >= 1 && Q <= NQ && P >= blk.pb && P < blk.pe) {
+                    const int64_t idx = N.at(r, Q, P);
+#pragma unroll
+                    for (int m = 0; m < NV; m++) {
+                        n1[idx + m * ncs] = fadm[m];
+                        n2[idx + m * ncs] = SPLIT ? fadp[m] : fadp[m] - fadm[m];
+                    }
+                }
+            } else {
+                __syncthreads();   // xs free
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
-                    const int64_t k = idx + m * ncs;
-                    if (d == 1) {
-                        n1[k] = fadm[m];
-                        n2[k] = fadp[m];
-                    } else {
+                    xs[m * SWEEP3_LANES + tid] = fadm[m];
+                    xs[(NV + m) * SWEEP3_LANES + tid] = fadp[m];
+                }
+                __syncthreads();
+                const int Q0 = rd * lines_per;
+                for (int t = tid; t < nl * (n + 1); t += SWEEP3_LANES) {
+                    const int qq = t % nl, rr = t / nl + 1, Qn = Q0 + qq;
+                    if (Qn < 1 || Qn > NQ || P < blk.pb || P >= blk.pe) continue;
+                    const int64_t idx = N.at(rr, Qn, P);
+                    const int src = qq * L + rr;
+#pragma unroll
+                    for (int m = 0; m < NV; m++) {
+                        const int64_t k = idx + m * ncs;
                         const T x0 = trans ? n1[k] : T(0);
-                        n1[k] = x0 + fadm[m];
-                        n2[k] = x0 + fadp[m];
+                        const T fm_ = xs[m * SWEEP3_LANES + src];
+                        const T fp_ = xs[(NV + m) * SWEEP3_LANES + src];
+                        n1[k] = x0 + fm_;
+                        // y-sweep, merged form: gp keeps faddp - faddm (the
+                        // z-sweep adds its increments to gm only)
+                        n2[k] = (SPLIT || D == 3) ? x0 + fp_ : fp_ - fm_;
                     }
                 }
             }
@@ -568,8 +596,13 @@ __global__ __launch_bounds__(256) void update3_kernel(T* q, const T* face, const
     const T dtdx = T(dtdx_d), dtdy = T(dtdy_d), dtdz = T(dtdz_d);
 #pragma unroll
     for (int m = 0; m < NV; m++) {
-        const T fmr = face[F.fm + m * F.sx + xf + 1], fpl = face[F.fp + m * F.sx + xf];
-        const T gmr = face[F.gm + m * F.sy + yf + mx], gpl = face[F.gp + m * F.sy + yf];
+        const T fmr = face[F.fm + m * F.sx + xf + 1];
+        const T gmr = face[F.gm + m * F.sy + yf + mx];
+        T fpl = face[F.fp + m * F.sx + xf], gpl = face[F.gp + m * F.sy + yf];
+        if (sizeof(T) != 8) {   // merged form: fp, gp hold faddp - faddm
+            fpl = face[F.fm + m * F.sx + xf] + fpl;
+            gpl = face[F.gm + m * F.sy + yf] + gpl;
+        }
         const T hmr = face[F.hm + m * F.sz + zf + (int64_t)mx * my], hpl = face[F.hp + m * F.sz + zf];
         T v = qc[m * plane];
         v = v - dtdx * (fmr - fpl) - dtdy * (gmr - gpl) - dtdz * (hmr - hpl);

@@ -281,7 +281,7 @@ def bench3(a):
                                "seed 20261203, radial bump, Roe+efix, MC, method(3)=22",
                    "patches": len(qs), "cells_per_step": cells, "nghost": 2,
                    "precision": a.precision, "launch": "batched",
-                   "plane_block": a.plane_block or 8},
+                   "plane_block": a.plane_block or 4},
         "kernels_ms_per_step": kms,
     }
     step_kern = sum(kms.values())

@@ -57,7 +57,7 @@ int amrgpu3_last_error(char* buf, int len);
 /* Define the batch: npatch grids of mx[p] x my[p] x mz[p] interior cells
  * (the grids of a level, node(ndilo..ndkhi)); allocates their solution and
  * flux arrays on the device.  plane_block: planes per workgroup of a sweep
- * (0: default 8); a smaller block gives more workgroups, a larger one fewer
+ * (0: default 4); a smaller block gives more workgroups, a larger one fewer
  * recomputed planes. */
 int amrgpu3_set_patches(int npatch, const int* mx, const int* my, const int* mz,
                         int plane_block);

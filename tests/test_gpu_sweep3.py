@@ -118,11 +118,15 @@ def test_stepgrid3_fp32_within_tolerance(gpu, case):
         print("stepgrid3 fp32", case, (mx, my, mz), flush=True)
         q = patch(rng, case[0], mx, my, mz)
         dx, dy, dz, dt = mesh(mx, my, mz)
-        qo, _, co, _ = orc.stepgrid3(p, q, dx, dy, dz, dt)
-        qg, _, cg, _ = capi3.stepgrid3(q, dx, dy, dz, dt)
+        qo, fo, co, _ = orc.stepgrid3(p, q, dx, dy, dz, dt)
+        qg, fg, cg, _ = capi3.stepgrid3(q, dx, dy, dz, dt, want_fluxes=True)
         s = (slice(2, -2),) * 3
         assert rel_l1(qg[s], qo[s]) <= 1e-5, (mx, my, mz)
         assert cg == pytest.approx(co, rel=1e-5)
+        # fp and gp are rebuilt from the merged form (fm + (faddp - faddm))
+        for k, (a, b) in enumerate(zip(interior_faces(fg, mx, my, mz),
+                                       interior_faces(fo, mx, my, mz))):
+            assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1.0), (mx, my, mz, k)
     capi3.finalize3()
 
 
