@@ -32,3 +32,28 @@ def test_shim_compiles_and_binds_exported_symbols(tmp_path):
                               text=True).stdout
     for s in bound:
         assert (" " + s + "\n") in exported, s
+
+
+SHIM3 = os.path.join(ROOT, "amrclaw_amd", "fortran", "amrgpu3_shim.f90")
+MODS3 = os.path.join(ROOT, "oracle", "_ref", "obj3")
+
+
+@pytest.mark.skipif(not os.path.exists(FC) or not os.path.exists(os.path.join(MODS3, "amr_module.mod")),
+                    reason="amdflang or the reference 3-D module build absent")
+def test_3d_shim_compiles_and_binds_exported_symbols(tmp_path):
+    """The 3-D shim defines stepgrid with src/3d/stepgrid.f's name against the
+    reference's 3-D amr_module and binds only exported amrgpu3_* symbols."""
+    obj = tmp_path / "shim3.o"
+    r = subprocess.run([FC, "-c", "-I" + MODS3, SHIM3, "-o", str(obj), "-module-dir", str(tmp_path)],
+                       capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    syms = subprocess.run(["nm", str(obj)], capture_output=True, text=True).stdout.split("\n")
+    defined = {l.split()[-1] for l in syms if " T " in l}
+    undefined = {l.split()[-1] for l in syms if " U " in l}
+    assert "stepgrid_" in defined
+    bound = {s for s in undefined if s.startswith("amrgpu3_")}
+    assert {"amrgpu3_stepgrid", "amrgpu3_last_error"} <= bound
+    exported = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
+                              text=True).stdout
+    for s in bound:
+        assert (" " + s + "\n") in exported, s
