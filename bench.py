@@ -36,6 +36,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel(s), from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of the default workloads (scaled by the
+# plain-copy calibration: FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB); the
+# summaries and commands are the files named here.
+MEASURED_TRAFFIC = {
+    ("euler2d", "fp32", 4096): (459126 * 2048 + 450853 * 1024,
+                                "profiles/level_update_euler_fp32_4096.md"),
+    ("euler3d", "fp32", 512): (
+        (271400 + 875200 + 1415000 + 330700) * 2048 + (801800 + 1136000 + 1143000 + 98580) * 1024,
+        "profiles/step3_euler_fp32_512.md"),
+}
 METRIC = "cell-updates/s, FP32-batched vs FP64 per-patch at fixed L1 err; % HBM peak"
 
 
@@ -288,8 +299,11 @@ def bench3(a):
     dom = max(names[:3], key=lambda nm: kms[nm])
     bpc = 2 * 5 * esize
     achieved = cells * bpc / (step_kern / 1e3) / 1e9
+    tr = MEASURED_TRAFFIC.get(("euler3d", a.precision, len(qs))) if not a.plane_block else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": tr[0] if tr else None,
+                          "traffic_source": tr[1] if tr else None,
                           "kernel": "sweep3_phase_kernel (x+y+z) + update3_kernel",
                           "kernel_ms": step_kern, "dominant": dom,
                           "bytes_per_cell_update": bpc,
@@ -369,11 +383,14 @@ def main():
                                 for k in ("sweep", "fill", "registers")},
     }
     if sweep_n:
+        tuned_default = not (a.txmax or a.tymax or a.per_patch)
+        tr = MEASURED_TRAFFIC.get(("euler2d", a.precision, npatch)) if tuned_default else None
         kern_ms = sweep_ms / sweep_n        # one timed record per step's sweep
         achieved = cells * bytes_per_cell / (kern_ms / 1e3) / 1e9
         result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                              "traffic": None,
+                              "traffic": tr[0] if tr else None,
+                              "traffic_source": tr[1] if tr else None,
                               "kernel": "sweep_kernel<EulerRoe>",
                               "kernel_ms": kern_ms,
                               "bytes_per_cell_update": bytes_per_cell,
@@ -406,6 +423,14 @@ def main():
         result["speedup_vs_fp64_batched"] = r64["wall"] / r["wall"]
         result["l1_rel_vs_fp64"] = num / den
         result["mass_rel_diff_vs_fp64"] = abs(m32 - m64) / abs(m64)
+        # the same measure on the initial data alone: rounding the FP64 state
+        # to FP32 storage already moves the level's mass by this much
+        r64_0 = r32_0 = 0.0
+        for k in range(npatch):
+            rho = H.grid(2, k, 0)[2:-2, 2:-2, 0]
+            r64_0 += float(rho.sum())
+            r32_0 += float(rho.astype(np.float32).astype(np.float64).sum())
+        result["mass_rel_diff_initial_rounding"] = abs(r32_0 - r64_0) / abs(r64_0)
         result["steps_compared"] = a.steps + a.warmup
     if a.cpu_seconds > 0:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
