@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libamrgpu.so")
 
 ADVECTION, ACOUSTICS, EULER = 1, 2, 3
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
 
 
 class AmrgpuConfig(C.Structure):

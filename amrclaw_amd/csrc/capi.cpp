@@ -125,6 +125,7 @@ struct Context {
     amrgpu_config cfg{};
     MethodArgs meth{};
     size_t esize = 8;
+    bool mixed = false;   // AMRGPU_MIXED: FP32 sweep on FP64 storage, flux form
     hipStream_t stream = nullptr;
     hipEvent_t ev[NEVENTS] = {};
     int txmax = 28, tymax = 28;
@@ -317,8 +318,8 @@ int plan_level(Level& L) {
                 t.wx = (int16_t)seg_width(t.tx);
                 t.wy = (int16_t)seg_width(t.ty);
                 L.tiles.push_back(t);
-                size_t b = g.esize == 8 ? sweep_lds_bytes<double>(nv, t.tx, t.ty)
-                                        : sweep_lds_bytes<float>(nv, t.tx, t.ty);
+                size_t b = (g.esize == 8 && !g.mixed) ? sweep_lds_bytes<double>(nv, t.tx, t.ty)
+                                                      : sweep_lds_bytes<float>(nv, t.tx, t.ty);
                 lds = std::max(lds, b);
             }
     }
@@ -748,6 +749,7 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     a.fine_patches = rl.fine_patches;
     a.sv = rl.sv;
     auto launch = [&](const SweepArgs& x) {
+        if (g.mixed) return launch_sweep_mixed(g.cfg.solver, x, L.lds, g.stream);
         return g.esize == 8 ? launch_sweep_f64(g.cfg.solver, x, L.lds, g.stream)
                             : launch_sweep_f32(g.cfg.solver, x, L.lds, g.stream);
     };
@@ -832,6 +834,7 @@ int launch_level_step(int level) {
         a.hx = L.hx;
         a.hy = L.hy;
         a.delt = L.possk;
+        a.flux_form = g.mixed ? 1 : 0;
         tbegin(AMRGPU_KT_REGISTERS);
         hipError_t e = launch_finereg(g.cfg.solver, (int)g.esize, a, g.meth.ph, g.stream);
         tend(AMRGPU_KT_REGISTERS);
@@ -1213,8 +1216,9 @@ int amrgpu_init(const amrgpu_config* cfg) {
         if (cfg->mthbc[k] < 1 || cfg->mthbc[k] > 3)
             return fail(AMRGPU_ERR_UNSUPPORTED, "mthbc(%d)=%d: supported are 1 extrap, 2 periodic, 3 wall",
                         k + 1, cfg->mthbc[k]);
-    if (cfg->precision != AMRGPU_FP64 && cfg->precision != AMRGPU_FP32)
-        return fail(AMRGPU_ERR_ARG, "precision must be AMRGPU_FP64 or AMRGPU_FP32");
+    if (cfg->precision != AMRGPU_FP64 && cfg->precision != AMRGPU_FP32 &&
+        cfg->precision != AMRGPU_MIXED)
+        return fail(AMRGPU_ERR_ARG, "precision must be AMRGPU_FP64, AMRGPU_FP32 or AMRGPU_MIXED");
     if (cfg->method[1] < 1 || cfg->method[1] > 2 || cfg->method[2] < 0 || cfg->method[2] > 2)
         return fail(AMRGPU_ERR_ARG, "method(2)=%d method(3)=%d invalid", cfg->method[1], cfg->method[2]);
     for (int mw = 0; mw < cfg->mwaves; mw++)
@@ -1222,7 +1226,8 @@ int amrgpu_init(const amrgpu_config* cfg) {
             return fail(AMRGPU_ERR_ARG, "mthlim(%d)=%d invalid", mw + 1, cfg->mthlim[mw]);
     g = Context();
     g.cfg = *cfg;
-    g.esize = cfg->precision == AMRGPU_FP64 ? 8 : 4;
+    g.esize = cfg->precision == AMRGPU_FP32 ? 4 : 8;
+    g.mixed = cfg->precision == AMRGPU_MIXED;
     MethodArgs& M = g.meth;
     M.ph.ubar = cfg->ubar;
     M.ph.vbar = cfg->vbar;
@@ -1244,9 +1249,10 @@ int amrgpu_init(const amrgpu_config* cfg) {
         SweepArgs none{};
         HIPCHK(launch_sweep_f64(g.cfg.solver, none, 0, g.stream));
         HIPCHK(launch_sweep_f32(g.cfg.solver, none, 0, g.stream));
+        HIPCHK(launch_sweep_mixed(g.cfg.solver, none, 0, g.stream));
     }
-    g.txmax = g.esize == 8 ? 28 : 28;
-    g.tymax = g.esize == 8 ? 12 : 28;
+    g.txmax = 28;
+    g.tymax = (g.esize == 8 && !g.mixed) ? 12 : 28;
     g.inited = true;
     return AMRGPU_OK;
 }

@@ -8,6 +8,7 @@
 namespace amrgpu {
 hipError_t launch_sweep_f64(int solver, const SweepArgs& a, size_t lds, hipStream_t st);
 hipError_t launch_sweep_f32(int solver, const SweepArgs& a, size_t lds, hipStream_t st);
+hipError_t launch_sweep_mixed(int solver, const SweepArgs& a, size_t lds, hipStream_t st);
 hipError_t launch_finereg(int solver, int esize, const QadArgs& a, const Physics& ph, hipStream_t st);
 hipError_t launch_update(int esize, const UpdateArgs& a, const int64_t* d_block_start,
                          int64_t navg_cells, hipStream_t st);

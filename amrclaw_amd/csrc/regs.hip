@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void finereg_kernel(QadArgs a, Physics ph) {
     double qcv[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) qcv[m] = qc[(int64_t)(it.slot - 1) * NV + m];
-    for (int l = 1; l <= r; l++) {
+    for (int l = 1; l <= (a.flux_form ? 0 : r); l++) {
         const int f = (it.seg - 1) * r + l;        // fine row (sides 1,3) or column (2,4)
         int I, J;
         if (it.side == 1) { I = 0; J = f; }

@@ -42,6 +42,11 @@ struct QadArgs {
     double* reg;            // the level's register arena
     int32_t ratx, raty;     // intratx/intraty(level-1)
     double hx, hy, delt;
+    // 1: the level's fluxes are in flux form (the mixed path), so fm/fp at
+    // the grid edges are the fluxes themselves and qad's Riemann problems,
+    // which turn fluctuations into fluxes, are skipped
+    int32_t flux_form;
+    int32_t pad;
 };
 
 struct UpdateArgs {

@@ -62,6 +62,8 @@ struct FastMathF32 {
 
 // ----------------------------------------------------------------------------
 template <typename T, class Mth = ExactMath<T>> struct Advection {
+    // the same solver in another precision (FP64 fluxes of the mixed path)
+    template <typename U> using Rebind = Advection<U, ExactMath<U>>;
     static constexpr int NV = 1, NW = 1;
     using Math = Mth;
     struct Cache {};
@@ -74,6 +76,10 @@ template <typename T, class Mth = ExactMath<T>> struct Advection {
         amdq[0] = dmin1(s[0], T(0)) * wave[0][0];
         apdq[0] = dmax1(s[0], T(0)) * wave[0][0];
     }
+    // physical flux f(q) in direction ixy (the flux form of the mixed path)
+    __device__ static void flux(const Physics& ph, int ixy, const T* q, T f[NV]) {
+        f[0] = ((ixy == 1) ? T(ph.ubar) : T(ph.vbar)) * q[0];
+    }
     __device__ static void rpt2(const Physics& ph, int ixy, int, const T*, const T*,
                                 const T*, const T*, const T*, const Cache&,
                                 const T asdq[NV], T bm[NV], T bp[NV]) {
@@ -85,6 +91,8 @@ template <typename T, class Mth = ExactMath<T>> struct Advection {
 
 // ----------------------------------------------------------------------------
 template <typename T, class Mth = ExactMath<T>> struct Acoustics {
+    // the same solver in another precision (FP64 fluxes of the mixed path)
+    template <typename U> using Rebind = Acoustics<U, ExactMath<U>>;
     static constexpr int NV = 3, NW = 2;
     using Math = Mth;
     struct Cache {};
@@ -106,6 +114,13 @@ template <typename T, class Mth = ExactMath<T>> struct Acoustics {
             apdq[m] = s[1] * wave[1][m];
         }
     }
+    // f(q) = A q: (K u, p / rho, 0) in x, (K v, 0, p / rho) in y
+    __device__ static void flux(const Physics& ph, int ixy, const T* q, T f[NV]) {
+        const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+        f[0] = T(ph.bulk) * q[mu];
+        f[mu] = Mth::div(q[0], T(ph.rho));
+        f[mv] = T(0);
+    }
     __device__ static void rpt2(const Physics& ph, int ixy, int, const T*, const T*,
                                 const T*, const T*, const T*, const Cache&,
                                 const T asdq[NV], T bm[NV], T bp[NV]) {
@@ -120,6 +135,8 @@ template <typename T, class Mth = ExactMath<T>> struct Acoustics {
 
 // ----------------------------------------------------------------------------
 template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
+    // the same solver in another precision (FP64 fluxes of the mixed path)
+    template <typename U> using Rebind = EulerRoe<U, ExactMath<U>>;
     static constexpr int NV = 4, NW = 4;
     using Math = Mth;
     struct Cache { T u, v, enth, a, g1a2, euv, u2v2; };
@@ -146,6 +163,17 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
 
     __device__ static T pressure(T gamma1, T rho, T mx, T my, T en) {
         return gamma1 * (en - Mth::div(T(0.5) * (mx * mx + my * my), rho));
+    }
+
+    // f(q) = (rho u, rho u^2 + p, rho u v, u (E + p)) along momentum mu
+    __device__ static void flux(const Physics& ph, int ixy, const T* q, T f[NV]) {
+        const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+        const T p = pressure(T(ph.gamma) - T(1), q[0], q[1], q[2], q[3]);
+        const T u = Mth::div(q[mu], q[0]);
+        f[0] = q[mu];
+        f[mu] = q[mu] * u + p;
+        f[mv] = q[mv] * u;
+        f[3] = u * (q[3] + p);
     }
 
     __device__ static void rpn2(const Physics& ph, int ixy, const T* L, const T* R,

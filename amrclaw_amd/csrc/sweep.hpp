@@ -89,7 +89,17 @@ template <class Sol, typename T> struct LaneResult {
 // flux2.f for one interface.  `inner` marks interfaces 1..n+1 of the slice
 // (the ones flux2 limits and uses); lanes outside still run the solver on
 // clamped, valid states so the shuffles see defined values.
-template <class Sol, typename T>
+//
+// FLUXFORM (the mixed path: FP32 waves, FP64 storage): both cells of the
+// interface take the same increment G = A-dq (+ the second-order and
+// transverse terms) instead of A-dq and -A+dq; the update adds the FP64
+// physical fluxes, so the interface flux is F = f(q_l) + G for both cells.
+// In exact arithmetic f(q_r) - F = A+dq (the Riemann solvers' conservation
+// property), so the scheme is the same; in finite precision the cells'
+// updates telescope and the totals are conserved to FP64 rounding although
+// the waves are FP32, and F carries only G's FP32 error, which scales with
+// the jump, not with f.
+template <class Sol, typename T, bool FLUXFORM = false>
 __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T* ql,
                                            const T* qr, bool inner, int w, T dtdx,
                                            LaneResult<Sol, T>& o) {
@@ -98,10 +108,18 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
     T wave[NW][NV], s[NW], amdq[NV], apdq[NV];
     Sol::rpn2(M.ph, ixy, ql, qr, nullptr, nullptr, cache, wave, s, amdq, apdq);
 
+    if (FLUXFORM) {
 #pragma unroll
-    for (int m = 0; m < NV; m++) {
-        o.fadm[m] = T(0) + amdq[m];
-        o.fadp[m] = T(0) - apdq[m];
+        for (int m = 0; m < NV; m++) {
+            o.fadm[m] = T(0) + amdq[m];
+            o.fadp[m] = o.fadm[m];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < NV; m++) {
+            o.fadm[m] = T(0) + amdq[m];
+            o.fadp[m] = T(0) - apdq[m];
+        }
     }
     T cfl = T(0);
     if (inner) {
@@ -197,6 +215,9 @@ __host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty) {
 template <class Sol, typename T, typename S>
 __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
     constexpr int NV = Sol::NV;
+    // compute precision below the storage precision: the mixed path (flux
+    // form, update in the storage precision)
+    constexpr bool MIXED = sizeof(T) < sizeof(S);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
 
@@ -215,6 +236,14 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
 
     const int tid = threadIdx.x;
     const S* qin = static_cast<const S*>(a.qin) + pd.off;
+    // mixed path: FP64 physical flux of the stored state at patch-local
+    // linear index li (x-flux for ixy = 1, y-flux for 2)
+    auto phys_flux = [&](int ixy, int64_t li, double* f) {
+        double qd[NV];
+#pragma unroll
+        for (int m = 0; m < NV; m++) qd[m] = double(qin[(int64_t)m * pd.plane + li]);
+        Sol::template Rebind<double>::flux(a.meth.ph, ixy, qd, f);
+    };
 
     // 1. stage tile + halo; zero gm (the x-phase accumulates into it)
     {
@@ -252,7 +281,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
             }
             const bool inner = live && c >= 1 && c <= tx + 1;
             LaneResult<Sol, T> o;
-            flux2_lane<Sol, T>(a.meth, 1, ql, qr, inner, w, dtdx, o);
+            flux2_lane<Sol, T, MIXED>(a.meth, 1, ql, qr, inner, w, dtdx, o);
             cfl = dmax1(cfl, o.cfl);
             if (inner && r >= 1 && r <= ty) {
 #pragma unroll
@@ -301,7 +330,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
             }
             const bool inner = live && e >= 1 && e <= ty + 1;
             LaneResult<Sol, T> o;
-            flux2_lane<Sol, T>(a.meth, 2, ql, qr, inner, w, dtdy, o);
+            flux2_lane<Sol, T, MIXED>(a.meth, 2, ql, qr, inner, w, dtdy, o);
             cfl = dmax1(cfl, o.cfl);
             if (inner && c >= 1 && c <= tx) {
 #pragma unroll
@@ -343,12 +372,30 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
         for (int idx = tid; idx < n; idx += 256) {
             const int j = idx / tx, k = idx - j * tx;   // 0-based interior of tile
             const int li = (td.j0 + j + 2) * pd.mitot + (td.i0 + k + 2);
+            double fxc[NV], fxw[NV], fyc[NV], fys[NV];
+            if constexpr (MIXED) {
+                phys_flux(1, li, fxc);
+                phys_flux(1, li - 1, fxw);
+                phys_flux(2, li, fyc);
+                phys_flux(2, li - pd.mitot, fys);
+            }
 #pragma unroll
             for (int m = 0; m < NV; m++) {
-                T q = qs[(m * qsy + j + 2) * qsx + k + 2];
-                T dx_ = fm[m * fpl + j * fstr + (k + 1)] - fp[m * fpl + j * fstr + k];
-                T dy_ = gm[m * gpl + (j + 1) * tx + k] - gp[m * gpl + j * tx + k];
-                qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
+                if constexpr (MIXED) {
+                    // update in FP64 from the stored state: F(i+1) - F(i) =
+                    // (f(q_i) - f(q_i-1)) + (G(i+1) - G(i)), f in FP64
+                    const double q = qin[(int64_t)m * pd.plane + li];
+                    const double dx_ = (fxc[m] - fxw[m]) + (double(fm[m * fpl + j * fstr + (k + 1)]) -
+                                                            double(fp[m * fpl + j * fstr + k]));
+                    const double dy_ = (fyc[m] - fys[m]) + (double(gm[m * gpl + (j + 1) * tx + k]) -
+                                                            double(gp[m * gpl + j * tx + k]));
+                    qout[(int64_t)m * pd.plane + li] = q - dtdx_d * dx_ - dtdy_d * dy_;
+                } else {
+                    T q = qs[(m * qsy + j + 2) * qsx + k + 2];
+                    T dx_ = fm[m * fpl + j * fstr + (k + 1)] - fp[m * fpl + j * fstr + k];
+                    T dy_ = gm[m * gpl + (j + 1) * tx + k] - gp[m * gpl + j * tx + k];
+                    qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
+                }
             }
         }
         // ghost cells on the patch edges keep their time-t values in the new
@@ -366,7 +413,9 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
                     const int li = (td.j0 + j + 2) * pd.mitot + (td.i0 + k + 2);
 #pragma unroll
                     for (int m = 0; m < NV; m++)
-                        qout[(int64_t)m * pd.plane + li] = S(qs[(m * qsy + j + 2) * qsx + k + 2]);
+                        qout[(int64_t)m * pd.plane + li] =
+                            MIXED ? qin[(int64_t)m * pd.plane + li]
+                                  : S(qs[(m * qsy + j + 2) * qsx + k + 2]);
                 }
             }
         }
@@ -392,28 +441,41 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
     }
     // 5. coarse-fine registers (reads the final fluxes in LDS only)
     if (a.reg) {
+        // mixed path: the flux through an edge is f(state left of / below
+        // it) + G (FP64 f), the same value the cells' update implies
         double* per = a.reg + perim_offset(pd, NV);
+        auto flux_at = [&](const T* g, int gstride, int m, int ixy, int64_t li_left) -> double {
+            double v = double(g[m * gstride]);
+            if constexpr (MIXED) {
+                double f[NV];
+                phys_flux(ixy, li_left, f);
+                v = f[m] + v;
+            }
+            return v;
+        };
         if (td.i0 == 0)
             for (int idx = tid; idx < NV * ty; idx += 256) {
                 const int j = idx / NV, m = idx - j * NV;
-                per[(int64_t)(td.j0 + j) * NV + m] = double(fm[m * fpl + j * fstr]);
+                per[(int64_t)(td.j0 + j) * NV + m] =
+                    flux_at(fm + j * fstr, fpl, m, 1, (int64_t)(td.j0 + j + 2) * pd.mitot + 1);
             }
         if (td.i0 + tx == pd.mx)
             for (int idx = tid; idx < NV * ty; idx += 256) {
                 const int j = idx / NV, m = idx - j * NV;
                 per[(int64_t)NV * (pd.my + pd.mx) + (int64_t)(td.j0 + j) * NV + m] =
-                    double(fp[m * fpl + j * fstr + tx]);
+                    flux_at(fp + j * fstr + tx, fpl, m, 1, (int64_t)(td.j0 + j + 2) * pd.mitot + pd.mx + 1);
             }
         if (td.j0 + ty == pd.my)
             for (int idx = tid; idx < NV * tx; idx += 256) {
                 const int k = idx / NV, m = idx - k * NV;
-                per[(int64_t)NV * pd.my + (int64_t)(td.i0 + k) * NV + m] = double(gp[m * gpl + ty * tx + k]);
+                per[(int64_t)NV * pd.my + (int64_t)(td.i0 + k) * NV + m] =
+                    flux_at(gp + ty * tx + k, gpl, m, 2, (int64_t)(pd.my + 1) * pd.mitot + (td.i0 + k + 2));
             }
         if (td.j0 == 0)
             for (int idx = tid; idx < NV * tx; idx += 256) {
                 const int k = idx / NV, m = idx - k * NV;
                 per[(int64_t)NV * (2 * pd.my + pd.mx) + (int64_t)(td.i0 + k) * NV + m] =
-                    double(gm[m * gpl + k]);
+                    flux_at(gm + k, gpl, m, 2, (int64_t)1 * pd.mitot + (td.i0 + k + 2));
             }
     }
     if (a.sv) {
@@ -432,7 +494,17 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
             else if (s.side == 2) { src = gm + (j + 1) * tx + k; stride = gpl; h = a.dx; }
             else if (s.side == 3) { src = fm + j * fstr + k + 1; stride = fpl; h = a.dy; }
             else { src = gp + j * tx + k; stride = gpl; h = a.dx; }
-            for (int m = 0; m < NV; m++) slot[m] = -double(src[m * stride]) * dtc * h;
+            if constexpr (MIXED) {
+                // F = f(state left of / below the edge) + G: sides 1 and 4
+                // look at the coarse cell's left / lower neighbour
+                const int64_t lc = (int64_t)(s.J + 1) * pd.mitot + (s.I + 1);
+                const int64_t ll = s.side == 1 ? lc - 1 : (s.side == 4 ? lc - pd.mitot : lc);
+                double f[NV];
+                phys_flux((s.side == 1 || s.side == 3) ? 1 : 2, ll, f);
+                for (int m = 0; m < NV; m++) slot[m] = -(f[m] + double(src[m * stride])) * dtc * h;
+            } else {
+                for (int m = 0; m < NV; m++) slot[m] = -double(src[m * stride]) * dtc * h;
+            }
         }
     }
     cfl = wave_max(cfl);

@@ -16,10 +16,13 @@ coarse step (level-2 time dt2, between the coarse times 0 and 2 dt2).
 Inputs are resident in HBM before the timed region; the CFL readback is
 not part of the step.
 
-Runs: FP32 batched (the headline), FP64 batched and FP64 with one sweep
-launch per patch (the plain per-patch GPU path) on the same layout and dt;
-the FP32 result's relative L1 error and mass difference against FP64 are
-reported.  The CPU baseline is the reference's own advanc (compiled from
+Runs: the mixed path (the headline: FP32 wave and flux arithmetic on FP64
+storage, flux form with an FP64 update, FP64 registers, coarse interpolation
+and totals), then FP64 batched, FP64 with one sweep launch per patch (the
+plain per-patch GPU path) and FP32 storage on the same layout and dt; the
+relative L1 error and level-mass difference against FP64 are reported, and
+the composite conservation drift of each precision on a periodic two-level
+hierarchy.  The CPU baseline is the reference's own advanc (compiled from
 its Fortran by oracle/Makefile) over a sample of the same hierarchy.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--npatch N]
@@ -41,6 +44,8 @@ HBM_PEAK_GBS = 8000.0
 # plain-copy calibration: FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB); the
 # summaries and commands are the files named here.
 MEASURED_TRAFFIC = {
+    ("euler2d", "mixed", 4096): (1135200 * 2048 + 768800 * 1024,
+                                 "profiles/level_update_euler_mixed_4096.md"),
     ("euler2d", "fp32", 4096): (459126 * 2048 + 450853 * 1024,
                                 "profiles/level_update_euler_fp32_4096.md"),
     ("euler3d", "fp32", 512): (
@@ -58,7 +63,9 @@ def parse():
     ap.add_argument("--npatch", type=int, default=4096)
     ap.add_argument("--txmax", type=int, default=0)
     ap.add_argument("--tymax", type=int, default=0)
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--precision", choices=["mixed", "fp32", "fp64"], default="mixed",
+                    help="mixed (default): FP32 waves and fluxes on FP64 storage, "
+                         "conservative to FP64 rounding; fp32: FP32 storage too")
     ap.add_argument("--per-patch", action="store_true",
                     help="time the one-launch-per-patch sweep instead")
     ap.add_argument("--graphs", action="store_true",
@@ -100,7 +107,7 @@ def install(capi, H, precision, per_patch, graphs, txmax, tymax):
     capi.init(cfg)
     if not (txmax or tymax):
         from amrclaw_amd import autotune
-        tuned = autotune.load(capi.EULER, "fp32" if precision == capi.FP32 else "fp64")
+        tuned = autotune.load(capi.EULER, "fp64" if precision == capi.FP64 else "fp32")
         if tuned:
             txmax, tymax = tuned
     if txmax or tymax:
@@ -139,6 +146,35 @@ def run(capi, dt2, steps, warmup, sync):
         out[name] = (ms, n)
     capi.set_timing(False)
     return out
+
+
+def conservation_drift(capi, precision, npatch=256, coarse_steps=4):
+    """Composite conservation on a periodic domain: the two-level batched
+    hierarchy with npatch level-2 patches and periodic boundaries, level 1
+    first averaged from level 2, then coarse_steps Berger-Oliger steps
+    (advance 1, advance 2 twice, update 1 with refluxing).  Returns
+    |M - M0| / |M0| of conck(1) (the FP64 total of level 1)."""
+    from amrclaw_amd import workload
+    H, _ = workload.batched_euler_hierarchy(npatch)
+    cfg = capi.make_config(capi.EULER, order=2, transverse=2, limiter=4, cfl=0.9, cflv1=1.0,
+                           gamma=1.4, efix=True, mthbc=(2, 2, 2, 2), precision=precision)
+    capi.init(cfg)
+    try:
+        capi.load_hierarchy(H)
+        dt2 = H.levels[1]["possk"]
+        capi.set_time(1, 0.0, 2.0 * dt2)
+        capi.set_time(2, 0.0, dt2)
+        capi.update_level(1)
+        m0 = capi.conck(1)
+        for _ in range(coarse_steps):
+            capi.advance_level(1, want_cfl=False)
+            capi.advance_level(2, want_cfl=False)
+            capi.advance_level(2, want_cfl=False)
+            capi.update_level(1)
+        m = capi.conck(1)
+    finally:
+        capi.finalize()
+    return abs(m - m0) / abs(m0)
 
 
 def interiors(capi, H):
@@ -246,7 +282,9 @@ def bench3(a):
     cells = int(np.prod(sizes, axis=1).sum())
     smax = max(workload.max_wave_speed3(q) for q in qs)
     dt = 0.5 * h / smax
-    prec = capi3.FP32 if a.precision == "fp32" else capi3.FP64
+    # the 3-D step has FP32 and FP64 paths (the mixed path is 2-D)
+    p3 = "fp64" if a.precision == "fp64" else "fp32"
+    prec = capi3.FP32 if p3 == "fp32" else capi3.FP64
     esize = 4 if prec == capi3.FP32 else 8
 
     def install(precision):
@@ -275,7 +313,7 @@ def bench3(a):
     def interiors():
         return [capi3.get_patch3(k, q.shape)[2:-2, 2:-2, 2:-2] for k, q in enumerate(qs)]
 
-    print("bench3: timed run", a.precision, file=sys.stderr, flush=True)
+    print("bench3: timed run", p3, file=sys.stderr, flush=True)
     install(prec)
     wall, kt = run(a.steps, a.warmup)
     q_main = interiors() if not a.no_compare else None
@@ -291,7 +329,7 @@ def bench3(a):
         "config": {"workload": "batched 3-D Euler step: 512 grids, mx,my,mz even U[8,32] "
                                "seed 20261203, radial bump, Roe+efix, MC, method(3)=22",
                    "patches": len(qs), "cells_per_step": cells, "nghost": 2,
-                   "precision": a.precision, "launch": "batched",
+                   "precision": p3, "launch": "batched",
                    "plane_block": a.plane_block or 4},
         "kernels_ms_per_step": kms,
     }
@@ -299,7 +337,7 @@ def bench3(a):
     dom = max(names[:3], key=lambda nm: kms[nm])
     bpc = 2 * 5 * esize
     achieved = cells * bpc / (step_kern / 1e3) / 1e9
-    tr = MEASURED_TRAFFIC.get(("euler3d", a.precision, len(qs))) if not a.plane_block else None
+    tr = MEASURED_TRAFFIC.get(("euler3d", p3, len(qs))) if not a.plane_block else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": tr[0] if tr else None,
@@ -353,8 +391,14 @@ def main():
     print("bench: building the workload", file=sys.stderr, flush=True)
     H, cells = workload.batched_euler_hierarchy(a.npatch)
     npatch = len(H.levels[1]["boxes"])
-    prec = capi.FP32 if a.precision == "fp32" else capi.FP64
-    esize = 4 if prec == capi.FP32 else 8
+    PREC = {"mixed": capi.MIXED, "fp32": capi.FP32, "fp64": capi.FP64}
+    DESC = {"mixed": "FP32 wave and flux arithmetic on FP64 storage (flux form, FP64 update, "
+                     "registers, coarse interpolation and totals)",
+            "fp32": "FP32 storage, wave and flux arithmetic; FP64 registers, coarse "
+                    "interpolation and totals",
+            "fp64": "FP64 throughout (the reference's arithmetic, bit for bit)"}
+    prec = PREC[a.precision]
+    esize = 4 if prec == capi.FP32 else 8     # storage bytes per value
     bytes_per_cell = 2 * 4 * esize            # (2 nvar + maux) * s
 
     print("bench: timed run", a.precision, file=sys.stderr, flush=True)
@@ -370,12 +414,12 @@ def main():
         "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": r["wall"] / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if prec == capi.FP32 else "f64", "data": "synthetic",
+        "dtype": "f64" if prec == capi.FP64 else "f32", "data": "synthetic",
         "config": {"workload": "batched 2-D Euler level update (level 2 of 2): %d patches, "
                                "mx,my even U[8,128] seed 20261202, Roe+efix, MC, "
                                "transverse 2, extrapolation BCs" % npatch,
                    "patches": npatch, "cells_per_step": cells, "nghost": 2,
-                   "precision": a.precision,
+                   "precision": a.precision, "precision_detail": DESC[a.precision],
                    "launch": "per-patch" if a.per_patch else "batched",
                    "graphs": bool(a.graphs)},
         "step_ms_events": r["events_ms"] / a.steps,
@@ -395,43 +439,68 @@ def main():
                               "kernel_ms": kern_ms,
                               "bytes_per_cell_update": bytes_per_cell,
                               "algorithmic_bytes_per_launch": cells * bytes_per_cell}
-    if not a.no_compare and prec == capi.FP32:
-        print("bench: FP64 batched", file=sys.stderr, flush=True)
-        install(capi, H, capi.FP64, False, False, 0, 0)
-        r64 = run(capi, dt2, a.steps, a.warmup, sync)
-        q64 = interiors(capi, H)
-        capi.finalize()
-        print("bench: FP64 per patch", file=sys.stderr, flush=True)
-        install(capi, H, capi.FP64, True, False, 0, 0)
-        rpp = run(capi, dt2, a.steps, a.warmup, sync)
-        qpp = interiors(capi, H)
-        capi.finalize()
-        same = all(np.array_equal(x, y) for x, y in zip(q64, qpp))
-        num = sum(float(np.abs(x - y).sum()) for x, y in zip(q_main, q64))
-        den = sum(float(np.abs(y).sum()) for y in q64)
+    if not a.no_compare and prec != capi.FP64:
         h2 = H.levels[1]["hx"]
-        m32 = sum(float(x[..., 0].sum()) for x in q_main) * h2 * h2
-        m64 = sum(float(y[..., 0].sum()) for y in q64) * h2 * h2
+
+        def mass(qs):
+            return sum(float(x[..., 0].sum()) for x in qs) * h2 * h2
+
+        runs = {}
+        for name, p_, per_patch in (("fp64_batched", capi.FP64, False),
+                                    ("fp64_per_patch", capi.FP64, True),
+                                    ("fp32" if prec == capi.MIXED else "mixed",
+                                     capi.FP32 if prec == capi.MIXED else capi.MIXED, False)):
+            print("bench:", name, file=sys.stderr, flush=True)
+            install(capi, H, p_, per_patch, False, 0, 0)
+            rr = run(capi, dt2, a.steps, a.warmup, sync)
+            runs[name] = (rr, interiors(capi, H))
+            capi.finalize()
+        r64, q64 = runs["fp64_batched"]
+        rpp, qpp = runs["fp64_per_patch"]
+        den = sum(float(np.abs(y).sum()) for y in q64)
+        m64 = mass(q64)
+
+        def l1(qs):
+            return sum(float(np.abs(x - y).sum()) for x, y in zip(qs, q64)) / den
+
         result["fp64_batched"] = {"value": cells * a.steps / r64["wall"],
                                   "ms_per_step": r64["wall"] / a.steps * 1e3,
                                   "sweep_ms": r64["sweep"][0] / max(r64["sweep"][1], 1)}
         result["fp64_per_patch"] = {"value": cells * a.steps / rpp["wall"],
                                     "ms_per_step": rpp["wall"] / a.steps * 1e3,
                                     "sweep_launches_per_step": npatch,
-                                    "identical_to_fp64_batched": same}
+                                    "identical_to_fp64_batched":
+                                        all(np.array_equal(x, y) for x, y in zip(q64, qpp))}
+        other = "fp32" if prec == capi.MIXED else "mixed"
+        ro, qo = runs[other]
+        result[other] = {"value": cells * a.steps / ro["wall"],
+                         "ms_per_step": ro["wall"] / a.steps * 1e3,
+                         "sweep_ms": ro["sweep"][0] / max(ro["sweep"][1], 1),
+                         "precision_detail": DESC[other],
+                         "speedup_vs_fp64_per_patch": rpp["wall"] / ro["wall"],
+                         "l1_rel_vs_fp64": l1(qo),
+                         "mass_rel_diff_vs_fp64": abs(mass(qo) - m64) / abs(m64)}
         result["speedup_vs_fp64_per_patch"] = rpp["wall"] / r["wall"]
         result["speedup_vs_fp64_batched"] = r64["wall"] / r["wall"]
-        result["l1_rel_vs_fp64"] = num / den
-        result["mass_rel_diff_vs_fp64"] = abs(m32 - m64) / abs(m64)
-        # the same measure on the initial data alone: rounding the FP64 state
-        # to FP32 storage already moves the level's mass by this much
+        result["l1_rel_vs_fp64"] = l1(q_main)
+        result["mass_rel_diff_vs_fp64"] = abs(mass(q_main) - m64) / abs(m64)
+        # FP32 storage: rounding the FP64 initial state alone moves the
+        # level's mass by this much
         r64_0 = r32_0 = 0.0
         for k in range(npatch):
             rho = H.grid(2, k, 0)[2:-2, 2:-2, 0]
             r64_0 += float(rho.sum())
             r32_0 += float(rho.astype(np.float32).astype(np.float64).sum())
-        result["mass_rel_diff_initial_rounding"] = abs(r32_0 - r64_0) / abs(r64_0)
+        result["fp32_storage_initial_mass_rel_diff"] = abs(r32_0 - r64_0) / abs(r64_0)
         result["steps_compared"] = a.steps + a.warmup
+        print("bench: periodic conservation", file=sys.stderr, flush=True)
+        result["conservation_drift"] = {
+            "setup": "periodic two-level hierarchy, 256 level-2 patches, 4 coarse steps "
+                     "(each: level 1, two level-2 steps, update with refluxing); "
+                     "|conck(1) - conck0(1)| / conck0(1)",
+            "mixed": conservation_drift(capi, capi.MIXED),
+            "fp32": conservation_drift(capi, capi.FP32),
+            "fp64": conservation_drift(capi, capi.FP64)}
     if a.cpu_seconds > 0:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline(H, dt2, a.cpu_seconds))

@@ -34,7 +34,13 @@ enum {
 };
 
 enum { AMRGPU_ADVECTION = 1, AMRGPU_ACOUSTICS = 2, AMRGPU_EULER = 3 };
-enum { AMRGPU_FP64 = 0, AMRGPU_FP32 = 1 };
+/* Precision of a run.  FP64: the reference's arithmetic, bit for bit.
+ * FP32: storage and wave/flux arithmetic in FP32; registers, coarse
+ * interpolation, averaging and totals FP64.  MIXED: wave/flux arithmetic in
+ * FP32 on FP64 storage, in flux form (both cells of an interface take
+ * f(q_l) + A-dq) with the update in FP64, so that totals are conserved to FP64
+ * rounding; everything else as FP64. */
+enum { AMRGPU_FP64 = 0, AMRGPU_FP32 = 1, AMRGPU_MIXED = 2 };
 
 /* Run parameters: the amr_module.f90 fields the level update reads, filled
  * by the shim from claw.data/amr.data (amr2.f90:152-392), plus the physics

@@ -123,11 +123,12 @@ def test_level_update_launch_variants(gpu, mode, graphs):
         capi.finalize()
 
 
+@pytest.mark.parametrize("precision", [capi.FP32, capi.MIXED])
 @pytest.mark.parametrize("phys", ["acoustics", "euler"])
 @pytest.mark.parametrize("case", BCS[:3])
-def test_level_update_fp32(gpu, phys, case):
-    print("amr fp32", phys, case, flush=True)
-    H, O, lo, ld = _run(phys, case, 1, capi.FP32)
+def test_level_update_fp32(gpu, phys, case, precision):
+    print("amr fp32", phys, case, precision, flush=True)
+    H, O, lo, ld = _run(phys, case, 1, precision)
     try:
         for l in range(1, H.nlev + 1):
             got = capi.read_level(H, l, 0)
@@ -140,14 +141,17 @@ def test_level_update_fp32(gpu, phys, case):
         capi.finalize()
 
 
-def test_periodic_mass_conserved_on_device(gpu):
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_periodic_mass_conserved_on_device(gpu, precision):
     """Refluxing keeps the level-1 mass constant to rounding (periodic
-    domain, coarse data first averaged from the finer levels)."""
-    print("amr conservation", flush=True)
+    domain, coarse data first averaged from the finer levels).  The mixed
+    path (FP32 waves, FP64 storage, flux form, no qad) conserves to FP64
+    rounding as well."""
+    print("amr conservation", precision, flush=True)
     solver, nvar, states, dt1 = PHYS["euler"]
     rng = np.random.default_rng(42)
     H = nested_hierarchy(rng, states, nvar, mthbc=(2, 2, 2, 2), xper=True, yper=True, dt1=dt1)
-    p, cfg = both_params(solver)
+    p, cfg = both_params(solver, precision=precision)
     for k in range(4):
         cfg.mthbc[k] = 2
     capi.init(cfg)
