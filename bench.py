@@ -287,9 +287,12 @@ def bench3(a):
     prec = capi3.FP32 if p3 == "fp32" else capi3.FP64
     esize = 4 if prec == capi3.FP32 else 8
 
+    from amrclaw_amd import autotune
+    plane_block = a.plane_block or autotune.load_plane_block(p3) or 0
+
     def install(precision):
         capi3.init3(capi3.make_config3(capi3.EULER, precision=precision))
-        capi3.set_patches3(sizes, a.plane_block)
+        capi3.set_patches3(sizes, plane_block)
         for k, q in enumerate(qs):
             capi3.put_patch3(k, q)
 
@@ -330,7 +333,7 @@ def bench3(a):
                                "seed 20261203, radial bump, Roe+efix, MC, method(3)=22",
                    "patches": len(qs), "cells_per_step": cells, "nghost": 2,
                    "precision": p3, "launch": "batched",
-                   "plane_block": a.plane_block or 4},
+                   "plane_block": plane_block or 4},
         "kernels_ms_per_step": kms,
     }
     step_kern = sum(kms.values())
