@@ -238,7 +238,7 @@ def advance_level(level, want_cfl=True):
     return dt.value, cfl.value
 
 
-LAUNCH_BATCHED, LAUNCH_PER_PATCH = 0, 1
+LAUNCH_BATCHED, LAUNCH_PER_PATCH, LAUNCH_PERSISTENT = 0, 1, 2
 KT_SWEEP, KT_FILL, KT_REGISTERS, KT_UPDATE, KT_FLAG = range(5)
 
 

@@ -135,7 +135,8 @@ struct Context {
     double hxposs[MAXLV] = {}, hyposs[MAXLV] = {};
     std::vector<unsigned long long> h_cfl;
     bool regs_valid = false;
-    int launch_mode = AMRGPU_LAUNCH_BATCHED;   // or one sweep launch per patch
+    int launch_mode = AMRGPU_LAUNCH_BATCHED;   // or per patch, or persistent
+    unsigned long long* d_next = nullptr;       // tile counter of the persistent sweep
     bool timing = false;                        // per-kernel-class HIP events
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[AMRGPU_KT_COUNT];
     size_t tev_used[AMRGPU_KT_COUNT] = {};
@@ -762,6 +763,12 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
             x.ntiles = L.patch_tile_start[p + 1] - L.patch_tile_start[p];
             if (x.ntiles > 0) e = launch(x);
         }
+    } else if (g.launch_mode == AMRGPU_LAUNCH_PERSISTENT) {
+        // a kernel, not a memset, clears the counter (safe under graph capture)
+        e = launch_fill_u64(g.d_next, 0ull, 1, g.stream);
+        SweepArgs x = a;
+        x.next = g.d_next;
+        if (e == hipSuccess) e = launch(x);
     } else {
         e = launch(a);
     }
@@ -1243,6 +1250,7 @@ int amrgpu_init(const amrgpu_config* cfg) {
     M.mwaves = cfg->mwaves;
     for (int mw = 0; mw < NWMAX; mw++) M.mthlim[mw] = mw < cfg->mwaves ? cfg->mthlim[mw] : 0;
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&g.d_next, sizeof(unsigned long long)));
     for (auto& e : g.ev) HIPCHK(hipEventCreate(&e));
     {
         // set the kernels' LDS limits now, outside any stream capture
@@ -1268,6 +1276,7 @@ int amrgpu_finalize(void) {
         }
     for (auto& e : g.ev)
         if (e) hipEventDestroy(e);
+    hipFree(g.d_next);
     hipStreamDestroy(g.stream);
     g = Context();
     return AMRGPU_OK;
@@ -1492,7 +1501,8 @@ int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richa
 
 int amrgpu_set_launch_mode(int mode) {
     if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
-    if (mode != AMRGPU_LAUNCH_BATCHED && mode != AMRGPU_LAUNCH_PER_PATCH)
+    if (mode != AMRGPU_LAUNCH_BATCHED && mode != AMRGPU_LAUNCH_PER_PATCH &&
+        mode != AMRGPU_LAUNCH_PERSISTENT)
         return fail(AMRGPU_ERR_ARG, "unknown launch mode %d", mode);
     if (mode == g.launch_mode) return AMRGPU_OK;
     g.launch_mode = mode;

@@ -48,6 +48,9 @@ struct SweepArgs {
     double* reg_fine;
     const PatchDesc* fine_patches;
     const SvEntry* sv;
+    // persistent launch when non-null: the tile counter, zeroed before the
+    // launch
+    unsigned long long* next;
 };
 
 // Perimeter flux block of a fine grid (FP64, after svdflx and qc1d):
@@ -212,16 +215,17 @@ __host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty) {
     return (qs + f + g) * sizeof(T);
 }
 
+// one tile of the sweep, run by one workgroup
 template <class Sol, typename T, typename S>
-__global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
+__device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
+                                           unsigned char* smem_raw) {
     constexpr int NV = Sol::NV;
     // compute precision below the storage precision: the mixed path (flux
     // form, update in the storage precision)
     constexpr bool MIXED = sizeof(T) < sizeof(S);
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     T* smem = reinterpret_cast<T*>(smem_raw);
 
-    const TileDesc td = a.tiles[blockIdx.x];
+    const TileDesc td = a.tiles[tile];
     const PatchDesc pd = a.patches[td.patch];
     const int tx = td.tx, ty = td.ty;
     const int qsx = sweep_qs_stride(tx), qsy = ty + 4;
@@ -509,6 +513,30 @@ __global__ __launch_bounds__(256) void sweep_kernel(SweepArgs a) {
     }
     cfl = wave_max(cfl);
     if ((tid & 63) == 0) atomic_max_nonneg(&a.cfl_patch[td.patch], double(cfl));
+}
+
+// batched launch: one workgroup per tile of the tile table
+template <class Sol, typename T, typename S>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1) void sweep_kernel(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    sweep_tile<Sol, T, S>(a, blockIdx.x, smem_raw);
+}
+
+// persistent launch: a grid sized to fill the device takes tiles from the
+// table in order (largest first) through the counter a.next, which starts
+// at zero; every workgroup leaves once the counter has passed the table
+template <class Sol, typename T, typename S>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1) void sweep_persistent_kernel(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int tile;
+    for (;;) {
+        if (threadIdx.x == 0) tile = (int)atomicAdd(a.next, 1ull);
+        __syncthreads();
+        const int t = tile;
+        if (t >= a.ntiles) break;   // the same value in every lane
+        sweep_tile<Sol, T, S>(a, t, smem_raw);
+        __syncthreads();            // LDS and `tile` are reused by the next tile
+    }
 }
 
 }  // namespace amrgpu

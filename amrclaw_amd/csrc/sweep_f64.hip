@@ -1,39 +1,12 @@
 // FP64 instantiations of the level sweep.  Built with -ffp-contract=off:
 // no multiply-add is fused, so results match the reference bit for bit.
-#include "sweep.hpp"
-#include "kernels.hpp"
+#include <algorithm>
+#include "sweep_launch.hpp"
 
 namespace amrgpu {
 
 hipError_t launch_sweep_f64(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
-    dim3 grid(a.ntiles), block(256);
-    static bool attr_set = false;
-    if (!attr_set) {
-        // allow tiles whose LDS footprint exceeds the 64 KiB default
-        const int max_lds = 160 * 1024;
-        hipFuncSetAttribute((const void*)sweep_kernel<Advection<double>, double, double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        hipFuncSetAttribute((const void*)sweep_kernel<Acoustics<double>, double, double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        hipFuncSetAttribute((const void*)sweep_kernel<EulerRoe<double>, double, double>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        attr_set = true;
-    }
-    if (a.ntiles == 0) return hipSuccess;   // (a call with no tiles only sets the attributes)
-    switch (solver) {
-    case SOLVER_ADVECTION:
-        hipLaunchKernelGGL((sweep_kernel<Advection<double>, double, double>), grid, block, lds, st, a);
-        break;
-    case SOLVER_ACOUSTICS:
-        hipLaunchKernelGGL((sweep_kernel<Acoustics<double>, double, double>), grid, block, lds, st, a);
-        break;
-    case SOLVER_EULER:
-        hipLaunchKernelGGL((sweep_kernel<EulerRoe<double>, double, double>), grid, block, lds, st, a);
-        break;
-    default:
-        return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
+    return sweep_dispatch<double, ExactMath<double>, double>(solver, a, lds, st);
 }
 
 }  // namespace amrgpu

@@ -1,0 +1,62 @@
+// sweep_launch.hpp -- launchers of the 2-D sweep for one compute precision T,
+// arithmetic policy Mth and storage precision S (included by sweep_f64.hip
+// and sweep_f32.hip, which differ in their compile flags).
+#pragma once
+#include <algorithm>
+#include "kernels.hpp"
+#include "sweep.hpp"
+
+namespace amrgpu {
+
+template <class Sol, typename T, typename S>
+static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        // allow tiles whose LDS footprint exceeds the 64 KiB default: the
+        // CU's 160 KiB less the kernel's static LDS
+        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S>,
+                              (const void*)sweep_persistent_kernel<Sol, T, S>};
+        for (const void* f : fns) {
+            hipFuncAttributes fa{};
+            hipError_t e = hipFuncGetAttributes(&fa, f);
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        160 * 1024 - (int)fa.sharedSizeBytes);
+            if (e != hipSuccess) return e;
+        }
+        attr_set = true;
+    }
+    if (a.ntiles == 0) return hipSuccess;   // (a call with no tiles only sets the attributes)
+    if (!a.next) {
+        hipLaunchKernelGGL((sweep_kernel<Sol, T, S>), dim3(a.ntiles), dim3(256), lds, st, a);
+        return hipGetLastError();
+    }
+    // persistent: as many workgroups as fit on the device at this LDS size
+    static size_t grid_lds = 0;
+    static int grid = 0;
+    if (grid == 0 || grid_lds != lds) {
+        int per_cu = 0, dev = 0, ncu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S>, 256, lds);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        grid = std::max(1, per_cu) * std::max(1, ncu);
+        grid_lds = lds;
+    }
+    const int g = std::min(grid, a.ntiles);
+    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S>), dim3(g), dim3(256), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, class Mth, typename S>
+static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
+    switch (solver) {
+    case SOLVER_ADVECTION: return sweep_go<Advection<T, Mth>, T, S>(a, lds, st);
+    case SOLVER_ACOUSTICS: return sweep_go<Acoustics<T, Mth>, T, S>(a, lds, st);
+    case SOLVER_EULER: return sweep_go<EulerRoe<T, Mth>, T, S>(a, lds, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace amrgpu

@@ -68,6 +68,8 @@ def parse():
                          "conservative to FP64 rounding; fp32: FP32 storage too")
     ap.add_argument("--per-patch", action="store_true",
                     help="time the one-launch-per-patch sweep instead")
+    ap.add_argument("--persistent", action="store_true",
+                    help="persistent sweep launch (workgroups take tiles from a counter)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay the level steps as hipGraphs in the timed region")
     ap.add_argument("--no-compare", action="store_true",
@@ -98,7 +100,7 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "dispatches": 5}))
 
 
-def install(capi, H, precision, per_patch, graphs, txmax, tymax):
+def install(capi, H, precision, launch, graphs, txmax, tymax):
     """Load the hierarchy and bring it to the timed state: level 1 advanced
     once from t = 0 (saveqc fills the level-2 registers' coarse ring),
     level 2 half way through that coarse step."""
@@ -112,8 +114,9 @@ def install(capi, H, precision, per_patch, graphs, txmax, tymax):
             txmax, tymax = tuned
     if txmax or tymax:
         capi.set_tiling(txmax, tymax)
-    if per_patch:
-        capi.set_launch_mode(capi.LAUNCH_PER_PATCH)
+    if launch != "batched":
+        capi.set_launch_mode({"per-patch": capi.LAUNCH_PER_PATCH,
+                              "persistent": capi.LAUNCH_PERSISTENT}[launch])
     capi.load_hierarchy(H)
     dt2 = H.levels[1]["possk"]
     capi.set_time(1, 0.0, 2.0 * dt2)
@@ -405,7 +408,8 @@ def main():
     bytes_per_cell = 2 * 4 * esize            # (2 nvar + maux) * s
 
     print("bench: timed run", a.precision, file=sys.stderr, flush=True)
-    dt2 = install(capi, H, prec, a.per_patch, a.graphs, a.txmax, a.tymax)
+    launch = "per-patch" if a.per_patch else ("persistent" if a.persistent else "batched")
+    dt2 = install(capi, H, prec, launch, a.graphs, a.txmax, a.tymax)
     r = run(capi, dt2, a.steps, a.warmup, sync)
     q_main = interiors(capi, H) if not a.no_compare else None
     capi.finalize()
@@ -423,14 +427,14 @@ def main():
                                "transverse 2, extrapolation BCs" % npatch,
                    "patches": npatch, "cells_per_step": cells, "nghost": 2,
                    "precision": a.precision, "precision_detail": DESC[a.precision],
-                   "launch": "per-patch" if a.per_patch else "batched",
+                   "launch": launch,
                    "graphs": bool(a.graphs)},
         "step_ms_events": r["events_ms"] / a.steps,
         "kernels_ms_per_step": {k: (r[k][0] / a.steps if r[k][1] else None)
                                 for k in ("sweep", "fill", "registers")},
     }
     if sweep_n:
-        tuned_default = not (a.txmax or a.tymax or a.per_patch)
+        tuned_default = not (a.txmax or a.tymax or a.per_patch or a.persistent)
         tr = MEASURED_TRAFFIC.get(("euler2d", a.precision, npatch)) if tuned_default else None
         kern_ms = sweep_ms / sweep_n        # one timed record per step's sweep
         achieved = cells * bytes_per_cell / (kern_ms / 1e3) / 1e9
@@ -449,10 +453,10 @@ def main():
             return sum(float(x[..., 0].sum()) for x in qs) * h2 * h2
 
         runs = {}
-        for name, p_, per_patch in (("fp64_batched", capi.FP64, False),
-                                    ("fp64_per_patch", capi.FP64, True),
+        for name, p_, per_patch in (("fp64_batched", capi.FP64, "batched"),
+                                    ("fp64_per_patch", capi.FP64, "per-patch"),
                                     ("fp32" if prec == capi.MIXED else "mixed",
-                                     capi.FP32 if prec == capi.MIXED else capi.MIXED, False)):
+                                     capi.FP32 if prec == capi.MIXED else capi.MIXED, launch)):
             print("bench:", name, file=sys.stderr, flush=True)
             install(capi, H, p_, per_patch, False, 0, 0)
             rr = run(capi, dt2, a.steps, a.warmup, sync)

@@ -162,10 +162,13 @@ int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richa
                       const int64_t* offsets, int64_t* nflagged);
 
 /* Sweep launch shape: AMRGPU_LAUNCH_BATCHED (default) puts all tiles of
- * all grids of a level into one launch; AMRGPU_LAUNCH_PER_PATCH issues one
- * launch per grid, the plain per-patch GPU path, kept as the comparison
- * baseline.  Results are identical.  No reference counterpart. */
-enum { AMRGPU_LAUNCH_BATCHED = 0, AMRGPU_LAUNCH_PER_PATCH = 1 };
+ * all grids of a level into one launch, one workgroup per tile;
+ * AMRGPU_LAUNCH_PERSISTENT launches as many workgroups as the device holds
+ * and lets them take the level's tiles (largest first) from a device
+ * counter; AMRGPU_LAUNCH_PER_PATCH issues one launch per grid, the plain
+ * per-patch GPU path, kept as the comparison baseline.  Results are
+ * identical.  No reference counterpart. */
+enum { AMRGPU_LAUNCH_BATCHED = 0, AMRGPU_LAUNCH_PER_PATCH = 1, AMRGPU_LAUNCH_PERSISTENT = 2 };
 int amrgpu_set_launch_mode(int mode);
 
 /* With on != 0, amrgpu_advance_level captures its launches (ghost fill,

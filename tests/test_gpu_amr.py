@@ -107,7 +107,8 @@ def test_level_update_fp64_multi_tile(gpu, tiling):
 
 
 @pytest.mark.parametrize("graphs", [False, True])
-@pytest.mark.parametrize("mode", [capi.LAUNCH_BATCHED, capi.LAUNCH_PER_PATCH])
+@pytest.mark.parametrize("mode", [capi.LAUNCH_BATCHED, capi.LAUNCH_PER_PATCH,
+                                  capi.LAUNCH_PERSISTENT])
 def test_level_update_launch_variants(gpu, mode, graphs):
     """One launch per patch and hipGraph replay of the level steps give the
     oracle's results bit for bit (graphs: every replay after the first of a
