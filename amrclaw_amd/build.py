@@ -33,6 +33,7 @@ UNITS = [
     ("capi.cpp", ["-O2", "-x", "hip"]),
     ("sweep3_f64.hip", ["-O3", "-ffp-contract=off"]),
     ("sweep3_f32.hip", ["-O3"]),
+    ("flag3.hip", ["-O3", "-ffp-contract=off"]),
     ("capi3.cpp", ["-O2", "-x", "hip"]),
 ]
 

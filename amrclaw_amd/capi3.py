@@ -52,6 +52,9 @@ def lib():
         L.amrgpu3_set_timing.argtypes = [C.c_int]
         L.amrgpu3_kernel_time.argtypes = [C.c_int, dpo, C.POINTER(C.c_int64)]
         L.amrgpu3_synchronize.argtypes = []
+        L.amrgpu3_flag_patches.argtypes = [C.c_double, C.POINTER(C.c_int64)]
+        L.amrgpu3_get_flags.argtypes = [C.c_int, _dp]
+        L.amrgpu3_conck.argtypes = [C.c_double] * 3 + [dpo]
         _bound = True
     return L
 
@@ -155,7 +158,27 @@ def synchronize3():
     _check(lib().amrgpu3_synchronize())
 
 
+def flag_patches3(tolsp):
+    """flag2refine on every grid of the batch; returns the number flagged."""
+    n = C.c_int64()
+    _check(lib().amrgpu3_flag_patches(tolsp, C.byref(n)))
+    return n.value
+
+
+def get_flags3(p, shape):
+    """Flags of grid p, shape (mz+4, my+4, mx+4)."""
+    out = np.empty(shape, dtype=np.float64)
+    _check(lib().amrgpu3_get_flags(p, out))
+    return out
+
+
+def conck3(h):
+    m = C.c_double()
+    _check(lib().amrgpu3_conck(h[0], h[1], h[2], C.byref(m)))
+    return m.value
+
+
 __all__ = ["ADVECTION", "EULER", "FP32", "FP64", "KT_X", "KT_Y", "KT_Z", "KT_UPDATE",
            "Amrgpu3Config", "make_config3", "init3", "finalize3", "set_patches3", "put_patch3",
            "get_patch3", "step_patches3", "get_fluxes3", "stepgrid3", "set_timing3",
-           "kernel_time3", "synchronize3"]
+           "kernel_time3", "synchronize3", "flag_patches3", "get_flags3", "conck3"]

@@ -42,6 +42,12 @@ struct Context3 {
     int64_t q_elems = 0, face_elems = 0;
     unsigned long long* d_cfl = nullptr;
     std::vector<unsigned long long> h_cfl;
+    // flagging (flag2refine) and conck state
+    double* d_flags = nullptr;          // FP64 flags of every grid, ghost border = goodpt
+    int64_t* d_foff = nullptr;
+    std::vector<int64_t> foff;
+    unsigned long long* d_nflag = nullptr;
+    double* d_partial = nullptr;        // conck partial sums, one per update block
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[NKIND];
     size_t tev_used[NKIND] = {};
@@ -83,6 +89,15 @@ void free_batch() {
     hipFree(g3.d_q);
     hipFree(g3.d_face);
     hipFree(g3.d_cfl);
+    hipFree(g3.d_flags);
+    hipFree(g3.d_foff);
+    hipFree(g3.d_nflag);
+    hipFree(g3.d_partial);
+    g3.d_flags = nullptr;
+    g3.d_foff = nullptr;
+    g3.d_nflag = nullptr;
+    g3.d_partial = nullptr;
+    g3.foff.clear();
     for (int k = 0; k < NKIND; k++) {
         hipFree(g3.d_blocks[k]);
         g3.d_blocks[k] = nullptr;
@@ -344,9 +359,68 @@ int get_fluxes(int p, double* const out[6]) {
     return AMRGPU_OK;
 }
 
+// flag and conck buffers, allocated on first use for the current batch
+int ensure_flag_buffers() {
+    if (g3.d_flags) return AMRGPU_OK;
+    g3.foff.assign(g3.npatch + 1, 0);
+    for (int p = 0; p < g3.npatch; p++) {
+        const Patch3& pt = g3.patches[p];
+        g3.foff[p + 1] = g3.foff[p] + (int64_t)(pt.mx + 4) * (pt.my + 4) * (pt.mz + 4);
+    }
+    const int64_t n = std::max<int64_t>(1, g3.foff[g3.npatch]);
+    HIPCHK3(hipMalloc(&g3.d_flags, n * sizeof(double)));
+    HIPCHK3(hipMemsetAsync(g3.d_flags, 0, n * sizeof(double), g3.stream));   // goodpt = 0
+    HIPCHK3(hipMalloc(&g3.d_foff, (g3.npatch + 1) * sizeof(int64_t)));
+    HIPCHK3(h2d3(g3.d_foff, g3.foff.data(), (g3.npatch + 1) * sizeof(int64_t)));
+    HIPCHK3(hipMalloc(&g3.d_nflag, sizeof(unsigned long long)));
+    HIPCHK3(hipMalloc(&g3.d_partial, std::max<size_t>(1, g3.blocks[3].size()) * sizeof(double)));
+    return AMRGPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int amrgpu3_flag_patches(double tolsp, int64_t* nflagged) {
+    int rc = need_init();
+    if (rc) return rc;
+    rc = ensure_flag_buffers();
+    if (rc) return rc;
+    HIPCHK3(hipMemsetAsync(g3.d_nflag, 0, sizeof(unsigned long long), g3.stream));
+    HIPCHK3(launch_flag3((int)g3.esize, g3.cfg.nvar, g3.d_q, g3.d_flags, g3.d_foff, g3.d_patches,
+                         g3.d_blocks[3], (int)g3.blocks[3].size(), tolsp, g3.d_nflag, g3.stream));
+    unsigned long long n = 0;
+    HIPCHK3(d2h3(&n, g3.d_nflag, sizeof n));
+    if (nflagged) *nflagged = (int64_t)n;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_get_flags(int patch, double* amrflags) {
+    int rc = check_patch(patch);
+    if (rc) return rc;
+    if (!amrflags) return fail3(AMRGPU_ERR_ARG, "null flags");
+    if (!g3.d_flags) return fail3(AMRGPU_ERR_STATE, "amrgpu3_flag_patches has not been called");
+    HIPCHK3(d2h3(amrflags, g3.d_flags + g3.foff[patch],
+                 (g3.foff[patch + 1] - g3.foff[patch]) * sizeof(double)));
+    return AMRGPU_OK;
+}
+
+int amrgpu3_conck(double hx, double hy, double hz, double* totmass) {
+    int rc = need_init();
+    if (rc) return rc;
+    if (!totmass) return fail3(AMRGPU_ERR_ARG, "null totmass");
+    rc = ensure_flag_buffers();
+    if (rc) return rc;
+    const int nb = (int)g3.blocks[3].size();
+    HIPCHK3(launch_conck3((int)g3.esize, g3.d_q, g3.d_patches, g3.d_blocks[3], nb, g3.d_partial,
+                          g3.stream));
+    std::vector<double> part(nb);
+    if (nb) HIPCHK3(d2h3(part.data(), g3.d_partial, nb * sizeof(double)));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *totmass = s * hx * hy * hz;
+    return AMRGPU_OK;
+}
 
 int amrgpu3_last_error(char* buf, int len) {
     if (!buf || len <= 0) return AMRGPU_ERR_ARG;

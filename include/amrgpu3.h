@@ -84,6 +84,21 @@ int amrgpu3_stepgrid(int mx, int my, int mz, const double* q_in, double* q_out, 
                      double dy, double dz, double dt, double* fm, double* fp, double* gm,
                      double* gp, double* hm, double* hp, double* cflgrid, double* dtnew);
 
+/* Flag every grid of the batch for refinement: flag2refine.f (3-D):3 as
+ * spest.f:71-84 calls it with the library's allowflag (always true), on the
+ * grids' current solution (ghost cells filled).  Flags are FP64 per grid,
+ * (mx+4, my+4, mz+4): interior cells 2 (DOFLAG, badpt) where the largest
+ * centred difference of any component in any direction exceeds tolsp, else
+ * 0 (DONTFLAG); the border stays 0 (goodpt).  nflagged (may be NULL)
+ * receives the number of flagged cells. */
+int amrgpu3_flag_patches(double tolsp, int64_t* nflagged);
+int amrgpu3_get_flags(int patch, double* amrflags);
+
+/* conck.f (3-D):6: hx hy hz times the sum of q(1) over the interiors of the
+ * batch's grids (FP64; summed in blocks, so it agrees with the reference's
+ * sequential sum to rounding). */
+int amrgpu3_conck(double hx, double hy, double hz, double* totmass);
+
 /* Device timing of the step's kernels: amrgpu3_set_timing(1) resets and
  * starts it; kind 0, 1, 2 = x-, y-, z-sweep, 3 = update. */
 int amrgpu3_set_timing(int on);

@@ -189,6 +189,10 @@ void orc_step3(const orc_params* p, int mx, int my, int mz, const double* qold, 
 void orc_stepgrid3(const orc_params* p, int mx, int my, int mz, double* q, double dx, double dy,
                    double dz, double dt, double* fm, double* fp, double* gm, double* gp, double* hm,
                    double* hp, double* cflgrid, double* dtnew);
+/* 3-D flagging and mass (orc_sweep3.c): flag2refine.f, conck.f */
+int orc_flag2refine3(int mx, int my, int mz, int mbc, int meqn, double tolsp, const double* q,
+                     double dontflag, double doflag, double* amrflags);
+double orc_mass3(int mx, int my, int mz, int mbc, int meqn, const double* q, double acc);
 
 /* cell flagging (orc_flag.c) */
 #define ORC_UNSET (-1.0)

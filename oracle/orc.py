@@ -639,3 +639,44 @@ def ref3_stepgrid3(p, q, dx, dy, dz, dt):
                                [C.POINTER(C.c_double), C.POINTER(C.c_double)]
     R.ref3_stepgrid(p.meqn, p.mbc, mx, my, mz, qn, dx, dy, dz, dt, *f, C.byref(cfl), C.byref(dtn))
     return qn, f, cfl.value, dtn.value
+
+
+def flag2refine3(q, tolsp, mbc=2):
+    """orc_flag2refine3 on one 3-D grid q (mz+4, my+4, mx+4, meqn): the flag
+    array (mz+4, my+4, mx+4) (0 = goodpt / DONTFLAG, 2 = DOFLAG) and the
+    number of DOFLAG cells."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    mz, my, mx = (n - 2 * mbc for n in q.shape[:3])
+    f = np.zeros(q.shape[:3])
+    L = lib()
+    L.orc_flag2refine3.restype = C.c_int
+    L.orc_flag2refine3.argtypes = [C.c_int] * 5 + [C.c_double, _dp, C.c_double, C.c_double, _dp]
+    n = L.orc_flag2refine3(mx, my, mz, mbc, q.shape[3], tolsp, q, 0.0, 2.0, f)
+    return f, n
+
+
+def mass3(qs, h, mbc=2):
+    """conck.f (3-D) over a list of grids: running sum of q(1) times hx hy hz."""
+    L = lib()
+    L.orc_mass3.restype = C.c_double
+    L.orc_mass3.argtypes = [C.c_int] * 5 + [_dp, C.c_double]
+    acc = 0.0
+    for q in qs:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        mz, my, mx = (n - 2 * mbc for n in q.shape[:3])
+        acc = L.orc_mass3(mx, my, mz, mbc, q.shape[3], q, acc)
+    return acc * h[0] * h[1] * h[2]
+
+
+def ref3_flag2refine3(q, tolsp, mbc=2):
+    """The reference's 3-D flag2refine.f on the same grid."""
+    global _ref3
+    if _ref3 is None:
+        lib()
+        _ref3 = C.CDLL(os.path.join(HERE, "_ref", "libamrref3.so"))
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    mz, my, mx = (n - 2 * mbc for n in q.shape[:3])
+    f = np.zeros(q.shape[:3])
+    _ref3.ref3_flag2refine.argtypes = [C.c_int] * 5 + [_dp, C.c_double, _dp]
+    _ref3.ref3_flag2refine(q.shape[3], mbc, mx, my, mz, q, tolsp, f)
+    return f

@@ -558,3 +558,46 @@ void orc_stepgrid3(const orc_params* p, int mx, int my, int mz, double* q, doubl
                         - dtdz * (A3(hm, m, i, j, k + 1) - A3(hp, m, i, j, k));
     *dtnew = *cflgrid > 0.0 ? dt * p->cfl / *cflgrid : ORC_RINFINITY;
 }
+
+/* ------------------------------------------------------- flag2refine ----- */
+/* src/3d/flag2refine.f:3-92 with the library's allowflag (allowflag.f: always
+ * true) as spest.f:71-84 calls it: amrflags(1-mbc:mx+mbc, 1-mbc:my+mbc,
+ * 1-mbc:mz+mbc) starts at goodpt everywhere; every interior cell gets
+ * DONTFLAG, or DOFLAG when the largest centred difference of any component in
+ * any direction exceeds tolsp.  Returns the number of DOFLAG cells. */
+int orc_flag2refine3(int mx, int my, int mz, int mbc, int meqn, double tolsp, const double* q,
+                     double dontflag, double doflag, double* amrflags) {
+    const int mit = mx + 2 * mbc, mjt = my + 2 * mbc;
+#define F3(i, j, k) amrflags[((i) + mbc - 1) + (size_t)mit * (((j) + mbc - 1) + (size_t)mjt * ((k) + mbc - 1))]
+    int nflag = 0;
+    for (int k = 1; k <= mz; k++)
+        for (int j = 1; j <= my; j++)
+            for (int i = 1; i <= mx; i++) {
+                F3(i, j, k) = dontflag;
+                double dq = 0.0;
+                for (int m = 1; m <= meqn; m++) {
+                    double dqi = fabs(A3(q, m, i + 1, j, k) - A3(q, m, i - 1, j, k));
+                    double dqj = fabs(A3(q, m, i, j + 1, k) - A3(q, m, i, j - 1, k));
+                    double dqk = fabs(A3(q, m, i, j, k + 1) - A3(q, m, i, j, k - 1));
+                    dq = dmax1(dmax1(dmax1(dq, dqi), dqj), dqk);
+                }
+                if (dq > tolsp) {
+                    F3(i, j, k) = doflag;
+                    nflag++;
+                }
+            }
+#undef F3
+    return nflag;
+}
+
+/* conck.f (3-D):47-60: the running sum of q(1) over the interior of one
+ * grid, in the reference's loop order (k, j, i), continued from acc (the
+ * sum of the grids before it); conck.f:77 then scales by hx hy hz */
+double orc_mass3(int mx, int my, int mz, int mbc, int meqn, const double* q, double acc) {
+    const int mit = mx + 2 * mbc, mjt = my + 2 * mbc;
+    double s = acc;
+    for (int k = 1; k <= mz; k++)
+        for (int j = 1; j <= my; j++)
+            for (int i = 1; i <= mx; i++) s = s + A3(q, 1, i, j, k);
+    return s;
+}

@@ -95,4 +95,19 @@ contains
     cflgrid = cfl_level
   end subroutine ref3_stepgrid
 
+  ! flag2refine.f (3-D) as spest.f:71-84 calls it: amrflags set to goodpt,
+  ! then the interior flagged goodpt / badpt
+  subroutine ref3_flag2refine(nvar, mbc, mx, my, mz, q, tolsp_in, amrflags) &
+       bind(C, name="ref3_flag2refine")
+    integer(c_int), value :: nvar, mbc, mx, my, mz
+    real(c_double), value :: tolsp_in
+    real(c_double) :: q(*), amrflags(*)
+    real(kind=8) :: aux1(1)
+    integer :: n
+    n = (mx + 2*mbc) * (my + 2*mbc) * (mz + 2*mbc)
+    amrflags(1:n) = goodpt
+    call flag2refine(mx, my, mz, mbc, nvar, 0, 0.d0, 0.d0, 0.d0, 1.d0, 1.d0, 1.d0, 0.d0, &
+         1, tolsp_in, q, aux1, amrflags, goodpt, badpt)
+  end subroutine ref3_flag2refine
+
 end module ref3_harness

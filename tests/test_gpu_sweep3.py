@@ -139,3 +139,35 @@ def test_uniform_state_invariant_fp32(gpu):
     assert np.allclose(qg, q, rtol=2e-6, atol=0)
     assert cg > 0
     capi3.finalize3()
+
+
+@pytest.mark.parametrize("precision", [capi3.FP64, capi3.FP32])
+def test_flag_patches3_matches_oracle(gpu, precision):
+    """flag2refine (3-D) on every grid of a batch: FP64 flags equal the
+    oracle's (pinned to the reference's flag2refine.f); with FP32 storage the
+    flags are those of the FP32-rounded state.  Flagged count and conck
+    (within rounding of the reference's sequential sum) too."""
+    c = capi3.make_config3(orc.EULER, precision=precision)
+    capi3.init3(c)
+    rng = np.random.default_rng(55)
+    sizes = [(int(a), int(b), int(d)) for a, b, d in rng.integers(1, 17, size=(7, 3))]
+    sizes[0] = (32, 20, 9)
+    qs = [patch(rng, orc.EULER, *s) for s in sizes]
+    if precision == capi3.FP32:
+        qs = [q.astype(np.float32).astype(np.float64) for q in qs]
+    capi3.set_patches3(sizes)
+    for k, q in enumerate(qs):
+        capi3.put_patch3(k, q)
+    tol = 1.3
+    print("flag3", precision, flush=True)
+    n = capi3.flag_patches3(tol)
+    total = 0
+    for k, q in enumerate(qs):
+        f, nk = orc.flag2refine3(q, tol)
+        total += nk
+        assert np.array_equal(capi3.get_flags3(k, f.shape), f), k
+    assert n == total and 0 < total
+    h = (1.0 / 32, 1.0 / 20, 1.0 / 9)
+    m = capi3.conck3(h)
+    assert m == pytest.approx(orc.mass3(qs, h), rel=1e-13)
+    capi3.finalize3()

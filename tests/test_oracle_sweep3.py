@@ -163,3 +163,40 @@ def test_euler3_roe_properties(ixyz):
             # along md maps the jump to that direction's flux jump
             df = _euler_flux3(q[i], md - 2) - _euler_flux3(q[i - 1], md - 2)
             assert np.allclose(bm[5 * i:5 * i + 5] + bp[5 * i:5 * i + 5], df, rtol=1e-11, atol=1e-11)
+
+
+@needs_ref3
+@pytest.mark.parametrize("seed", range(4))
+def test_flag2refine3_matches_reference(seed):
+    """orc_flag2refine3 against the reference's src/3d flag2refine.f."""
+    rng = np.random.default_rng(40 + seed)
+    mx, my, mz = (int(v) for v in rng.integers(2, 10, 3))
+    q = patch(rng, orc.EULER if seed % 2 else orc.ADVECTION, mx, my, mz)
+    tol = float(np.quantile(np.abs(q[:, :, 2:] - q[:, :, :-2]).max(axis=-1), 0.9))
+    a, n = orc.flag2refine3(q, tol)
+    b = orc.ref3_flag2refine3(q, tol)
+    assert np.array_equal(a, b)
+    assert n == int((b == 2.0).sum()) and 0 < n < mx * my * mz
+
+
+def test_flag2refine3_closed_form():
+    """A step in q(1) at x = 5.5: exactly the two cells beside it are
+    flagged (centred differences), at every (j, k) including edges; a
+    uniform state flags nothing."""
+    mx, my, mz = 10, 6, 5
+    q = np.ones((mz + 4, my + 4, mx + 4, 1))
+    q[:, :, 2 + 5:, 0] = 3.0         # cells i >= 6 (index i + 1)
+    f, n = orc.flag2refine3(q, 1.0)
+    want = np.zeros_like(f)
+    want[2:-2, 2:-2, 2 + 4:2 + 6] = 2.0   # cells i = 5, 6
+    assert np.array_equal(f, want) and n == 2 * my * mz
+    f, n = orc.flag2refine3(np.ones_like(q), 0.0)
+    assert n == 0 and not f.any()
+
+
+def test_mass3_closed_form():
+    q = np.zeros((7, 8, 9, 5))
+    q[2:-2, 2:-2, 2:-2, 0] = 1.5
+    q[..., 1] = 9.0                   # other components do not count
+    m = orc.mass3([q, q], (0.5, 0.25, 0.125))
+    assert m == 2 * 1.5 * (3 * 4 * 5) * 0.5 * 0.25 * 0.125
