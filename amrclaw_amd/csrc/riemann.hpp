@@ -53,11 +53,22 @@ template <typename T> struct ExactMath {
     using type = T;
     __device__ static __forceinline__ T div(T a, T b) { return a / b; }
     __device__ static __forceinline__ T sqrt(T x) { return dsqrt(x); }
+    // the wave limiter's ratio r = dot / |wave|^2 (inlinelimiter.f:36-50)
+    __device__ static __forceinline__ T ratio(T a, T b) { return a / b; }
 };
 struct FastMathF32 {
     using type = float;
     __device__ static __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // The limiter ratio r = dot / |wave|^2.  A wave below about 1e-19 has a
+    // subnormal |wave|^2, whose hardware reciprocal overflows to inf, and a
+    // zero neighbour dot would then give 0 * inf = NaN.  Both operands are
+    // scaled by 2^64 there (exact: powers of two), which makes the divisor
+    // normal and leaves every other ratio bit for bit as before.
+    __device__ static __forceinline__ float ratio(float a, float b) {
+        const float sc = b < 1.17549435e-38f ? 0x1p64f : 1.0f;
+        return (a * sc) * __builtin_amdgcn_rcpf(b * sc);
+    }
 };
 
 // ----------------------------------------------------------------------------

@@ -148,7 +148,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
             const int mth = M.mthlim[mw];
             if (mth != 0 && inner && wn2 != T(0)) {
                 using Mth = typename Sol::Math;
-                T r = Mth::div((s[mw] > T(0)) ? dotl : dotr, wn2);
+                T r = Mth::ratio((s[mw] > T(0)) ? dotl : dotr, wn2);
                 T wl = phi_limiter<Mth>(mth, r);
 #pragma unroll
                 for (int m = 0; m < NV; m++) wave[mw][m] = wl * wave[mw][m];

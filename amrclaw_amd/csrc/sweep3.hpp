@@ -312,7 +312,7 @@ __global__ __launch_bounds__(SWEEP3_LANES, sizeof(T) == 4 ? 3 : 1) void sweep3_p
                             }
                             if (wn2 == T(0)) continue;
                             using Mth = typename Sol::Math;
-                            T rr = Mth::div(s[w] > T(0) ? dotl : dotr, wn2);
+                            T rr = Mth::ratio(s[w] > T(0) ? dotl : dotr, wn2);
                             T wl = phi_limiter3<Mth>(mth, rr);
 #pragma unroll
                             for (int m = 0; m < NV; m++) wave[w][m] = wl * wave[w][m];

@@ -176,3 +176,31 @@ def test_stepgrid_mixed_conserves_to_fp64_rounding(gpu):
     m1 = float(qg[2:-2, 2:-2, 0].sum())
     # the boundary fluxes of a uniform state cancel between opposite sides
     assert abs(m1 - m0) <= 1e-13 * abs(m0), (m1 - m0) / m0
+
+
+@pytest.mark.parametrize("precision", [capi.FP32, capi.MIXED])
+def test_fp32_limiter_tiny_waves(gpu, precision):
+    """Tails below 1e-19 next to exact zeros (as an advected step profile
+    leaves on a fine level): in FP32 |wave|^2 is subnormal there and the
+    neighbour dot product is zero; the limiter ratio must stay 0, not
+    0 * (1 / subnormal) = NaN.  The step matches the FP64 oracle."""
+    p, c = both_params(orc.ADVECTION, precision=precision)
+    capi.init(c)
+    mx, my = 40, 24
+    q = np.zeros((my + 4, mx + 4, 1))
+    q[6:14, 8:20, 0] = 1.0
+    tails = [4.2e-21, 3.5e-28, 1e-20, 6e-38, 2e-25]
+    for k, v in enumerate(tails):
+        q[4 + 3 * k, 26 + k, 0] = v
+        q[20, 5 + 4 * k, 0] = v
+    if precision == capi.FP32:
+        q = q.astype(np.float32).astype(np.float64)
+    dx, dy = 1.0 / mx, 1.0 / my
+    dt = 0.4 * dy
+    print("tiny waves", precision, flush=True)
+    qo, _, co, _ = orc.stepgrid(p, q, dx, dy, dt)
+    qg, _, cg, _ = capi.stepgrid(q, dx, dy, dt)
+    capi.finalize()
+    assert np.isfinite(qg).all()
+    assert np.abs(qg - qo)[2:-2, 2:-2].max() <= 1e-6
+    assert rel_l1(qg[2:-2, 2:-2], qo[2:-2, 2:-2]) <= 1e-5
