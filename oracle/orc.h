@@ -194,6 +194,54 @@ int orc_flag2refine3(int mx, int my, int mz, int mbc, int meqn, double tolsp, co
                      double dontflag, double doflag, double* amrflags);
 double orc_mass3(int mx, int my, int mz, int mbc, int meqn, const double* q, double acc);
 
+/* ---- 3-D level update around the step (orc_amr3.c, src/3d) ---------------
+ * As orc_hier with a third direction: boxes[l][6k..6k+5] = (ilo, ihi, jlo,
+ * jhi, klo, khi); AoS arrays (nvar, mx+2ng, my+2ng, mz+2ng); intratz/kregsz
+ * and hz; mthbc(1:6) = x-lo, x-hi, y-lo, y-hi, z-lo, z-hi; periodic in a
+ * direction when its flag is set (all three or none, see orc_amr3.c). */
+typedef struct {
+    int nlev, mxnest;
+    int ngrids[ORC_MAXLV];
+    const int* boxes[ORC_MAXLV];
+    double* qnew[ORC_MAXLV];
+    double* qold[ORC_MAXLV];
+    const int64_t* qoff[ORC_MAXLV];
+    double hx[ORC_MAXLV], hy[ORC_MAXLV], hz[ORC_MAXLV], possk[ORC_MAXLV], time[ORC_MAXLV];
+    int iregsz[ORC_MAXLV], jregsz[ORC_MAXLV], kregsz[ORC_MAXLV];
+    int intratx[ORC_MAXLV], intraty[ORC_MAXLV], intratz[ORC_MAXLV];
+    double xlower, xupper, ylower, yupper, zlower, zupper;
+    int xperdom, yperdom, zperdom;
+    int mthbc[6];
+} orc_hier3;
+
+/* registers of a 3-D hierarchy: lenbc = 2 (ik jk + jk kk + kk ik)
+ * (prepf.f); listbc entries (i, j, k, side, kid, lkid) (setuse.f) */
+typedef struct {
+    int nlev;
+    int* lenbc[ORC_MAXLV];
+    int64_t* roff[ORC_MAXLV];
+    double* reg[ORC_MAXLV];
+    int64_t nreg[ORC_MAXLV];
+    int* lst_start[ORC_MAXLV];
+    int* lst_count[ORC_MAXLV];
+    int* listbc[ORC_MAXLV];
+    int nlist[ORC_MAXLV];
+} orc_regs3;
+
+int orc_bc3amr(const orc_params* p, const orc_hier3* h, double* val, int nrow, int ncol,
+               int nfil, double hx, double hy, double hz, double xlo_patch, double xhi_patch,
+               double ylo_patch, double yhi_patch, double zlo_patch, double zhi_patch);
+int orc_bound3(const orc_params* p, const orc_hier3* h, int level, int grid, double time,
+               double* valbig);
+int orc_prep_regs3(const orc_params* p, const orc_hier3* h, orc_regs3* r);
+void orc_free_regs3(orc_regs3* r);
+int orc_saveqc3(const orc_params* p, const orc_hier3* h, orc_regs3* r, int level);
+int orc_qad3(const orc_params* p, const orc_hier3* h, orc_regs3* r, int level, int grid);
+int orc_advanc3(const orc_params* p, orc_hier3* h, orc_regs3* r, int level, double* dtlevnew,
+                double* cfl_level);
+int orc_update3(const orc_params* p, const orc_hier3* h, orc_regs3* r, int level);
+double orc_conck3(const orc_params* p, const orc_hier3* h, int level);
+
 /* cell flagging (orc_flag.c) */
 #define ORC_UNSET (-1.0)
 #define ORC_DONTFLAG 0.0

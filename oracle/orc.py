@@ -680,3 +680,220 @@ def ref3_flag2refine3(q, tolsp, mbc=2):
     _ref3.ref3_flag2refine.argtypes = [C.c_int] * 5 + [_dp, C.c_double, _dp]
     _ref3.ref3_flag2refine(q.shape[3], mbc, mx, my, mz, q, tolsp, f)
     return f
+
+
+# ------------------------------------------------- 3-D level update ----
+class OrcHier3(C.Structure):
+    _fields_ = [
+        ("nlev", C.c_int), ("mxnest", C.c_int),
+        ("ngrids", C.c_int * MAXLV),
+        ("boxes", C.POINTER(C.c_int) * MAXLV),
+        ("qnew", C.POINTER(C.c_double) * MAXLV),
+        ("qold", C.POINTER(C.c_double) * MAXLV),
+        ("qoff", C.POINTER(C.c_int64) * MAXLV),
+        ("hx", C.c_double * MAXLV), ("hy", C.c_double * MAXLV), ("hz", C.c_double * MAXLV),
+        ("possk", C.c_double * MAXLV), ("time", C.c_double * MAXLV),
+        ("iregsz", C.c_int * MAXLV), ("jregsz", C.c_int * MAXLV), ("kregsz", C.c_int * MAXLV),
+        ("intratx", C.c_int * MAXLV), ("intraty", C.c_int * MAXLV), ("intratz", C.c_int * MAXLV),
+        ("xlower", C.c_double), ("xupper", C.c_double),
+        ("ylower", C.c_double), ("yupper", C.c_double),
+        ("zlower", C.c_double), ("zupper", C.c_double),
+        ("xperdom", C.c_int), ("yperdom", C.c_int), ("zperdom", C.c_int),
+        ("mthbc", C.c_int * 6),
+    ]
+
+
+class OrcRegs3(C.Structure):
+    _fields_ = OrcRegs._fields_
+
+
+def hier3_struct(H):
+    """OrcHier3 view of an amrclaw_amd.hierarchy.Hier3 (shares its arrays)."""
+    h = OrcHier3()
+    h.nlev = h.mxnest = H.nlev
+    for l, L in enumerate(H.levels, start=1):
+        h.ngrids[l] = len(L["boxes"])
+        h.boxes[l] = L["boxes"].ctypes.data_as(C.POINTER(C.c_int))
+        h.qnew[l] = L["qnew"].ctypes.data_as(C.POINTER(C.c_double))
+        h.qold[l] = L["qold"].ctypes.data_as(C.POINTER(C.c_double))
+        h.qoff[l] = L["off"].ctypes.data_as(C.POINTER(C.c_int64))
+        h.hx[l], h.hy[l], h.hz[l] = L["hx"], L["hy"], L["hz"]
+        h.possk[l], h.time[l] = L["possk"], L["time"]
+        h.iregsz[l], h.jregsz[l], h.kregsz[l] = L["iregsz"], L["jregsz"], L["kregsz"]
+        h.intratx[l], h.intraty[l], h.intratz[l] = L["ratx"], L["raty"], L["ratz"]
+    (h.xlower, h.xupper, h.ylower, h.yupper, h.zlower, h.zupper) = H.domain
+    h.xperdom, h.yperdom, h.zperdom = int(H.xper), int(H.yper), int(H.zper)
+    for k in range(6):
+        h.mthbc[k] = H.mthbc[k]
+    return h
+
+
+def bound3(p, H, level, grid, time):
+    """orc_bound3 on grid `grid` (0-based) of `level`, in place on H."""
+    L = lib()
+    L.orc_bound3.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcHier3), C.c_int, C.c_int,
+                             C.c_double, C.c_void_p]
+    h = hier3_struct(H)
+    val = H.grid(level, grid, 0)
+    rc = L.orc_bound3(C.byref(p), C.byref(h), level, grid, time, val.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError("orc_bound3 error %d" % rc)
+
+
+class OracleAMR3:
+    """The oracle's 3-D level update (advanc / update / conck) in place on
+    the arrays of a Hier3."""
+
+    def __init__(self, p, H):
+        L = lib()
+        P, Hh, Rr = C.POINTER(OrcParams), C.POINTER(OrcHier3), C.POINTER(OrcRegs3)
+        L.orc_prep_regs3.argtypes = [P, Hh, Rr]
+        L.orc_free_regs3.argtypes = [Rr]
+        L.orc_advanc3.argtypes = [P, Hh, Rr, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_update3.argtypes = [P, Hh, Rr, C.c_int]
+        L.orc_conck3.argtypes = [P, Hh, C.c_int]
+        L.orc_conck3.restype = C.c_double
+        self.p, self.H = p, H
+        self.h = hier3_struct(H)
+        self.r = OrcRegs3()
+        rc = L.orc_prep_regs3(C.byref(p), C.byref(self.h), C.byref(self.r))
+        if rc:
+            raise RuntimeError("orc_prep_regs3 error %d" % rc)
+
+    def __del__(self):
+        try:
+            lib().orc_free_regs3(C.byref(self.r))
+        except Exception:
+            pass
+
+    def advanc(self, level):
+        dtn, cfl = C.c_double(), C.c_double()
+        rc = lib().orc_advanc3(C.byref(self.p), C.byref(self.h), C.byref(self.r), level,
+                               C.byref(dtn), C.byref(cfl))
+        if rc:
+            raise RuntimeError("orc_advanc3 error %d" % rc)
+        self.H.levels[level - 1]["time"] = self.h.time[level]
+        return dtn.value, cfl.value
+
+    def update(self, level):
+        rc = lib().orc_update3(C.byref(self.p), C.byref(self.h), C.byref(self.r), level)
+        if rc:
+            raise RuntimeError("orc_update3 error %d" % rc)
+
+    def conck(self, level):
+        return lib().orc_conck3(C.byref(self.p), C.byref(self.h), level)
+
+    def reg(self, level, k):
+        n = 2 * self.p.meqn * self.r.lenbc[level][k]
+        off = self.r.roff[level][k]
+        return np.ctypeslib.as_array(self.r.reg[level], shape=(self.r.nreg[level],))[off:off + n].copy()
+
+    def listbc(self, level, k):
+        if not self.r.listbc[level]:
+            return np.zeros((0, 6), dtype=np.int32)
+        a = np.ctypeslib.as_array(self.r.listbc[level], shape=(6 * self.r.nlist[level],))
+        s, c = self.r.lst_start[level][k], self.r.lst_count[level][k]
+        return a[6 * s:6 * (s + c)].reshape(-1, 6).copy()
+
+
+def _ref3_lib(p):
+    global _ref3
+    if _ref3 is None:
+        lib()
+        _ref3 = C.CDLL(os.path.join(HERE, "_ref", "libamrref3.so"))
+    R = _ref3
+    lib().orc_set_plugin(C.byref(p))
+    R.ref3_set_method.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                  C.c_int, C.c_double, C.c_double, C.c_int]
+    meth = (C.c_int * 7)(*[p.method[k] for k in range(1, 8)])
+    lim = (C.c_int * 10)(*list(p.mthlim))
+    R.ref3_set_method(meth, lim, p.mwaves, p.mcapa, p.use_fwaves, p.cfl, p.cflv1, p.mbc)
+    return R
+
+
+def ref3_load_hier(p, H):
+    """Install a Hier3 in the reference's 3-D amr_module (harness)."""
+    R = _ref3_lib(p)
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    R.ref3_hier_setup.argtypes = [C.c_int] * 4 + [ip, ip] + [dp] * 5 + [ip] * 6 + \
+        [C.c_double] * 6 + [C.c_int] * 3 + [ip]
+    R.ref3_hier_xfer.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
+    lv = H.levels
+    f64 = lambda k: np.array([L[k] for L in lv], dtype=np.float64)
+    i32 = lambda k: np.array([L[k] for L in lv], dtype=np.int32)
+    boxes = np.ascontiguousarray(np.concatenate([L["boxes"].ravel() for L in lv]).astype(np.int32))
+    ngr = np.array([len(L["boxes"]) for L in lv], dtype=np.int32)
+    R.ref3_hier_setup(H.nlev, H.nlev, H.nvar, H.ng, ngr, boxes, f64("hx"), f64("hy"), f64("hz"),
+                      f64("possk"), f64("time"), i32("iregsz"), i32("jregsz"), i32("kregsz"),
+                      i32("ratx"), i32("raty"), i32("ratz"), *H.domain, int(H.xper), int(H.yper),
+                      int(H.zper), np.array(H.mthbc, dtype=np.int32))
+    for l in range(1, H.nlev + 1):
+        for k in range(len(lv[l - 1]["boxes"])):
+            for which in (0, 1):
+                g = np.ascontiguousarray(H.grid(l, k, which))
+                R.ref3_hier_xfer(l, k, which, H.nvar, g.ctypes.data_as(C.c_void_p), 1)
+    return R
+
+
+def ref3_get_grid(H, level, grid, which=0):
+    out = np.zeros(H.levels[level - 1]["shapes"][grid])
+    _ref3.ref3_hier_xfer(level, grid, which, H.nvar, out.ctypes.data_as(C.c_void_p), 0)
+    return out
+
+
+def ref3_bound(p, H, level, grid, time):
+    """The reference's 3-D bound.f on a freshly installed copy of H; returns
+    the grid's new array after the call."""
+    R = ref3_load_hier(p, H)
+    R.ref3_bound.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
+    R.ref3_bound(level, grid, time, H.nvar)
+    return ref3_get_grid(H, level, grid)
+
+
+class RefAMR3:
+    """The reference's own 3-D routines over a copy of a Hier3 installed in
+    amr_module (prepf/prepc, bound, saveqc, qad, stepgrid, fluxsv, fluxad,
+    update)."""
+
+    def __init__(self, p, H):
+        self.p, self.H = p, H
+        R = ref3_load_hier(p, H)
+        R.ref3_prep_regs.argtypes = [C.c_int, C.c_int]
+        R.ref3_advanc.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        R.ref3_update.argtypes = [C.c_int, C.c_int]
+        R.ref3_get_reg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
+        R.ref3_get_listbc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
+        R.ref3_prep_regs(H.nlev, H.nvar)
+        self.R = R
+
+    def advanc(self, level):
+        dtn, cfl = C.c_double(), C.c_double()
+        self.R.ref3_advanc(level, self.H.nvar, C.byref(dtn), C.byref(cfl))
+        return dtn.value, cfl.value
+
+    def update(self, level):
+        self.R.ref3_update(level, self.H.nvar)
+
+    def grid(self, level, k, which=0):
+        return ref3_get_grid(self.H, level, k, which)
+
+    def reg(self, level, k):
+        b = self.H.levels[level - 1]["boxes"][k]
+        L = self.H.levels[level - 2]
+        ik = (b[1] - b[0] + 1) // L["ratx"]
+        jk = (b[3] - b[2] + 1) // L["raty"]
+        kk = (b[5] - b[4] + 1) // L["ratz"]
+        lenbc = 2 * (ik * jk + jk * kk + kk * ik)
+        out = np.zeros(2 * self.H.nvar * lenbc)
+        self.R.ref3_get_reg(level, k, self.H.nvar, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def listbc(self, level, k):
+        n = C.c_int()
+        buf = np.zeros(6 * 200000, dtype=np.int32)
+        self.R.ref3_get_listbc(level, k, 200000, buf.ctypes.data_as(C.c_void_p), C.byref(n))
+        out = buf[:6 * n.value].reshape(-1, 6).copy()
+        first = 1 + sum(len(L["boxes"]) for L in self.H.levels[:level])
+        out[:, 4] -= first
+        return out

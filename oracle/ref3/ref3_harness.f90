@@ -1,10 +1,12 @@
 ! ref3_harness.f90 -- C-callable harness around the reference's own 3-D
 ! Fortran (TEST INFRASTRUCTURE ONLY; built into oracle/_ref, never shipped).
 !
-! stepgrid, step3, flux3 and inlinelimiter are compiled unmodified from the
-! reference's src/3d by oracle/Makefile (target ref3).  rpn3/rpt3/rptt3 are
-! the physics plugin the application links (flux3.f); here they forward to
-! the oracle's solvers.
+! stepgrid, step3, flux3, inlinelimiter, flag2refine and the level update
+! around the step (bound, filpatch, intfil, prefilp, bc3amr, saveqc, cstore,
+! qad, fluxsv, fluxad, upbnd, update, prepf, prepc, setuse, ...) are
+! compiled unmodified from the reference's src/3d by oracle/Makefile
+! (target ref3).  rpn3/rpt3/rptt3 are the physics plugin the application
+! links (flux3.f, qad.f); here they forward to the oracle's solvers.
 
 subroutine rpn3(ixyz,maxm,meqn,mwaves,maux,mbc,mx,ql,qr,auxl,auxr,wave,s,amdq,apdq)
   use iso_c_binding
@@ -110,4 +112,310 @@ contains
          1, tolsp_in, q, aux1, amrflags, goodpt, badpt)
   end subroutine ref3_flag2refine
 
+  ! ---------------------------------------------------------------------
+  ! Hierarchy set-up as in the 2-D harness: node/rnode/alloc/level lists
+  ! the way regrid leaves them (src/3d/amr_module.f90), then makeBndryList
+  ! (init_bndryList.f).  Grid k of level l is node gbase(l)+k; its new/old
+  ! solution blocks are consecutive in alloc.
+  subroutine ref3_hier_setup(nlev, mxnest_in, nvar, ng, ngr, boxes, hx, hy, hz, pk, tm, &
+       isz, jsz, ksz, ratx, raty, ratz, xlo, xhi, ylo, yhi, zlo, zhi, xper, yper, zper, bc) &
+       bind(C, name="ref3_hier_setup")
+    integer(c_int), value :: nlev, mxnest_in, nvar, ng
+    integer(c_int), intent(in) :: ngr(nlev), boxes(*), isz(nlev), jsz(nlev), ksz(nlev)
+    integer(c_int), intent(in) :: ratx(nlev), raty(nlev), ratz(nlev), bc(6)
+    real(c_double), intent(in) :: hx(nlev), hy(nlev), hz(nlev), pk(nlev), tm(nlev)
+    real(c_double), value :: xlo, xhi, ylo, yhi, zlo, zhi
+    integer(c_int), value :: xper, yper, zper
+    integer :: l, k, mptr, nb, ntot, loc, mitot, mjtot, mktot, mprev
+    ntot = sum(ngr(1:nlev))
+    maxgr = max(ntot + 10, 16)
+    if (allocated(node)) deallocate(node)
+    if (allocated(rnode)) deallocate(rnode)
+    if (allocated(listOfGrids)) deallocate(listOfGrids)
+    allocate(node(nsize, maxgr), rnode(rsize, maxgr), listOfGrids(maxgr))
+    node = 0
+    rnode = 0.d0
+    memsize = 1
+    nb = 0
+    do l = 1, nlev
+       do k = 1, ngr(l)
+          mitot = boxes(6*nb+2) - boxes(6*nb+1) + 1 + 2*ng
+          mjtot = boxes(6*nb+4) - boxes(6*nb+3) + 1 + 2*ng
+          mktot = boxes(6*nb+6) - boxes(6*nb+5) + 1 + 2*ng
+          memsize = memsize + 2*nvar*mitot*mjtot*mktot
+          nb = nb + 1
+       end do
+    end do
+    memsize = memsize + 8000000    ! room for registers and saveqc scratch
+    if (allocated(alloc)) deallocate(alloc)
+    allocate(alloc(memsize))
+    alloc = 0.d0
+    mxnest = mxnest_in
+    nghost = ng
+    xlower = xlo
+    xupper = xhi
+    ylower = ylo
+    yupper = yhi
+    zlower = zlo
+    zupper = zhi
+    xperdom = (xper /= 0)
+    yperdom = (yper /= 0)
+    zperdom = (zper /= 0)
+    mthbc(1:6) = bc(1:6)
+    lstart = 0
+    numgrids = 0
+    listStart = 0
+    nb = 0
+    loc = 2
+    mptr = 0
+    do l = 1, nlev
+       hxposs(l) = hx(l)
+       hyposs(l) = hy(l)
+       hzposs(l) = hz(l)
+       possk(l) = pk(l)
+       iregsz(l) = isz(l)
+       jregsz(l) = jsz(l)
+       kregsz(l) = ksz(l)
+       intratx(l) = ratx(l)
+       intraty(l) = raty(l)
+       intratz(l) = ratz(l)
+       numgrids(l) = ngr(l)
+       listStart(l) = mptr + 1
+       mprev = 0
+       do k = 1, ngr(l)
+          mptr = mptr + 1
+          listOfGrids(mptr) = mptr
+          if (k == 1) lstart(l) = mptr
+          if (mprev /= 0) node(levelptr, mprev) = mptr
+          node(levelptr, mptr) = null
+          mprev = mptr
+          node(nestlevel, mptr) = l
+          node(ndilo, mptr) = boxes(6*nb+1)
+          node(ndihi, mptr) = boxes(6*nb+2)
+          node(ndjlo, mptr) = boxes(6*nb+3)
+          node(ndjhi, mptr) = boxes(6*nb+4)
+          node(ndklo, mptr) = boxes(6*nb+5)
+          node(ndkhi, mptr) = boxes(6*nb+6)
+          rnode(cornxlo, mptr) = node(ndilo, mptr)*hx(l) + xlower
+          rnode(cornxhi, mptr) = (node(ndihi, mptr)+1)*hx(l) + xlower
+          rnode(cornylo, mptr) = node(ndjlo, mptr)*hy(l) + ylower
+          rnode(cornyhi, mptr) = (node(ndjhi, mptr)+1)*hy(l) + ylower
+          rnode(cornzlo, mptr) = node(ndklo, mptr)*hz(l) + zlower
+          rnode(cornzhi, mptr) = (node(ndkhi, mptr)+1)*hz(l) + zlower
+          rnode(timemult, mptr) = tm(l)
+          mitot = node(ndihi, mptr) - node(ndilo, mptr) + 1 + 2*ng
+          mjtot = node(ndjhi, mptr) - node(ndjlo, mptr) + 1 + 2*ng
+          mktot = node(ndkhi, mptr) - node(ndklo, mptr) + 1 + 2*ng
+          node(store1, mptr) = loc
+          node(store2, mptr) = loc + nvar*mitot*mjtot*mktot
+          node(storeaux, mptr) = 1
+          node(bndListSt, mptr) = null
+          node(bndListNum, mptr) = 0
+          loc = loc + 2*nvar*mitot*mjtot*mktot
+          nb = nb + 1
+       end do
+    end do
+    listStart(nlev+1) = mptr + 1
+    lenf = 1
+    lfree(1,1) = loc
+    lfree(1,2) = memsize - loc + 1
+    lendim = loc
+    if (allocated(bndList)) deallocate(bndList)
+    call initBndryList()
+    do l = 1, nlev
+       call makeBndryList(l)
+    end do
+  end subroutine ref3_hier_setup
+
+  integer function ref3_gridlen(mptr, nvar)
+    integer :: mptr, nvar
+    ref3_gridlen = nvar*(node(ndihi,mptr)-node(ndilo,mptr)+1+2*nghost) &
+         *(node(ndjhi,mptr)-node(ndjlo,mptr)+1+2*nghost)*(node(ndkhi,mptr)-node(ndklo,mptr)+1+2*nghost)
+  end function ref3_gridlen
+
+  ! copy grid (level l, index k from 0) new (which=0) / old (1) to or from q
+  subroutine ref3_hier_xfer(l, k, which, nvar, q, to_ref) bind(C, name="ref3_hier_xfer")
+    integer(c_int), value :: l, k, which, nvar, to_ref
+    real(c_double) :: q(*)
+    integer :: mptr, loc, n
+    mptr = listOfGrids(listStart(l) + k)
+    loc = node(store1, mptr)
+    if (which == 1) loc = node(store2, mptr)
+    n = ref3_gridlen(mptr, nvar)
+    if (to_ref /= 0) then
+       alloc(loc:loc+n-1) = q(1:n)
+    else
+       q(1:n) = alloc(loc:loc+n-1)
+    end if
+  end subroutine ref3_hier_xfer
+
+  ! bound.f (3-D) on grid k of level l at time t, in place on its new array
+  subroutine ref3_bound(l, k, t, nvar) bind(C, name="ref3_bound")
+    integer(c_int), value :: l, k, nvar
+    real(c_double), value :: t
+    integer :: mptr, mitot, mjtot, mktot
+    mptr = listOfGrids(listStart(l) + k)
+    mitot = node(ndihi,mptr)-node(ndilo,mptr)+1+2*nghost
+    mjtot = node(ndjhi,mptr)-node(ndjlo,mptr)+1+2*nghost
+    mktot = node(ndkhi,mptr)-node(ndklo,mptr)+1+2*nghost
+    call bound(t, nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, mktot, mptr, &
+         alloc(node(storeaux,mptr)), 0)
+  end subroutine ref3_bound
+
+  ! prepf (fine levels) and prepc (coarse levels), as regrid does
+  subroutine ref3_prep_regs(nlev, nvar) bind(C, name="ref3_prep_regs")
+    integer(c_int), value :: nlev, nvar
+    integer :: l
+    do l = 2, nlev
+       call prepf(l, nvar, 0)
+    end do
+    do l = 1, nlev - 1
+       call prepc(l, nvar)
+    end do
+  end subroutine ref3_prep_regs
+
+  ! advanc.f (3-D) restated around the reference's own routines (advanc.f
+  ! itself needs gauges_module, whose utility_module is absent)
+  subroutine ref3_advanc(level, nvar, dtlevnew, cflout) bind(C, name="ref3_advanc")
+    integer(c_int), value :: level, nvar
+    real(c_double) :: dtlevnew, cflout
+    integer :: j, mptr, mitot, mjtot, mktot
+    real(kind=8) :: dtnew
+    do j = 1, numgrids(level)
+       mptr = listOfGrids(listStart(level)+j-1)
+       mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
+       mjtot = node(ndjhi,mptr) - node(ndjlo,mptr) + 1 + 2*nghost
+       mktot = node(ndkhi,mptr) - node(ndklo,mptr) + 1 + 2*nghost
+       call bound(rnode(timemult,mptr), nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, &
+            mktot, mptr, alloc(node(storeaux,mptr)), 0)
+    end do
+    if (level+1 .le. mxnest) then
+       if (lstart(level+1) .ne. null) call saveqc(level+1, nvar, 0)
+    endif
+    dtlevnew = rinfinity
+    cfl_level = 0.d0
+    do j = 1, numgrids(level)
+       mptr = listOfGrids(listStart(level)+j-1)
+       mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
+       mjtot = node(ndjhi,mptr) - node(ndjlo,mptr) + 1 + 2*nghost
+       mktot = node(ndkhi,mptr) - node(ndklo,mptr) + 1 + 2*nghost
+       call harness3_par_advanc(mptr, mitot, mjtot, mktot, nvar, dtnew)
+       dtlevnew = dmin1(dtlevnew, dtnew)
+    end do
+    cflout = cfl_level
+  end subroutine ref3_advanc
+
+  ! advanc.f (3-D) par_advanc without gauges and timers
+  subroutine harness3_par_advanc(mptr, mitot, mjtot, mktot, nvar, dtnew)
+    integer :: mptr, mitot, mjtot, mktot, nvar
+    real(kind=8) :: dtnew
+    real(kind=8), allocatable :: fp(:), fm(:), gp(:), gm(:), hp(:), hm(:)
+    integer :: level, nx, ny, nz, locold, locnew, locaux, ntot, i, lenbc, locsvf, locsvq, locx1d
+    real(kind=8) :: hx, hy, hz, delt, time, xlow, ylow, zlow
+    ntot = nvar*mitot*mjtot*mktot
+    allocate(fp(ntot), fm(ntot), gp(ntot), gm(ntot), hp(ntot), hm(ntot))
+    fp = 0.d0
+    fm = 0.d0
+    gp = 0.d0
+    gm = 0.d0
+    hp = 0.d0
+    hm = 0.d0
+    level = node(nestlevel,mptr)
+    hx = hxposs(level)
+    hy = hyposs(level)
+    hz = hzposs(level)
+    delt = possk(level)
+    nx = node(ndihi,mptr) - node(ndilo,mptr) + 1
+    ny = node(ndjhi,mptr) - node(ndjlo,mptr) + 1
+    nz = node(ndkhi,mptr) - node(ndklo,mptr) + 1
+    time = rnode(timemult,mptr)
+    locold = node(store2, mptr)
+    locnew = node(store1, mptr)
+    if (level .lt. mxnest) then
+       do i = 1, ntot
+          alloc(locold + i - 1) = alloc(locnew + i - 1)
+       end do
+    endif
+    xlow = rnode(cornxlo,mptr) - nghost*hx
+    ylow = rnode(cornylo,mptr) - nghost*hy
+    zlow = rnode(cornzlo,mptr) - nghost*hz
+    locaux = node(storeaux,mptr)
+    if (node(ffluxptr,mptr) .ne. 0) then
+       lenbc = 2*( (nx/intratx(level-1))*(nz/intratz(level-1)) &
+            +(ny/intraty(level-1))*(nz/intratz(level-1)) &
+            +(nx/intratx(level-1))*(ny/intraty(level-1)))
+       locsvf = node(ffluxptr,mptr)
+       locsvq = locsvf + nvar*lenbc
+       locx1d = locsvq + nvar*lenbc
+       call qad(alloc(locnew),mitot,mjtot,mktot,nvar,alloc(locsvf),alloc(locsvq),lenbc, &
+            intratx(level-1),intraty(level-1),intratz(level-1),hx,hy,hz,0,alloc(locaux), &
+            alloc(locx1d),delt,mptr)
+    endif
+    call stepgrid(alloc(locnew),fm,fp,gm,gp,hm,hp,mitot,mjtot,mktot,nghost,delt,dtnew, &
+         hx,hy,hz,nvar,xlow,ylow,zlow,time,mptr,0,alloc(locaux))
+    if (node(cfluxptr,mptr) .ne. 0) call fluxsv(mptr,fm,fp,gm,gp,hm,hp, &
+         alloc(node(cfluxptr,mptr)),mitot,mjtot,mktot,nvar,listsp(level),delt,hx,hy,hz)
+    if (node(ffluxptr,mptr) .ne. 0) then
+       lenbc = 2*( (nx/intratx(level-1))*(nz/intratz(level-1)) &
+            +(ny/intraty(level-1))*(nz/intratz(level-1)) &
+            +(nx/intratx(level-1))*(ny/intraty(level-1)))
+       locsvf = node(ffluxptr,mptr)
+       call fluxad(fm,fp,gm,gp,hm,hp,alloc(locsvf),mptr,mitot,mjtot,mktot,nvar,lenbc, &
+            intratx(level-1),intraty(level-1),intratz(level-1),nghost,delt,hx,hy,hz)
+    endif
+    rnode(timemult,mptr) = rnode(timemult,mptr)+delt
+    deallocate(fp, fm, gp, gm, hp, hm)
+  end subroutine harness3_par_advanc
+
+  subroutine ref3_update(level, nvar) bind(C, name="ref3_update")
+    integer(c_int), value :: level, nvar
+    call update(level, nvar, 0)
+  end subroutine ref3_update
+
+  ! register block (svdflx then qc1d) of grid k (0-based) of fine level l
+  subroutine ref3_get_reg(l, k, nvar, out) bind(C, name="ref3_get_reg")
+    integer(c_int), value :: l, k, nvar
+    real(c_double) :: out(*)
+    integer :: mptr, lenbc, nx, ny, nz
+    mptr = listOfGrids(listStart(l) + k)
+    nx = node(ndihi,mptr) - node(ndilo,mptr) + 1
+    ny = node(ndjhi,mptr) - node(ndjlo,mptr) + 1
+    nz = node(ndkhi,mptr) - node(ndklo,mptr) + 1
+    lenbc = 2*((nx/intratx(l-1))*(ny/intraty(l-1)) + (ny/intraty(l-1))*(nz/intratz(l-1)) &
+         + (nz/intratz(l-1))*(nx/intratx(l-1)))
+    out(1:2*nvar*lenbc) = alloc(node(ffluxptr,mptr):node(ffluxptr,mptr)+2*nvar*lenbc-1)
+  end subroutine ref3_get_reg
+
+  ! listbc of grid k (0-based) of coarse level l (6 integers per entry)
+  subroutine ref3_get_listbc(l, k, nmax, out, n) bind(C, name="ref3_get_listbc")
+    integer(c_int), value :: l, k, nmax
+    integer(c_int) :: out(*), n
+    integer :: mptr
+    external copy_listbc3
+    mptr = listOfGrids(listStart(l) + k)
+    n = 0
+    if (node(cfluxptr,mptr) .eq. 0) return
+    call copy_listbc3(alloc(node(cfluxptr,mptr)), listsp(l), nmax, out, n)
+  end subroutine ref3_get_listbc
+
+  subroutine ref3_get_time(l, k, t) bind(C, name="ref3_get_time")
+    integer(c_int), value :: l, k
+    real(c_double) :: t
+    t = rnode(timemult, listOfGrids(listStart(l) + k))
+  end subroutine ref3_get_time
+
 end module ref3_harness
+
+! listbc lives in the double heap as integers (prepc.f, setuse.f); read it
+! through an integer dummy as fluxsv does
+subroutine copy_listbc3(listbc, maxsp, nmax, out, n)
+  implicit none
+  integer :: maxsp, nmax, n, listbc(6, maxsp), out(6, *)
+  integer :: e
+  n = 0
+  do e = 1, maxsp
+     if (listbc(1, e) .eq. 0) exit
+     n = n + 1
+     if (n .le. nmax) out(:, n) = listbc(:, e)
+  end do
+end subroutine copy_listbc3
