@@ -35,6 +35,9 @@ UNITS = [
     ("sweep3_f32.hip", ["-O3"]),
     ("flag3.hip", ["-O3", "-ffp-contract=off"]),
     ("capi3.cpp", ["-O2", "-x", "hip"]),
+    ("amr3.hip", ["-O3", "-ffp-contract=off"]),
+    ("amr3_plan.cpp", ["-O2", "-x", "hip"]),
+    ("capi3_amr.cpp", ["-O2", "-x", "hip"]),
 ]
 
 

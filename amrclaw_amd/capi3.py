@@ -55,6 +55,18 @@ def lib():
         L.amrgpu3_flag_patches.argtypes = [C.c_double, C.POINTER(C.c_int64)]
         L.amrgpu3_get_flags.argtypes = [C.c_int, _dp]
         L.amrgpu3_conck.argtypes = [C.c_double] * 3 + [dpo]
+        L.amrgpu3_set_geometry.argtypes = [C.c_int] + [_ip] * 6 + [_dp] * 4 + [_ip]
+        L.amrgpu3_set_level.argtypes = [C.c_int, C.c_int, _ip]
+        L.amrgpu3_set_time.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.amrgpu3_get_time.argtypes = [C.c_int, dpo]
+        _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.amrgpu3_put_level.argtypes = [C.c_int, C.c_int, _dp, _i64]
+        L.amrgpu3_get_level.argtypes = [C.c_int, C.c_int, _dp, _i64]
+        L.amrgpu3_bound_level.argtypes = [C.c_int, C.c_double]
+        L.amrgpu3_advance_level.argtypes = [C.c_int, dpo, dpo]
+        L.amrgpu3_update_level.argtypes = [C.c_int]
+        L.amrgpu3_conck_level.argtypes = [C.c_int, dpo]
+        L.amrgpu3_get_registers.argtypes = [C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
         _bound = True
     return L
 
@@ -182,3 +194,92 @@ __all__ = ["ADVECTION", "EULER", "FP32", "FP64", "KT_X", "KT_Y", "KT_Z", "KT_UPD
            "Amrgpu3Config", "make_config3", "init3", "finalize3", "set_patches3", "put_patch3",
            "get_patch3", "step_patches3", "get_fluxes3", "stepgrid3", "set_timing3",
            "kernel_time3", "synchronize3", "flag_patches3", "get_flags3", "conck3"]
+
+
+# ------------------------------------------------------------ hierarchy ---
+def load_hierarchy3(H):
+    """Install an amrclaw_amd.hierarchy.Hier3: geometry (domain, mthbc,
+    ratios), the grids of every level, both time levels and the times."""
+    lv = H.levels
+    i32 = lambda k: np.ascontiguousarray([L[k] for L in lv], dtype=np.int32)
+    f64 = lambda k: np.ascontiguousarray([L[k] for L in lv], dtype=np.float64)
+    _check(lib().amrgpu3_set_geometry(H.nlev, i32("iregsz"), i32("jregsz"), i32("kregsz"),
+                                      i32("ratx"), i32("raty"), i32("ratz"), f64("hx"), f64("hy"),
+                                      f64("hz"), np.ascontiguousarray(H.domain, dtype=np.float64),
+                                      np.ascontiguousarray(H.mthbc, dtype=np.int32)))
+    for l, L in enumerate(lv, start=1):
+        b = np.ascontiguousarray(L["boxes"], dtype=np.int32).reshape(-1)
+        _check(lib().amrgpu3_set_level(l, len(L["boxes"]), b if b.size else np.zeros(6, np.int32)))
+        _check(lib().amrgpu3_set_time(l, L["time"], L["possk"]))
+        if len(L["boxes"]):
+            _check(lib().amrgpu3_put_level(l, 0, L["qnew"], L["off"]))
+            _check(lib().amrgpu3_put_level(l, 1, L["qold"], L["off"]))
+
+
+def read_level3(H, l, which=0):
+    """Download level l (new: which = 0, old: 1) laid out like H.levels[l-1]."""
+    L = H.levels[l - 1]
+    out = np.zeros_like(L["qnew"])
+    _check(lib().amrgpu3_get_level(l, which, out, L["off"]))
+    return out
+
+
+def set_time3(level, time, possk):
+    _check(lib().amrgpu3_set_time(level, time, possk))
+
+
+def get_time3(level):
+    t = C.c_double()
+    _check(lib().amrgpu3_get_time(level, C.byref(t)))
+    return t.value
+
+
+def bound_level3(level, time):
+    """bound.f (3-D) for every grid of the level at `time` (ghost cells only)."""
+    _check(lib().amrgpu3_bound_level(level, time))
+
+
+def advance_level3(level):
+    """advanc.f (3-D) for one level: returns (dtlevnew, cfl_level)."""
+    dtn, cfl = C.c_double(), C.c_double()
+    _check(lib().amrgpu3_advance_level(level, C.byref(dtn), C.byref(cfl)))
+    return dtn.value, cfl.value
+
+
+def update_level3(level):
+    """update.f with upbnd.f (3-D): reflux and average onto `level`."""
+    _check(lib().amrgpu3_update_level(level))
+
+
+def conck_level3(level):
+    m = C.c_double()
+    _check(lib().amrgpu3_conck_level(level, C.byref(m)))
+    return m.value
+
+
+def get_registers3(level, grid, nvar):
+    """(svdflx, qc1d) of one fine grid, each (lenbc, nvar)."""
+    n = C.c_int()
+    _check(lib().amrgpu3_get_registers(level, grid, None, C.byref(n)))
+    buf = np.zeros(2 * nvar * n.value)
+    _check(lib().amrgpu3_get_registers(level, grid, buf.ctypes.data_as(C.c_void_p), C.byref(n)))
+    lb = n.value
+    return buf[:nvar * lb].reshape(lb, nvar), buf[nvar * lb:].reshape(lb, nvar)
+
+
+class DeviceAMR3:
+    """The library's 3-D level update over a Hier3 loaded on the device
+    (the advanc / update / conck interface of the reference's driver)."""
+
+    def __init__(self, H):
+        self.H = H
+        load_hierarchy3(H)
+
+    def advanc(self, level):
+        return advance_level3(level)
+
+    def update(self, level):
+        update_level3(level)
+
+    def conck(self, level):
+        return conck_level3(level)

@@ -16,6 +16,7 @@
 
 #include "../../include/amrgpu.h"
 #include "../../include/amrgpu3.h"
+#include "capi3_internal.hpp"
 #include "kernels3.hpp"
 
 using namespace amrgpu;
@@ -169,26 +170,27 @@ int64_t block_cost(const Patch3& pt, int d, const Block3& b) {
 // the batch's work lists: per sweep, (patch, plane block) in decreasing
 // cost, so that the longest workgroups start first; update, (patch, 256
 // cells)
-void plan_blocks() {
-    for (int k = 0; k < NKIND; k++) g3.blocks[k].clear();
-    const int pbk = g3.plane_block;
-    for (int p = 0; p < g3.npatch; p++) {
-        const Patch3& pt = g3.patches[p];
+void plan_blocks_of(const std::vector<Patch3>& patches, int pbk, std::vector<Block3> out[NKIND]) {
+    for (int k = 0; k < NKIND; k++) out[k].clear();
+    for (int p = 0; p < (int)patches.size(); p++) {
+        const Patch3& pt = patches[p];
         const int np[3] = {pt.mz, pt.mz, pt.my};
         for (int d = 0; d < 3; d++)
             for (int b = 1; b <= np[d]; b += pbk)
-                g3.blocks[d].push_back(Block3{p, b, std::min(b + pbk, np[d] + 1), 0});
+                out[d].push_back(Block3{p, b, std::min(b + pbk, np[d] + 1), 0});
         const int ncell = pt.mx * pt.my * pt.mz;
         for (int c = 0; c < ncell; c += 256)
-            g3.blocks[3].push_back(Block3{p, c, std::min(c + 256, ncell), 0});
+            out[3].push_back(Block3{p, c, std::min(c + 256, ncell), 0});
     }
     for (int d = 0; d < 3; d++) {
-        auto& v = g3.blocks[d];
-        std::stable_sort(v.begin(), v.end(), [d](const Block3& a, const Block3& b) {
-            return block_cost(g3.patches[a.patch], d, a) > block_cost(g3.patches[b.patch], d, b);
+        auto& v = out[d];
+        std::stable_sort(v.begin(), v.end(), [&patches, d](const Block3& a, const Block3& b) {
+            return block_cost(patches[a.patch], d, a) > block_cost(patches[b.patch], d, b);
         });
     }
 }
+
+void plan_blocks() { plan_blocks_of(g3.patches, g3.plane_block, g3.blocks); }
 
 int define_batch(int npatch, const int* mx, const int* my, const int* mz) {
     free_batch();
@@ -265,25 +267,26 @@ int get_q(int p, double* q) {
     return AMRGPU_OK;
 }
 
-int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
+int step_io(const Step3IO& io, double dx, double dy, double dz, double dt) {
     if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !(dt >= 0))
         return fail3(AMRGPU_ERR_ARG, "bad mesh widths or time step");
-    HIPCHK3(hipMemsetAsync(g3.d_cfl, 0, std::max(1, g3.npatch) * sizeof(unsigned long long),
+    HIPCHK3(hipMemsetAsync(io.d_cfl, 0, std::max(1, io.npatch) * sizeof(unsigned long long),
                            g3.stream));
     Sweep3Args a{};
-    a.q = g3.d_q;
-    a.face = g3.d_face;
-    a.patches = g3.d_patches;
+    a.q = io.q;
+    a.face = io.face;
+    a.patches = io.d_patches;
     a.dtd[0] = dt / dx;
     a.dtd[1] = dt / dy;
     a.dtd[2] = dt / dz;
     a.meth = g3.meth;
-    a.cfl_patch = g3.d_cfl;
+    a.cfl_patch = io.d_cfl;
     const int solver = g3.cfg.solver;
     for (int d = 1; d <= 3; d++) {
         a.phase = d;
-        a.blocks = g3.d_blocks[d - 1];
-        a.nblocks = (int)g3.blocks[d - 1].size();
+        a.blocks = io.d_blocks[d - 1];
+        a.nblocks = io.nblocks[d - 1];
+        if (a.nblocks == 0) continue;
         hipEvent_t stop;
         int rc = tbegin(d - 1, &stop);
         if (rc) return rc;
@@ -292,24 +295,40 @@ int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
         rc = tend(stop);
         if (rc) return rc;
     }
-    {
+    const int nb = io.nblocks[3];
+    if (nb > 0) {
         hipEvent_t stop;
         int rc = tbegin(3, &stop);
         if (rc) return rc;
-        const int nb = (int)g3.blocks[3].size();
         if (g3.esize == 8)
-            HIPCHK3(launch_update3_f64(g3.cfg.nvar, static_cast<double*>(g3.d_q),
-                                       static_cast<const double*>(g3.d_face), g3.d_patches,
-                                       g3.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
+            HIPCHK3(launch_update3_f64(g3.cfg.nvar, static_cast<double*>(io.q),
+                                       static_cast<const double*>(io.face), io.d_patches,
+                                       io.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
                                        g3.stream));
         else
-            HIPCHK3(launch_update3_f32(g3.cfg.nvar, static_cast<float*>(g3.d_q),
-                                       static_cast<const float*>(g3.d_face), g3.d_patches,
-                                       g3.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
+            HIPCHK3(launch_update3_f32(g3.cfg.nvar, static_cast<float*>(io.q),
+                                       static_cast<const float*>(io.face), io.d_patches,
+                                       io.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
                                        g3.stream));
         rc = tend(stop);
         if (rc) return rc;
     }
+    return AMRGPU_OK;
+}
+
+int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
+    Step3IO io{};
+    io.q = g3.d_q;
+    io.face = g3.d_face;
+    io.d_patches = g3.d_patches;
+    for (int k = 0; k < NKIND; k++) {
+        io.d_blocks[k] = g3.d_blocks[k];
+        io.nblocks[k] = (int)g3.blocks[k].size();
+    }
+    io.npatch = g3.npatch;
+    io.d_cfl = g3.d_cfl;
+    int rc = step_io(io, dx, dy, dz, dt);
+    if (rc) return rc;
     if (cflgrid) {
         HIPCHK3(d2h3(g3.h_cfl.data(), g3.d_cfl, g3.npatch * sizeof(unsigned long long)));
         for (int p = 0; p < g3.npatch; p++) {
@@ -378,6 +397,32 @@ int ensure_flag_buffers() {
 }
 
 }  // namespace
+
+namespace amrgpu {
+bool ctx3_inited() { return g3.inited; }
+const amrgpu3_config& ctx3_cfg() { return g3.cfg; }
+size_t ctx3_esize() { return g3.esize; }
+hipStream_t ctx3_stream() { return g3.stream; }
+int ctx3_plane_block() { return g3.plane_block; }
+int ctx3_fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g3_err = buf;
+    return code;
+}
+hipError_t ctx3_h2d(void* dst, const void* src, size_t n) { return h2d3(dst, src, n); }
+hipError_t ctx3_d2h(void* dst, const void* src, size_t n) { return d2h3(dst, src, n); }
+int ctx3_step(const Step3IO& io, double dx, double dy, double dz, double dt) {
+    return step_io(io, dx, dy, dz, dt);
+}
+void ctx3_plan_blocks(const std::vector<Patch3>& patches, int plane_block,
+                      std::vector<Block3> out[4]) {
+    plan_blocks_of(patches, plane_block, out);
+}
+}  // namespace amrgpu
 
 extern "C" {
 
@@ -485,6 +530,7 @@ int amrgpu3_init(const amrgpu3_config* cfg) {
 
 int amrgpu3_finalize(void) {
     if (!g3.inited) return AMRGPU_OK;
+    amr3_release();
     free_batch();
     free_events();
     if (g3.stream) hipStreamDestroy(g3.stream);

@@ -99,6 +99,50 @@ int amrgpu3_get_flags(int patch, double* amrflags);
  * sequential sum to rounding). */
 int amrgpu3_conck(double hx, double hy, double hz, double* totmass);
 
+/* ---- Hierarchy: the level update around the step (src/3d advanc.f) -----
+ * Drop-in for the reference's 3-D advanc(level, nvar, dtlevnew, ...) and
+ * update(level, nvar, naux) (tick.f calls both by name): the grids of every
+ * level live on the device with both time levels; advance_level fills
+ * every grid's ghost cells (bound.f, filpatch.f, intfil.f, prefilp.f,
+ * bc3amr.f90), saves the coarse perimeter of the next level (saveqc.f,
+ * cstore.f), copies new to old, adds qad.f's coarse-fine Riemann fluxes,
+ * steps every grid (stepgrid.f) and saves fluxes in the registers
+ * (fluxsv.f, fluxad.f); update_level refluxes (upbnd.f) and averages
+ * (update.f).  Periodic domains must be periodic in all three directions
+ * (saveqc.f wraps only then); mthbc 0 (user) and 4 are refused. */
+
+/* Levels 1..nlev: index-space sizes, ratios to the next finer level
+ * (intratx/y/z), mesh widths (hxposs/hyposs/hzposs); domain = (xlower,
+ * xupper, ylower, yupper, zlower, zupper); mthbc(1:6).  Drops every level. */
+int amrgpu3_set_geometry(int nlev, const int* iregsz, const int* jregsz, const int* kregsz,
+                         const int* ratx, const int* raty, const int* ratz, const double* hx,
+                         const double* hy, const double* hz, const double* domain,
+                         const int* mthbc);
+/* The grids of a level: boxes[6g..6g+5] = node(ndilo, ndihi, ndjlo, ndjhi,
+ * ndklo, ndkhi); allocates their storage (new and old) on the device. */
+int amrgpu3_set_level(int level, int ngrids, const int* boxes);
+/* rnode(timemult) of the level's grids and possk(level). */
+int amrgpu3_set_time(int level, double time, double possk);
+int amrgpu3_get_time(int level, double* time);
+/* Copy a level's new (which = 0) or old (1) solutions to / from the
+ * device: grid g is the reference's alloc(node(store1|store2)) array
+ * q(nvar, mx+4, my+4, mz+4) at alloc_base + offsets[g]. */
+int amrgpu3_put_level(int level, int which, const double* alloc_base, const int64_t* offsets);
+int amrgpu3_get_level(int level, int which, double* alloc_base, const int64_t* offsets);
+/* bound.f (3-D) for every grid of the level at `time`. */
+int amrgpu3_bound_level(int level, double time);
+/* advanc.f (3-D): dtlevnew = min over grids of possk*cfl/cflgrid, the
+ * level's largest Courant number; advances the level's time by possk. */
+int amrgpu3_advance_level(int level, double* dtlevnew, double* cfl_level);
+/* update.f (3-D) with upbnd.f: onto `level` from level+1. */
+int amrgpu3_update_level(int level);
+/* conck.f (3-D): hx hy hz times the sum of q(1) over the level's grids. */
+int amrgpu3_conck_level(int level, double* totmass);
+/* Register block of fine grid `grid` of `level` (prepf.f): svdflx(nvar,
+ * lenbc) then qc1d(nvar, lenbc); lenbc = 2 (ik jk + jk kk + kk ik).  With
+ * svdflx_qc1d NULL only lenbc is returned. */
+int amrgpu3_get_registers(int level, int grid, double* svdflx_qc1d, int* lenbc);
+
 /* Device timing of the step's kernels: amrgpu3_set_timing(1) resets and
  * starts it; kind 0, 1, 2 = x-, y-, z-sweep, 3 = update. */
 int amrgpu3_set_timing(int on);
