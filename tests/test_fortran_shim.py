@@ -57,3 +57,34 @@ def test_3d_shim_compiles_and_binds_exported_symbols(tmp_path):
                               text=True).stdout
     for s in bound:
         assert (" " + s + "\n") in exported, s
+
+
+SHIM3L = os.path.join(ROOT, "amrclaw_amd", "fortran", "amrgpu3_level_shim.f90")
+
+
+@pytest.mark.skipif(not os.path.exists(FC) or not os.path.exists(os.path.join(MODS3, "amr_module.mod")),
+                    reason="amdflang or the reference 3-D module build absent")
+def test_3d_level_shim_compiles_and_binds_exported_symbols(tmp_path):
+    """The 3-D level shim defines advanc and update with src/3d's names
+    against the reference's 3-D amr_module and binds only exported
+    amrgpu3_* symbols."""
+    objs = []
+    for src in (SHIM3, SHIM3L):
+        obj = tmp_path / (os.path.basename(src) + ".o")
+        r = subprocess.run([FC, "-c", "-I" + MODS3, src, "-o", str(obj), "-module-dir",
+                            str(tmp_path), "-I" + str(tmp_path)],
+                           capture_output=True, text=True, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        objs.append(obj)
+    syms = subprocess.run(["nm", str(objs[1])], capture_output=True, text=True).stdout.split("\n")
+    defined = {l.split()[-1] for l in syms if " T " in l}
+    undefined = {l.split()[-1] for l in syms if " U " in l}
+    assert {"advanc_", "update_", "amrgpu3_upload_hierarchy_",
+            "amrgpu3_download_hierarchy_"} <= defined
+    bound = {s for s in undefined if s.startswith("amrgpu3_") and not s.endswith("_")}
+    assert {"amrgpu3_advance_level", "amrgpu3_update_level", "amrgpu3_set_geometry",
+            "amrgpu3_set_level", "amrgpu3_put_level", "amrgpu3_get_level"} <= bound
+    exported = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
+                              text=True).stdout
+    for s in bound:
+        assert (" " + s + "\n") in exported, s
