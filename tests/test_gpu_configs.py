@@ -95,3 +95,43 @@ def test_config_periodic_mass_conserved(gpu, precision):
             assert abs(m - m0) <= 1e-13 * abs(m0), (m, m0)
     finally:
         capi.finalize()
+
+
+@pytest.mark.parametrize("precision", [capi.FP64, capi.FP32])
+def test_config_c5_3d(gpu, precision):
+    """C5 (3-D Euler radial bump, 64^3 level 1, two levels at ratio 2): one
+    coarse Berger-Oliger step on the device against the oracle; FP64 bit for
+    bit (solution, registers, dt / CFL), FP32 within 1e-5 relative L1."""
+    from amrclaw_amd import capi3
+    from tests.common3 import berger_oliger3
+    from tests.configs3 import config_hierarchy3
+    print("config C5", precision, flush=True)
+    H = config_hierarchy3()
+    if precision == capi.FP32:
+        for L in H.levels:
+            L["qnew"][:] = L["qnew"].astype(np.float32)
+            L["qold"][:] = L["qold"].astype(np.float32)
+    p = orc.make_params3(orc.EULER)
+    O = orc.OracleAMR3(p, H.copy())
+    capi3.init3(capi3.make_config3(orc.EULER, precision=precision))
+    try:
+        D = capi3.DeviceAMR3(H)
+        lo, ld = [], []
+        berger_oliger3(D, H, 1, ld)
+        berger_oliger3(O, O.H, 1, lo)
+        for l in range(1, H.nlev + 1):
+            got = capi3.read_level3(H, l, 0)
+            want = O.H.levels[l - 1]["qnew"]
+            if precision == capi.FP64:
+                assert np.array_equal(got, want), l
+            else:
+                assert rel_l1(got, want) <= 1e-5, (l, rel_l1(got, want))
+        if precision == capi.FP64:
+            assert lo == ld
+            for l in range(2, H.nlev + 1):
+                for k in range(len(H.levels[l - 1]["boxes"])):
+                    sv, qc = capi3.get_registers3(l, k, H.nvar)
+                    ref = O.reg(l, k).reshape(2, -1, H.nvar)
+                    assert np.array_equal(sv, ref[0]) and np.array_equal(qc, ref[1]), (l, k)
+    finally:
+        capi3.finalize3()

@@ -63,3 +63,29 @@ def test_config_periodic_mass_conserved(name):
     for _ in range(3):
         berger_oliger(O, H, 1)
         assert abs(O.conck(1) - m0) <= 1e-13 * abs(m0)
+
+
+needs_ref3 = pytest.mark.skipif(not orc.ref3_available(), reason="3-D reference build absent")
+
+
+@needs_ref3
+def test_config_c5_level_update_matches_reference():
+    """C5 (3-D Euler radial, 64^3 level 1 in eight 32^3 grids, two levels at
+    ratio 2, walls below and extrapolation above): one coarse step of the
+    oracle bit for bit with the reference's src/3d Fortran."""
+    from tests.common3 import berger_oliger3
+    from tests.configs3 import config_hierarchy3
+    H = config_hierarchy3()
+    assert [len(L["boxes"]) for L in H.levels] == [8, 17, 38]
+    p = orc.make_params3(orc.EULER)
+    O = orc.OracleAMR3(p, H.copy())
+    R = orc.RefAMR3(p, H)
+    lo, lr = [], []
+    berger_oliger3(O, O.H, 1, lo)
+    berger_oliger3(R, H, 1, lr)
+    assert lo == lr
+    for l in range(1, H.nlev + 1):
+        for k in range(len(H.levels[l - 1]["boxes"])):
+            assert np.array_equal(O.H.grid(l, k, 0), R.grid(l, k, 0)), (l, k)
+            if l > 1:
+                assert np.array_equal(O.reg(l, k), R.reg(l, k)), ("reg", l, k)
