@@ -58,6 +58,7 @@ def lib():
         L.amrgpu3_set_geometry.argtypes = [C.c_int] + [_ip] * 6 + [_dp] * 4 + [_ip]
         L.amrgpu3_set_level.argtypes = [C.c_int, C.c_int, _ip]
         L.amrgpu3_set_time.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.amrgpu3_set_plane_block.argtypes = [C.c_int]
         L.amrgpu3_get_time.argtypes = [C.c_int, dpo]
         _i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.amrgpu3_put_level.argtypes = [C.c_int, C.c_int, _dp, _i64]
@@ -222,6 +223,11 @@ def read_level3(H, l, which=0):
     out = np.zeros_like(L["qnew"])
     _check(lib().amrgpu3_get_level(l, which, out, L["off"]))
     return out
+
+
+def set_plane_block3(plane_block):
+    """Planes per sweep workgroup of the hierarchy's levels (0: per level)."""
+    _check(lib().amrgpu3_set_plane_block(plane_block))
 
 
 def set_time3(level, time, possk):

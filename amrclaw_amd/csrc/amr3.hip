@@ -66,7 +66,8 @@ __device__ __forceinline__ double mc_slope(double vp, double vc, double vm) {
 
 template <typename S>
 __global__ __launch_bounds__(256) void fill3_stage_kernel(Fill3Args a) {
-    const Fill3Op op = a.ops[blockIdx.x];
+    const Fill3Work w = a.work[blockIdx.x];
+    const Fill3Op op = a.ops[w.op];
     const int tid = threadIdx.x, nv = a.nv;
     const Buf3<S> dst = buf3<S>(a, op.dst);
     const int64_t ncell = (int64_t)op.ni * op.nj * op.nk;
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void fill3_stage_kernel(Fill3Args a) {
         const int mode = (op.flags & F3F_FORCE_NEW) ? T3_NEW : a.tmode[op.src_level];
         const double ai = a.alphai[op.src_level], ac = a.alphac[op.src_level];
         const int64_t smit = sp.mx + 4, smij = smit * (sp.my + 4), spl = qplane3(sp);
-        for (int64_t t = tid; t < ncell * nv; t += 256) {
+        for (int64_t t = w.lo + tid; t < w.hi; t += 256) {
             const int m = (int)(t / ncell);
             const int64_t c = t - m * ncell;
             const int di = (int)(c % op.ni), r = (int)(c / op.ni);
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void fill3_stage_kernel(Fill3Args a) {
         const double zfc = a.zlower + op.kplo * hzc;
         const double ratiox = (double)op.ratx, ratioy = (double)op.raty, ratioz = (double)op.ratz;
         const unsigned char* mask = a.masks + op.mask_off;
-        for (int64_t c = tid; c < ncell; c += 256) {
+        for (int64_t c = w.lo + tid; c < w.hi; c += 256) {
             if (mask[c]) continue;
             const int iff = (int)(c % op.ni) + 1, r = (int)(c / op.ni);
             const int jf = r % op.nj + 1, kf = r / op.nj + 1;   // 1-based in the box
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void fill3_stage_kernel(Fill3Args a) {
         }
     } else if (op.type == F3_COPYBACK) {
         const Buf3<S> src = buf3<S>(a, op.src);
-        for (int64_t t = tid; t < ncell * nv; t += 256) {
+        for (int64_t t = w.lo + tid; t < w.hi; t += 256) {
             const int m = (int)(t / ncell);
             const int64_t c = t - m * ncell;
             const int di = (int)(c % op.ni), r = (int)(c / op.ni);
@@ -396,11 +397,11 @@ __global__ __launch_bounds__(256) void avg3_kernel(Reg3Args a) {
 
 // ---------------------------------------------------------------- launchers
 hipError_t launch_fill3_stage(int esize, const Fill3Args& a, hipStream_t st) {
-    if (a.nops <= 0) return hipSuccess;
+    if (a.nwork <= 0) return hipSuccess;
     if (esize == 8)
-        hipLaunchKernelGGL(fill3_stage_kernel<double>, dim3(a.nops), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(fill3_stage_kernel<double>, dim3(a.nwork), dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL(fill3_stage_kernel<float>, dim3(a.nops), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(fill3_stage_kernel<float>, dim3(a.nwork), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

@@ -9,8 +9,10 @@
 // regrid, and emits operations on buffers (the level's grids and FP64
 // scratch arrays for the coarse patches valcrse and periodic pieces
 // valPatch), each tagged with a dependency stage.  At every level step one
-// launch per stage executes all operations of that stage, one workgroup per
-// operation, in FP64 and in the reference's order.
+// launch per stage executes all operations of that stage, copies and
+// interpolations split into workgroup items of at most 2048 elements,
+// boundary conditions one workgroup each, in FP64 and in the reference's
+// order.
 #pragma once
 #include <cstdint>
 #include "sweep3.hpp"
@@ -54,6 +56,13 @@ struct Fill3Op {
 
 enum Time3Mode : int32_t { T3_NEW = 0, T3_OLD = 1, T3_BLEND = 2 };
 
+// one workgroup's share of an operation: elements [lo, hi) of its linear
+// range (copies: cells x components; interpolation: cells); a boundary
+// condition or ring op is one item
+struct Fill3Work {
+    int32_t op, lo, hi, pad;
+};
+
 // Per-grid register layout of a fine level (prepf.f): svdflx(nv, lenbc)
 // at reg_off, qc1d(nv, lenbc) after it.
 struct RegGrid3 {
@@ -62,7 +71,8 @@ struct RegGrid3 {
 };
 
 struct Fill3Args {
-    const Fill3Op* ops;
+    const Fill3Work* work;       // this launch's items
+    const Fill3Op* ops;          // all ops of the program
     const Fill3Buf* bufs;
     double* scratch;
     const unsigned char* masks;
@@ -76,7 +86,7 @@ struct Fill3Args {
     double hx[AMR3_MAXLV], hy[AMR3_MAXLV], hz[AMR3_MAXLV];
     double xlower, ylower, zlower;
     int32_t mthbc[6];
-    int32_t nv, nops;
+    int32_t nv, nwork;
 };
 
 // fluxsv.f: coarse cell (I, J, K) (1-based with ghosts) of grid `grid`,

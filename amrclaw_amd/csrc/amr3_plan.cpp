@@ -526,12 +526,31 @@ struct Planner3 {
     }
 };
 
-void finish_plan(Fill3Plan& P) {
+void finish_plan(Fill3Plan& P, int nv) {
     std::stable_sort(P.ops.begin(), P.ops.end(),
                      [](const Fill3Op& a, const Fill3Op& b) { return a.stage < b.stage; });
     P.nstages = P.ops.empty() ? 0 : P.ops.back().stage + 1;
+    // split copies and interpolations into items of at most CHUNK elements
+    // so that a large ghost slab is spread over several workgroups
+    constexpr int CHUNK = 2048;
+    P.work.clear();
     P.stage_start.assign(P.nstages + 1, 0);
-    for (const auto& op : P.ops) P.stage_start[op.stage + 1]++;
+    for (int o = 0; o < (int)P.ops.size(); o++) {
+        const Fill3Op& op = P.ops[o];
+        const int64_t ncell = (int64_t)op.ni * op.nj * op.nk;
+        int64_t n = 0;
+        if (op.type == F3_COPY || op.type == F3_COPYBACK) n = ncell * nv;
+        else if (op.type == F3_INTERP) n = ncell;
+        if (n == 0) {
+            P.work.push_back(Fill3Work{o, 0, 0, 0});
+            P.stage_start[op.stage + 1]++;
+            continue;
+        }
+        for (int64_t a = 0; a < n; a += CHUNK) {
+            P.work.push_back(Fill3Work{o, (int32_t)a, (int32_t)std::min<int64_t>(a + CHUNK, n), 0});
+            P.stage_start[op.stage + 1]++;
+        }
+    }
     for (int s = 0; s < P.nstages; s++) P.stage_start[s + 1] += P.stage_start[s];
 }
 
@@ -603,7 +622,7 @@ bool plan_fill3_level(const Hier3Geom& H, int level, int nvar, Fill3Plan& P, std
             err = pl.err.empty() ? "3-D ghost-fill planning failed" : pl.err;
             return false;
         }
-    finish_plan(P);
+    finish_plan(P, nvar);
     return true;
 }
 
@@ -616,7 +635,7 @@ bool plan_saveqc3(const Hier3Geom& H, int level, int nvar, Fill3Plan& P, std::st
             err = pl.err.empty() ? "3-D saveqc planning failed" : pl.err;
             return false;
         }
-    finish_plan(P);
+    finish_plan(P, nvar);
     return true;
 }
 

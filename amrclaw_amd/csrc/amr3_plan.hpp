@@ -29,7 +29,8 @@ struct Fill3Plan {
     std::vector<Fill3Buf> bufs;
     std::vector<Fill3Op> ops;                // sorted by stage
     std::vector<unsigned char> masks;
-    std::vector<int> stage_start;            // ops of stage s: [stage_start[s], stage_start[s+1])
+    std::vector<Fill3Work> work;             // workgroup items, by stage
+    std::vector<int> stage_start;            // items of stage s: [stage_start[s], stage_start[s+1])
     int nstages = 0;
     int64_t scratch_elems = 0;
     uint32_t src_levels_mask = 0;            // levels read as time sources

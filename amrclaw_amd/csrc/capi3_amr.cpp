@@ -38,6 +38,7 @@ const double RINF = (double)10.e32f;   // amr_module's rinfinity
 
 struct DevFill {
     Fill3Plan plan;
+    Fill3Work* d_work = nullptr;
     Fill3Op* d_ops = nullptr;
     Fill3Buf* d_bufs = nullptr;
     unsigned char* d_masks = nullptr;
@@ -77,6 +78,7 @@ struct Amr3 {
     int64_t scratch_cap = 0;
     bool planned = false;        // ghost-fill programs
     bool planned_regs = false;   // saveqc programs and registers
+    int plane_block = 0;         // sweep plane block of the levels (0: per level)
 };
 
 Amr3 A;
@@ -95,6 +97,7 @@ void dfree(P*& p) {
 }
 
 void free_fill(DevFill& f) {
+    dfree(f.d_work);
     dfree(f.d_ops);
     dfree(f.d_bufs);
     dfree(f.d_masks);
@@ -148,6 +151,7 @@ int upload(T*& d, const std::vector<T>& h) {
 
 int upload_fill(DevFill& f) {
     int rc = upload(f.d_ops, f.plan.ops);
+    if (!rc) rc = upload(f.d_work, f.plan.work);
     if (!rc) rc = upload(f.d_bufs, f.plan.bufs);
     if (!rc) rc = upload(f.d_masks, f.plan.masks);
     if (!rc && f.plan.scratch_elems > A.scratch_cap) {
@@ -265,8 +269,8 @@ int run_fill(int level, double t, const DevFill& D) {
     int rc = fill_args(level, t, P, D, a);
     if (rc) return rc;
     for (int s = 0; s < P.nstages; s++) {
-        a.ops = D.d_ops + P.stage_start[s];
-        a.nops = P.stage_start[s + 1] - P.stage_start[s];
+        a.work = D.d_work + P.stage_start[s];
+        a.nwork = P.stage_start[s + 1] - P.stage_start[s];
         HIPCHKA(launch_fill3_stage((int)ctx3_esize(), a, ctx3_stream()));
     }
     return AMRGPU_OK;
@@ -443,7 +447,14 @@ int amrgpu3_set_level(int level, int ngrids, const int* boxes) {
     }
     L.q_elems = qo;
     L.face_elems = fo;
-    ctx3_plan_blocks(L.patches, ctx3_plane_block(), L.blocks);
+    // planes per sweep workgroup: a level of few grids needs small blocks to
+    // fill the chip (measured on the radial run configuration: FP32 best at
+    // 1, FP64 at 2); many grids keep the batch's block
+    int planes = 0;
+    for (const auto& p : L.patches) planes += p.mz;
+    int pb = A.plane_block;
+    if (pb <= 0) pb = planes >= 4096 ? ctx3_plane_block() : (ctx3_esize() == 4 ? 1 : 2);
+    ctx3_plan_blocks(L.patches, pb, L.blocks);
     const size_t es = ctx3_esize();
     rc = upload(L.d_patches, L.patches);
     for (int k = 0; k < 4 && !rc; k++) rc = upload(L.d_blocks[k], L.blocks[k]);
@@ -465,6 +476,13 @@ int amrgpu3_set_level(int level, int ngrids, const int* boxes) {
     HIPCHKA(hipStreamSynchronize(ctx3_stream()));
     A.planned = false;
     A.planned_regs = false;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_set_plane_block(int plane_block) {
+    if (!ctx3_inited()) return ctx3_fail(AMRGPU_ERR_STATE, "amrgpu3_init has not been called");
+    if (plane_block < 0) return ctx3_fail(AMRGPU_ERR_ARG, "plane_block must be >= 0");
+    A.plane_block = plane_block;
     return AMRGPU_OK;
 }
 

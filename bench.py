@@ -76,9 +76,11 @@ def parse():
                     help="skip the FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 disables it)")
-    ap.add_argument("--workload", choices=["euler2d", "euler3d"], default="euler2d",
-                    help="euler3d: the batched 3-D Euler step (512 grids) instead of "
-                         "the headline 2-D level update")
+    ap.add_argument("--workload", choices=["euler2d", "euler3d", "amr3d"], default="euler2d",
+                    help="euler3d: the batched 3-D Euler step (512 grids); amr3d: coarse "
+                         "Berger-Oliger steps of the 3-D radial run configuration (three "
+                         "levels, the whole 3-D level update); default: the headline 2-D "
+                         "level update")
     ap.add_argument("--plane-block", type=int, default=0,
                     help="3-D: planes per workgroup of a sweep (0: library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -375,6 +377,122 @@ def bench3(a):
     print(json.dumps(result))
 
 
+def bench_amr3(a):
+    """3-D AMR level update on the radial run configuration
+    (amrclaw_amd.workload.radial_euler3_hierarchy: 3-D Euler, 64^3 level 1 in
+    eight 32^3 grids, two finer levels at ratio 2, 8 + 17 + 38 grids).  One
+    step = one coarse Berger-Oliger step: advanc of level 1, two of level 2,
+    four of level 3 (each: ghost fill, saveqc, qad, the three sweeps and the
+    update, fluxsv, fluxad) and the three updates (upbnd, averaging); the
+    cell updates it counts are 262,144 + 2 x 69,632 + 4 x 125,640."""
+    import torch
+    from amrclaw_amd import capi3, workload
+
+    H = workload.radial_euler3_hierarchy(courant=0.3)
+    nlev = H.nlev
+    ncell = [int(sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1)
+                     for b in L["boxes"])) for L in H.levels]
+    cells = sum(c * 2 ** l for l, c in enumerate(ncell))
+    p3 = "fp64" if a.precision == "fp64" else "fp32"
+    prec = capi3.FP32 if p3 == "fp32" else capi3.FP64
+    esize = 4 if prec == capi3.FP32 else 8
+    cfl_seen = [0.0]
+
+    def coarse(level=1):
+        dtn, cfl = capi3.advance_level3(level)
+        cfl_seen[0] = max(cfl_seen[0], cfl)
+        if level < nlev:
+            for _ in range(2):
+                coarse(level + 1)
+            capi3.update_level3(level)
+
+    def run(steps, warmup):
+        capi3.init3(capi3.make_config3(capi3.EULER, precision=prec))
+        if a.plane_block:
+            capi3.set_plane_block3(a.plane_block)
+        capi3.load_hierarchy3(H)
+        for _ in range(warmup):
+            coarse()
+        capi3.synchronize3()
+        torch.cuda.synchronize()
+        capi3.set_timing3(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            coarse()
+        capi3.synchronize3()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kt = [capi3.kernel_time3(k) for k in (capi3.KT_X, capi3.KT_Y, capi3.KT_Z,
+                                               capi3.KT_UPDATE)]
+        capi3.set_timing3(False)
+        mass = capi3.conck_level3(1)
+        capi3.finalize3()
+        return wall, kt, mass
+
+    print("bench_amr3: timed run", p3, file=sys.stderr, flush=True)
+    wall, kt, mass = run(a.steps, a.warmup)
+    names = ("x_sweep", "y_sweep", "z_sweep", "update")
+    kms = {nm: t[0] / a.steps for nm, t in zip(names, kt)}
+    result = {
+        "metric": METRIC,
+        "value": cells * a.steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == capi3.FP32 else "f64", "data": "synthetic",
+        "config": {"workload": "3-D AMR level update, radial Euler run configuration: "
+                               "64^3 level 1 (8 grids), levels 2-3 at ratio 2 (17 and 38 "
+                               "grids), walls below / extrapolation above, Roe+efix, MC, "
+                               "method(3)=22; one step = one coarse Berger-Oliger step",
+                   "grids_per_level": [len(L["boxes"]) for L in H.levels],
+                   "cells_per_level": ncell, "cell_updates_per_step": cells,
+                   "precision": p3, "max_cfl": cfl_seen[0]},
+        "kernels_ms_per_step": kms,
+        "level1_mass_after": mass,
+    }
+    step_kern = sum(kms.values())
+    bpc = 2 * 5 * esize
+    achieved = cells * bpc / (step_kern / 1e3) / 1e9
+    result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                          "kernel": "sweep3_phase_kernel (x+y+z) + update3_kernel over the "
+                                    "seven level steps",
+                          "kernel_ms": step_kern, "bytes_per_cell_update": bpc,
+                          "algorithmic_bytes_per_step": cells * bpc,
+                          "non_sweep_ms": wall / a.steps * 1e3 - step_kern}
+    if a.cpu_seconds > 0:
+        print("bench_amr3: CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline_amr3(H, cells, a.cpu_seconds))
+    print(json.dumps(result))
+
+
+def cpu_baseline_amr3(H, cells, budget):
+    """The reference's own 3-D level update (bound, saveqc, qad, stepgrid,
+    fluxsv, fluxad, upbnd, update of src/3d, compiled from its Fortran by
+    oracle/Makefile; one thread) over whole coarse Berger-Oliger steps of
+    the same hierarchy, for about `budget` seconds (at least one step)."""
+    from oracle import orc
+    p = orc.make_params3(orc.EULER)
+    kind = "reference" if orc.ref3_available() else "port"
+    impl = orc.RefAMR3(p, H.copy()) if kind == "reference" else orc.OracleAMR3(p, H.copy())
+
+    def coarse(level=1):
+        impl.advanc(level)
+        if level < H.nlev:
+            for _ in range(2):
+                coarse(level + 1)
+            impl.update(level)
+
+    n = 0
+    t0 = time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < budget:
+        coarse()
+        n += 1
+    wall = time.perf_counter() - t0
+    return {"value": cells * n / wall, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": "%d coarse Berger-Oliger steps of the same hierarchy (%d cell "
+                      "updates each) in %.1f s" % (n, cells, wall)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -386,6 +504,9 @@ def main():
         return
     if a.workload == "euler3d":
         bench3(a)
+        return
+    if a.workload == "amr3d":
+        bench_amr3(a)
         return
     import torch
     from amrclaw_amd import capi, workload

@@ -121,6 +121,11 @@ int amrgpu3_set_geometry(int nlev, const int* iregsz, const int* jregsz, const i
 /* The grids of a level: boxes[6g..6g+5] = node(ndilo, ndihi, ndjlo, ndjhi,
  * ndklo, ndkhi); allocates their storage (new and old) on the device. */
 int amrgpu3_set_level(int level, int ngrids, const int* boxes);
+/* Planes per workgroup of the levels' sweeps, applied by later
+ * amrgpu3_set_level calls (no reference counterpart: a launch shape).  0
+ * (default): per level, 1 (FP32) or 2 (FP64) for levels of few grids, the
+ * batch default for levels of 4096 planes or more. */
+int amrgpu3_set_plane_block(int plane_block);
 /* rnode(timemult) of the level's grids and possk(level). */
 int amrgpu3_set_time(int level, double time, double possk);
 int amrgpu3_get_time(int level, double* time);
