@@ -184,45 +184,42 @@ def max_wave_speed3(q, gamma=GAMMA):
 
 
 # ---------------------------------------------------------------- 3-D AMR --
-# The 3-D Euler radial run configuration as a static three-level hierarchy:
-# the radial bump of rho = 1 + 10 exp(-20 r^2) (r from the corner at the
-# origin, u = v = w = 0, E = rho) on [0, 2]^3, walls at the lower sides and
-# extrapolation at the upper sides, level 1 64^3 in eight 32^3 grids, two
-# finer levels at ratio 2 covering the cells whose centred pressure
-# difference exceeds 0.05 (buffered by 3 cells) with tiles of 8^3 coarse
-# cells, each nested one coarse cell inside a single grid of the level below
-# except at the domain edge.  Time step: CFL `courant` of the initial state on
-# level 1, halved per level.
+# The 3-D Euler run configuration as a static three-level hierarchy: the
+# radial bump of bump3_state (rho = p = 2 inside r < 0.2 about the centre of
+# the unit cube, 1 outside, at rest), extrapolation at every side, level 1
+# 64^3 in eight 32^3 grids, two finer levels at ratio 2 covering the cells
+# flag2refine's test flags (largest centred difference of any component
+# above `tol`), buffered by 3 cells, with tiles of 8^3 coarse cells, each
+# nested one coarse cell inside a single grid of the level below except at
+# the domain edge.  The coarse step is t_end / nsteps (t_end = 0.2: initial
+# Courant number 0.58 on level 1), halved per level.
 
-def radial3_state(x, y, z):
-    r = np.sqrt(x * x + y * y + z * z)
-    rho = 1.0 + 10.0 * np.exp(-20.0 * r * r)
-    zero = np.zeros_like(rho)
-    return np.stack([rho, zero, zero, zero, rho], -1)
+C5_T_END = 0.2
+C5_STEPS = 26
 
 
-def _cells3(box, h, lo, ng):
+def _cells3(box, h, ng):
     i = np.arange(box[0] - ng, box[1] + ng + 1)
     j = np.arange(box[2] - ng, box[3] + ng + 1)
     k = np.arange(box[4] - ng, box[5] + ng + 1)
-    return np.meshgrid(lo[2] + (k + 0.5) * h[2], lo[1] + (j + 0.5) * h[1], lo[0] + (i + 0.5) * h[0],
-                       indexing="ij")
+    return np.meshgrid((k + 0.5) * h, (j + 0.5) * h, (i + 0.5) * h, indexing="ij")
 
 
-def _pflags3(box, h, lo, tol):
-    Z, Y, X = _cells3(box, h, lo, 1)
-    q = radial3_state(X, Y, Z)
-    p = (GAMMA - 1.0) * (q[..., 4] - 0.5 * (q[..., 1] ** 2 + q[..., 2] ** 2 + q[..., 3] ** 2) / q[..., 0])
-    d = np.maximum.reduce([np.abs(p[1:-1, 1:-1, 2:] - p[1:-1, 1:-1, :-2]),
-                           np.abs(p[1:-1, 2:, 1:-1] - p[1:-1, :-2, 1:-1]),
-                           np.abs(p[2:, 1:-1, 1:-1] - p[:-2, 1:-1, 1:-1])])
-    return d > tol
+def _flags3(box, h, tol):
+    """flag2refine.f's test on the interior of `box`: any component's
+    centred difference in any direction above tol."""
+    Z, Y, X = _cells3(box, h, 1)
+    q = bump3_state(X, Y, Z)
+    d = np.maximum.reduce([np.abs(q[1:-1, 1:-1, 2:] - q[1:-1, 1:-1, :-2]),
+                           np.abs(q[1:-1, 2:, 1:-1] - q[1:-1, :-2, 1:-1]),
+                           np.abs(q[2:, 1:-1, 1:-1] - q[:-2, 1:-1, 1:-1])])
+    return d.max(axis=-1) > tol
 
 
-def radial_euler3_hierarchy(n=64, nlev=3, tile=8, ibuff=3, tol=0.05, split=2, courant=0.5):
-    lo = (0.0, 0.0, 0.0)
-    H = Hier3(5, domain=(0.0, 2.0, 0.0, 2.0, 0.0, 2.0), mthbc=(3, 1, 3, 1, 3, 1))
-    h = 2.0 / n
+def radial_euler3_hierarchy(n=64, nlev=3, tile=8, ibuff=3, tol=0.1, split=2,
+                            t_end=C5_T_END, nsteps=C5_STEPS):
+    H = Hier3(5, domain=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), mthbc=(1,) * 6)
+    h = 1.0 / n
     s = n // split
     boxes = [(a, a + s - 1, b, b + s - 1, c, c + s - 1)
              for c in range(0, n, s) for b in range(0, n, s) for a in range(0, n, s)]
@@ -231,7 +228,7 @@ def radial_euler3_hierarchy(n=64, nlev=3, tile=8, ibuff=3, tol=0.05, split=2, co
         boxes, h, sz = levels[-1]
         fine = []
         for b in boxes:
-            f = _pflags3(b, (h, h, h), lo, tol)
+            f = _flags3(b, h, tol)
             if not f.any():
                 continue
             fb = np.zeros_like(f)
@@ -260,17 +257,15 @@ def radial_euler3_hierarchy(n=64, nlev=3, tile=8, ibuff=3, tol=0.05, split=2, co
                             fine.append((2 * ti, 2 * hi[0] + 1, 2 * tj, 2 * hi[1] + 1,
                                          2 * tk, 2 * hi[2] + 1))
         levels.append((fine, h / 2, sz * 2))
-    # CFL 0.5 of the initial state (zero velocity: the sound speed)
-    cmax = float(np.sqrt(GAMMA * 0.4 * 11.0 / 11.0))
-    dt = courant * levels[0][1] / cmax
+    dt = t_end / nsteps
     for l, (boxes, hl, sz) in enumerate(levels, start=1):
         H.add_level(boxes, (hl, hl, hl), dt, 0.0, (sz, sz, sz), (2, 2, 2))
         dt = dt / 2
     for l in range(1, H.nlev + 1):
         L = H.levels[l - 1]
         for k, b in enumerate(L["boxes"]):
-            Z, Y, X = _cells3(b, (L["hx"],) * 3, lo, 2)
-            q = radial3_state(X, Y, Z)
+            Z, Y, X = _cells3(b, L["hx"], 2)
+            q = bump3_state(X, Y, Z)
             H.grid(l, k, 0)[:] = q
             H.grid(l, k, 1)[:] = q
     return H

@@ -378,18 +378,22 @@ def bench3(a):
 
 
 def bench_amr3(a):
-    """3-D AMR level update on the radial run configuration
-    (amrclaw_amd.workload.radial_euler3_hierarchy: 3-D Euler, 64^3 level 1 in
-    eight 32^3 grids, two finer levels at ratio 2, 8 + 17 + 38 grids).  One
-    step = one coarse Berger-Oliger step: advanc of level 1, two of level 2,
-    four of level 3 (each: ghost fill, saveqc, qad, the three sweeps and the
-    update, fluxsv, fluxad) and the three updates (upbnd, averaging); the
-    cell updates it counts are 262,144 + 2 x 69,632 + 4 x 125,640."""
+    """3-D AMR level update on the 3-D Euler run configuration
+    (amrclaw_amd.workload.radial_euler3_hierarchy: rho = p = 2 inside
+    r < 0.2 about the centre of the unit cube, 1 outside, extrapolation
+    BCs, 64^3 level 1 in eight 32^3 grids, two finer levels at ratio 2).
+    The timed run goes from t = 0 to 0.2 in 26 coarse Berger-Oliger steps
+    (advanc of level 1, 2 of level 2, 4 of level 3, each: ghost fill,
+    saveqc, qad, the three sweeps and the update, fluxsv, fluxad; and the
+    updates); --warmup coarse steps run first on a separate copy.  Reports
+    the FP32 path's relative L1 error against FP64 at t = 0.2 and the
+    level-1 mass drift (conck) of both."""
     import torch
     from amrclaw_amd import capi3, workload
 
-    H = workload.radial_euler3_hierarchy(courant=0.3)
+    H = workload.radial_euler3_hierarchy()
     nlev = H.nlev
+    steps = workload.C5_STEPS
     ncell = [int(sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1)
                      for b in L["boxes"])) for L in H.levels]
     cells = sum(c * 2 ** l for l, c in enumerate(ncell))
@@ -406,13 +410,22 @@ def bench_amr3(a):
                 coarse(level + 1)
             capi3.update_level3(level)
 
-    def run(steps, warmup):
-        capi3.init3(capi3.make_config3(capi3.EULER, precision=prec))
+    def load(precision):
+        capi3.init3(capi3.make_config3(capi3.EULER, precision=precision))
         if a.plane_block:
             capi3.set_plane_block3(a.plane_block)
         capi3.load_hierarchy3(H)
-        for _ in range(warmup):
-            coarse()
+        for l in range(nlev - 1, 0, -1):      # setgrd: coarse data averaged from the finer
+            capi3.update_level3(l)
+
+    def run(precision, timed):
+        if timed and a.warmup:
+            load(precision)
+            for _ in range(a.warmup):
+                coarse()
+            capi3.finalize3()
+        load(precision)
+        m0 = capi3.conck_level3(1)
         capi3.synchronize3()
         torch.cuda.synchronize()
         capi3.set_timing3(True)
@@ -425,29 +438,31 @@ def bench_amr3(a):
         kt = [capi3.kernel_time3(k) for k in (capi3.KT_X, capi3.KT_Y, capi3.KT_Z,
                                                capi3.KT_UPDATE)]
         capi3.set_timing3(False)
-        mass = capi3.conck_level3(1)
+        m1 = capi3.conck_level3(1)
+        levels = [capi3.read_level3(H, l, 0) for l in range(1, nlev + 1)]
         capi3.finalize3()
-        return wall, kt, mass
+        return wall, kt, (m1 - m0) / m0, levels
 
     print("bench_amr3: timed run", p3, file=sys.stderr, flush=True)
-    wall, kt, mass = run(a.steps, a.warmup)
+    wall, kt, drift, lv = run(prec, True)
     names = ("x_sweep", "y_sweep", "z_sweep", "update")
-    kms = {nm: t[0] / a.steps for nm, t in zip(names, kt)}
+    kms = {nm: t[0] / steps for nm, t in zip(names, kt)}
     result = {
         "metric": METRIC,
-        "value": cells * a.steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "value": cells * steps / wall, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": a.warmup, "ms_per_step": wall / steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if prec == capi3.FP32 else "f64", "data": "synthetic",
-        "config": {"workload": "3-D AMR level update, radial Euler run configuration: "
-                               "64^3 level 1 (8 grids), levels 2-3 at ratio 2 (17 and 38 "
-                               "grids), walls below / extrapolation above, Roe+efix, MC, "
-                               "method(3)=22; one step = one coarse Berger-Oliger step",
+        "config": {"workload": "3-D AMR level update, 3-D Euler run configuration: radial "
+                               "bump rho=p=2 for r<0.2 about the centre of [0,1]^3, "
+                               "extrapolation BCs, 64^3 level 1 (8 grids of 32^3), levels 2-3 "
+                               "at ratio 2, Roe+efix, MC, method(3)=22, t = 0 to 0.2 in 26 "
+                               "coarse Berger-Oliger steps; one step = one coarse step",
                    "grids_per_level": [len(L["boxes"]) for L in H.levels],
                    "cells_per_level": ncell, "cell_updates_per_step": cells,
                    "precision": p3, "max_cfl": cfl_seen[0]},
         "kernels_ms_per_step": kms,
-        "level1_mass_after": mass,
+        "level1_mass_drift": drift,
     }
     step_kern = sum(kms.values())
     bpc = 2 * 5 * esize
@@ -458,7 +473,18 @@ def bench_amr3(a):
                                     "seven level steps",
                           "kernel_ms": step_kern, "bytes_per_cell_update": bpc,
                           "algorithmic_bytes_per_step": cells * bpc,
-                          "non_sweep_ms": wall / a.steps * 1e3 - step_kern}
+                          "non_sweep_ms": wall / steps * 1e3 - step_kern}
+    if not a.no_compare and prec == capi3.FP32:
+        print("bench_amr3: FP64 run", file=sys.stderr, flush=True)
+        wall64, _, drift64, lv64 = run(capi3.FP64, False)
+        rel = [float(np.abs(x - y).sum() / np.abs(y).sum()) for x, y in zip(lv, lv64)]
+        result["fp64"] = {"value": cells * steps / wall64, "ms_per_step": wall64 / steps * 1e3,
+                          "level1_mass_drift": drift64}
+        result["speedup_vs_fp64"] = wall64 / wall
+        result["l1_rel_vs_fp64_per_level"] = rel
+        # the outflow through the extrapolation boundaries is the same in
+        # both; their difference is the FP32 path's conservation error
+        result["mass_drift_minus_fp64"] = drift - drift64
     if a.cpu_seconds > 0:
         print("bench_amr3: CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline_amr3(H, cells, a.cpu_seconds))

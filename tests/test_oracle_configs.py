@@ -70,19 +70,22 @@ needs_ref3 = pytest.mark.skipif(not orc.ref3_available(), reason="3-D reference 
 
 @needs_ref3
 def test_config_c5_level_update_matches_reference():
-    """C5 (3-D Euler radial, 64^3 level 1 in eight 32^3 grids, two levels at
-    ratio 2, walls below and extrapolation above): one coarse step of the
-    oracle bit for bit with the reference's src/3d Fortran."""
+    """C5 (3-D Euler, radial bump rho = p = 2 for r < 0.2 about the centre
+    of the unit cube, extrapolation BCs, 64^3 level 1 in eight 32^3 grids,
+    two levels at ratio 2): the first level steps of a coarse step (level 1,
+    level 2, two of level 3, the level-2 update) of the oracle bit for bit
+    with the reference's src/3d Fortran."""
     from tests.common3 import berger_oliger3
     from tests.configs3 import config_hierarchy3
     H = config_hierarchy3()
-    assert [len(L["boxes"]) for L in H.levels] == [8, 17, 38]
+    assert [len(L["boxes"]) for L in H.levels] == [8, 76, 263]
     p = orc.make_params3(orc.EULER)
     O = orc.OracleAMR3(p, H.copy())
     R = orc.RefAMR3(p, H)
     lo, lr = [], []
-    berger_oliger3(O, O.H, 1, lo)
-    berger_oliger3(R, H, 1, lr)
+    for impl, log in ((O, lo), (R, lr)):
+        log.append(impl.advanc(1))
+        berger_oliger3(impl, H, 2, log)
     assert lo == lr
     for l in range(1, H.nlev + 1):
         for k in range(len(H.levels[l - 1]["boxes"])):
