@@ -76,10 +76,12 @@ def parse():
                     help="skip the FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 disables it)")
-    ap.add_argument("--workload", choices=["euler2d", "euler3d", "amr3d"], default="euler2d",
+    ap.add_argument("--workload", choices=["euler2d", "euler3d", "amr3d", "C1", "C2", "C3"],
+                    default="euler2d",
                     help="euler3d: the batched 3-D Euler step (512 grids); amr3d: coarse "
                          "Berger-Oliger steps of the 3-D radial run configuration (three "
-                         "levels, the whole 3-D level update); default: the headline 2-D "
+                         "levels, the whole 3-D level update); C1, C2, C3: the 2-D run "
+                         "configurations to their end times; default: the headline 2-D "
                          "level update")
     ap.add_argument("--plane-block", type=int, default=0,
                     help="3-D: planes per workgroup of a sweep (0: library default)")
@@ -203,11 +205,12 @@ def _quiet_stdout(fn):
         os.close(saved)
 
 
-def cpu_baseline(H, dt2, budget, nsample=256):
+def cpu_baseline(H, dt2, budget, nsample=256, parallel=True):
     """The reference's advanc (its own Fortran, OpenMP over grids as
-    advanc.f:60-129) on level 2 of a sample of the hierarchy: level 1 whole,
-    the first nsample level-2 patches; same state and times as the GPU
-    step.  Falls back to the oracle's C port (serial) without the build."""
+    advanc.f:60-129, or serial with parallel=False) on level 2 of a sample of
+    the hierarchy: level 1 whole, the first nsample level-2 patches; same
+    state and times as the GPU step.  Falls back to the oracle's C port
+    (serial) without the build."""
     from oracle import orc
     from amrclaw_amd.hierarchy import Hier
     p = orc.make_params(orc.EULER, order=2, transverse=2, limiter=4)
@@ -221,11 +224,11 @@ def cpu_baseline(H, dt2, budget, nsample=256):
             S.grid(l, k, 1)[:] = H.grid(l, k, 1)
     b = S.levels[1]["boxes"]
     cells = int(((b[:, 1] - b[:, 0] + 1) * (b[:, 3] - b[:, 2] + 1)).sum())
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)) if parallel else 1
     if orc.ref_available():
         kind = "reference"
         A = orc.RefAMR(p, S)
-        A.set_parallel(True)
+        A.set_parallel(parallel)
         A.advanc(1)                           # level 1: 0 -> 2 dt2, saveqc(2)
         reset = lambda: A.set_time(2, dt2)
     else:
@@ -519,6 +522,114 @@ def cpu_baseline_amr3(H, cells, budget):
                       "updates each) in %.1f s" % (n, cells, wall)}
 
 
+def bench_config(a):
+    """A 2-D run configuration (amrclaw_amd.workload.config_hierarchy: C1
+    advection, periodic, 50^2 + two levels at ratio 2; C2 acoustics, 50^2 +
+    ratios 2, 4; C3 Euler quadrants, 80^2 in four grids + ratios 2, 4) run
+    from t = 0 to its end time (1.0, 0.6, 0.8) in coarse Berger-Oliger steps
+    of fixed size on static layouts.  FP64 (the configurations' precision)
+    unless --precision says otherwise.  One step = one coarse step."""
+    import torch
+    from amrclaw_amd import capi, workload
+
+    name = a.workload
+    c = workload.CONFIGS[name]
+    H = workload.config_hierarchy(name)
+    nlev = H.nlev
+    dt1 = H.levels[0]["possk"]
+    steps = int(round(c["t_end"] / dt1))
+    ncell = [int(((L["boxes"][:, 1] - L["boxes"][:, 0] + 1) *
+                  (L["boxes"][:, 3] - L["boxes"][:, 2] + 1)).sum()) for L in H.levels]
+    sub = [1]
+    for L in H.levels[:-1]:
+        sub.append(sub[-1] * L["ratx"])
+    cells = sum(n * r for n, r in zip(ncell, sub))
+    pname = a.precision if a.precision != "mixed" or "--precision" in sys.argv else "fp64"
+    prec = {"mixed": capi.MIXED, "fp32": capi.FP32, "fp64": capi.FP64}[pname]
+
+    def coarse(level=1):
+        capi.advance_level(level)
+        if level < nlev:
+            for _ in range(H.levels[level - 1]["ratx"]):
+                coarse(level + 1)
+            capi.update_level(level)
+
+    def load(precision):
+        capi.init(capi.make_config(c["solver"], mthbc=c["mthbc"], domain=c["domain"],
+                                   precision=precision))
+        if a.graphs:
+            capi.set_graphs(True)
+        capi.load_hierarchy(H)
+
+    if a.warmup:
+        load(prec)
+        for _ in range(a.warmup):
+            coarse()
+        capi.finalize()
+    load(prec)
+    capi.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        coarse()
+    capi.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    capi.finalize()
+    result = {
+        "metric": METRIC, "value": cells * steps / wall, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32"}[pname], "data": "synthetic",
+        "config": {"workload": "2-D run configuration %s to t = %g on static layouts "
+                               "(levels %s, ratios %s)" % (name, c["t_end"],
+                                                          [len(L["boxes"]) for L in H.levels],
+                                                          list(c["ratios"])),
+                   "cells_per_level": ncell, "cell_updates_per_step": cells,
+                   "precision": pname, "graphs": bool(a.graphs)},
+    }
+    if a.cpu_seconds > 0:
+        print("bench_config: CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline_config(name, H, cells,
+                                                                           a.cpu_seconds))
+    print(json.dumps(result))
+
+
+def cpu_baseline_config(name, H, cells, budget):
+    """The reference's own advanc/update (src/2d Fortran, OpenMP over grids)
+    over coarse Berger-Oliger steps of the same configuration, about
+    `budget` seconds."""
+    from oracle import orc
+    from amrclaw_amd import workload
+    c = workload.CONFIGS[name]
+    p = orc.make_params(c["solver"])
+    kind = "reference" if orc.ref_available() else "port"
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if kind == "reference":
+        impl = orc.RefAMR(p, H.copy())
+        impl.set_parallel(True)
+    else:
+        impl, threads = orc.OracleAMR(p, H.copy()), 1
+
+    def coarse(level=1):
+        impl.advanc(level)
+        if level < H.nlev:
+            for _ in range(H.levels[level - 1]["ratx"]):
+                coarse(level + 1)
+            impl.update(level)
+
+    n = 0
+    t0 = time.perf_counter()
+    while n == 0 or time.perf_counter() - t0 < budget:
+        coarse()
+        n += 1
+    wall = time.perf_counter() - t0
+    if kind == "reference":
+        impl.flush()
+    return {"value": cells * n / wall, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+            "sample": "%d coarse steps of %s (%d cell updates each) in %.1f s" % (n, name, cells, wall)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -533,6 +644,9 @@ def main():
         return
     if a.workload == "amr3d":
         bench_amr3(a)
+        return
+    if a.workload in ("C1", "C2", "C3"):
+        bench_config(a)
         return
     import torch
     from amrclaw_amd import capi, workload
@@ -658,6 +772,9 @@ def main():
     if a.cpu_seconds > 0:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline(H, dt2, a.cpu_seconds))
+        # the same sample on one core (the reference's OpenMP loops serial)
+        result["cpu_baseline_1thread"] = _quiet_stdout(
+            lambda: cpu_baseline(H, dt2, a.cpu_seconds / 2, nsample=64, parallel=False))
     print(json.dumps(result))
 
 
