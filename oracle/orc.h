@@ -241,6 +241,13 @@ int orc_advanc3(const orc_params* p, orc_hier3* h, orc_regs3* r, int level, doub
                 double* cfl_level);
 int orc_update3(const orc_params* p, const orc_hier3* h, orc_regs3* r, int level);
 double orc_conck3(const orc_params* p, const orc_hier3* h, int level);
+int orc_bound3_ng(const orc_params* p, const orc_hier3* h, int level, int grid, double time,
+                  double* valbig, int ng);
+/* spest.f / errest.f + errf1.f (3-D) for one grid (orc_amr3.c) */
+int orc_spest3(const orc_params* p, const orc_hier3* h, int level, int grid, double tolsp,
+               double* big, double* flags, int* nflag);
+int orc_errest3(const orc_params* p, const orc_hier3* h, int level, int grid, double tol,
+                int iorder, double* big, double* rflags);
 
 /* cell flagging (orc_flag.c) */
 #define ORC_UNSET (-1.0)

@@ -897,3 +897,63 @@ class RefAMR3:
         first = 1 + sum(len(L["boxes"]) for L in self.H.levels[:level])
         out[:, 4] -= first
         return out
+
+
+def flag_level3(O, level, tolsp, tol=0.0, iorder=2, richardson=False):
+    """spest.f (3-D) and, with richardson, errest.f + errf1.f for every grid
+    of `level` of an OracleAMR3: returns (flags, rflags, nflag) with one
+    (mz+4, my+4, mx+4) array per grid (goodpt 0 / badpt 2); rflags is None
+    without richardson."""
+    L = lib()
+    P, Hh = C.POINTER(OrcParams), C.POINTER(OrcHier3)
+    L.orc_spest3.argtypes = [P, Hh, C.c_int, C.c_int, C.c_double, _dp, _dp, C.POINTER(C.c_int)]
+    L.orc_errest3.argtypes = [P, Hh, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp]
+    H = O.H
+    flags, rflags, total = [], [], 0
+    for k in range(len(H.levels[level - 1]["boxes"])):
+        shape = H.levels[level - 1]["shapes"][k]
+        big = np.zeros(shape)
+        f = np.zeros(shape[:3])
+        n = C.c_int()
+        rc = L.orc_spest3(C.byref(O.p), C.byref(O.h), level, k, tolsp, big, f, C.byref(n))
+        if rc:
+            raise RuntimeError("orc_spest3 error %d" % rc)
+        flags.append(f)
+        total += n.value
+        if richardson:
+            r = np.zeros(shape[:3])
+            rc = L.orc_errest3(C.byref(O.p), C.byref(O.h), level, k, tol, iorder, big, r)
+            if rc:
+                raise RuntimeError("orc_errest3 error %d" % rc)
+            rflags.append(r)
+    return flags, (rflags if richardson else None), total
+
+
+def ref3_flag_level(R, level, tolsp, tol=0.0, iorder=2, richardson=False):
+    """The reference's 3-D flagging of `level` on a RefAMR3's hierarchy:
+    (dom1flags over the level's index space with a one-cell border,
+    per-grid errf1 flags or None)."""
+    H = R.H
+    Lv = H.levels[level - 1]
+    isz, jsz, ksz = Lv["iregsz"], Lv["jregsz"], Lv["kregsz"]
+    dom = np.zeros((ksz + 2, jsz + 2, isz + 2), dtype=np.int8)
+    sizes = [int(np.prod(s[:3])) for s in Lv["shapes"]]
+    rf = np.zeros(max(sum(sizes), 1))
+    R.R.ref3_flag_level.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS"), _dp]
+    R.R.ref3_flag_level(level, H.nvar, tolsp, tol, iorder, int(bool(richardson)), dom, rf)
+    rflags = None
+    if richardson:
+        offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(int)
+        rflags = [rf[o:o + n].reshape(s[:3]) for o, n, s in zip(offs, sizes, Lv["shapes"])]
+    return dom, rflags
+
+
+def domain_flags3(H, level, flags):
+    """setflags.f (3-D): the grids' interior flags summed into the level's
+    dom1flags(0:isize+1, 0:jsize+1, 0:ksize+1)."""
+    Lv = H.levels[level - 1]
+    dom = np.zeros((Lv["kregsz"] + 2, Lv["jregsz"] + 2, Lv["iregsz"] + 2), dtype=np.int8)
+    for b, f in zip(Lv["boxes"], flags):
+        dom[b[4] + 1:b[5] + 2, b[2] + 1:b[3] + 2, b[0] + 1:b[1] + 2] += f[2:-2, 2:-2, 2:-2].astype(np.int8)
+    return dom

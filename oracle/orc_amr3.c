@@ -461,7 +461,13 @@ static int prefilrecur3(const orc_params* p, const orc_hier3* h, int level, doub
  * interior k), bottom, top (all i and j), then bc3amr on the whole grid */
 int orc_bound3(const orc_params* p, const orc_hier3* h, int level, int grid, double time,
                double* valbig) {
-    const int ng = p->mbc;
+    return orc_bound3_ng(p, h, level, grid, time, valbig, p->mbc);
+}
+
+/* bound.f (3-D) with a ghost ring of width ng around the grid (the
+ * storage keeps p->mbc = nghost; errest.f calls bound with 2 nghost) */
+int orc_bound3_ng(const orc_params* p, const orc_hier3* h, int level, int grid, double time,
+                  double* valbig, int ng) {
     if (partial_periodic(h)) return ORC_E_PERIODIC;
     const int* b = h->boxes[level] + 6 * grid;
     const int ilo = b[0], ihi = b[1], jlo = b[2], jhi = b[3], klo = b[4], khi = b[5];
@@ -1138,4 +1144,110 @@ double orc_conck3(const orc_params* p, const orc_hier3* h, int level) {
                     totmass = totmass + V3(q, nv, mitot, mjtot, 1, i, j, k);
     }
     return totmass * h->hx[level] * h->hy[level] * h->hz[level];
+}
+
+/* ===================================================== flagging ======== */
+
+/* spest.f (3-D) for one grid: big = the grid's new values with ghost cells
+ * filled at its time (bound), flags(mitot, mjtot, mktot) = goodpt (0), then
+ * flag2refine.f's badpt (2) / goodpt on the interior; returns the number
+ * of flagged cells in *nflag */
+int orc_spest3(const orc_params* p, const orc_hier3* h, int level, int grid, double tolsp,
+               double* big, double* flags, int* nflag) {
+    const int nv = p->meqn, ng = p->mbc;
+    const int* b = h->boxes[level] + 6 * grid;
+    const int mx = b[1] - b[0] + 1, my = b[3] - b[2] + 1, mz = b[5] - b[4] + 1;
+    const size_t n = (size_t)(mx + 2 * ng) * (my + 2 * ng) * (mz + 2 * ng);
+    memcpy(big, h->qnew[level] + h->qoff[level][grid], n * nv * sizeof(double));
+    int rc = orc_bound3(p, h, level, grid, h->time[level], big);
+    if (rc) return rc;
+    for (size_t c = 0; c < n; c++) flags[c] = 0.0;
+    int k = orc_flag2refine3(mx, my, mz, ng, nv, tolsp, big, 0.0, 2.0, flags);
+    if (nflag) *nflag = k;
+    return 0;
+}
+
+/* errest.f (3-D) with coarsen.f, copysol.f and errf1.f for one grid:
+ * the old values with 2 nghost ghost cells filled at t - possk, coarsened
+ * by 2 and stepped by 2 possk on the coarsened grid; `big` (spest's array)
+ * stepped by possk on the grid; their difference flags 2x2x2 blocks whose
+ * Richardson estimate reaches tol.  rflags(mitot, mjtot, mktot) receives
+ * errf1's rctflg component 1 (goodpt 0 / badpt 2), which the reference
+ * leaves in the grid's store2 */
+int orc_errest3(const orc_params* p, const orc_hier3* h, int level, int grid, double tol,
+                int iorder, double* big, double* rflags) {
+    const int nv = p->meqn, ng = p->mbc, ngbig = 2 * p->mbc;
+    const int* b = h->boxes[level] + 6 * grid;
+    const int nx = b[1] - b[0] + 1, ny = b[3] - b[2] + 1, nz = b[5] - b[4] + 1;
+    if (nx % 2 || ny % 2 || nz % 2) return ORC_E_SCRATCH;
+    const int mitot = nx + 2 * ng, mjtot = ny + 2 * ng, mktot = nz + 2 * ng;
+    const int mi2 = nx / 2 + 2 * ng, mj2 = ny / 2 + 2 * ng, mk2 = nz / 2 + 2 * ng;
+    const int midub = nx + 2 * ngbig, mjdub = ny + 2 * ngbig, mkdub = nz + 2 * ngbig;
+    const double hx = h->hx[level], hy = h->hy[level], hz = h->hz[level];
+    const double dt = h->possk[level], time = h->time[level], tpre = time - dt;
+    double* dub = calloc((size_t)midub * mjdub * mkdub * nv, sizeof(double));
+    double* crs = calloc((size_t)mi2 * mj2 * mk2 * nv, sizeof(double));
+    const size_t nf = (size_t)mitot * mjtot * mktot * nv, nc = (size_t)mi2 * mj2 * mk2 * nv;
+    double* f = calloc(6 * (nf > nc ? nf : nc), sizeof(double));
+    if (!dub || !crs || !f) { free(dub); free(crs); free(f); return ORC_E_ALLOC; }
+    /* copysol: old interior into the centre of the wider array */
+    const double* old = h->qold[level] + h->qoff[level][grid];
+    for (int k = ng + 1; k <= mktot - ng; k++)
+        for (int j = ng + 1; j <= mjtot - ng; j++)
+            for (int i = ng + 1; i <= mitot - ng; i++)
+                for (int m = 1; m <= nv; m++)
+                    V3(dub, nv, midub, mjdub, m, i - ng + ngbig, j - ng + ngbig, k - ng + ngbig) =
+                        V3(old, nv, mitot, mjtot, m, i, j, k);
+    int rc = orc_bound3_ng(p, h, level, grid, tpre, dub, ngbig);
+    if (rc) { free(dub); free(crs); free(f); return rc; }
+    /* coarsen.f */
+    for (int m = 1; m <= nv; m++)
+        for (int k = 1; k <= mk2; k++)
+            for (int j = 1; j <= mj2; j++)
+                for (int i = 1; i <= mi2; i++) {
+                    int fi = 2 * (i - 1) + 1, fj = 2 * (j - 1) + 1, fk = 2 * (k - 1) + 1;
+                    V3(crs, nv, mi2, mj2, m, i, j, k) =
+                        (V3(dub, nv, midub, mjdub, m, fi, fj, fk) + V3(dub, nv, midub, mjdub, m, fi + 1, fj, fk) +
+                         V3(dub, nv, midub, mjdub, m, fi, fj + 1, fk) + V3(dub, nv, midub, mjdub, m, fi + 1, fj + 1, fk) +
+                         V3(dub, nv, midub, mjdub, m, fi, fj, fk + 1) + V3(dub, nv, midub, mjdub, m, fi + 1, fj, fk + 1) +
+                         V3(dub, nv, midub, mjdub, m, fi, fj + 1, fk + 1) + V3(dub, nv, midub, mjdub, m, fi + 1, fj + 1, fk + 1)) / 8.0;
+                }
+    free(dub);
+    double cfl, dtn;
+    orc_stepgrid3(p, nx / 2, ny / 2, nz / 2, crs, 2.0 * hx, 2.0 * hy, 2.0 * hz, 2. * dt, f, f + nc,
+                  f + 2 * nc, f + 3 * nc, f + 4 * nc, f + 5 * nc, &cfl, &dtn);
+    memset(f, 0, 6 * (nf > nc ? nf : nc) * sizeof(double));
+    orc_stepgrid3(p, nx, ny, nz, big, hx, hy, hz, dt, f, f + nf, f + 2 * nf, f + 3 * nf, f + 4 * nf,
+                  f + 5 * nf, &cfl, &dtn);
+    free(f);
+    /* errf1.f (3-D) */
+    const double order = (double)((1 << (iorder + 1)) - 2);
+    for (size_t c = 0; c < (size_t)mitot * mjtot * mktot; c++) rflags[c] = 0.0;
+#define RF(i, j, k) rflags[((i) - 1) + (size_t)mitot * (((j) - 1) + (size_t)mjtot * ((k) - 1))]
+    int kfine = ng + 1;
+    for (int k = ng + 1; k <= mk2 - ng; k++, kfine += 2) {
+        int jfine = ng + 1;
+        for (int j = ng + 1; j <= mj2 - ng; j++, jfine += 2) {
+            int ifine = ng + 1;
+            for (int i = ng + 1; i <= mi2 - ng; i++, ifine += 2) {
+                double t1 = V3(big, nv, mitot, mjtot, 1, ifine, jfine, kfine);
+                double t2 = V3(big, nv, mitot, mjtot, 1, ifine + 1, jfine, kfine);
+                double t3 = V3(big, nv, mitot, mjtot, 1, ifine + 1, jfine + 1, kfine);
+                double t4 = V3(big, nv, mitot, mjtot, 1, ifine, jfine + 1, kfine);
+                double t5 = V3(big, nv, mitot, mjtot, 1, ifine, jfine, kfine + 1);
+                double t6 = V3(big, nv, mitot, mjtot, 1, ifine + 1, jfine, kfine + 1);
+                double t7 = V3(big, nv, mitot, mjtot, 1, ifine + 1, jfine + 1, kfine + 1);
+                double t8 = V3(big, nv, mitot, mjtot, 1, ifine, jfine + 1, kfine + 1);
+                double aval = (t1 + t2 + t3 + t4 + t5 + t6 + t7 + t8) / 8.0;
+                double est = fabs((aval - V3(crs, nv, mi2, mj2, 1, i, j, k)) / order);
+                if (est >= tol)
+                    for (int dk = 0; dk < 2; dk++)
+                        for (int dj = 0; dj < 2; dj++)
+                            for (int di = 0; di < 2; di++) RF(ifine + di, jfine + dj, kfine + dk) = 2.0;
+            }
+        }
+    }
+#undef RF
+    free(crs);
+    return 0;
 }

@@ -404,6 +404,92 @@ contains
     t = rnode(timemult, listOfGrids(listStart(l) + k))
   end subroutine ref3_get_time
 
+  ! spest.f (3-D) for every grid of `level` and, with frich, errest.f +
+  ! errf1.f (whose flags errf1 leaves in component 1 of each grid's store2);
+  ! domflags is dom1flags(0:isize+1, 0:jsize+1, 0:ksize+1), rflags the
+  ! grids' store2 component-1 values one after the other.  spest.f itself
+  ! does not compile here (its dummy argument t0 clashes with amr_module's
+  ! t0, spest.f:4-6), so its per-grid steps are performed below with the
+  ! reference's own bound, flag2refine and setflags (spest.f:14-99).
+  subroutine ref3_flag_level(level, nvar, tolsp_in, tol_in, iorder_in, frich, domflags, rflags) &
+       bind(C, name="ref3_flag_level")
+    integer(c_int), value :: level, nvar, iorder_in, frich
+    real(c_double), value :: tolsp_in, tol_in
+    integer(c_int8_t) :: domflags(*)
+    real(c_double) :: rflags(*)
+    integer :: j, mptr, n, off, c, loc
+    tolsp = tolsp_in
+    tol = tol_in
+    iorder = iorder_in
+    flag_gradient = .true.
+    flag_richardson = frich .ne. 0
+    eprint = .false.
+    edebug = .false.
+    call harness3_spest(nvar, level, domflags, iregsz(level), jregsz(level), kregsz(level))
+    if (frich .ne. 0) call errest(nvar, 0, level)
+    off = 0
+    do j = 1, numgrids(level)
+       mptr = listOfGrids(listStart(level) + j - 1)
+       n = ref3_gridlen(mptr, 1)
+       if (frich .ne. 0) then
+          loc = node(store2, mptr)
+          do c = 1, n
+             rflags(off + c) = alloc(loc + (c - 1) * nvar)
+          end do
+       end if
+       call reclam(node(tempptr, mptr), n * nvar)
+       off = off + n
+    end do
+  end subroutine ref3_flag_level
+
+  subroutine harness3_spest(nvar, lcheck, dom1flags, isize, jsize, ksize)
+    integer :: nvar, lcheck, isize, jsize, ksize
+    integer(c_int8_t) :: dom1flags(0:isize+1, 0:jsize+1, 0:ksize+1)
+    integer :: i, j, k, mptr, nx, ny, nz, mitot, mjtot, mktot, locnew, locbig, locamrflags
+    integer, external :: igetsp
+    real(kind=8) :: time, dx, dy, dz, xlow, ylow, zlow, aux1(1)
+    external flag2refine
+    do i = 1, isize
+       do j = 1, jsize
+          do k = 1, ksize
+             dom1flags(i, j, k) = 0
+          end do
+       end do
+    end do
+    mptr = lstart(lcheck)
+    do while (mptr .ne. 0)
+       nx = node(ndihi,mptr) - node(ndilo,mptr) + 1
+       ny = node(ndjhi,mptr) - node(ndjlo,mptr) + 1
+       nz = node(ndkhi,mptr) - node(ndklo,mptr) + 1
+       mitot = nx + 2*nghost
+       mjtot = ny + 2*nghost
+       mktot = nz + 2*nghost
+       locnew = node(store1,mptr)
+       time = rnode(timemult,mptr)
+       dx = hxposs(lcheck)
+       dy = hyposs(lcheck)
+       dz = hzposs(lcheck)
+       xlow = rnode(cornxlo,mptr) - nghost*dx
+       ylow = rnode(cornylo,mptr) - nghost*dy
+       zlow = rnode(cornzlo,mptr) - nghost*dz
+       locbig = igetsp(mitot*mjtot*mktot*nvar)
+       node(tempptr,mptr) = locbig
+       do i = 1, mitot*mjtot*mktot*nvar
+          alloc(locbig+i-1) = alloc(locnew+i-1)
+       end do
+       call bound(time,nvar,nghost,alloc(locbig),mitot,mjtot,mktot,mptr,aux1,0)
+       locamrflags = igetsp(mitot*mjtot*mktot)
+       do i = 1, mitot*mjtot*mktot
+          alloc(locamrflags+i-1) = goodpt
+       end do
+       call flag2refine(nx,ny,nz,nghost,nvar,0,xlow,ylow,zlow,dx,dy,dz,time,lcheck,tolsp, &
+            alloc(locbig),aux1,alloc(locamrflags),goodpt,badpt)
+       call setflags(dom1flags,isize,jsize,ksize,alloc(locamrflags),1,mitot,mjtot,mktot,mptr)
+       call reclam(locamrflags, mitot*mjtot*mktot)
+       mptr = node(levelptr,mptr)
+    end do
+  end subroutine harness3_spest
+
 end module ref3_harness
 
 ! listbc lives in the double heap as integers (prepc.f, setuse.f); read it

@@ -192,3 +192,56 @@ def test_bound3_time_tolerance(offset, blends):
     if orc.ref3_available():
         for k in range(len(H.levels[2]["boxes"])):
             assert np.array_equal(A.grid(3, k), orc.ref3_bound(p, H, 3, k, t)), k
+
+
+@needs_ref3
+@pytest.mark.parametrize("case", [BCS[0], BCS[1], BCS[3]])
+@pytest.mark.parametrize("seed", range(2))
+def test_flagging3_matches_reference(case, seed):
+    """spest (bound + flag2refine + setflags) and errest (copysol, bound with
+    2 nghost at t - possk, coarsen, the two stepgrids, errf1) of every level
+    after a coarse step, bit for bit with the reference's 3-D Fortran."""
+    bc, per = case
+    rng = np.random.default_rng(600 + seed)
+    H = nested_hierarchy3(rng, smooth_euler3, 5, n=(12, 12, 8), mthbc=bc, per=per, dt1=0.004)
+    for l, L in enumerate(H.levels, start=1):          # Richardson needs even grids
+        assert all((b[1] - b[0] + 1) % 2 == 0 and (b[3] - b[2] + 1) % 2 == 0 and
+                   (b[5] - b[4] + 1) % 2 == 0 for b in L["boxes"])
+    p = orc.make_params3(orc.EULER)
+    O = orc.OracleAMR3(p, H.copy())
+    R = orc.RefAMR3(p, H)
+    berger_oliger3(O, O.H, 1)
+    berger_oliger3(R, H, 1)
+    # regrid flags the levels from fine to coarse (regrid.f); errf1 leaves
+    # its flags in component 1 of the flagged level's store2, which a later
+    # flagging of a finer level would read as old values
+    for level in range(H.nlev - 1, 0, -1):
+        tolsp, tol = 0.05, 1e-4
+        f, rf, n = orc.flag_level3(O, level, tolsp, tol=tol, richardson=True)
+        dom, rref = orc.ref3_flag_level(R, level, tolsp, tol=tol, richardson=True)
+        assert np.array_equal(orc.domain_flags3(O.H, level, f), dom), level
+        assert n > 0
+        for k, (a, b) in enumerate(zip(rf, rref)):
+            assert np.array_equal(a, b), (level, k)
+        assert sum(int((r == 2.0).sum()) for r in rf) > 0
+
+
+def test_errest3_uniform_state_flags_nothing():
+    """A uniform state has no Richardson error: no flags at any positive
+    tolerance, every interior 2x2x2 block flagged at tol = 0."""
+    rng = np.random.default_rng(5)
+    H = nested_hierarchy3(rng, smooth_euler3, 5, n=(12, 12, 8), dt1=0.004)
+    for L in H.levels:
+        if len(L["boxes"]):
+            for which in ("qnew", "qold"):
+                L[which].reshape(-1, 5)[:] = (1.1, 0.2, -0.1, 0.05, 2.9)
+    O = orc.OracleAMR3(orc.make_params3(orc.EULER), H)
+    for L in H.levels:
+        L["time"] = L["possk"]
+    O.h = orc.hier3_struct(H)
+    f, rf, _ = orc.flag_level3(O, 1, 0.05, tol=1e-12, richardson=True)
+    assert all(int((r != 0).sum()) == 0 for r in rf)
+    assert all(int((x != 0).sum()) == 0 for x in f)
+    f, rf, _ = orc.flag_level3(O, 1, 0.05, tol=0.0, richardson=True)
+    for b, r in zip(H.levels[0]["boxes"], rf):
+        assert int((r == 2.0).sum()) == (b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1)
