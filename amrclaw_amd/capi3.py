@@ -68,6 +68,9 @@ def lib():
         L.amrgpu3_update_level.argtypes = [C.c_int]
         L.amrgpu3_conck_level.argtypes = [C.c_int, dpo]
         L.amrgpu3_get_registers.argtypes = [C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int)]
+        L.amrgpu3_flag_level.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.amrgpu3_get_level_flags.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         _bound = True
     return L
 
@@ -273,6 +276,24 @@ def get_registers3(level, grid, nvar):
     return buf[:nvar * lb].reshape(lb, nvar), buf[nvar * lb:].reshape(lb, nvar)
 
 
+def flag_level3(level, tolsp, *, richardson=False, tol=0.0, iorder=2):
+    """spest (and with richardson errest/errf1) on every grid of the level;
+    returns (flagged cells, Richardson-flagged cells)."""
+    n, r = C.c_int64(), C.c_int64()
+    _check(lib().amrgpu3_flag_level(level, tolsp, 1 if richardson else 0, tol, iorder,
+                                    C.byref(n), C.byref(r)))
+    return n.value, r.value
+
+
+def get_level_flags3(level, grid, shape):
+    """(amrflags, rflags) of one grid, each of shape (mz+4, my+4, mx+4)."""
+    f = np.empty(shape, dtype=np.float64)
+    r = np.empty(shape, dtype=np.float64)
+    _check(lib().amrgpu3_get_level_flags(level, grid, f.ctypes.data_as(C.c_void_p),
+                                         r.ctypes.data_as(C.c_void_p)))
+    return f, r
+
+
 class DeviceAMR3:
     """The library's 3-D level update over a Hier3 loaded on the device
     (the advanc / update / conck interface of the reference's driver)."""
@@ -289,3 +310,6 @@ class DeviceAMR3:
 
     def conck(self, level):
         return conck_level3(level)
+
+    def flag(self, level, tolsp, **kw):
+        return flag_level3(level, tolsp, **kw)

@@ -44,7 +44,7 @@ __device__ Buf3<S> buf3(const Fill3Args& a, int b) {
         r.off = a.lvl_patches[fb.level][fb.patch].qoff;
     } else {
         r.lvl = nullptr;
-        r.scr = a.scratch;
+        r.scr = fb.kind == 1 ? a.scratch : a.arena;
         r.off = fb.off;
     }
     r.plane = fb.plane;
@@ -76,7 +76,8 @@ __global__ __launch_bounds__(256) void fill3_stage_kernel(Fill3Args a) {
         const Patch3 sp = a.lvl_patches[op.src_level][op.src];
         const S* qn = static_cast<const S*>(a.lvl_new[op.src_level]) + sp.qoff;
         const S* qo = static_cast<const S*>(a.lvl_old[op.src_level]) + sp.qoff;
-        const int mode = (op.flags & F3F_FORCE_NEW) ? T3_NEW : a.tmode[op.src_level];
+        const int mode = (op.flags & F3F_FORCE_NEW) ? T3_NEW
+                         : (op.flags & F3F_FORCE_OLD) ? T3_OLD : a.tmode[op.src_level];
         const double ai = a.alphai[op.src_level], ac = a.alphac[op.src_level];
         const int64_t smit = sp.mx + 4, smij = smit * (sp.my + 4), spl = qplane3(sp);
         for (int64_t t = w.lo + tid; t < w.hi; t += 256) {
@@ -151,6 +152,21 @@ __global__ __launch_bounds__(256) void fill3_stage_kernel(Fill3Args a) {
             else if ((x -= s2) < s5) { j = 1 + x / nr; i = 1 + x % nr; k = 0; }
             else { x -= s5; j = 1 + x / nr; i = 1 + x % nr; k = op.nk - 1; }
             qc1d[(int64_t)idx * nv + m] = src.load(m, i, j, k);
+        }
+    } else if (op.type == F3_COARSEN) {
+        // coarsen.f (3-D): dst(i, j, k) = mean of src(2i..2i+1, 2j.., 2k..)
+        const Buf3<S> src = buf3<S>(a, op.src);
+        for (int64_t t = w.lo + tid; t < w.hi; t += 256) {
+            const int m = (int)(t / ncell);
+            const int64_t c = t - m * ncell;
+            const int i = (int)(c % op.ni), r = (int)(c / op.ni);
+            const int j = r % op.nj, k = r / op.nj;
+            const int fi = 2 * i, fj = 2 * j, fk = 2 * k;
+            const double v = (src.load(m, fi, fj, fk) + src.load(m, fi + 1, fj, fk) +
+                              src.load(m, fi, fj + 1, fk) + src.load(m, fi + 1, fj + 1, fk) +
+                              src.load(m, fi, fj, fk + 1) + src.load(m, fi + 1, fj, fk + 1) +
+                              src.load(m, fi, fj + 1, fk + 1) + src.load(m, fi + 1, fj + 1, fk + 1)) / 8.0;
+            dst.store(m, i, j, k, v);
         }
     } else if (op.type == F3_BC) {
         // bc3amr.f90: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi in turn, each
@@ -395,7 +411,59 @@ __global__ __launch_bounds__(256) void avg3_kernel(Reg3Args a) {
     }
 }
 
+// errf1.f (3-D): one coarsened interior cell per thread; the Richardson
+// estimate |mean of the 8 fine values - coarse value| / (2^(iorder+1) - 2)
+// of component 1 flags (badpt = 2) its 2x2x2 fine cells in rflags (zeroed
+// before the launch); fine and coarse solutions in the FP64 arena
+__global__ __launch_bounds__(256) void errf1_3_kernel(const double* arena, const Patch3* fine,
+                                                      const Patch3* crs, const int64_t* foff,
+                                                      const int64_t* cstart, int ngrids,
+                                                      double tol, double order, double* rflags,
+                                                      unsigned long long* nflag) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cstart[ngrids]) return;
+    int lo = 0, hi = ngrids - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (cstart[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int g = lo;
+    const Patch3 pf = fine[g], pc = crs[g];
+    const int64_t c = t - cstart[g];
+    const int i = (int)(c % pc.mx), r = (int)(c / pc.mx);
+    const int j = r % pc.my, k = r / pc.my;          // 0-based interior coarse cell
+    const int64_t fmit = pf.mx + 4, fmij = fmit * (pf.my + 4);
+    const int64_t cmit = pc.mx + 4, cmij = cmit * (pc.my + 4);
+    const double* qf = arena + pf.qoff;              // component 1 = plane 0
+    const double* qc = arena + pc.qoff;
+    const int fi = 2 + 2 * i, fj = 2 + 2 * j, fk = 2 + 2 * k;   // 0-based with ghosts
+    auto F = [&](int a, int b, int d) { return qf[(int64_t)d * fmij + (int64_t)b * fmit + a]; };
+    const double t1 = F(fi, fj, fk), t2 = F(fi + 1, fj, fk), t3 = F(fi + 1, fj + 1, fk);
+    const double t4 = F(fi, fj + 1, fk), t5 = F(fi, fj, fk + 1), t6 = F(fi + 1, fj, fk + 1);
+    const double t7 = F(fi + 1, fj + 1, fk + 1), t8 = F(fi, fj + 1, fk + 1);
+    const double aval = (t1 + t2 + t3 + t4 + t5 + t6 + t7 + t8) / 8.0;
+    const double est = fabs((aval - qc[(int64_t)(k + 2) * cmij + (int64_t)(j + 2) * cmit + (i + 2)]) / order);
+    if (est >= tol) {
+        double* rf = rflags + foff[g];
+        for (int dk = 0; dk < 2; dk++)
+            for (int dj = 0; dj < 2; dj++)
+                for (int di = 0; di < 2; di++)
+                    rf[(int64_t)(fk + dk) * fmij + (int64_t)(fj + dj) * fmit + (fi + di)] = 2.0;
+        atomicAdd(nflag, 8ull);
+    }
+}
+
 // ---------------------------------------------------------------- launchers
+hipError_t launch_errf1_3(const double* arena, const Patch3* fine, const Patch3* crs,
+                          const int64_t* foff, const int64_t* cstart, int ngrids, int64_t ncells,
+                          double tol, double order, double* rflags, unsigned long long* nflag,
+                          hipStream_t st) {
+    if (ncells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(errf1_3_kernel, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, st,
+                       arena, fine, crs, foff, cstart, ngrids, tol, order, rflags, nflag);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill3_stage(int esize, const Fill3Args& a, hipStream_t st) {
     if (a.nwork <= 0) return hipSuccess;
     if (esize == 8)

@@ -26,14 +26,19 @@ enum Fill3OpType : int32_t {
     F3_INTERP = 1,    // filrecur: limited linear interpolation (filpatch.f:160-260)
     F3_BC = 2,        // bc3amr on a whole buffer (bc3amr.f90)
     F3_COPYBACK = 3,  // prefilrecur: patchCopyOut (prefilp.f)
-    F3_RING = 4       // cstore: the coarse perimeter of a buffer -> qc1d (cstore.f)
+    F3_RING = 4,      // cstore: the coarse perimeter of a buffer -> qc1d (cstore.f)
+    F3_COARSEN = 5    // coarsen.f: 2x2x2 means of a buffer into another (errest.f)
 };
-enum : int32_t { F3F_FORCE_NEW = 1 };   // icall: store1, no time blend
+enum : int32_t {
+    F3F_FORCE_NEW = 1,   // icall: store1, no time blend
+    F3F_FORCE_OLD = 2    // copysol of store2 (errest.f)
+};
 
 // A destination/scratch array, structure of arrays: cell (i, j, k), 0-based
 // with ghosts, component m at off + m*plane + (k*ncol + j)*nrow + i.
 struct Fill3Buf {
-    int32_t kind;          // 0: grid `patch` of level `level`; 1: FP64 scratch
+    int32_t kind;          // 0: grid `patch` of level `level`; 1: FP64 scratch;
+                           // 2: FP64 arena read afterwards by other kernels
     int32_t level, patch;
     int32_t nrow, ncol, nfil;
     int64_t plane;
@@ -87,6 +92,7 @@ struct Fill3Args {
     double xlower, ylower, zlower;
     int32_t mthbc[6];
     int32_t nv, nwork;
+    double* arena;               // kind-2 buffers
 };
 
 // fluxsv.f: coarse cell (I, J, K) (1-based with ghosts) of grid `grid`,

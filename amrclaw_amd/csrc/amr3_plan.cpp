@@ -327,12 +327,14 @@ struct Planner3 {
     }
 
     // bound.f (3-D) for grid `grid` of `level` into its own storage
-    bool bound(int level, int grid) {
+    bool bound(int level, int grid) { return bound_into(level, grid, pbuf(level, grid), NG3); }
+
+    // bound.f (3-D) with a ghost ring of width ng into buffer `buf`
+    // ((mx + 2 ng) x (my + 2 ng) x (mz + 2 ng), the interior at (ng, ng, ng))
+    bool bound_into(int level, int grid, int buf, int ng) {
         const Level3Geom& G = H.lev[level];
         const auto& b = G.boxes[grid];
-        const int ng = NG3;
         const int ilo = b[0], ihi = b[1], jlo = b[2], jhi = b[3], klo = b[4], khi = b[5];
-        const int buf = pbuf(level, grid);
         const int mitot = ihi - ilo + 1 + 2 * ng, mjtot = jhi - jlo + 1 + 2 * ng;
         const int mktot = khi - klo + 1 + 2 * ng;
         const double xlo = ilo * G.hx + H.xlower, xhi = (ihi + 1) * G.hx + H.xlower;
@@ -382,6 +384,21 @@ struct Planner3 {
         for (int s = 0; s < 6; s++) ready = std::max(ready, r[s]);
         const double lo[3] = {xl, yf, zb}, hi[3] = {xr, yr, zt};
         return bc_op(buf, G.hx, G.hy, G.hz, lo, hi, ready + 1);
+    }
+
+    // an FP64 arena buffer (kind 2) laid out like a grid
+    int arena(int nrow, int ncol, int nfil, int64_t& arena_elems) {
+        Fill3Buf fb{};
+        fb.kind = 2;
+        fb.patch = -1;
+        fb.nrow = nrow;
+        fb.ncol = ncol;
+        fb.nfil = nfil;
+        fb.plane = (int64_t)nrow * ncol * nfil;
+        fb.off = arena_elems;
+        arena_elems += ((int64_t)nv * fb.plane + 63) / 64 * 64;
+        P.bufs.push_back(fb);
+        return (int)P.bufs.size() - 1;
     }
 
     // icall.f (3-D): copy the level's grids (store1) over the box into
@@ -539,7 +556,7 @@ void finish_plan(Fill3Plan& P, int nv) {
         const Fill3Op& op = P.ops[o];
         const int64_t ncell = (int64_t)op.ni * op.nj * op.nk;
         int64_t n = 0;
-        if (op.type == F3_COPY || op.type == F3_COPYBACK) n = ncell * nv;
+        if (op.type == F3_COPY || op.type == F3_COPYBACK || op.type == F3_COARSEN) n = ncell * nv;
         else if (op.type == F3_INTERP) n = ncell;
         if (n == 0) {
             P.work.push_back(Fill3Work{o, 0, 0, 0});
@@ -635,6 +652,88 @@ bool plan_saveqc3(const Hier3Geom& H, int level, int nvar, Fill3Plan& P, std::st
             err = pl.err.empty() ? "3-D saveqc planning failed" : pl.err;
             return false;
         }
+    finish_plan(P, nvar);
+    return true;
+}
+
+bool plan_richardson3(const Hier3Geom& H, int level, int nvar, Rich3Plan& R, std::string& err) {
+    R = Rich3Plan();
+    if (!periodic_ok(H, err)) return false;
+    Fill3Plan& P = R.fill;
+    Planner3 pl(H, P, nvar);
+    const Level3Geom& G = H.lev[level];
+    const int ng = NG3, ngbig = 2 * NG3;
+    const int n = (int)G.boxes.size();
+    std::vector<int> dub(n);
+    for (int g = 0; g < n; g++) {
+        const auto& b = G.boxes[g];
+        const int mx = b[1] - b[0] + 1, my = b[3] - b[2] + 1, mz = b[5] - b[4] + 1;
+        if (mx % 2 || my % 2 || mz % 2) {
+            err = "Richardson estimation needs grids of even size (errest.f: nx/2)";
+            return false;
+        }
+        // copysol.f: the old interior into the centre of the wider array
+        const int d = pl.arena(mx + 2 * ngbig, my + 2 * ngbig, mz + 2 * ngbig, R.arena_elems);
+        dub[g] = d;
+        Fill3Op op{};
+        op.type = F3_COPY;
+        op.flags = F3F_FORCE_OLD;
+        op.dst = d;
+        op.src = g;
+        op.src_level = level;
+        op.di0 = op.dj0 = op.dk0 = ngbig;
+        op.ni = mx;
+        op.nj = my;
+        op.nk = mz;
+        op.si0 = op.sj0 = op.sk0 = ng;
+        op.stage = 0;
+        pl.push(op);
+        if (!pl.bound_into(level, g, d, ngbig)) {
+            err = pl.err.empty() ? "Richardson ghost-fill planning failed" : pl.err;
+            return false;
+        }
+    }
+    int last = 0;
+    for (const auto& op : P.ops) last = std::max(last, op.stage);
+    for (int g = 0; g < n; g++) {
+        const auto& b = G.boxes[g];
+        const int mx = b[1] - b[0] + 1, my = b[3] - b[2] + 1, mz = b[5] - b[4] + 1;
+        // coarsen.f into (mx/2 + 2 ng)^3
+        const int c = pl.arena(mx / 2 + 2 * ng, my / 2 + 2 * ng, mz / 2 + 2 * ng, R.arena_elems);
+        Fill3Op op{};
+        op.type = F3_COARSEN;
+        op.dst = c;
+        op.src = dub[g];
+        op.ni = mx / 2 + 2 * ng;
+        op.nj = my / 2 + 2 * ng;
+        op.nk = mz / 2 + 2 * ng;
+        op.stage = last + 1;
+        pl.push(op);
+        // spest's locbig: the grid's new values with their ghost cells
+        const int f = pl.arena(mx + 2 * ng, my + 2 * ng, mz + 2 * ng, R.arena_elems);
+        Fill3Op cp{};
+        cp.type = F3_COPY;
+        cp.flags = F3F_FORCE_NEW;
+        cp.dst = f;
+        cp.src = g;
+        cp.src_level = level;
+        cp.ni = mx + 2 * ng;
+        cp.nj = my + 2 * ng;
+        cp.nk = mz + 2 * ng;
+        cp.stage = 0;
+        pl.push(cp);
+        Patch3 pc{}, pf{};
+        pc.qoff = P.bufs[c].off;
+        pc.mx = mx / 2;
+        pc.my = my / 2;
+        pc.mz = mz / 2;
+        pf.qoff = P.bufs[f].off;
+        pf.mx = mx;
+        pf.my = my;
+        pf.mz = mz;
+        R.crs.push_back(pc);
+        R.fine.push_back(pf);
+    }
     finish_plan(P, nvar);
     return true;
 }

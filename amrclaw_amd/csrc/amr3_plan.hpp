@@ -41,6 +41,19 @@ bool plan_fill3_level(const Hier3Geom& H, int level, int nvar, Fill3Plan& P, std
 // saveqc.f (3-D) for the grids of fine level `level`
 bool plan_saveqc3(const Hier3Geom& H, int level, int nvar, Fill3Plan& P, std::string& err);
 
+// errest.f (3-D) for every grid of `level`: the fill program builds, in an
+// FP64 arena, the coarsened old solution of each grid (copysol of store2,
+// bound with 2 nghost ghost cells at t - possk, coarsen.f) and the copy of
+// the grid's new values with ghost cells (spest's locbig); crs / fine
+// describe those arrays as grids (qoff into the arena) for the two
+// stepgrid calls of errest.f
+struct Rich3Plan {
+    Fill3Plan fill;
+    std::vector<Patch3> crs, fine;
+    int64_t arena_elems = 0;
+};
+bool plan_richardson3(const Hier3Geom& H, int level, int nvar, Rich3Plan& R, std::string& err);
+
 struct Reg3Level {
     // fine side (level > 1): prepf.f
     std::vector<RegGrid3> regs;

@@ -267,7 +267,8 @@ int get_q(int p, double* q) {
     return AMRGPU_OK;
 }
 
-int step_io(const Step3IO& io, double dx, double dy, double dz, double dt) {
+int step_io(const Step3IO& io, double dx, double dy, double dz, double dt, int es_override = 0) {
+    const size_t es = es_override ? (size_t)es_override : g3.esize;
     if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !(dt >= 0))
         return fail3(AMRGPU_ERR_ARG, "bad mesh widths or time step");
     HIPCHK3(hipMemsetAsync(io.d_cfl, 0, std::max(1, io.npatch) * sizeof(unsigned long long),
@@ -290,8 +291,8 @@ int step_io(const Step3IO& io, double dx, double dy, double dz, double dt) {
         hipEvent_t stop;
         int rc = tbegin(d - 1, &stop);
         if (rc) return rc;
-        HIPCHK3(g3.esize == 8 ? launch_sweep3_f64(solver, a, g3.stream)
-                              : launch_sweep3_f32(solver, a, g3.stream));
+        HIPCHK3(es == 8 ? launch_sweep3_f64(solver, a, g3.stream)
+                        : launch_sweep3_f32(solver, a, g3.stream));
         rc = tend(stop);
         if (rc) return rc;
     }
@@ -300,7 +301,7 @@ int step_io(const Step3IO& io, double dx, double dy, double dz, double dt) {
         hipEvent_t stop;
         int rc = tbegin(3, &stop);
         if (rc) return rc;
-        if (g3.esize == 8)
+        if (es == 8)
             HIPCHK3(launch_update3_f64(g3.cfg.nvar, static_cast<double*>(io.q),
                                        static_cast<const double*>(io.face), io.d_patches,
                                        io.d_blocks[3], nb, a.dtd[0], a.dtd[1], a.dtd[2],
@@ -415,8 +416,8 @@ int ctx3_fail(int code, const char* fmt, ...) {
 }
 hipError_t ctx3_h2d(void* dst, const void* src, size_t n) { return h2d3(dst, src, n); }
 hipError_t ctx3_d2h(void* dst, const void* src, size_t n) { return d2h3(dst, src, n); }
-int ctx3_step(const Step3IO& io, double dx, double dy, double dz, double dt) {
-    return step_io(io, dx, dy, dz, dt);
+int ctx3_step(const Step3IO& io, double dx, double dy, double dz, double dt, int esize) {
+    return step_io(io, dx, dy, dz, dt, esize);
 }
 void ctx3_plan_blocks(const std::vector<Patch3>& patches, int plane_block,
                       std::vector<Block3> out[4]) {

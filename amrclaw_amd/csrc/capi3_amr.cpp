@@ -28,6 +28,10 @@ hipError_t launch_qad3(int solver, const Reg3Args& a, const Physics3& ph, hipStr
 hipError_t launch_fluxsv3(const Reg3Args& a, hipStream_t st);
 hipError_t launch_fluxad3(const Reg3Args& a, hipStream_t st);
 hipError_t launch_update3_level(const Reg3Args& a, hipStream_t st);
+hipError_t launch_errf1_3(const double* arena, const Patch3* fine, const Patch3* crs,
+                          const int64_t* foff, const int64_t* cstart, int ngrids, int64_t ncells,
+                          double tol, double order, double* rflags, unsigned long long* nflag,
+                          hipStream_t st);
 }  // namespace amrgpu
 
 using namespace amrgpu;
@@ -66,6 +70,11 @@ struct Lev3 {
     Upbnd3Cell* d_ucells = nullptr;
     Upbnd3Entry* d_uents = nullptr;
     Avg3Block* d_avg = nullptr;
+    // flagging (flag2refine, errf1) of the level's grids
+    double* d_flags = nullptr;
+    double* d_rflags = nullptr;
+    int64_t* d_foff = nullptr;
+    std::vector<int64_t> foff;
 };
 
 struct Amr3 {
@@ -125,6 +134,9 @@ void free_level(Lev3& L) {
     dfree(L.d_old);
     dfree(L.d_cfl);
     dfree(L.d_partial);
+    dfree(L.d_flags);
+    dfree(L.d_rflags);
+    dfree(L.d_foff);
     L = Lev3();
 }
 
@@ -218,9 +230,11 @@ int ensure_plans() {
 
 // intfil.f (3-D):95-150 time selection for the source levels of a fill at
 // time t: new, old or the linear blend, with the tolerance possk/10
-int fill_args(int level, double t, const Fill3Plan& P, const DevFill& D, Fill3Args& a) {
+int fill_args(int level, double t, const Fill3Plan& P, const DevFill& D, Fill3Args& a,
+              double* arena = nullptr) {
     const amrgpu3_config& c = ctx3_cfg();
     a = Fill3Args{};
+    a.arena = arena;
     a.ops = D.d_ops;
     a.bufs = D.d_bufs;
     a.scratch = A.d_scratch;
@@ -262,11 +276,11 @@ int fill_args(int level, double t, const Fill3Plan& P, const DevFill& D, Fill3Ar
     return AMRGPU_OK;
 }
 
-int run_fill(int level, double t, const DevFill& D) {
+int run_fill(int level, double t, const DevFill& D, double* arena = nullptr) {
     const Fill3Plan& P = D.plan;
     if (P.ops.empty()) return AMRGPU_OK;
     Fill3Args a;
-    int rc = fill_args(level, t, P, D, a);
+    int rc = fill_args(level, t, P, D, a, arena);
     if (rc) return rc;
     for (int s = 0; s < P.nstages; s++) {
         a.work = D.d_work + P.stage_start[s];
@@ -654,6 +668,175 @@ int amrgpu3_conck_level(int level, double* totmass) {
     }
     const Level3Geom& G = A.H.lev[level];
     *totmass = s * G.hx * G.hy * G.hz;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_flag_level(int level, double tolsp, int flag_richardson, double tol, int iorder,
+                       int64_t* nflag, int64_t* nrich) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    rc = ensure_fill_plans();
+    if (rc) return rc;
+    Lev3& L = A.lev[level];
+    const hipStream_t st = ctx3_stream();
+    const int n = (int)L.patches.size();
+    // per-grid flag arrays (mx+4, my+4, mz+4), goodpt = 0 outside the interior
+    if (!L.d_flags) {
+        L.foff.assign(n + 1, 0);
+        for (int g = 0; g < n; g++)
+            L.foff[g + 1] = L.foff[g] + (int64_t)(L.patches[g].mx + 4) * (L.patches[g].my + 4) *
+                                            (L.patches[g].mz + 4);
+        const int64_t tot = std::max<int64_t>(1, L.foff[n]);
+        HIPCHKA(hipMalloc((void**)&L.d_flags, tot * sizeof(double)));
+        HIPCHKA(hipMalloc((void**)&L.d_rflags, tot * sizeof(double)));
+        rc = upload(L.d_foff, L.foff);
+        if (rc) return rc;
+    }
+    const int64_t tot = std::max<int64_t>(1, L.foff[n]);
+    HIPCHKA(hipMemsetAsync(L.d_flags, 0, tot * sizeof(double), st));
+    HIPCHKA(hipMemsetAsync(L.d_rflags, 0, tot * sizeof(double), st));
+    unsigned long long* d_cnt = nullptr;
+    HIPCHKA(hipMalloc((void**)&d_cnt, 2 * sizeof(unsigned long long)));
+    HIPCHKA(hipMemsetAsync(d_cnt, 0, 2 * sizeof(unsigned long long), st));
+    // spest.f (3-D): ghost cells at the grids' time, then flag2refine
+    rc = run_fill(level, L.time, L.fill);
+    if (rc) { hipFree(d_cnt); return rc; }
+    if (n)
+        HIPCHKA(launch_flag3((int)ctx3_esize(), ctx3_cfg().nvar, L.d_new, L.d_flags, L.d_foff,
+                             L.d_patches, L.d_blocks[3], (int)L.blocks[3].size(), tolsp, d_cnt, st));
+    if (flag_richardson && n) {
+        // errest.f (3-D) + errf1.f
+        if (level == A.H.nlev) {
+            hipFree(d_cnt);
+            return ctx3_fail(AMRGPU_ERR_ARG, "Richardson estimation on the finest level needs its "
+                             "old time, which the finest level does not keep");
+        }
+        Rich3Plan R;
+        std::string err;
+        if (!plan_richardson3(A.H, level, ctx3_cfg().nvar, R, err)) {
+            hipFree(d_cnt);
+            return ctx3_fail(AMRGPU_ERR_ARG, "level %d Richardson: %s", level, err.c_str());
+        }
+        DevFill D;
+        D.plan = R.fill;
+        double* arena = nullptr;
+        void* face = nullptr;
+        Patch3* d_pf = nullptr;
+        Patch3* d_pc = nullptr;
+        Block3* d_bf[4] = {};
+        Block3* d_bc[4] = {};
+        int64_t* d_cstart = nullptr;
+        auto cleanup = [&]() {
+            hipStreamSynchronize(st);
+            free_fill(D);
+            hipFree(arena);
+            hipFree(face);
+            hipFree(d_pf);
+            hipFree(d_pc);
+            for (int k = 0; k < 4; k++) {
+                hipFree(d_bf[k]);
+                hipFree(d_bc[k]);
+            }
+            hipFree(d_cstart);
+            hipFree(d_cnt);
+        };
+        const int nv = ctx3_cfg().nvar;
+        int64_t fo = 0;
+        for (auto& p : R.crs) {
+            p.foff = fo;
+            fo += (faces3_elems(p, nv) + 63) / 64 * 64;
+        }
+        int64_t fo2 = 0;
+        for (auto& p : R.fine) {
+            p.foff = fo2;
+            fo2 += (faces3_elems(p, nv) + 63) / 64 * 64;
+        }
+        std::vector<Block3> bf[4], bc[4];
+        ctx3_plan_blocks(R.fine, 2, bf);
+        ctx3_plan_blocks(R.crs, 2, bc);
+        std::vector<int64_t> cstart(n + 1, 0);
+        for (int g = 0; g < n; g++)
+            cstart[g + 1] = cstart[g] + (int64_t)R.crs[g].mx * R.crs[g].my * R.crs[g].mz;
+        hipError_t e = hipMalloc((void**)&arena, std::max<int64_t>(1, R.arena_elems) * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&face, std::max<int64_t>(1, std::max(fo, fo2)) * sizeof(double));
+        if (e != hipSuccess) {
+            cleanup();
+            return ctx3_fail(AMRGPU_ERR_HIP, "Richardson buffers: %s", hipGetErrorString(e));
+        }
+        rc = upload_fill(D);
+        if (!rc) rc = upload(d_pf, R.fine);
+        if (!rc) rc = upload(d_pc, R.crs);
+        for (int k = 0; k < 4 && !rc; k++) {
+            rc = upload(d_bf[k], bf[k]);
+            if (!rc) rc = upload(d_bc[k], bc[k]);
+        }
+        if (!rc) rc = upload(d_cstart, cstart);
+        // the fill at t - possk: copysol of the old values, bound with 2
+        // nghost, coarsen; the copy of the new values with their ghost cells
+        if (!rc) rc = run_fill(level, L.time - L.possk, D, arena);
+        const Level3Geom& G = A.H.lev[level];
+        if (!rc) {
+            Step3IO io{};
+            io.q = arena;
+            io.face = face;
+            io.d_patches = d_pc;
+            for (int k = 0; k < 4; k++) {
+                io.d_blocks[k] = d_bc[k];
+                io.nblocks[k] = (int)bc[k].size();
+            }
+            io.npatch = n;
+            io.d_cfl = L.d_cfl;
+            rc = ctx3_step(io, 2.0 * G.hx, 2.0 * G.hy, 2.0 * G.hz, 2. * L.possk, 8);
+        }
+        if (!rc) {
+            Step3IO io{};
+            io.q = arena;
+            io.face = face;
+            io.d_patches = d_pf;
+            for (int k = 0; k < 4; k++) {
+                io.d_blocks[k] = d_bf[k];
+                io.nblocks[k] = (int)bf[k].size();
+            }
+            io.npatch = n;
+            io.d_cfl = L.d_cfl;
+            rc = ctx3_step(io, G.hx, G.hy, G.hz, L.possk, 8);
+        }
+        if (!rc) {
+            const double order = (double)((1 << (iorder + 1)) - 2);
+            e = launch_errf1_3(arena, d_pf, d_pc, L.d_foff, d_cstart, n, cstart[n], tol, order,
+                               L.d_rflags, d_cnt + 1, st);
+            if (e != hipSuccess) rc = ctx3_fail(AMRGPU_ERR_HIP, "errf1: %s", hipGetErrorString(e));
+        }
+        unsigned long long cnt[2] = {0, 0};
+        if (!rc) {
+            e = ctx3_d2h(cnt, d_cnt, sizeof cnt);
+            if (e != hipSuccess) rc = ctx3_fail(AMRGPU_ERR_HIP, "counts: %s", hipGetErrorString(e));
+        }
+        cleanup();
+        if (rc) return rc;
+        if (nflag) *nflag = (int64_t)cnt[0];
+        if (nrich) *nrich = (int64_t)cnt[1];
+        return AMRGPU_OK;
+    }
+    unsigned long long cnt[2] = {0, 0};
+    hipError_t e = ctx3_d2h(cnt, d_cnt, sizeof cnt);
+    hipFree(d_cnt);
+    if (e != hipSuccess) return ctx3_fail(AMRGPU_ERR_HIP, "counts: %s", hipGetErrorString(e));
+    if (nflag) *nflag = (int64_t)cnt[0];
+    if (nrich) *nrich = 0;
+    return AMRGPU_OK;
+}
+
+int amrgpu3_get_level_flags(int level, int grid, double* amrflags, double* rflags) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Lev3& L = A.lev[level];
+    if (!L.d_flags) return ctx3_fail(AMRGPU_ERR_STATE, "amrgpu3_flag_level has not been called");
+    if (grid < 0 || grid >= (int)L.patches.size())
+        return ctx3_fail(AMRGPU_ERR_ARG, "grid %d out of range", grid);
+    const size_t n = (size_t)(L.foff[grid + 1] - L.foff[grid]);
+    if (amrflags) HIPCHKA(ctx3_d2h(amrflags, L.d_flags + L.foff[grid], n * sizeof(double)));
+    if (rflags) HIPCHKA(ctx3_d2h(rflags, L.d_rflags + L.foff[grid], n * sizeof(double)));
     return AMRGPU_OK;
 }
 

@@ -29,8 +29,9 @@ int ctx3_fail(int code, const char* fmt, ...);
 hipError_t ctx3_h2d(void* dst, const void* src, size_t n);
 hipError_t ctx3_d2h(void* dst, const void* src, size_t n);
 // stepgrid.f (3-D) for every grid of the batch: clears d_cfl, launches the
-// x-, y- and z-sweeps and the conservative update
-int ctx3_step(const Step3IO& io, double dx, double dy, double dz, double dt);
+// x-, y- and z-sweeps and the conservative update; esize 8 runs the FP64
+// kernels whatever the library's precision (0: the library's)
+int ctx3_step(const Step3IO& io, double dx, double dy, double dz, double dt, int esize = 0);
 // the work lists of a batch: per sweep (grid, plane block), largest first;
 // update and totals (grid, 256 cells)
 void ctx3_plan_blocks(const std::vector<Patch3>& patches, int plane_block,

@@ -147,6 +147,21 @@ int amrgpu3_conck_level(int level, double* totmass);
  * lenbc) then qc1d(nvar, lenbc); lenbc = 2 (ik jk + jk kk + kk ik).  With
  * svdflx_qc1d NULL only lenbc is returned. */
 int amrgpu3_get_registers(int level, int grid, double* svdflx_qc1d, int* lenbc);
+/* Flagging of the level's grids, replacing the per-grid loops of spest.f
+ * (3-D):56-84 and errest.f (3-D):68-216.  spest: every grid's ghost cells
+ * at the level's time (bound.f), then flag2refine.f into a per-grid FP64
+ * array (mx+4, my+4, mz+4) of badpt (2) / goodpt (0).  With flag_richardson
+ * (levels below the finest): errest.f's copysol of the old values, bound
+ * with 2 nghost ghost cells at time - possk, coarsen.f, the two FP64
+ * stepgrid calls (2 possk on the coarsened grid, possk on the grid's new
+ * values) and errf1.f into a second array, which the reference leaves in
+ * component 1 of store2 and the device keeps apart (store2 is unchanged).
+ * nflag / nrich (may be NULL) receive the flagged cell counts.  Grids
+ * need even sizes for the Richardson estimate. */
+int amrgpu3_flag_level(int level, double tolsp, int flag_richardson, double tol, int iorder,
+                       int64_t* nflag, int64_t* nrich);
+/* The two flag arrays of grid `grid` (either pointer may be NULL). */
+int amrgpu3_get_level_flags(int level, int grid, double* amrflags, double* rflags);
 
 /* Device timing of the step's kernels: amrgpu3_set_timing(1) resets and
  * starts it; kind 0, 1, 2 = x-, y-, z-sweep, 3 = update. */

@@ -174,3 +174,71 @@ def test_periodic_mass_conserved3_on_device(gpu, solver):
             assert abs(m - m0) <= 1e-13 * max(abs(m0), 1.0), (m, m0)
     finally:
         capi3.finalize3()
+
+
+@pytest.mark.parametrize("case", [BCS[0], BCS[1], BCS[3]])
+@pytest.mark.parametrize("seed", range(2))
+def test_flag_level3_fp64_bit_exact(gpu, case, seed):
+    """amrgpu3_flag_level after a coarse step: spest's flags and errest /
+    errf1's Richardson flags of every level below the finest bit for bit
+    with the oracle (flagged fine to coarse, as regrid.f does, since the
+    reference's errf1 overwrites component 1 of store2)."""
+    bc, per = case
+    print("flag3 fp64", case, seed, flush=True)
+    rng = np.random.default_rng(600 + seed)
+    H = nested_hierarchy3(rng, smooth_euler3, 5, n=(12, 12, 8), mthbc=bc, per=per, dt1=0.004)
+    p = orc.make_params3(orc.EULER)
+    O = orc.OracleAMR3(p, H.copy())
+    _init(orc.EULER)
+    try:
+        D = capi3.DeviceAMR3(H)
+        berger_oliger3(O, O.H, 1)
+        berger_oliger3(D, H, 1)
+        for level in range(H.nlev - 1, 0, -1):
+            tolsp, tol = 0.05, 1e-4
+            f, rf, n = orc.flag_level3(O, level, tolsp, tol=tol, richardson=True)
+            nd, rd = D.flag(level, tolsp, richardson=True, tol=tol)
+            assert nd == n and n > 0, (level, nd, n)
+            assert rd == sum(int((r == 2.0).sum()) for r in rf) and rd > 0, level
+            for k, shape in enumerate(H.levels[level - 1]["shapes"]):
+                gf, gr = capi3.get_level_flags3(level, k, shape[:3])
+                assert np.array_equal(gf, f[k]), ("spest", level, k)
+                assert np.array_equal(gr, rf[k]), ("errf1", level, k)
+        # the finest level: spest only (no old time for errest)
+        f, _, n = orc.flag_level3(O, H.nlev, 0.05)
+        nd, rd = D.flag(H.nlev, 0.05)
+        assert nd == n and rd == 0
+        for k, shape in enumerate(H.levels[H.nlev - 1]["shapes"]):
+            assert np.array_equal(capi3.get_level_flags3(H.nlev, k, shape[:3])[0], f[k]), k
+        # flagging leaves the solutions as they were (bound fills ghost cells only)
+        for l in range(1, H.nlev):
+            assert np.array_equal(capi3.read_level3(H, l, 1), O.H.levels[l - 1]["qold"]), l
+    finally:
+        capi3.finalize3()
+
+
+@pytest.mark.parametrize("precision", [capi3.FP64, capi3.FP32])
+def test_flag_level3_uniform_state(gpu, precision):
+    """A uniform state has no Richardson error in either precision: nothing
+    flagged at tol 1e-12, every interior cell at tol 0."""
+    print("flag3 uniform", precision, flush=True)
+    rng = np.random.default_rng(5)
+    H = nested_hierarchy3(rng, smooth_euler3, 5, n=(12, 12, 8), dt1=0.004)
+    for L in H.levels:
+        if len(L["boxes"]):
+            for which in ("qnew", "qold"):
+                L[which].reshape(-1, 5)[:] = (1.1, 0.2, -0.1, 0.05, 2.9)
+        L["time"] = L["possk"]
+    _init(orc.EULER, precision)
+    try:
+        D = capi3.DeviceAMR3(H)
+        nd, rd = D.flag(1, 0.05, richardson=True, tol=1e-12)
+        assert nd == 0 and rd == 0
+        nd, rd = D.flag(1, 0.05, richardson=True, tol=0.0)
+        boxes = H.levels[0]["boxes"]
+        assert rd == sum((b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1) for b in boxes)
+        for k, shape in enumerate(H.levels[0]["shapes"]):
+            _, r = capi3.get_level_flags3(1, k, shape[:3])
+            assert (r[2:-2, 2:-2, 2:-2] == 2.0).all() and r.sum() == 2.0 * r[2:-2, 2:-2, 2:-2].size
+    finally:
+        capi3.finalize3()
