@@ -320,7 +320,7 @@ int plan_level(Level& L) {
                 t.wy = (int16_t)seg_width(t.ty);
                 L.tiles.push_back(t);
                 size_t b = (g.esize == 8 && !g.mixed) ? sweep_lds_bytes<double>(nv, t.tx, t.ty)
-                                                      : sweep_lds_bytes<float>(nv, t.tx, t.ty);
+                                                      : sweep_lds_bytes<float>(nv, t.tx, t.ty, g.mixed);
                 lds = std::max(lds, b);
             }
     }

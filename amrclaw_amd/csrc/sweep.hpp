@@ -207,11 +207,15 @@ __device__ __forceinline__ T wave_max(T v) {
 
 // LDS bytes needed for a tile of tx * ty cells (compute type T)
 __host__ __device__ inline int sweep_qs_stride(int tx) { return (tx + 4) | 1; }
+// LDS of one tile: the staged states and the edge increments; `merged` (the
+// mixed path, where both cells of an edge take the same increment) keeps one
+// array per direction instead of the minus / plus pair
 template <typename T>
-__host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty) {
+__host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty, bool merged = false) {
+    const size_t k = merged ? 1 : 2;
     size_t qs = (size_t)nv * (ty + 4) * sweep_qs_stride(tx);
-    size_t f = 2 * (size_t)nv * ty * (tx + 1);
-    size_t g = 2 * (size_t)nv * (ty + 1) * tx;
+    size_t f = k * (size_t)nv * ty * (tx + 1);
+    size_t g = k * (size_t)nv * (ty + 1) * tx;
     return (qs + f + g) * sizeof(T);
 }
 
@@ -231,9 +235,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
     const int qsx = sweep_qs_stride(tx), qsy = ty + 4;
     T* qs = smem;                          // [NV][qsy][qsx], cell (k,r) at [r+1][k+1]
     T* fm = qs + NV * qsy * qsx;           // [NV][ty][tx+1]  x-edge e=1..tx+1, row j
-    T* fp = fm + NV * ty * (tx + 1);
+    // mixed path: fp == fm and gp == gm (flux form), one array each
+    T* fp = MIXED ? fm : fm + NV * ty * (tx + 1);
     T* gm = fp + NV * ty * (tx + 1);       // [NV][ty+1][tx]  y-edge e=1..ty+1, col k
-    T* gp = gm + NV * (ty + 1) * tx;
+    T* gp = MIXED ? gm : gm + NV * (ty + 1) * tx;
     const int fstr = tx + 1;               // row stride of fm/fp
     const int fpl = ty * (tx + 1);         // component stride of fm/fp
     const int gpl = (ty + 1) * tx;         // component stride of gm/gp
@@ -291,7 +296,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
                     fm[m * fpl + (r - 1) * fstr + (c - 1)] = o.fadm[m];
-                    fp[m * fpl + (r - 1) * fstr + (c - 1)] = o.fadp[m];
+                    if (!MIXED) fp[m * fpl + (r - 1) * fstr + (c - 1)] = o.fadp[m];
                 }
             }
             const bool cell = live && c >= 1 && c <= tx;       // cell k = c
@@ -342,7 +347,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                     const int gi = m * gpl + (e - 1) * tx + (c - 1);
                     T xg = gm[gi];
                     gm[gi] = xg + o.fadm[m];
-                    gp[gi] = xg + o.fadp[m];
+                    if (!MIXED) gp[gi] = xg + o.fadp[m];
                 }
             }
             const bool cell = live && e >= 1 && e <= ty;       // cell j = e
@@ -352,7 +357,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                 for (int m = 0; m < NV; m++) {
                     const int fi = m * fpl + (e - 1) * fstr + c;
                     fm[fi] = fm[fi] + o.g2[m];
-                    fp[fi] = fp[fi] + o.g2[m];
+                    if (!MIXED) fp[fi] = fp[fi] + o.g2[m];
                 }
             }
             __syncthreads();
@@ -362,7 +367,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                 for (int m = 0; m < NV; m++) {
                     const int fi = m * fpl + (e - 1) * fstr + (c - 1);
                     fm[fi] = fm[fi] + o.g1[m];
-                    fp[fi] = fp[fi] + o.g1[m];
+                    if (!MIXED) fp[fi] = fp[fi] + o.g1[m];
                 }
             }
             __syncthreads();
@@ -515,9 +520,21 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
     if ((tid & 63) == 0) atomic_max_nonneg(&a.cfl_patch[td.patch], double(cfl));
 }
 
+// workgroups per CU the register allocation is bounded for: FP32 arithmetic
+// on FP64 storage (the mixed path, merged edge arrays of about 20 KB of LDS
+// at 28x12 tiles) fits 7 without spilling; the FP32 path's two edge arrays
+// per direction keep it at 5 by LDS
+#ifndef AMRGPU_MIXED_WGS
+#define AMRGPU_MIXED_WGS 7
+#endif
+template <typename T, typename S>
+constexpr int sweep_min_wgs() {
+    return sizeof(T) == 4 ? (sizeof(S) == 8 ? AMRGPU_MIXED_WGS : 5) : 1;
+}
+
 // batched launch: one workgroup per tile of the tile table
 template <class Sol, typename T, typename S>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1) void sweep_kernel(SweepArgs a) {
+__global__ __launch_bounds__(256, (sweep_min_wgs<T, S>())) void sweep_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     sweep_tile<Sol, T, S>(a, blockIdx.x, smem_raw);
 }
@@ -526,7 +543,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1) void sweep_kernel(Swee
 // table in order (largest first) through the counter a.next, which starts
 // at zero; every workgroup leaves once the counter has passed the table
 template <class Sol, typename T, typename S>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 5 : 1) void sweep_persistent_kernel(SweepArgs a) {
+__global__ __launch_bounds__(256, (sweep_min_wgs<T, S>())) void sweep_persistent_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int tile;
     for (;;) {
