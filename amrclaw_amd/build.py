@@ -5,7 +5,10 @@
 Every HIP translation unit is built with packed FP32 instructions disabled
 (-Xclang -target-feature -Xclang -packed-fp32-ops); the FP64 sweep is built
 with -ffp-contract=off so that no multiply-add is fused and its results match
-the reference's FP64 arithmetic bit for bit.
+the reference's FP64 arithmetic bit for bit.  The FP32 and mixed sweeps keep
+IEEE semantics too (no -ffinite-math-only / -ffast-math): the Roe solver's
+entropy fix takes its branches on comparisons that see NaN sound speeds of
+non-physical states, as the reference's does.
 """
 import concurrent.futures as cf
 import os

@@ -176,11 +176,13 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
         return gamma1 * (en - Mth::div(T(0.5) * (mx * mx + my * my), rho));
     }
 
-    // f(q) = (rho u, rho u^2 + p, rho u v, u (E + p)) along momentum mu
+    // f(q) = (rho u, rho u^2 + p, rho u v, u (E + p)) along momentum mu (the
+    // mixed path's FP64 fluxes): one division, by rho, shared by p and u
     __device__ static void flux(const Physics& ph, int ixy, const T* q, T f[NV]) {
         const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
-        const T p = pressure(T(ph.gamma) - T(1), q[0], q[1], q[2], q[3]);
-        const T u = Mth::div(q[mu], q[0]);
+        const T rinv = Mth::div(T(1), q[0]);
+        const T p = (T(ph.gamma) - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * rinv);
+        const T u = q[mu] * rinv;
         f[0] = q[mu];
         f[mu] = q[mu] * u + p;
         f[mv] = q[mv] * u;
