@@ -78,6 +78,8 @@ template <typename T, class Mth = ExactMath<T>> struct Advection {
     static constexpr int NV = 1, NW = 1;
     using Math = Mth;
     struct Cache {};
+    // components of a wave that are zero by construction (none here)
+    __device__ static constexpr bool wave_zero(int, int, int) { return false; }
     __device__ static Cache prep(const Physics&, int, const T*, const T*) { return {}; }
     __device__ static void rpn2(const Physics& ph, int ixy, const T* ql, const T* qr,
                                 const T*, const T*, const Cache&, T wave[NW][NV],
@@ -107,6 +109,9 @@ template <typename T, class Mth = ExactMath<T>> struct Acoustics {
     static constexpr int NV = 3, NW = 2;
     using Math = Mth;
     struct Cache {};
+    // components of a wave that are zero by construction: the transverse
+    // momentum of both waves
+    __device__ static constexpr bool wave_zero(int ixy, int, int m) { return m == (ixy == 1 ? 2 : 1); }
     __device__ static Cache prep(const Physics&, int, const T*, const T*) { return {}; }
     __device__ static void rpn2(const Physics& ph, int ixy, const T* ql, const T* qr,
                                 const T*, const T*, const Cache&, T wave[NW][NV],
@@ -151,6 +156,14 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
     static constexpr int NV = 4, NW = 4;
     using Math = Mth;
     struct Cache { T u, v, enth, a, g1a2, euv, u2v2; };
+    // components of a wave that are zero by construction: density and normal
+    // momentum of the shear wave (wave 2)
+    __device__ static constexpr bool wave_zero(int ixy, int mw, int m) {
+        return mw == 1 && (m == 0 || m == (ixy == 1 ? 1 : 2));
+    }
+    // FP32 arithmetic leaves out the products with those zero components
+    // (x + s*0); the FP64 instantiation keeps the reference's sequence
+    static constexpr bool SKIPZ = sizeof(T) == 4;
 
     __device__ static Cache prep(const Physics& ph, int ixy, const T* L, const T* R) {
         const int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
@@ -217,7 +230,9 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
 #pragma unroll
             for (int mw = 0; mw < 4; mw++)
 #pragma unroll
-                for (int m = 0; m < 4; m++) am[m] = am[m] + dmin1(s[mw], T(0)) * wave[mw][m];
+                for (int m = 0; m < 4; m++)
+                    if (!(SKIPZ && wave_zero(ixy, mw, m)))
+                        am[m] = am[m] + dmin1(s[mw], T(0)) * wave[mw][m];
         } else {
             T rhoim1 = L[0];
             T pim1 = pressure(gamma1, L[0], L[1], L[2], L[3]);
@@ -243,7 +258,8 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
                 if (s[1] < T(0)) {
 #pragma unroll
                     for (int m = 0; m < 4; m++)
-                        am[m] = am[m] + s[1] * wave[1][m] + s[2] * wave[2][m];
+                        am[m] = (SKIPZ && wave_zero(ixy, 1, m) ? am[m] : am[m] + s[1] * wave[1][m]) +
+                                s[2] * wave[2][m];
                     T rhoi = R[0];
                     T pi_ = pressure(gamma1, R[0], R[1], R[2], R[3]);
                     T ci = Mth::sqrt(Mth::div(gamma * pi_, rhoi));
@@ -276,12 +292,14 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
             if (ph.efix) {
                 T df = T(0);
 #pragma unroll
-                for (int mw = 0; mw < 4; mw++) df = df + s[mw] * wave[mw][m];
+                for (int mw = 0; mw < 4; mw++)
+                    if (!(SKIPZ && wave_zero(ixy, mw, m))) df = df + s[mw] * wave[mw][m];
                 apdq[m] = df - am[m];
             } else {
                 T ap = T(0);
 #pragma unroll
-                for (int mw = 0; mw < 4; mw++) ap = ap + dmax1(s[mw], T(0)) * wave[mw][m];
+                for (int mw = 0; mw < 4; mw++)
+                    if (!(SKIPZ && wave_zero(ixy, mw, m))) ap = ap + dmax1(s[mw], T(0)) * wave[mw][m];
                 apdq[m] = ap;
             }
         }
@@ -311,6 +329,9 @@ template <typename T, class Mth = ExactMath<T>> struct EulerRoe {
             T xm = T(0), xp = T(0);
 #pragma unroll
             for (int mw = 0; mw < 4; mw++) {
+                // wave 3 of the transverse split carries no density and no
+                // transverse momentum
+                if (SKIPZ && mw == 2 && (m == 0 || m == mv)) continue;
                 xm = xm + dmin1(sb[mw], T(0)) * wb[mw][m];
                 xp = xp + dmax1(sb[mw], T(0)) * wb[mw][m];
             }

@@ -130,6 +130,10 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
                                            const T* qr, bool inner, int w, T dtdx,
                                            LaneResult<Sol, T>& o) {
     constexpr int NV = Sol::NV, NW = Sol::NW;
+    // FP32 arithmetic (no bit-exactness requirement) leaves out the terms of
+    // the wave components that are zero by construction and the additions
+    // of zero that the reference's sums start from
+    constexpr bool SKIPZ = sizeof(T) == 4;
     typename Sol::Cache cache = Sol::prep(M.ph, ixy, ql, qr);
     T wave[NW][NV], s[NW], amdq[NV], apdq[NV];
     Sol::rpn2(M.ph, ixy, ql, qr, nullptr, nullptr, cache, wave, s, amdq, apdq);
@@ -137,14 +141,14 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
     if (FLUXFORM) {
 #pragma unroll
         for (int m = 0; m < NV; m++) {
-            o.fadm[m] = T(0) + amdq[m];
+            o.fadm[m] = SKIPZ ? amdq[m] : T(0) + amdq[m];
             o.fadp[m] = o.fadm[m];
         }
     } else {
 #pragma unroll
         for (int m = 0; m < NV; m++) {
-            o.fadm[m] = T(0) + amdq[m];
-            o.fadp[m] = T(0) - apdq[m];
+            o.fadm[m] = SKIPZ ? amdq[m] : T(0) + amdq[m];
+            o.fadp[m] = SKIPZ ? -apdq[m] : T(0) - apdq[m];
         }
     }
     T cfl = T(0);
@@ -163,6 +167,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
             T wn2 = T(0), dotr = T(0);
 #pragma unroll
             for (int m = 0; m < NV; m++) {
+                if (SKIPZ && Sol::wave_zero(ixy, mw, m)) continue;
                 T wnext = lane_down(wave[mw][m], w);
                 wn2 = wn2 + wave[mw][m] * wave[mw][m];
                 dotr = dotr + wave[mw][m] * wnext;
@@ -174,7 +179,8 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
                 T r = Mth::ratio((s[mw] > T(0)) ? dotl : dotr, wn2);
                 T wl = phi_limiter<Mth>(mth, r);
 #pragma unroll
-                for (int m = 0; m < NV; m++) wave[mw][m] = wl * wave[mw][m];
+                for (int m = 0; m < NV; m++)
+                    if (!(SKIPZ && Sol::wave_zero(ixy, mw, m))) wave[mw][m] = wl * wave[mw][m];
             }
         }
         const T dtdxave = T(0.5) * (dtdx + dtdx);
@@ -183,6 +189,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
             T c = T(0);
 #pragma unroll
             for (int mw = 0; mw < NW; mw++) {
+                if (SKIPZ && Sol::wave_zero(ixy, mw, m)) continue;
                 T abs_sign = M.use_fwaves ? copysign(T(1), s[mw]) : fabs(s[mw]);
                 c = c + abs_sign * (T(1) - fabs(s[mw]) * dtdxave) * wave[mw][m];
             }
@@ -208,8 +215,8 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
         for (int m = 0; m < NV; m++) {
             T nbm = lane_down(bm1[m], w);   // imp=1 result of interface k+1
             T nbp = lane_down(bp1[m], w);
-            o.g1[m] = (T(0) - hdt * nbm) - hdt * bm2[m];
-            o.g2[m] = (T(0) - hdt * nbp) - hdt * bp2[m];
+            o.g1[m] = (SKIPZ ? -(hdt * nbm) : T(0) - hdt * nbm) - hdt * bm2[m];
+            o.g2[m] = (SKIPZ ? -(hdt * nbp) : T(0) - hdt * nbp) - hdt * bp2[m];
         }
     } else {
 #pragma unroll
