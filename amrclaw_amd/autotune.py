@@ -1,10 +1,12 @@
-"""Autotuner for the sweep's tile shape.
+"""Autotuner for the sweep's tile shape and waves per CU.
 
 Times the level sweep of a representative level for each candidate pair of
-tile caps (txmax, tymax) and records the fastest in tuning.json, keyed by
-solver and precision.  The tile caps fix the LDS footprint of a workgroup
-and with it how many workgroups (waves) share a CU, so this also picks the
-waves per CU.
+tile caps (txmax, tymax) and, on the mixed path, each occupancy bound (the
+workgroups per CU, i.e. waves per SIMD, its register allocation is bounded
+for: 5, 6 or 7), and records the fastest in tuning.json, keyed by solver and
+precision.  The tile caps fix the LDS footprint of a workgroup, which also
+caps the workgroups per CU; on the FP32 and FP64 paths that is the only
+occupancy knob.
 
     python -m amrclaw_amd.autotune [--precision mixed|fp32|fp64] [--npatch N] [--out PATH]
     python -m amrclaw_amd.autotune --workload euler3d   # plane block of the 3-D step
@@ -29,6 +31,8 @@ CANDIDATES = {
 # the mixed path's tiles hold FP32 copies, like the FP32 path's
 CANDIDATES["mixed"] = CANDIDATES["fp32"]
 PLANE_BLOCKS = [2, 3, 4, 6, 8]
+# occupancy bounds per precision (0: the library's fixed bound)
+OCCUPANCY = {"fp32": [0], "fp64": [0], "mixed": [5, 6, 7]}
 PRECISIONS = {"fp32": capi.FP32, "fp64": capi.FP64, "mixed": capi.MIXED}
 
 
@@ -40,6 +44,16 @@ def load(solver, precision):
         d = json.load(f)
     v = d.get("%d/%s" % (solver, precision))
     return tuple(v["tiling"]) if v else None
+
+
+def load_occupancy(solver, precision):
+    """Tuned occupancy bound (workgroups per CU) for a solver/precision, or None."""
+    if not os.path.exists(CACHE):
+        return None
+    with open(CACHE) as f:
+        d = json.load(f)
+    v = d.get("%d/%s" % (solver, precision))
+    return int(v["wgs"]) if v and v.get("wgs") else None
 
 
 def load_plane_block(precision):
@@ -97,38 +111,54 @@ def tune3(precision="fp32", npatch=512, steps=10, verbose=True, out=CACHE):
 
 
 def tune(precision="fp32", npatch=4096, steps=10, verbose=True, out=CACHE):
+    """Tile caps and occupancy bound of the 2-D level update, timed on the
+    step the benchmark measures: advance_level of level 2 of the two-level
+    batched Euler hierarchy (ghost fill with time-interpolated coarse values,
+    the sweep with its register fluxes, qad + fluxad), level 1 advanced once
+    so that level 2 sits inside the coarse step."""
     prec = PRECISIONS[precision]
-    boxes, h, mx, my = workload.batched_euler_level(npatch)
-    capi.init(capi.make_config(capi.EULER, precision=prec))
-    capi.set_level(1, boxes, h, h)
-    qs = [workload.patch_initial_state(boxes[k], h, 1000 + k) for k in range(len(boxes))]
-    capi.put_level(1, qs, which=0)
-    capi.put_level(1, qs, which=1)
-    cells = int((mx * my).sum())
-    dt = 0.2 * h
+    H, cells = workload.batched_euler_hierarchy(npatch)
+    dt2 = H.levels[1]["possk"]
+    cfg = capi.make_config(capi.EULER, order=2, transverse=2, limiter=4, cfl=0.9, cflv1=1.0,
+                           gamma=1.4, efix=True, precision=prec)
     results = []
-    for txmax, tymax in CANDIDATES[precision]:
-        try:
-            capi.set_tiling(txmax, tymax)
-        except capi.AmrgpuError:
-            continue
-        for _ in range(2):
-            capi.sweep_level(1, dt, want_cfl=False)
-        capi.synchronize()
-        capi.event_record(0)
-        for _ in range(steps):
-            capi.sweep_level(1, dt, want_cfl=False)
-        capi.event_record(1)
-        ms = capi.event_elapsed(0, 1) / steps
-        results.append({"tiling": [txmax, tymax], "ms": ms, "cell_updates_per_s": cells / ms * 1e3})
-        if verbose:
-            print("tiling %2dx%-2d  %8.3f ms  %.3e cell-updates/s" % (txmax, tymax, ms, cells / ms * 1e3),
-                  flush=True)
-    capi.finalize()
+    for wgs in OCCUPANCY[precision]:
+        for txmax, tymax in CANDIDATES[precision]:
+            capi.init(cfg)
+            try:
+                capi.set_tiling(txmax, tymax)
+                if wgs:
+                    capi.set_occupancy(wgs)
+            except capi.AmrgpuError:
+                capi.finalize()
+                continue
+            capi.load_hierarchy(H)
+            capi.set_time(1, 0.0, 2.0 * dt2)
+            capi.advance_level(1)
+
+            def step():
+                capi.set_time(2, dt2, dt2)
+                capi.advance_level(2, want_cfl=False)
+
+            for _ in range(3):
+                step()
+            capi.synchronize()
+            capi.event_record(0)
+            for _ in range(steps):
+                step()
+            capi.event_record(1)
+            ms = capi.event_elapsed(0, 1) / steps
+            capi.finalize()
+            results.append({"tiling": [txmax, tymax], "wgs": wgs, "ms": ms,
+                            "cell_updates_per_s": cells / ms * 1e3})
+            if verbose:
+                print("tiling %2dx%-2d  wgs %d  %8.3f ms  %.3e cell-updates/s"
+                      % (txmax, tymax, wgs, ms, cells / ms * 1e3), flush=True)
     best = min(results, key=lambda r: r["ms"])
     _store("%d/%s" % (capi.EULER, precision),
-           {"tiling": best["tiling"], "ms": best["ms"], "npatch": npatch, "all": results,
-            "when": time.strftime("%Y-%m-%d %H:%M:%S")}, out)
+           {"tiling": best["tiling"], "wgs": best["wgs"], "ms": best["ms"], "npatch": npatch,
+            "step": "advance_level(2) of the two-level batched hierarchy",
+            "all": results, "when": time.strftime("%Y-%m-%d %H:%M:%S")}, out)
     return best
 
 

@@ -68,6 +68,7 @@ def lib():
         L.amrgpu_finalize.argtypes = []
         L.amrgpu_last_error.argtypes = [C.c_char_p, C.c_int]
         L.amrgpu_set_tiling.argtypes = [C.c_int, C.c_int]
+        L.amrgpu_set_occupancy.argtypes = [C.c_int]
         L.amrgpu_stepgrid.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double,
                                       C.c_double, C.c_double, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p,
@@ -149,6 +150,11 @@ def finalize():
 
 def set_tiling(txmax, tymax):
     _check(lib().amrgpu_set_tiling(txmax, tymax))
+
+
+def set_occupancy(wgs):
+    """Workgroups per CU the mixed sweep is register-bounded for (5, 6, 7; 0 default)."""
+    _check(lib().amrgpu_set_occupancy(wgs))
 
 
 def stepgrid(q, dx, dy, dt, want_fluxes=False):

@@ -129,6 +129,7 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev[NEVENTS] = {};
     int txmax = 28, tymax = 28;
+    int sweep_wgs = 0;   // occupancy bound of the mixed sweep (0: its default)
     Level levels[MAXLV];
     int mxnest = 0;
     int iregsz[MAXLV] = {}, jregsz[MAXLV] = {}, ratx[MAXLV] = {}, raty[MAXLV] = {};
@@ -750,7 +751,7 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     a.fine_patches = rl.fine_patches;
     a.sv = rl.sv;
     auto launch = [&](const SweepArgs& x) {
-        if (g.mixed) return launch_sweep_mixed(g.cfg.solver, x, L.lds, g.stream);
+        if (g.mixed) return launch_sweep_mixed(g.cfg.solver, x, L.lds, g.stream, g.sweep_wgs);
         return g.esize == 8 ? launch_sweep_f64(g.cfg.solver, x, L.lds, g.stream)
                             : launch_sweep_f32(g.cfg.solver, x, L.lds, g.stream);
     };
@@ -1257,7 +1258,7 @@ int amrgpu_init(const amrgpu_config* cfg) {
         SweepArgs none{};
         HIPCHK(launch_sweep_f64(g.cfg.solver, none, 0, g.stream));
         HIPCHK(launch_sweep_f32(g.cfg.solver, none, 0, g.stream));
-        HIPCHK(launch_sweep_mixed(g.cfg.solver, none, 0, g.stream));
+        for (int w : {5, 6, 7}) HIPCHK(launch_sweep_mixed(g.cfg.solver, none, 0, g.stream, w));
     }
     g.txmax = 28;
     g.tymax = (g.esize == 8 && !g.mixed) ? 12 : 28;
@@ -1285,6 +1286,16 @@ int amrgpu_finalize(void) {
 int amrgpu_last_error(char* buf, int len) {
     if (!buf || len <= 0) return AMRGPU_ERR_ARG;
     std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_occupancy(int wgs) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (wgs != 0 && (wgs < 5 || wgs > 7))
+        return fail(AMRGPU_ERR_ARG, "occupancy bound must be 5, 6 or 7 workgroups per CU (0: default)");
+    if (wgs == g.sweep_wgs) return AMRGPU_OK;
+    g.sweep_wgs = wgs;
+    for (auto& L : g.levels) free_graphs(L);   // captured launches name the kernel
     return AMRGPU_OK;
 }
 

@@ -8,7 +8,8 @@
 namespace amrgpu {
 hipError_t launch_sweep_f64(int solver, const SweepArgs& a, size_t lds, hipStream_t st);
 hipError_t launch_sweep_f32(int solver, const SweepArgs& a, size_t lds, hipStream_t st);
-hipError_t launch_sweep_mixed(int solver, const SweepArgs& a, size_t lds, hipStream_t st);
+// wgs: the occupancy bound of the mixed sweep, 5, 6 or 7 workgroups per CU (0: 7)
+hipError_t launch_sweep_mixed(int solver, const SweepArgs& a, size_t lds, hipStream_t st, int wgs);
 hipError_t launch_finereg(int solver, int esize, const QadArgs& a, const Physics& ph, hipStream_t st);
 hipError_t launch_update(int esize, const UpdateArgs& a, const int64_t* d_block_start,
                          int64_t navg_cells, hipStream_t st);

@@ -550,21 +550,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
     if ((tid & 63) == 0) atomic_max_nonneg(&a.cfl_patch[td.patch], double(cfl));
 }
 
-// workgroups per CU the register allocation is bounded for: FP32 arithmetic
-// on FP64 storage (the mixed path, merged edge arrays of about 20 KB of LDS
-// at 28x12 tiles) fits 7 without spilling; the FP32 path's two edge arrays
-// per direction keep it at 5 by LDS
-#ifndef AMRGPU_MIXED_WGS
-#define AMRGPU_MIXED_WGS 7
-#endif
-template <typename T, typename S>
-constexpr int sweep_min_wgs() {
-    return sizeof(T) == 4 ? (sizeof(S) == 8 ? AMRGPU_MIXED_WGS : 5) : 1;
-}
-
-// batched launch: one workgroup per tile of the tile table
-template <class Sol, typename T, typename S>
-__global__ __launch_bounds__(256, (sweep_min_wgs<T, S>())) void sweep_kernel(SweepArgs a) {
+// batched launch: one workgroup per tile of the tile table.  WGS: the
+// workgroups per CU the register allocation is bounded for (a tuning knob:
+// the mixed path is instantiated for 5, 6 and 7)
+template <class Sol, typename T, typename S, int WGS>
+__global__ __launch_bounds__(256, WGS) void sweep_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     sweep_tile<Sol, T, S>(a, blockIdx.x, smem_raw);
 }
@@ -572,8 +562,8 @@ __global__ __launch_bounds__(256, (sweep_min_wgs<T, S>())) void sweep_kernel(Swe
 // persistent launch: a grid sized to fill the device takes tiles from the
 // table in order (largest first) through the counter a.next, which starts
 // at zero; every workgroup leaves once the counter has passed the table
-template <class Sol, typename T, typename S>
-__global__ __launch_bounds__(256, (sweep_min_wgs<T, S>())) void sweep_persistent_kernel(SweepArgs a) {
+template <class Sol, typename T, typename S, int WGS>
+__global__ __launch_bounds__(256, WGS) void sweep_persistent_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int tile;
     for (;;) {

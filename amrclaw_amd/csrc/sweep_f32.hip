@@ -7,11 +7,12 @@
 namespace amrgpu {
 
 hipError_t launch_sweep_f32(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
-    return sweep_dispatch<float, FastMathF32, float>(solver, a, lds, st);
+    return sweep_dispatch<float, FastMathF32, float>(solver, a, lds, st, 0);
 }
 
-hipError_t launch_sweep_mixed(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
-    return sweep_dispatch<float, FastMathF32, double>(solver, a, lds, st);
+hipError_t launch_sweep_mixed(int solver, const SweepArgs& a, size_t lds, hipStream_t st,
+                              int wgs) {
+    return sweep_dispatch<float, FastMathF32, double>(solver, a, lds, st, wgs);
 }
 
 }  // namespace amrgpu

@@ -6,7 +6,7 @@
 namespace amrgpu {
 
 hipError_t launch_sweep_f64(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
-    return sweep_dispatch<double, ExactMath<double>, double>(solver, a, lds, st);
+    return sweep_dispatch<double, ExactMath<double>, double>(solver, a, lds, st, 0);
 }
 
 }  // namespace amrgpu

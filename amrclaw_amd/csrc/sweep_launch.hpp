@@ -8,14 +8,14 @@
 
 namespace amrgpu {
 
-template <class Sol, typename T, typename S>
+template <class Sol, typename T, typename S, int WGS>
 static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     static bool attr_set = false;
     if (!attr_set) {
         // allow tiles whose LDS footprint exceeds the 64 KiB default: the
         // CU's 160 KiB less the kernel's static LDS
-        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S>,
-                              (const void*)sweep_persistent_kernel<Sol, T, S>};
+        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S, WGS>,
+                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS>};
         for (const void* f : fns) {
             hipFuncAttributes fa{};
             hipError_t e = hipFuncGetAttributes(&fa, f);
@@ -28,7 +28,7 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     }
     if (a.ntiles == 0) return hipSuccess;   // (a call with no tiles only sets the attributes)
     if (!a.next) {
-        hipLaunchKernelGGL((sweep_kernel<Sol, T, S>), dim3(a.ntiles), dim3(256), lds, st, a);
+        hipLaunchKernelGGL((sweep_kernel<Sol, T, S, WGS>), dim3(a.ntiles), dim3(256), lds, st, a);
         return hipGetLastError();
     }
     // persistent: as many workgroups as fit on the device at this LDS size
@@ -37,7 +37,7 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     if (grid == 0 || grid_lds != lds) {
         int per_cu = 0, dev = 0, ncu = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S>, 256, lds);
+            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S, WGS>, 256, lds);
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
@@ -45,17 +45,38 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
         grid_lds = lds;
     }
     const int g = std::min(grid, a.ntiles);
-    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S>), dim3(g), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S, WGS>), dim3(g), dim3(256), lds, st, a);
     return hipGetLastError();
 }
 
-template <typename T, class Mth, typename S>
-static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
+template <typename T, class Mth, typename S, int WGS>
+static hipError_t sweep_dispatch_w(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
     switch (solver) {
-    case SOLVER_ADVECTION: return sweep_go<Advection<T, Mth>, T, S>(a, lds, st);
-    case SOLVER_ACOUSTICS: return sweep_go<Acoustics<T, Mth>, T, S>(a, lds, st);
-    case SOLVER_EULER: return sweep_go<EulerRoe<T, Mth>, T, S>(a, lds, st);
+    case SOLVER_ADVECTION: return sweep_go<Advection<T, Mth>, T, S, WGS>(a, lds, st);
+    case SOLVER_ACOUSTICS: return sweep_go<Acoustics<T, Mth>, T, S, WGS>(a, lds, st);
+    case SOLVER_EULER: return sweep_go<EulerRoe<T, Mth>, T, S, WGS>(a, lds, st);
     default: return hipErrorInvalidValue;
+    }
+}
+
+// the occupancy bound: the mixed path (FP32 arithmetic on FP64 storage, one
+// edge array per direction, about 20 KB of LDS per tile) comes in 5, 6 and 7
+// workgroups per CU (0: 7, measured best on the batched Euler level); the
+// FP32 path's two edge arrays per direction keep it at 5 by LDS; the FP64
+// path is left unbounded
+template <typename T, class Mth, typename S>
+static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hipStream_t st,
+                                 int wgs) {
+    if constexpr (sizeof(T) == 4 && sizeof(S) == 8) {
+        if (wgs == 5) return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
+        if (wgs == 6) return sweep_dispatch_w<T, Mth, S, 6>(solver, a, lds, st);
+        return sweep_dispatch_w<T, Mth, S, 7>(solver, a, lds, st);
+    } else if constexpr (sizeof(T) == 4) {
+        (void)wgs;
+        return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
+    } else {
+        (void)wgs;
+        return sweep_dispatch_w<T, Mth, S, 1>(solver, a, lds, st);
     }
 }
 

@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--npatch", type=int, default=4096)
+    ap.add_argument("--occupancy", type=int, default=0,
+                    help="mixed sweep's workgroups per CU (5, 6, 7; default: tuned)")
     ap.add_argument("--txmax", type=int, default=0)
     ap.add_argument("--tymax", type=int, default=0)
     ap.add_argument("--precision", choices=["mixed", "fp32", "fp64"], default="mixed",
@@ -104,20 +106,24 @@ def copy_calibration(mib):
     print(json.dumps({"copy_bytes": n * 4, "dispatches": 5}))
 
 
-def install(capi, H, precision, launch, graphs, txmax, tymax):
+def install(capi, H, precision, launch, graphs, txmax, tymax, occupancy=0):
     """Load the hierarchy and bring it to the timed state: level 1 advanced
     once from t = 0 (saveqc fills the level-2 registers' coarse ring),
     level 2 half way through that coarse step."""
     cfg = capi.make_config(capi.EULER, order=2, transverse=2, limiter=4, cfl=0.9,
                            cflv1=1.0, gamma=1.4, efix=True, precision=precision)
     capi.init(cfg)
+    from amrclaw_amd import autotune
+    key = {capi.FP64: "fp64", capi.FP32: "fp32", capi.MIXED: "mixed"}[precision]
     if not (txmax or tymax):
-        from amrclaw_amd import autotune
-        tuned = autotune.load(capi.EULER, "fp64" if precision == capi.FP64 else "fp32")
+        tuned = autotune.load(capi.EULER, key) or autotune.load(capi.EULER, "fp32")
         if tuned:
             txmax, tymax = tuned
     if txmax or tymax:
         capi.set_tiling(txmax, tymax)
+    wgs = occupancy or autotune.load_occupancy(capi.EULER, key)
+    if wgs and precision == capi.MIXED:
+        capi.set_occupancy(wgs)
     if launch != "batched":
         capi.set_launch_mode({"per-patch": capi.LAUNCH_PER_PATCH,
                               "persistent": capi.LAUNCH_PERSISTENT}[launch])
@@ -670,7 +676,7 @@ def main():
 
     print("bench: timed run", a.precision, file=sys.stderr, flush=True)
     launch = "per-patch" if a.per_patch else ("persistent" if a.persistent else "batched")
-    dt2 = install(capi, H, prec, launch, a.graphs, a.txmax, a.tymax)
+    dt2 = install(capi, H, prec, launch, a.graphs, a.txmax, a.tymax, a.occupancy)
     r = run(capi, dt2, a.steps, a.warmup, sync)
     q_main = interiors(capi, H) if not a.no_compare else None
     capi.finalize()

@@ -79,6 +79,11 @@ int amrgpu_last_error(char* buf, int len);
 
 /* Tile-size caps of the sweep (autotuner knob); 0 keeps the current value. */
 int amrgpu_set_tiling(int txmax, int tymax);
+/* Occupancy bound of the AMRGPU_MIXED sweep (autotuner knob, no counterpart
+ * in the reference): the workgroups per CU its register allocation is
+ * bounded for, 5, 6 or 7 (0: the default, 7).  The other precisions ignore
+ * it. */
+int amrgpu_set_occupancy(int wgs);
 
 /* Single-patch time step: stepgrid.f:24 (step2 + conservative update + dtnew)
  * on one grid with ghost cells already filled.  q_in/q_out: (nvar, mx+4,

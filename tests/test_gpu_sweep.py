@@ -204,3 +204,33 @@ def test_fp32_limiter_tiny_waves(gpu, precision):
     assert np.isfinite(qg).all()
     assert np.abs(qg - qo)[2:-2, 2:-2].max() <= 1e-6
     assert rel_l1(qg[2:-2, 2:-2], qo[2:-2, 2:-2]) <= 1e-5
+
+
+def test_mixed_occupancy_bounds_agree(gpu):
+    """The mixed sweep's occupancy variants (register bounds of 5, 6 and 7
+    workgroups per CU, the autotuner's knob) run the same arithmetic: each
+    stays within 1e-5 of the FP64 oracle and within FP32 rounding of the
+    others; out-of-range bounds are refused."""
+    solver = orc.EULER
+    p, c = both_params(solver, precision=capi.MIXED)
+    capi.init(c)
+    try:
+        rng = np.random.default_rng(77)
+        q = random_patch(rng, solver, 61, 37)
+        dx, dy, dt = 1.0 / 61, 1.0 / 37, 0.2 / 61
+        qo, _, co, _ = orc.stepgrid(p, q, dx, dy, dt)
+        got = {}
+        for wgs in (5, 6, 7, 0):
+            print("mixed occupancy", wgs, flush=True)
+            capi.set_occupancy(wgs)
+            got[wgs], _, cg, _ = capi.stepgrid(q, dx, dy, dt)
+            assert rel_l1(got[wgs][2:-2, 2:-2], qo[2:-2, 2:-2]) <= 1e-5, wgs
+            assert abs(cg - co) <= 1e-5 * co
+        for wgs in (5, 6):
+            assert rel_l1(got[wgs], got[7]) <= 1e-6, wgs
+        assert np.array_equal(got[0], got[7])
+        for bad in (4, 8, -1):
+            with pytest.raises(capi.AmrgpuError):
+                capi.set_occupancy(bad)
+    finally:
+        capi.finalize()
