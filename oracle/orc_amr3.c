@@ -3,8 +3,9 @@
  * INFRASTRUCTURE ONLY): ghost filling (bound.f, filpatch.f, intfil.f,
  * trimbd.f, prefilp.f, bc3amr.f90), the coarse-fine flux registers
  * (prepf.f, prepc.f, setuse.f, saveqc.f, icall.f, preicall.f, cstore.f,
- * qad.f, fluxsv.f, fluxad.f), refluxing and averaging (upbnd.f, update.f)
- * and conck.f, all of src/3d.  Plain-C restatements in the reference's
+ * qad.f, fluxsv.f, fluxad.f), refluxing and averaging (upbnd.f, update.f),
+ * conck.f and flagging (spest.f's bound + flag2refine.f; errest.f with
+ * copysol.f, coarsen.f and errf1.f), all of src/3d.  Plain-C restatements in the reference's
  * evaluation order; arrays in the reference's AoS layout
  * val(nvar, nrow, ncol, nfil), component fastest.
  *
