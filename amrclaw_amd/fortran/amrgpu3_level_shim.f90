@@ -60,6 +60,19 @@ module amrgpu3_level_iface
        integer(c_int), value :: level
        real(c_double) :: totmass
      end function
+     integer(c_int) function amrgpu3_flag_level(level, tolsp, flag_richardson, tol, iorder, &
+          nflag, nrich) bind(C, name="amrgpu3_flag_level")
+       import :: c_int, c_double, c_int64_t
+       integer(c_int), value :: level, flag_richardson, iorder
+       real(c_double), value :: tolsp, tol
+       integer(c_int64_t) :: nflag, nrich
+     end function
+     integer(c_int) function amrgpu3_get_level_flags(level, grid, amrflags, rflags) &
+          bind(C, name="amrgpu3_get_level_flags")
+       import :: c_int, c_double
+       integer(c_int), value :: level, grid
+       real(c_double) :: amrflags(*), rflags(*)
+     end function
   end interface
 end module amrgpu3_level_iface
 
@@ -169,3 +182,46 @@ subroutine update(level, nvar, maux)
   integer, intent(in) :: level, nvar, maux
   call amrgpu3_check(amrgpu3_update_level(level), 'update')
 end subroutine update
+
+! Replaces, in flglvl.f (3-D):64-67, the calls of spest and errest: the
+! device flags every grid of lcheck from its resident solution (spest: bound
+! at the grid's time and flag2refine; errest: copysol, bound with 2 nghost at
+! time - possk, coarsen, the two stepgrids, errf1).  As spest does, the
+! gradient flags go into dom1flags through setflags and a locbig array is
+! left in node(tempptr) for bufnst to reclaim; as errf1 does, the
+! Richardson flags go into component 1 of store2, where bufnst reads them.
+! The device's grid j-1 of lcheck is listOfGrids(listStart(lcheck)+j-1),
+! the order amrgpu3_upload_hierarchy uploads them in.
+subroutine amrgpu3_spest_errest(nvar, naux, lcheck, dom1flags, isize, jsize, ksize)
+  use amr_module
+  use amrgpu3_iface, only: amrgpu3_check
+  use amrgpu3_level_iface
+  implicit none
+  integer, intent(in) :: nvar, naux, lcheck, isize, jsize, ksize
+  integer*1 :: dom1flags(isize+2, jsize+2, ksize+2)
+  integer :: mptr, j, mitot, mjtot, mktot, n, i, loc2
+  integer(c_int64_t) :: nflag, nrich
+  real(c_double), allocatable :: fl(:), rf(:)
+  integer, external :: igetsp
+  call amrgpu3_check(amrgpu3_flag_level(lcheck, tolsp, merge(1, 0, flag_richardson), tol, &
+       iorder, nflag, nrich), 'flag_level')
+  do j = 1, numgrids(lcheck)
+     mptr = listOfGrids(listStart(lcheck) + j - 1)
+     mitot = node(ndihi, mptr) - node(ndilo, mptr) + 1 + 2 * nghost
+     mjtot = node(ndjhi, mptr) - node(ndjlo, mptr) + 1 + 2 * nghost
+     mktot = node(ndkhi, mptr) - node(ndklo, mptr) + 1 + 2 * nghost
+     n = mitot * mjtot * mktot
+     allocate(fl(n), rf(n))
+     call amrgpu3_check(amrgpu3_get_level_flags(lcheck, j - 1, fl, rf), 'get_level_flags')
+     node(tempptr, mptr) = igetsp(n * nvar)
+     if (flag_gradient) call setflags(dom1flags, isize, jsize, ksize, fl, 1, mitot, mjtot, &
+          mktot, mptr)
+     if (flag_richardson) then
+        loc2 = node(store2, mptr)
+        do i = 1, n
+           alloc(loc2 + (i - 1) * nvar) = rf(i)
+        end do
+     end if
+     deallocate(fl, rf)
+  end do
+end subroutine amrgpu3_spest_errest

@@ -80,10 +80,13 @@ def test_3d_level_shim_compiles_and_binds_exported_symbols(tmp_path):
     defined = {l.split()[-1] for l in syms if " T " in l}
     undefined = {l.split()[-1] for l in syms if " U " in l}
     assert {"advanc_", "update_", "amrgpu3_upload_hierarchy_",
-            "amrgpu3_download_hierarchy_"} <= defined
+            "amrgpu3_download_hierarchy_", "amrgpu3_spest_errest_"} <= defined
     bound = {s for s in undefined if s.startswith("amrgpu3_") and not s.endswith("_")}
     assert {"amrgpu3_advance_level", "amrgpu3_update_level", "amrgpu3_set_geometry",
-            "amrgpu3_set_level", "amrgpu3_put_level", "amrgpu3_get_level"} <= bound
+            "amrgpu3_set_level", "amrgpu3_put_level", "amrgpu3_get_level",
+            "amrgpu3_flag_level", "amrgpu3_get_level_flags"} <= bound
+    # the flagging replacement hands its flags to the reference's own routines
+    assert {"setflags_", "igetsp_"} <= undefined
     exported = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
                               text=True).stdout
     for s in bound:
