@@ -80,7 +80,7 @@ struct Level {
     FillOp* d_sq_ops = nullptr;
     FillBuf* d_sq_bufs = nullptr;
     double* d_sq_scratch = nullptr;
-    double* d_partial = nullptr;        // conck partial sums, one per grid
+    double* d_partial = nullptr;        // conck total (one double; sized one per grid)
     // grid corners rnode(cornxlo/cornylo) (default ilo*hx + xlower)
     std::vector<double> cxlo, cylo;
     // flagging: FP64 flags of every grid, region rectangles, counters
@@ -1474,11 +1474,9 @@ int amrgpu_conck(int level, double* totmass) {
     if (L.ngrids == 0) return AMRGPU_OK;
     hipError_t e = launch_conck((int)g.esize, L.d_patches, L.ngrids, L.buf[L.cur], L.d_partial, g.stream);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "conck: %s", hipGetErrorString(e));
-    std::vector<double> h(L.ngrids);
-    HIPCHK(hipMemcpyAsync(h.data(), L.d_partial, L.ngrids * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    double tot = 0.0;   // the sequential sum over the level (conck.f:36-60)
+    HIPCHK(hipMemcpyAsync(&tot, L.d_partial, sizeof(double), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
-    double tot = 0.0;
-    for (double v : h) tot = tot + v;
     *totmass = tot * L.hx * L.hy;                                   // conck.f:63
     return AMRGPU_OK;
 }

@@ -48,7 +48,7 @@ struct Context3 {
     int64_t* d_foff = nullptr;
     std::vector<int64_t> foff;
     unsigned long long* d_nflag = nullptr;
-    double* d_partial = nullptr;        // conck partial sums, one per update block
+    double* d_partial = nullptr;        // conck total (sized one per update block)
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[NKIND];
     size_t tev_used[NKIND] = {};
@@ -457,13 +457,12 @@ int amrgpu3_conck(double hx, double hy, double hz, double* totmass) {
     if (!totmass) return fail3(AMRGPU_ERR_ARG, "null totmass");
     rc = ensure_flag_buffers();
     if (rc) return rc;
-    const int nb = (int)g3.blocks[3].size();
-    HIPCHK3(launch_conck3((int)g3.esize, g3.d_q, g3.d_patches, g3.d_blocks[3], nb, g3.d_partial,
-                          g3.stream));
-    std::vector<double> part(nb);
-    if (nb) HIPCHK3(d2h3(part.data(), g3.d_partial, nb * sizeof(double)));
-    double s = 0.0;
-    for (double v : part) s += v;
+    const int np = (int)g3.patches.size();
+    double s = 0.0;   // the sequential sum over the batch (conck.f (3-D):47-60)
+    if (np) {
+        HIPCHK3(launch_conck3((int)g3.esize, g3.d_q, g3.d_patches, np, g3.d_partial, g3.stream));
+        HIPCHK3(d2h3(&s, g3.d_partial, sizeof(double)));
+    }
     *totmass = s * hx * hy * hz;
     return AMRGPU_OK;
 }

@@ -657,14 +657,12 @@ int amrgpu3_conck_level(int level, double* totmass) {
     if (rc) return rc;
     if (!totmass) return ctx3_fail(AMRGPU_ERR_ARG, "null totmass");
     Lev3& L = A.lev[level];
-    const int nb = (int)L.blocks[3].size();
-    double s = 0.0;
-    if (nb) {
-        HIPCHKA(launch_conck3((int)ctx3_esize(), L.d_new, L.d_patches, L.d_blocks[3], nb,
-                              L.d_partial, ctx3_stream()));
-        std::vector<double> part(nb);
-        HIPCHKA(ctx3_d2h(part.data(), L.d_partial, nb * sizeof(double)));
-        for (double v : part) s += v;
+    const int np = (int)L.patches.size();
+    double s = 0.0;   // the sequential sum over the level (conck.f (3-D):47-60)
+    if (np) {
+        HIPCHKA(launch_conck3((int)ctx3_esize(), L.d_new, L.d_patches, np, L.d_partial,
+                              ctx3_stream()));
+        HIPCHKA(ctx3_d2h(&s, L.d_partial, sizeof(double)));
     }
     const Level3Geom& G = A.H.lev[level];
     *totmass = s * G.hx * G.hy * G.hz;

@@ -4,8 +4,8 @@
 // thread per interior cell: the largest centred difference of any component
 // in any direction against tolsp, DOFLAG (2) or DONTFLAG (0) into the grid's
 // FP64 flag array (mx+4, my+4, mz+4), whose ghost border stays goodpt (0).
-// conck3_kernel: conck.f (3-D):47-60, the sum of q(1) over a block of 256
-// interior cells of one grid (one partial per block; the host adds them).
+// conck3_kernel: conck.f (3-D):47-60, the sum of q(1) over the interiors of
+// the grids in the reference's sequential order (one wave).
 #include "kernels3.hpp"
 
 namespace amrgpu {
@@ -41,27 +41,36 @@ __global__ __launch_bounds__(256) void flag3_kernel(const T* q, double* flags, c
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(nflag, (unsigned long long)__popcll(b));
 }
 
+// a lane's double, read by the whole wave (lane index uniform)
+__device__ __forceinline__ double lane_value3(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// conck.f (3-D):47-60 in the reference's order: one wave walks the grids in
+// order, planes k, rows j, columns i in 64-cell chunks loaded by the lanes
+// and added lane by lane: the sequential FP64 sum bit for bit
 template <typename T>
-__global__ __launch_bounds__(256) void conck3_kernel(const T* q, const Patch3* patches,
-                                                     const Block3* blocks, double* partial) {
-    __shared__ double red[256];
-    const Block3 blk = blocks[blockIdx.x];
-    const int c = blk.pb + (int)threadIdx.x;
-    double v = 0.0;
-    if (c < blk.pe) {
-        const Patch3 pt = patches[blk.patch];
-        const int mx = pt.mx, my = pt.my;
-        const int i = c % mx + 1, t = c / mx, j = t % my + 1, k = t / my + 1;
-        const int64_t mit = mx + 4, mij = mit * (my + 4);
-        v = double(q[pt.qoff + (int64_t)(k + 1) * mij + (j + 1) * mit + (i + 1)]);
+__global__ __launch_bounds__(64) void conck3_kernel(const T* q, const Patch3* patches, int npatch,
+                                                    double* out) {
+    const int lane = threadIdx.x;
+    double tot = 0.0;
+    for (int p = 0; p < npatch; p++) {
+        const Patch3 pt = patches[p];
+        const int64_t mit = pt.mx + 4, mij = mit * (pt.my + 4);
+        for (int k = 0; k < pt.mz; k++)
+            for (int j = 0; j < pt.my; j++) {
+                const T* row = q + pt.qoff + (int64_t)(k + 2) * mij + (int64_t)(j + 2) * mit + 2;
+                for (int i0 = 0; i0 < pt.mx; i0 += 64) {
+                    const int n = min(64, pt.mx - i0);
+                    const double v = lane < n ? double(row[i0 + lane]) : 0.0;
+                    for (int c = 0; c < n; c++) tot = tot + lane_value3(v, c);
+                }
+            }
     }
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (lane == 0) out[0] = tot;
 }
 
 template <typename T>
@@ -93,16 +102,15 @@ hipError_t launch_flag3(int esize, int nv, const void* q, double* flags, const i
                                  nblocks, tolsp, nflag, st);
 }
 
-hipError_t launch_conck3(int esize, const void* q, const Patch3* patches, const Block3* blocks,
-                         int nblocks, double* partial, hipStream_t st) {
-    if (nblocks == 0) return hipSuccess;
-    dim3 grid(nblocks), block(256);
+hipError_t launch_conck3(int esize, const void* q, const Patch3* patches, int npatch, double* out,
+                         hipStream_t st) {
+    if (npatch <= 0) return hipSuccess;
     if (esize == 8)
-        hipLaunchKernelGGL((conck3_kernel<double>), grid, block, 0, st,
-                           static_cast<const double*>(q), patches, blocks, partial);
+        hipLaunchKernelGGL((conck3_kernel<double>), dim3(1), dim3(64), 0, st,
+                           static_cast<const double*>(q), patches, npatch, out);
     else
-        hipLaunchKernelGGL((conck3_kernel<float>), grid, block, 0, st,
-                           static_cast<const float*>(q), patches, blocks, partial);
+        hipLaunchKernelGGL((conck3_kernel<float>), dim3(1), dim3(64), 0, st,
+                           static_cast<const float*>(q), patches, npatch, out);
     return hipGetLastError();
 }
 

@@ -15,6 +15,8 @@ hipError_t launch_update(int esize, const UpdateArgs& a, const int64_t* d_block_
                          int64_t navg_cells, hipStream_t st);
 hipError_t launch_flag(int esize, const FlagArgs& a, int ngrids, hipStream_t st);
 hipError_t launch_errf1(const ErrfArgs& a, int ngrids, hipStream_t st);
+// conck.f: the level's sum of q(1) over the interiors, in the reference's
+// sequential order, into out[0]
 hipError_t launch_conck(int esize, const PatchDesc* patches, int npatch, const void* q,
-                        double* partial, hipStream_t st);
+                        double* out, hipStream_t st);
 }  // namespace amrgpu

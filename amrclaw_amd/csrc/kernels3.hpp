@@ -18,6 +18,6 @@ hipError_t launch_update3_f32(int nv, float* q, const float* face, const Patch3*
 hipError_t launch_flag3(int esize, int nv, const void* q, double* flags, const int64_t* foff,
                         const Patch3* patches, const Block3* blocks, int nblocks, double tolsp,
                         unsigned long long* nflag, hipStream_t st);
-hipError_t launch_conck3(int esize, const void* q, const Patch3* patches, const Block3* blocks,
-                         int nblocks, double* partial, hipStream_t st);
+hipError_t launch_conck3(int esize, const void* q, const Patch3* patches, int npatch, double* out,
+                         hipStream_t st);
 }  // namespace amrgpu

@@ -123,37 +123,46 @@ __global__ __launch_bounds__(256) void average_kernel(UpdateArgs a, const int64_
     }
 }
 
-// conck.f:36-63: sum of component 1 over the interior of every grid; one
-// workgroup per grid, fixed-shape tree so the total is reproducible (the
-// reference sums sequentially, so the two agree to rounding, not bitwise)
+// a lane's double, read by the whole wave (lane index uniform)
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// conck.f:36-63 in the reference's order: one wave walks the grids in level
+// order, rows j, then columns i; each lane loads one cell of a 64-cell chunk
+// of the row (coalesced) and the chunk is added lane by lane in ascending i
+// by a loop uniform over the wave, so the total is the reference's
+// sequential FP64 sum bit for bit.  A diagnostic: one wave, latency-bound.
 template <typename S>
-__global__ __launch_bounds__(256) void conck_kernel(const PatchDesc* patches, const void* qv,
-                                                    double* partial) {
-    __shared__ double red[256];
-    const PatchDesc pd = patches[blockIdx.x];
-    const S* q = static_cast<const S*>(qv) + pd.off;
-    double acc = 0.0;
-    const int n = pd.mx * pd.my;
-    for (int c = threadIdx.x; c < n; c += 256) {
-        const int j = c / pd.mx, i = c - j * pd.mx;
-        acc += (double)q[(int64_t)(j + NGHOST) * pd.mitot + (i + NGHOST)];
+__global__ __launch_bounds__(64) void conck_kernel(const PatchDesc* patches, int npatch,
+                                                   const void* qv, double* out) {
+    const int lane = threadIdx.x;
+    double tot = 0.0;
+    for (int p = 0; p < npatch; p++) {
+        const PatchDesc pd = patches[p];
+        const S* q = static_cast<const S*>(qv) + pd.off;
+        for (int j = 0; j < pd.my; j++) {
+            const S* row = q + (int64_t)(j + NGHOST) * pd.mitot + NGHOST;
+            for (int i0 = 0; i0 < pd.mx; i0 += 64) {
+                const int n = min(64, pd.mx - i0);
+                const double v = lane < n ? (double)row[i0 + lane] : 0.0;
+                for (int k = 0; k < n; k++) tot = tot + lane_value(v, k);
+            }
+        }
     }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (lane == 0) out[0] = tot;
 }
 
 hipError_t launch_conck(int esize, const PatchDesc* patches, int npatch, const void* q,
-                        double* partial, hipStream_t st) {
+                        double* out, hipStream_t st) {
     if (npatch <= 0) return hipSuccess;
     if (esize == 8)
-        hipLaunchKernelGGL(conck_kernel<double>, dim3(npatch), dim3(256), 0, st, patches, q, partial);
+        hipLaunchKernelGGL(conck_kernel<double>, dim3(1), dim3(64), 0, st, patches, npatch, q, out);
     else
-        hipLaunchKernelGGL(conck_kernel<float>, dim3(npatch), dim3(256), 0, st, patches, q, partial);
+        hipLaunchKernelGGL(conck_kernel<float>, dim3(1), dim3(64), 0, st, patches, npatch, q, out);
     return hipGetLastError();
 }
 

@@ -105,7 +105,7 @@ def test_level_update3_fp64_bit_exact(gpu, solver, case, seed):
                     ref = O.reg(l, k).reshape(2, -1, H.nvar)
                     assert np.array_equal(qc, ref[1]), ("qc1d", l, k)
                     assert np.array_equal(sv, ref[0]), ("svdflx", l, k)
-        assert capi3.conck_level3(1) == pytest.approx(O.conck(1), rel=1e-13, abs=1e-14)
+        assert capi3.conck_level3(1) == O.conck(1)   # sequential sum, bit for bit
     finally:
         capi3.finalize3()
 

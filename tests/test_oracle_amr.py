@@ -99,3 +99,24 @@ def test_same_level_copy_is_exact():
     assert np.array_equal(left[2:12, 12:14], right[2:12, 2:4])
     # extrapolation at the physical boundary: ghost = first interior cell
     assert np.array_equal(left[2:12, 0], left[2:12, 2]) and np.array_equal(left[2:12, 1], left[2:12, 2])
+
+
+def test_conck_closed_form():
+    """conck.f:36-63: with integer densities the sum is exact in any order,
+    so the oracle's total must be sum * hx * hy exactly; ghost cells hold
+    1e30 and the other components random values, so a total that reads a
+    ghost cell or another component fails."""
+    from tests.common import nested_hierarchy, smooth_euler
+    rng = np.random.default_rng(3)
+    H = nested_hierarchy(rng, smooth_euler, 4, nlev=2, dt1=0.004)
+    total = 0
+    for k in range(len(H.levels[0]["boxes"])):
+        g = H.grid(1, k, 0)
+        g[:] = rng.normal(size=g.shape)
+        g[..., 0] = 1e30
+        dens = rng.integers(1, 1000, size=g[2:-2, 2:-2, 0].shape)
+        g[2:-2, 2:-2, 0] = dens
+        total += int(dens.sum())
+    O = orc.OracleAMR(orc.make_params(orc.EULER), H)
+    L = H.levels[0]
+    assert O.conck(1) == float(total) * L["hx"] * L["hy"]

@@ -245,3 +245,24 @@ def test_errest3_uniform_state_flags_nothing():
     f, rf, _ = orc.flag_level3(O, 1, 0.05, tol=0.0, richardson=True)
     for b, r in zip(H.levels[0]["boxes"], rf):
         assert int((r == 2.0).sum()) == (b[1] - b[0] + 1) * (b[3] - b[2] + 1) * (b[5] - b[4] + 1)
+
+
+def test_conck3_closed_form():
+    """conck.f (3-D): integer densities make the sum exact in any order, so
+    the oracle's total is sum * hx * hy * hz exactly; poisoned ghost cells
+    and random other components catch a wrong index or component."""
+    rng = np.random.default_rng(4)
+    H = nested_hierarchy3(rng, smooth_euler3, 5, nlev=2, dt1=0.004)
+    total = 0
+    for k in range(len(H.levels[0]["boxes"])):
+        g = H.grid(1, k, 0)
+        g[:] = rng.normal(size=g.shape)
+        g[..., 0] = 1e30
+        dens = rng.integers(1, 1000, size=g[2:-2, 2:-2, 2:-2, 0].shape)
+        g[2:-2, 2:-2, 2:-2, 0] = dens
+        total += int(dens.sum())
+    O = orc.OracleAMR3(orc.make_params3(orc.EULER), H)
+    L = H.levels[0]
+    assert O.conck(1) == float(total) * L["hx"] * L["hy"] * L["hz"]
+    assert orc.mass3([H.grid(1, k, 0) for k in range(len(L["boxes"]))],
+                     (L["hx"], L["hy"], L["hz"])) == O.conck(1)
