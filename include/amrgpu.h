@@ -135,8 +135,9 @@ int amrgpu_advance_level(int level, double* dtlevnew, double* cfl_level);
 int amrgpu_update_level(int level);
 
 /* conck.f:36-63: sum of component 1 over the interior of all grids of
- * `level`, times hx*hy.  Summed on the device in a fixed tree order, so it
- * agrees with the reference's sequential sum to rounding. */
+ * `level`, times hx*hy.  Summed on the device in the reference's sequential
+ * order (grids in level order, then j, then i), so the total is the
+ * reference's bit for bit.  A diagnostic: one wave, latency-bound. */
 int amrgpu_conck(int level, double* totmass);
 
 /* Lower-left corners of the grids of a level, rnode(cornxlo/cornylo)

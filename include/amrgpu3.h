@@ -95,8 +95,8 @@ int amrgpu3_flag_patches(double tolsp, int64_t* nflagged);
 int amrgpu3_get_flags(int patch, double* amrflags);
 
 /* conck.f (3-D):6: hx hy hz times the sum of q(1) over the interiors of the
- * batch's grids (FP64; summed in blocks, so it agrees with the reference's
- * sequential sum to rounding). */
+ * batch's grids, in the reference's sequential order (grids, then k, j, i):
+ * the reference's FP64 total bit for bit. */
 int amrgpu3_conck(double hx, double hy, double hz, double* totmass);
 
 /* ---- Hierarchy: the level update around the step (src/3d advanc.f) -----
@@ -141,7 +141,8 @@ int amrgpu3_bound_level(int level, double time);
 int amrgpu3_advance_level(int level, double* dtlevnew, double* cfl_level);
 /* update.f (3-D) with upbnd.f: onto `level` from level+1. */
 int amrgpu3_update_level(int level);
-/* conck.f (3-D): hx hy hz times the sum of q(1) over the level's grids. */
+/* conck.f (3-D): hx hy hz times the sum of q(1) over the level's grids, in
+ * the reference's sequential order (bit for bit). */
 int amrgpu3_conck_level(int level, double* totmass);
 /* Register block of fine grid `grid` of `level` (prepf.f): svdflx(nvar,
  * lenbc) then qc1d(nvar, lenbc); lenbc = 2 (ik jk + jk kk + kk ik).  With
