@@ -134,7 +134,8 @@ struct Context {
     int mxnest = 0;
     int iregsz[MAXLV] = {}, jregsz[MAXLV] = {}, ratx[MAXLV] = {}, raty[MAXLV] = {};
     double hxposs[MAXLV] = {}, hyposs[MAXLV] = {};
-    std::vector<unsigned long long> h_cfl;
+    unsigned long long* h_cfl = nullptr;   // pinned: the per-level-step Courant read-back
+    int h_cfl_cap = 0;
     bool regs_valid = false;
     int launch_mode = AMRGPU_LAUNCH_BATCHED;   // or per patch, or persistent
     unsigned long long* d_next = nullptr;       // tile counter of the persistent sweep
@@ -788,9 +789,17 @@ int sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev
 }
 
 int level_cfl(Level& L, double dt, double* cfl_level, double* dtlevnew) {
-    g.h_cfl.resize(L.ngrids);
-    HIPCHK(hipMemcpyAsync(g.h_cfl.data(), L.d_cfl, L.ngrids * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, g.stream));
+    if (L.ngrids > g.h_cfl_cap) {
+        if (g.h_cfl) hipHostFree(g.h_cfl);
+        g.h_cfl = nullptr;
+        g.h_cfl_cap = 0;
+        HIPCHK(hipHostMalloc((void**)&g.h_cfl, (size_t)L.ngrids * sizeof(unsigned long long),
+                             hipHostMallocDefault));
+        g.h_cfl_cap = L.ngrids;
+    }
+    if (L.ngrids > 0)
+        HIPCHK(hipMemcpyAsync(g.h_cfl, L.d_cfl, L.ngrids * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     double cmax = 0.0, dmin = RINFINITY;
     for (int p = 0; p < L.ngrids; p++) {
@@ -1278,6 +1287,7 @@ int amrgpu_finalize(void) {
     for (auto& e : g.ev)
         if (e) hipEventDestroy(e);
     hipFree(g.d_next);
+    if (g.h_cfl) hipHostFree(g.h_cfl);
     hipStreamDestroy(g.stream);
     g = Context();
     return AMRGPU_OK;
