@@ -52,7 +52,9 @@ struct Context3 {
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> tev[NKIND];
     size_t tev_used[NKIND] = {};
+    void* h_pin = nullptr;              // pinned staging of small read-backs
 };
+constexpr size_t PIN3_BYTES = 1 << 16;
 
 Context3 g3;
 std::string g3_err;
@@ -79,7 +81,21 @@ hipError_t h2d3(void* dst, const void* src, size_t n) {
     hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, g3.stream);
     return e != hipSuccess ? e : hipStreamSynchronize(g3.stream);
 }
+// device -> host, synchronous on the library stream.  Small read-backs (the
+// per-level-step Courant numbers, counts, totals) go through a pinned
+// staging block: a copy into pageable memory is a slower staged transfer,
+// and these sit on the critical path of every level step.
 hipError_t d2h3(void* dst, const void* src, size_t n) {
+    if (n <= PIN3_BYTES) {
+        if (!g3.h_pin) {
+            hipError_t e = hipHostMalloc(&g3.h_pin, PIN3_BYTES, hipHostMallocDefault);
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipMemcpyAsync(g3.h_pin, src, n, hipMemcpyDeviceToHost, g3.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g3.stream);
+        if (e == hipSuccess) std::memcpy(dst, g3.h_pin, n);
+        return e;
+    }
     hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, g3.stream);
     return e != hipSuccess ? e : hipStreamSynchronize(g3.stream);
 }
@@ -535,6 +551,8 @@ int amrgpu3_finalize(void) {
     free_events();
     if (g3.stream) hipStreamDestroy(g3.stream);
     g3.stream = nullptr;
+    if (g3.h_pin) hipHostFree(g3.h_pin);
+    g3.h_pin = nullptr;
     g3.inited = false;
     g3.timing = false;
     return AMRGPU_OK;
