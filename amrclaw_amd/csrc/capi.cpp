@@ -593,8 +593,12 @@ int time_weights(double t, FillArgs& a, uint32_t mask) {
 }
 
 // launch the stages of a fill program (ghost fill or saveqc) at time t
+// clear / nclear: an array the first stage's launch zeroes on the way (the
+// level's Courant maxima before its sweep); *cleared reports whether it ran
 int run_fill(const FillPlan& P, const FillOp* d_ops, const FillBuf* d_bufs,
-             const unsigned char* d_masks, double* d_scratch, double t) {
+             const unsigned char* d_masks, double* d_scratch, double t,
+             unsigned long long* clear = nullptr, int64_t nclear = 0, bool* cleared = nullptr) {
+    if (cleared) *cleared = false;
     FillArgs a{};
     int rc = time_weights(t, a, P.src_levels_mask);
     if (rc) return rc;
@@ -618,8 +622,13 @@ int run_fill(const FillPlan& P, const FillOp* d_ops, const FillBuf* d_bufs,
     for (int s = 0; s < P.nstages; s++) {
         a.ops = d_ops + P.stage_start[s];
         a.nops = P.stage_start[s + 1] - P.stage_start[s];
+        const bool clr = clear && nclear > 0 && a.nops > 0 && (!cleared || !*cleared);
+        a.clear = clr ? clear : nullptr;
+        a.nclear = clr ? nclear : 0;
         hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
         if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "fill stage %d: %s", s, hipGetErrorString(e));
+        if (clr && cleared) *cleared = true;
+        if (clr && !cleared) clear = nullptr;
     }
     tend(AMRGPU_KT_FILL);
     return AMRGPU_OK;
@@ -731,9 +740,9 @@ struct RegLink {
 // the sweep of every grid of a level; buf[1-cur] receives the new values
 // (the caller swaps the roles afterwards)
 int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_off_dev,
-                 const RegLink& rl = RegLink()) {
+                 const RegLink& rl = RegLink(), bool cfl_cleared = false) {
     if (L.ngrids == 0) return AMRGPU_OK;
-    HIPCHK(launch_fill_u64(L.d_cfl, 0ull, L.ngrids, g.stream));
+    if (!cfl_cleared) HIPCHK(launch_fill_u64(L.d_cfl, 0ull, L.ngrids, g.stream));
     SweepArgs a{};
     a.qin = L.buf[L.cur];
     a.qout = L.buf[1 - L.cur];
@@ -823,7 +832,9 @@ int launch_level_step(int level) {
     Level& L = g.levels[level];
     int rc = ensure_plan(level);
     if (rc) return rc;
-    rc = run_fill(L.plan, L.d_ops, L.d_bufs, L.d_masks, L.d_scratch, L.time);
+    bool cfl_cleared = false;   // the fill's first launch zeroes the Courant maxima
+    rc = run_fill(L.plan, L.d_ops, L.d_bufs, L.d_masks, L.d_scratch, L.time, L.d_cfl,
+                  L.ngrids, &cfl_cleared);
     if (rc) return rc;
     const bool has_fine = level + 1 <= g.mxnest && level + 1 < MAXLV && g.levels[level + 1].ngrids > 0;
     RegLink rl;
@@ -836,7 +847,7 @@ int launch_level_step(int level) {
         rl.sv = L.d_sv;
     }
     if (level > 1) rl.reg = L.d_reg;
-    rc = launch_sweep(L, L.possk, nullptr, nullptr, rl);
+    rc = launch_sweep(L, L.possk, nullptr, nullptr, rl, cfl_cleared);
     if (rc) return rc;
     if (level > 1 && L.nqad > 0) {
         QadArgs a{};

@@ -44,6 +44,8 @@ template <typename S>
 __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
     const FillOp op = a.ops[blockIdx.x];
     const int tid = threadIdx.x, nv = a.nv;
+    if (a.clear && blockIdx.x == 0)
+        for (int64_t t = tid; t < a.nclear; t += 256) a.clear[t] = 0ull;
     const BufRef<S> dst = bufref<S>(a, op.dst);
     const int ncell = op.ni * op.nj;
 

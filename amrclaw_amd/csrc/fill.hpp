@@ -82,6 +82,10 @@ struct FillArgs {
     double* arena;                        // kind-2 buffers
     int32_t big_old;                      // FOPF_BIG copies read store2
     int32_t pad;
+    // zeroed by workgroup 0 of the launch (the level's per-grid Courant
+    // maxima, cleared here for the sweep that follows: one launch fewer)
+    unsigned long long* clear;
+    int64_t nclear;
 };
 
 }  // namespace amrgpu
