@@ -48,6 +48,7 @@ struct Level {
     int64_t nelem = 0;
     unsigned long long* d_cfl = nullptr;
     size_t lds = 0;
+    bool layout28x12 = false;   // every tile fits 28x12 (mixed Euler sweep, fixed LDS layout)
     // bulk transfer tables: patch element offsets, planes, first cell
     int64_t* d_off = nullptr;
     int32_t* d_plane = nullptr;
@@ -326,6 +327,12 @@ int plan_level(Level& L) {
                 lds = std::max(lds, b);
             }
     }
+    // the mixed Euler sweep has a variant with its LDS laid out for 28x12
+    // tiles (constant strides) when every tile fits
+    L.layout28x12 = g.mixed && g.cfg.solver == SOLVER_EULER && !L.tiles.empty();
+    for (const auto& t : L.tiles)
+        if (t.tx > 28 || t.ty > 12) L.layout28x12 = false;
+    if (L.layout28x12) lds = std::max(lds, sweep_lds_bytes<float>(nv, 28, 12, true));
     if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH) {
         // one launch per patch: the patch's tiles are contiguous
         L.patch_tile_start.assign(L.ngrids + 1, 0);
@@ -749,6 +756,7 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     a.patches = L.d_patches;
     a.tiles = L.d_tiles;
     a.ntiles = (int)L.tiles.size();
+    a.layout28x12 = L.layout28x12 ? 1 : 0;
     a.dt = dt;
     a.dx = L.hx;
     a.dy = L.hy;

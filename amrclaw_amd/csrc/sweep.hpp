@@ -32,7 +32,7 @@ struct SweepArgs {
     const PatchDesc* patches;
     const TileDesc* tiles;
     int32_t ntiles;
-    int32_t pad0;
+    int32_t layout28x12;   // every tile fits 28x12: the fixed-layout kernel may run
     double dt, dx, dy;
     MethodArgs meth;
     unsigned long long* cfl_patch;  // per-patch max Courant number (double bits)
@@ -249,8 +249,10 @@ __host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty, bool m
     return (qs + f + g) * sizeof(T);
 }
 
-// one tile of the sweep, run by one workgroup
-template <class Sol, typename T, typename S>
+// one tile of the sweep, run by one workgroup.  LX, LY: compile-time extents
+// of the LDS layout (the tile caps; every tile fits in them), so that the
+// LDS strides are constants; 0: the tile's own extents
+template <class Sol, typename T, typename S, int LX = 0, int LY = 0>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                                            unsigned char* smem_raw) {
     constexpr int NV = Sol::NV;
@@ -262,16 +264,18 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
     const TileDesc td = a.tiles[tile];
     const PatchDesc pd = a.patches[td.patch];
     const int tx = td.tx, ty = td.ty;
-    const int qsx = sweep_qs_stride(tx), qsy = ty + 4;
+    const int sx = LX ? LX : tx, sy = LY ? LY : ty;   // layout extents
+    const int qsx = sweep_qs_stride(sx), qsy = sy + 4;
     T* qs = smem;                          // [NV][qsy][qsx], cell (k,r) at [r+1][k+1]
-    T* fm = qs + NV * qsy * qsx;           // [NV][ty][tx+1]  x-edge e=1..tx+1, row j
+    T* fm = qs + NV * qsy * qsx;           // [NV][sy][sx+1]  x-edge e=1..tx+1, row j
     // mixed path: fp == fm and gp == gm (flux form), one array each
-    T* fp = MIXED ? fm : fm + NV * ty * (tx + 1);
-    T* gm = fp + NV * ty * (tx + 1);       // [NV][ty+1][tx]  y-edge e=1..ty+1, col k
-    T* gp = MIXED ? gm : gm + NV * (ty + 1) * tx;
-    const int fstr = tx + 1;               // row stride of fm/fp
-    const int fpl = ty * (tx + 1);         // component stride of fm/fp
-    const int gpl = (ty + 1) * tx;         // component stride of gm/gp
+    T* fp = MIXED ? fm : fm + NV * sy * (sx + 1);
+    T* gm = fp + NV * sy * (sx + 1);       // [NV][sy+1][sx]  y-edge e=1..ty+1, col k
+    T* gp = MIXED ? gm : gm + NV * (sy + 1) * sx;
+    const int fstr = sx + 1;               // row stride of fm/fp
+    const int fpl = sy * (sx + 1);         // component stride of fm/fp
+    const int gsx = sx;                    // row stride of gm/gp
+    const int gpl = (sy + 1) * sx;         // component stride of gm/gp
 
     const int tid = threadIdx.x;
     const S* qin = static_cast<const S*>(a.qin) + pd.off;
@@ -286,10 +290,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
 
     // 1. stage tile + halo; zero gm (the x-phase accumulates into it)
     {
-        const int w4 = tx + 4, n = NV * qsy * w4;
+        const int w4 = tx + 4, rows = ty + 4, n = NV * rows * w4;
         for (int idx = tid; idx < n; idx += 256) {
-            int m = idx / (qsy * w4);
-            int rem = idx - m * qsy * w4;
+            int m = idx / (rows * w4);
+            int rem = idx - m * rows * w4;
             int rr = rem / w4, kk = rem - rr * w4;
             int li = (td.j0 + rr) * pd.mitot + (td.i0 + kk);   // patch-local, ghost-origin
             qs[(m * qsy + rr) * qsx + kk] = T(qin[(int64_t)m * pd.plane + li]);
@@ -334,7 +338,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             if (a.meth.transverse != 0 && cell && r <= ty) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
-                    T* g = &gm[m * gpl + r * tx + (c - 1)];
+                    T* g = &gm[m * gpl + r * gsx + (c - 1)];
                     *g = *g + o.g2[m];
                 }
             }
@@ -343,7 +347,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             if (a.meth.transverse != 0 && cell && r >= 1) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
-                    T* g = &gm[m * gpl + (r - 1) * tx + (c - 1)];
+                    T* g = &gm[m * gpl + (r - 1) * gsx + (c - 1)];
                     *g = *g + o.g1[m];
                 }
             }
@@ -374,7 +378,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             if (inner && c >= 1 && c <= tx) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
-                    const int gi = m * gpl + (e - 1) * tx + (c - 1);
+                    const int gi = m * gpl + (e - 1) * gsx + (c - 1);
                     T xg = gm[gi];
                     gm[gi] = xg + o.fadm[m];
                     if (!MIXED) gp[gi] = xg + o.fadp[m];
@@ -426,13 +430,13 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                     const double q = qin[(int64_t)m * pd.plane + li];
                     const double dx_ = (fxc[m] - fxw[m]) + (double(fm[m * fpl + j * fstr + (k + 1)]) -
                                                             double(fp[m * fpl + j * fstr + k]));
-                    const double dy_ = (fyc[m] - fys[m]) + (double(gm[m * gpl + (j + 1) * tx + k]) -
-                                                            double(gp[m * gpl + j * tx + k]));
+                    const double dy_ = (fyc[m] - fys[m]) + (double(gm[m * gpl + (j + 1) * gsx + k]) -
+                                                            double(gp[m * gpl + j * gsx + k]));
                     qout[(int64_t)m * pd.plane + li] = q - dtdx_d * dx_ - dtdy_d * dy_;
                 } else {
                     T q = qs[(m * qsy + j + 2) * qsx + k + 2];
                     T dx_ = fm[m * fpl + j * fstr + (k + 1)] - fp[m * fpl + j * fstr + k];
-                    T dy_ = gm[m * gpl + (j + 1) * tx + k] - gp[m * gpl + j * tx + k];
+                    T dy_ = gm[m * gpl + (j + 1) * gsx + k] - gp[m * gpl + j * gsx + k];
                     qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
                 }
             }
@@ -464,17 +468,20 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             double* fo = a.flux_out + a.flux_off[td.patch];
             const int64_t fpln = (int64_t)pd.plane;
             const int64_t arr = NV * fpln;
-            for (int idx = tid; idx < NV * ty * (tx + 1); idx += 256) {
-                int m = idx / fpl, rem = idx - m * fpl, j = rem / fstr, e = rem - j * fstr;
+            const int fcells = ty * (tx + 1), gcells = (ty + 1) * tx;
+            for (int idx = tid; idx < NV * fcells; idx += 256) {
+                int m = idx / fcells, rem = idx - m * fcells, j = rem / (tx + 1), e = rem - j * (tx + 1);
                 int64_t li = (int64_t)(td.j0 + j + 2) * pd.mitot + (td.i0 + e + 2) + m * fpln;
-                fo[li] = double(fm[idx]);
-                fo[arr + li] = double(fp[idx]);
+                const int lx = m * fpl + j * fstr + e;
+                fo[li] = double(fm[lx]);
+                fo[arr + li] = double(fp[lx]);
             }
-            for (int idx = tid; idx < NV * gpl; idx += 256) {
-                int m = idx / gpl, rem = idx - m * gpl, e = rem / tx, k = rem - e * tx;
+            for (int idx = tid; idx < NV * gcells; idx += 256) {
+                int m = idx / gcells, rem = idx - m * gcells, e = rem / tx, k = rem - e * tx;
                 int64_t li = (int64_t)(td.j0 + e + 2) * pd.mitot + (td.i0 + k + 2) + m * fpln;
-                fo[2 * arr + li] = double(gm[idx]);
-                fo[3 * arr + li] = double(gp[idx]);
+                const int lx = m * gpl + e * gsx + k;
+                fo[2 * arr + li] = double(gm[lx]);
+                fo[3 * arr + li] = double(gp[lx]);
             }
         }
     }
@@ -508,7 +515,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             for (int idx = tid; idx < NV * tx; idx += 256) {
                 const int k = idx / NV, m = idx - k * NV;
                 per[(int64_t)NV * pd.my + (int64_t)(td.i0 + k) * NV + m] =
-                    flux_at(gp + ty * tx + k, gpl, m, 2, (int64_t)(pd.my + 1) * pd.mitot + (td.i0 + k + 2));
+                    flux_at(gp + ty * gsx + k, gpl, m, 2, (int64_t)(pd.my + 1) * pd.mitot + (td.i0 + k + 2));
             }
         if (td.j0 == 0)
             for (int idx = tid; idx < NV * tx; idx += 256) {
@@ -530,9 +537,9 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             int stride;
             double h;
             if (s.side == 1) { src = fp + j * fstr + k; stride = fpl; h = a.dy; }
-            else if (s.side == 2) { src = gm + (j + 1) * tx + k; stride = gpl; h = a.dx; }
+            else if (s.side == 2) { src = gm + (j + 1) * gsx + k; stride = gpl; h = a.dx; }
             else if (s.side == 3) { src = fm + j * fstr + k + 1; stride = fpl; h = a.dy; }
-            else { src = gp + j * tx + k; stride = gpl; h = a.dx; }
+            else { src = gp + j * gsx + k; stride = gpl; h = a.dx; }
             if constexpr (MIXED) {
                 // F = f(state left of / below the edge) + G: sides 1 and 4
                 // look at the coarse cell's left / lower neighbour
@@ -553,10 +560,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
 // batched launch: one workgroup per tile of the tile table.  WGS: the
 // workgroups per CU the register allocation is bounded for (a tuning knob:
 // the mixed path is instantiated for 5, 6 and 7)
-template <class Sol, typename T, typename S, int WGS>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0>
 __global__ __launch_bounds__(256, WGS) void sweep_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    sweep_tile<Sol, T, S>(a, blockIdx.x, smem_raw);
+    sweep_tile<Sol, T, S, LX, LY>(a, blockIdx.x, smem_raw);
 }
 
 // persistent launch: a grid sized to fill the device takes tiles from the

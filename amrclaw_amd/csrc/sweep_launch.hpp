@@ -8,14 +8,15 @@
 
 namespace amrgpu {
 
-template <class Sol, typename T, typename S, int WGS>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0>
 static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     static bool attr_set = false;
     if (!attr_set) {
         // allow tiles whose LDS footprint exceeds the 64 KiB default: the
         // CU's 160 KiB less the kernel's static LDS
-        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S, WGS>,
-                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS>};
+        const void* fns[3] = {(const void*)sweep_kernel<Sol, T, S, WGS>,
+                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS>,
+                              (const void*)sweep_kernel<Sol, T, S, WGS, LX, LY>};
         for (const void* f : fns) {
             hipFuncAttributes fa{};
             hipError_t e = hipFuncGetAttributes(&fa, f);
@@ -28,7 +29,8 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     }
     if (a.ntiles == 0) return hipSuccess;   // (a call with no tiles only sets the attributes)
     if (!a.next) {
-        hipLaunchKernelGGL((sweep_kernel<Sol, T, S, WGS>), dim3(a.ntiles), dim3(256), lds, st, a);
+        hipLaunchKernelGGL((sweep_kernel<Sol, T, S, WGS, LX, LY>), dim3(a.ntiles), dim3(256), lds,
+                           st, a);
         return hipGetLastError();
     }
     // persistent: as many workgroups as fit on the device at this LDS size
@@ -49,12 +51,12 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <typename T, class Mth, typename S, int WGS>
+template <typename T, class Mth, typename S, int WGS, int LX = 0, int LY = 0>
 static hipError_t sweep_dispatch_w(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
     switch (solver) {
     case SOLVER_ADVECTION: return sweep_go<Advection<T, Mth>, T, S, WGS>(a, lds, st);
     case SOLVER_ACOUSTICS: return sweep_go<Acoustics<T, Mth>, T, S, WGS>(a, lds, st);
-    case SOLVER_EULER: return sweep_go<EulerRoe<T, Mth>, T, S, WGS>(a, lds, st);
+    case SOLVER_EULER: return sweep_go<EulerRoe<T, Mth>, T, S, WGS, LX, LY>(a, lds, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -71,6 +73,9 @@ static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hip
     if constexpr (sizeof(T) == 4 && sizeof(S) == 8) {
         if (wgs == 5) return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
         if (wgs == 6) return sweep_dispatch_w<T, Mth, S, 6>(solver, a, lds, st);
+        // the tuned tile caps of the Euler level: LDS strides fixed at
+        // compile time when every tile fits 28x12 (a.layout28x12)
+        if (a.layout28x12) return sweep_dispatch_w<T, Mth, S, 7, 28, 12>(solver, a, lds, st);
         return sweep_dispatch_w<T, Mth, S, 7>(solver, a, lds, st);
     } else if constexpr (sizeof(T) == 4) {
         (void)wgs;
