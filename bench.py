@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # plain-copy calibration: FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB); the
 # summaries and commands are the files named here.
 MEASURED_TRAFFIC = {
-    ("euler2d", "mixed", 4096): (1202400 * 2048 + 784000 * 1024,
+    ("euler2d", "mixed", 4096): (1200800 * 2048 + 783700 * 1024,
                                  "profiles/level_update_euler_mixed_4096.md"),
     ("euler2d", "fp32", 4096): (459126 * 2048 + 450853 * 1024,
                                 "profiles/level_update_euler_fp32_4096.md"),
