@@ -327,12 +327,14 @@ int plan_level(Level& L) {
                 lds = std::max(lds, b);
             }
     }
-    // the mixed Euler sweep has a variant with its LDS laid out for 28x12
-    // tiles (constant strides) when every tile fits
-    L.layout28x12 = g.mixed && g.cfg.solver == SOLVER_EULER && !L.tiles.empty();
+    // the FP32-arithmetic Euler sweeps (mixed and FP32 storage) have a
+    // variant with their LDS laid out for 28x12 tiles (constant strides),
+    // used when every tile fits
+    const bool f32 = g.mixed || g.esize == 4;
+    L.layout28x12 = f32 && g.cfg.solver == SOLVER_EULER && !L.tiles.empty();
     for (const auto& t : L.tiles)
         if (t.tx > 28 || t.ty > 12) L.layout28x12 = false;
-    if (L.layout28x12) lds = std::max(lds, sweep_lds_bytes<float>(nv, 28, 12, true));
+    if (L.layout28x12) lds = std::max(lds, sweep_lds_bytes<float>(nv, 28, 12, g.mixed));
     if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH) {
         // one launch per patch: the patch's tiles are contiguous
         L.patch_tile_start.assign(L.ngrids + 1, 0);

@@ -79,6 +79,7 @@ static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hip
         return sweep_dispatch_w<T, Mth, S, 7>(solver, a, lds, st);
     } else if constexpr (sizeof(T) == 4) {
         (void)wgs;
+        if (a.layout28x12) return sweep_dispatch_w<T, Mth, S, 5, 28, 12>(solver, a, lds, st);
         return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
     } else {
         (void)wgs;
