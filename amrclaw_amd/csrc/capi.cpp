@@ -48,7 +48,7 @@ struct Level {
     int64_t nelem = 0;
     unsigned long long* d_cfl = nullptr;
     size_t lds = 0;
-    bool layout28x12 = false;   // every tile fits 28x12 (mixed Euler sweep, fixed LDS layout)
+    int layout = 0;   // fixed LDS layout of the Euler sweep every tile fits (1 28x12, 2 12x12)
     // bulk transfer tables: patch element offsets, planes, first cell
     int64_t* d_off = nullptr;
     int32_t* d_plane = nullptr;
@@ -327,14 +327,17 @@ int plan_level(Level& L) {
                 lds = std::max(lds, b);
             }
     }
-    // the FP32-arithmetic Euler sweeps (mixed and FP32 storage) have a
-    // variant with their LDS laid out for 28x12 tiles (constant strides),
-    // used when every tile fits
+    // the Euler sweeps have variants with their LDS laid out for the tuned
+    // tile caps (constant strides), used when every tile fits: 28x12 for
+    // FP32 arithmetic (mixed and FP32 storage), 12x12 for FP64
     const bool f32 = g.mixed || g.esize == 4;
-    L.layout28x12 = f32 && g.cfg.solver == SOLVER_EULER && !L.tiles.empty();
+    const int lx = f32 ? 28 : 12, ly = 12;
+    L.layout = (g.cfg.solver == SOLVER_EULER && !L.tiles.empty()) ? (f32 ? 1 : 2) : 0;
     for (const auto& t : L.tiles)
-        if (t.tx > 28 || t.ty > 12) L.layout28x12 = false;
-    if (L.layout28x12) lds = std::max(lds, sweep_lds_bytes<float>(nv, 28, 12, g.mixed));
+        if (t.tx > lx || t.ty > ly) L.layout = 0;
+    if (L.layout)
+        lds = std::max(lds, f32 ? sweep_lds_bytes<float>(nv, lx, ly, g.mixed)
+                                : sweep_lds_bytes<double>(nv, lx, ly));
     if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH) {
         // one launch per patch: the patch's tiles are contiguous
         L.patch_tile_start.assign(L.ngrids + 1, 0);
@@ -758,7 +761,7 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     a.patches = L.d_patches;
     a.tiles = L.d_tiles;
     a.ntiles = (int)L.tiles.size();
-    a.layout28x12 = L.layout28x12 ? 1 : 0;
+    a.layout = L.layout;
     a.dt = dt;
     a.dx = L.hx;
     a.dy = L.hy;
@@ -1286,9 +1289,12 @@ int amrgpu_init(const amrgpu_config* cfg) {
     {
         // set the kernels' LDS limits now, outside any stream capture
         SweepArgs none{};
-        HIPCHK(launch_sweep_f64(g.cfg.solver, none, 0, g.stream));
-        HIPCHK(launch_sweep_f32(g.cfg.solver, none, 0, g.stream));
-        for (int w : {5, 6, 7}) HIPCHK(launch_sweep_mixed(g.cfg.solver, none, 0, g.stream, w));
+        for (int lay : {0, 1, 2}) {   // run-time and fixed LDS layouts
+            none.layout = lay;
+            HIPCHK(launch_sweep_f64(g.cfg.solver, none, 0, g.stream));
+            HIPCHK(launch_sweep_f32(g.cfg.solver, none, 0, g.stream));
+            for (int w : {5, 6, 7}) HIPCHK(launch_sweep_mixed(g.cfg.solver, none, 0, g.stream, w));
+        }
     }
     g.txmax = 28;
     g.tymax = (g.esize == 8 && !g.mixed) ? 12 : 28;

@@ -32,7 +32,7 @@ struct SweepArgs {
     const PatchDesc* patches;
     const TileDesc* tiles;
     int32_t ntiles;
-    int32_t layout28x12;   // every tile fits 28x12: the fixed-layout kernel (FP32 arithmetic) may run
+    int32_t layout;        // fixed LDS layout every tile fits: 1 28x12 (FP32 arithmetic), 2 12x12 (FP64); 0 none
     double dt, dx, dy;
     MethodArgs meth;
     unsigned long long* cfl_patch;  // per-patch max Courant number (double bits)

@@ -74,15 +74,16 @@ static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hip
         if (wgs == 5) return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
         if (wgs == 6) return sweep_dispatch_w<T, Mth, S, 6>(solver, a, lds, st);
         // the tuned tile caps of the Euler level: LDS strides fixed at
-        // compile time when every tile fits 28x12 (a.layout28x12)
-        if (a.layout28x12) return sweep_dispatch_w<T, Mth, S, 7, 28, 12>(solver, a, lds, st);
+        // compile time when every tile fits 28x12 (a.layout == 1)
+        if (a.layout == 1) return sweep_dispatch_w<T, Mth, S, 7, 28, 12>(solver, a, lds, st);
         return sweep_dispatch_w<T, Mth, S, 7>(solver, a, lds, st);
     } else if constexpr (sizeof(T) == 4) {
         (void)wgs;
-        if (a.layout28x12) return sweep_dispatch_w<T, Mth, S, 5, 28, 12>(solver, a, lds, st);
+        if (a.layout == 1) return sweep_dispatch_w<T, Mth, S, 5, 28, 12>(solver, a, lds, st);
         return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
     } else {
         (void)wgs;
+        if (a.layout == 2) return sweep_dispatch_w<T, Mth, S, 4, 12, 12>(solver, a, lds, st);
         return sweep_dispatch_w<T, Mth, S, 4>(solver, a, lds, st);
     }
 }
