@@ -236,6 +236,48 @@ def flux(p, ixy, q):
     return f
 
 
+def _slice3(p, q):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    n = q.shape[0]
+    return q, n, n - 2 * p.mbc
+
+
+def rpn3(p, ixyz, ql, qr):
+    """orc_rpn3 over consecutive slice cells (rows = cells 1-mbc..): returns
+    wave (n, mwaves, meqn), s (n, mwaves), amdq, apdq (n, meqn); row i holds
+    the interface between rows i-1 and i."""
+    ql, n, mx = _slice3(p, ql)
+    qr, _, _ = _slice3(p, qr)
+    wave = np.zeros((n, p.mwaves, p.meqn))
+    s = np.zeros((n, p.mwaves))
+    amdq = np.zeros((n, p.meqn))
+    apdq = np.zeros((n, p.meqn))
+    L = lib()
+    L.orc_rpn3.argtypes = [C.POINTER(OrcParams)] + [C.c_int] * 6 + [_dp] * 6
+    L.orc_rpn3(C.byref(p), ixyz, mx, p.meqn, p.mwaves, p.mbc, mx, ql, qr, wave, s, amdq, apdq)
+    return wave, s, amdq, apdq
+
+
+def rpt3(p, ixyz, icoor, ql, qr, asdq, double=False):
+    """orc_rpt3 (or orc_rptt3 with double=True): split asdq along direction
+    icoor (2: the sweep's mv, 3: its mw) into (bm, bp), rows as in rpn3."""
+    ql, n, mx = _slice3(p, ql)
+    qr, _, _ = _slice3(p, qr)
+    asdq = np.ascontiguousarray(asdq, dtype=np.float64)
+    bm = np.zeros((n, p.meqn))
+    bp = np.zeros((n, p.meqn))
+    L = lib()
+    if double:
+        L.orc_rptt3.argtypes = [C.POINTER(OrcParams)] + [C.c_int] * 9 + [_dp] * 5
+        L.orc_rptt3(C.byref(p), ixyz, icoor, 1, 1, mx, p.meqn, p.mwaves, p.mbc, mx, ql, qr,
+                    asdq, bm, bp)
+    else:
+        L.orc_rpt3.argtypes = [C.POINTER(OrcParams)] + [C.c_int] * 8 + [_dp] * 5
+        L.orc_rpt3(C.byref(p), ixyz, icoor, 1, mx, p.meqn, p.mwaves, p.mbc, mx, ql, qr, asdq,
+                   bm, bp)
+    return bm, bp
+
+
 # ------------------------------------------------------------ hierarchy ---
 MAXLV = 12
 

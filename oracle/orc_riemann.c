@@ -10,6 +10,16 @@
  *   qr(:,i-1) and its right state is ql(:,i); AMRClaw passes the same slice
  *   for both (flux2.f:125), so ql == qr == q1d there.
  * Loops run over i = 2-mbc .. mx+mbc as in the library solvers.
+ *
+ * What pins them (the reference holds no solver source or fixture): the
+ * advection and acoustics solvers by exact shift, slab and eigen identities
+ * (tests/test_oracle_sweep.py); the Euler rpn2 / rpt2 by closed forms
+ * (tests/test_oracle_euler_eigen.py): each wave is the part of the jump in
+ * its eigenspace of the Roe matrix (complex-step Jacobian of the flux at the
+ * Roe average), the shear and entropy waves have the textbook eigenvector
+ * forms, A-dq / A+dq and B-asdq / B+asdq are the spectral splits, and the
+ * Harten-Hyman fix is pinned by its two-speed identity, continuity at the
+ * sonic points and reflection symmetry (1-wave branch vs 4-wave branch).
  */
 #include <math.h>
 #include "orc.h"

@@ -20,7 +20,11 @@
  * transverse solvers split a fluctuation along velocity direction mv
  * (icoor = 2) or mw (icoor = 3) of the sweep's cyclic triple (mu, mv, mw)
  * with the interface's Roe average.  Their arithmetic is pinned by closed
- * forms only (parity unpinned against clawpack's sources).
+ * forms (tests/test_oracle_euler_eigen.py): every wave is the spectral part
+ * of the jump for its eigenvalue of the Roe matrix (the complex-step flux
+ * Jacobian at the Roe average), A-dq / A+dq and each rpt3 / rptt3 split
+ * are the spectral splits, and the entropy fix satisfies its two-speed
+ * identity and reflection symmetry.
  */
 #include <math.h>
 #include <stdlib.h>
