@@ -76,6 +76,24 @@ def build(verbose=False):
     return LIB
 
 
+def build_solver_plugin(src, out, verbose=False):
+    """Build a plug-in Riemann solver (a HIP translation unit written against
+    include/amrgpu_solver.hpp) into its own shared library, FP64-exact
+    (-ffp-contract=off).  The library is registered at run time through
+    amrgpu_register_solver; it does not link libamrgpu."""
+    os.makedirs(OBJ, exist_ok=True)
+    tmp = out + ".tmp"
+    cmd = [_hipcc()] + COMMON + ["-O3", "-ffp-contract=off", "-I" + CSRC, "-shared", src,
+                                 "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("plug-in build failed: %s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

@@ -46,6 +46,19 @@ class FillBuf(C.Structure):
                [("off", C.c_int64)]
 
 
+class AmrgpuSolverOps(C.Structure):
+    """amrgpu_solver_ops: the launcher table of a plug-in Riemann solver."""
+    _fields_ = [("abi", C.c_int), ("nvar", C.c_int), ("mwaves", C.c_int),
+                ("sweep_args_bytes", C.c_int), ("qad_args_bytes", C.c_int),
+                ("physics_bytes", C.c_int), ("name", C.c_char_p),
+                ("sweep_f64", C.c_void_p), ("sweep_f32", C.c_void_p),
+                ("sweep_mixed", C.c_void_p), ("qad", C.c_void_p)]
+
+
+SOLVER_ABI = 1
+SOLVER_PLUGIN0 = 100
+
+
 class AmrgpuError(RuntimeError):
     pass
 
@@ -65,6 +78,8 @@ def lib():
         L = C.CDLL(LIBPATH)
         dpo = C.POINTER(C.c_double)
         L.amrgpu_init.argtypes = [C.POINTER(AmrgpuConfig)]
+        L.amrgpu_register_solver.argtypes = [C.POINTER(AmrgpuSolverOps), C.POINTER(C.c_int)]
+        L.amrgpu_set_solver_params.argtypes = [C.c_int, C.c_void_p]
         L.amrgpu_finalize.argtypes = []
         L.amrgpu_last_error.argtypes = [C.c_char_p, C.c_int]
         L.amrgpu_set_tiling.argtypes = [C.c_int, C.c_int]
@@ -115,13 +130,16 @@ def _check(rc):
 def make_config(solver, *, order=2, transverse=2, limiter=4, cfl=0.9,
                 cflv1=1.0, mthbc=(1, 1, 1, 1), domain=(0.0, 1.0, 0.0, 1.0),
                 ubar=0.5, vbar=1.0, rho=1.0, bulk=4.0, gamma=1.4, efix=True,
-                precision=FP64):
+                precision=FP64, nvar=None, mwaves=None):
     """Configuration of one run (claw.data / amr.data fields, setprob
-    constants).  limiter: one mthlim code for every wave, or a list."""
+    constants).  limiter: one mthlim code for every wave, or a list.
+    A plug-in solver (id from register_solver) needs nvar and mwaves."""
     c = AmrgpuConfig()
     c.solver = solver
-    c.nvar, c.mwaves = {ADVECTION: (1, 1), ACOUSTICS: (3, 2),
-                        EULER: (4, 4)}[solver]
+    if solver in (ADVECTION, ACOUSTICS, EULER):
+        c.nvar, c.mwaves = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4)}[solver]
+    else:
+        c.nvar, c.mwaves = nvar, mwaves
     c.naux, c.nghost = 0, 2
     meth = [0, order, transverse, 0, 0, 0, 0]
     for k in range(7):
@@ -138,6 +156,34 @@ def make_config(solver, *, order=2, transverse=2, limiter=4, cfl=0.9,
     c.gamma, c.efix = gamma, int(bool(efix))
     c.precision = precision
     return c
+
+
+def load_solver_plugin(path, entry):
+    """Load a plug-in solver library (built against include/amrgpu_solver.hpp)
+    and fill its launcher table through its exported `entry` function.
+    Returns (table, the loaded library); keep both alive while the solver
+    is registered."""
+    P = C.CDLL(path)
+    fn = getattr(P, entry)
+    fn.argtypes = [C.POINTER(AmrgpuSolverOps)]
+    fn.restype = C.c_int
+    ops = AmrgpuSolverOps()
+    if fn(C.byref(ops)) != 0:
+        raise AmrgpuError("%s(%s) failed" % (entry, path))
+    return ops, P
+
+
+def register_solver(ops):
+    """amrgpu_register_solver: returns the solver id for make_config."""
+    sid = C.c_int()
+    _check(lib().amrgpu_register_solver(C.byref(ops), C.byref(sid)))
+    return sid.value
+
+
+def set_solver_params(params):
+    """amrgpu_set_solver_params: the plug-in solver's constants (/cparam/)."""
+    a = np.ascontiguousarray(params, dtype=np.float64)
+    _check(lib().amrgpu_set_solver_params(len(a), a.ctypes.data_as(C.c_void_p) if len(a) else None))
 
 
 def init(cfg):

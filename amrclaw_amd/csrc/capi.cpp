@@ -121,9 +121,17 @@ struct Level {
     std::vector<Graph> graphs;
 };
 
+// Plug-in solvers (amrgpu_register_solver): the application's tables, ids
+// AMRGPU_SOLVER_PLUGIN0 + k; registrations and constants outlive finalize.
+constexpr int NPLUGIN = 16;
+amrgpu_solver_ops g_plugins[NPLUGIN];
+int g_nplugins = 0;
+double g_solver_params[AMRGPU_SOLVER_PARAMS_MAX] = {};
+
 struct Context {
     bool inited = false;
     amrgpu_config cfg{};
+    const amrgpu_solver_ops* plugin = nullptr;   // the run's solver when it is a plug-in
     MethodArgs meth{};
     size_t esize = 8;
     bool mixed = false;   // AMRGPU_MIXED: FP32 sweep on FP64 storage, flux form
@@ -148,6 +156,28 @@ struct Context {
 };
 
 Context g;
+
+// The sweep of the run's solver in one arithmetic: 0 FP64, 1 FP32 storage,
+// 2 FP32 arithmetic on FP64 storage (mixed); wgs: the mixed occupancy bound.
+hipError_t run_sweep(int prec, const SweepArgs& x, size_t lds, int wgs) {
+    if (g.plugin) {
+        const int e = prec == 0 ? g.plugin->sweep_f64(&x, lds, g.stream)
+                    : prec == 1 ? g.plugin->sweep_f32(&x, lds, g.stream)
+                                : g.plugin->sweep_mixed(&x, lds, g.stream, wgs);
+        return (hipError_t)e;
+    }
+    if (prec == 2) return launch_sweep_mixed(g.cfg.solver, x, lds, g.stream, wgs);
+    return prec == 0 ? launch_sweep_f64(g.cfg.solver, x, lds, g.stream)
+                     : launch_sweep_f32(g.cfg.solver, x, lds, g.stream);
+}
+
+int run_precision() { return g.mixed ? 2 : (g.esize == 8 ? 0 : 1); }
+
+// qad + fluxad of a fine level (the solver's rpn2 at the coarse-fine edges)
+hipError_t run_qad(const QadArgs& a) {
+    if (g.plugin) return (hipError_t)g.plugin->qad((int)g.esize, &a, &g.meth.ph, g.stream);
+    return launch_finereg(g.cfg.solver, (int)g.esize, a, g.meth.ph, g.stream);
+}
 std::string g_err;
 
 int fail(int code, const char* fmt, ...) {
@@ -306,13 +336,23 @@ std::vector<std::pair<int, int>> split_axis(int n, int cap) {
     return out;
 }
 
-int plan_level(Level& L) {
-    L.tiles.clear();
+// The tiling of a level for tile caps (txmax, tymax) and a launch mode:
+// the tile table, the LDS bytes of its largest tile and the fixed-layout
+// variant every tile fits.  Pure: nothing of the level changes, so a caller
+// can refuse caps before any level has been re-planned.
+struct Tiling {
+    std::vector<TileDesc> tiles;
+    std::vector<int> patch_tile_start;
     size_t lds = 0;
+    int layout = 0;
+};
+
+Tiling make_tiling(const Level& L, int txmax, int tymax, int launch_mode) {
+    Tiling T;
     const int nv = g.cfg.nvar;
     for (int p = 0; p < L.ngrids; p++) {
         const PatchDesc& pd = L.patches[p];
-        auto xs = split_axis(pd.mx, g.txmax), ys = split_axis(pd.my, g.tymax);
+        auto xs = split_axis(pd.mx, txmax), ys = split_axis(pd.my, tymax);
         for (auto& y : ys)
             for (auto& x : xs) {
                 TileDesc t{};
@@ -321,10 +361,10 @@ int plan_level(Level& L) {
                 t.j0 = (int16_t)y.first; t.ty = (int16_t)y.second;
                 t.wx = (int16_t)seg_width(t.tx);
                 t.wy = (int16_t)seg_width(t.ty);
-                L.tiles.push_back(t);
+                T.tiles.push_back(t);
                 size_t b = (g.esize == 8 && !g.mixed) ? sweep_lds_bytes<double>(nv, t.tx, t.ty)
                                                       : sweep_lds_bytes<float>(nv, t.tx, t.ty, g.mixed);
-                lds = std::max(lds, b);
+                T.lds = std::max(T.lds, b);
             }
     }
     // the Euler sweeps have variants with their LDS laid out for the tuned
@@ -332,37 +372,62 @@ int plan_level(Level& L) {
     // FP32 arithmetic (mixed and FP32 storage), 12x12 for FP64
     const bool f32 = g.mixed || g.esize == 4;
     const int lx = f32 ? 28 : 12, ly = 12;
-    L.layout = (g.cfg.solver == SOLVER_EULER && !L.tiles.empty()) ? (f32 ? 1 : 2) : 0;
-    for (const auto& t : L.tiles)
-        if (t.tx > lx || t.ty > ly) L.layout = 0;
-    if (L.layout)
-        lds = std::max(lds, f32 ? sweep_lds_bytes<float>(nv, lx, ly, g.mixed)
-                                : sweep_lds_bytes<double>(nv, lx, ly));
-    if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH) {
+    T.layout = (g.cfg.solver == SOLVER_EULER && !T.tiles.empty()) ? (f32 ? 1 : 2) : 0;
+    for (const auto& t : T.tiles)
+        if (t.tx > lx || t.ty > ly) T.layout = 0;
+    if (T.layout)
+        T.lds = std::max(T.lds, f32 ? sweep_lds_bytes<float>(nv, lx, ly, g.mixed)
+                                    : sweep_lds_bytes<double>(nv, lx, ly));
+    if (launch_mode == AMRGPU_LAUNCH_PER_PATCH) {
         // one launch per patch: the patch's tiles are contiguous
-        L.patch_tile_start.assign(L.ngrids + 1, 0);
-        for (const auto& t : L.tiles) L.patch_tile_start[t.patch + 1]++;
-        for (int p = 0; p < L.ngrids; p++) L.patch_tile_start[p + 1] += L.patch_tile_start[p];
+        T.patch_tile_start.assign(L.ngrids + 1, 0);
+        for (const auto& t : T.tiles) T.patch_tile_start[t.patch + 1]++;
+        for (int p = 0; p < L.ngrids; p++) T.patch_tile_start[p + 1] += T.patch_tile_start[p];
     } else {
         // largest tiles first (the reference orders grids by cost, regrid.f:132-169)
-        std::stable_sort(L.tiles.begin(), L.tiles.end(), [](const TileDesc& a, const TileDesc& b) {
+        std::stable_sort(T.tiles.begin(), T.tiles.end(), [](const TileDesc& a, const TileDesc& b) {
             return a.tx * a.ty > b.tx * b.ty;
         });
-        L.patch_tile_start.clear();
     }
+    return T;
+}
+
+constexpr size_t LDS_MAX = 160 * 1024;
+
+// Re-tile a level with the current caps and launch mode.  A tiling whose
+// largest tile needs more LDS than a CU has is refused before anything of
+// the level changes.
+int plan_level(Level& L) {
+    Tiling T = make_tiling(L, g.txmax, g.tymax, g.launch_mode);
+    if (T.lds > LDS_MAX)
+        return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", g.txmax, g.tymax, T.lds);
     g.layout_version++;
     free_graphs(L);
-    if (lds > 160 * 1024)
-        return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", g.txmax, g.tymax, lds);
-    L.lds = lds;
     g.regs_valid = false;
     quiesce();
+    L.tiles = std::move(T.tiles);
+    L.patch_tile_start = std::move(T.patch_tile_start);
+    L.lds = T.lds;
+    L.layout = T.layout;
     if (L.d_tiles) hipFree(L.d_tiles);
     L.d_tiles = nullptr;
     if (!L.tiles.empty()) {
         HIPCHK(hipMalloc(&L.d_tiles, L.tiles.size() * sizeof(TileDesc)));
         HIPCHK(h2d(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(TileDesc)));
     }
+    return AMRGPU_OK;
+}
+
+// Whether every defined level can be tiled with caps (txmax, tymax) in a
+// launch mode; the caller changes nothing when one cannot.
+int check_tiling(int txmax, int tymax, int launch_mode) {
+    for (const auto& L : g.levels)
+        if (L.ngrids) {
+            Tiling T = make_tiling(L, txmax, tymax, launch_mode);
+            if (T.lds > LDS_MAX)
+                return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", txmax, tymax,
+                            T.lds);
+        }
     return AMRGPU_OK;
 }
 
@@ -433,7 +498,9 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
         HIPCHK(h2d(L.d_plane, pl.data(), ngrids * sizeof(int32_t)));
         HIPCHK(h2d(L.d_cell_start, cs.data(), ngrids * sizeof(int64_t)));
     }
-    return plan_level(L);
+    int rc = plan_level(L);
+    if (rc) free_level(L);   // refused tiling: leave the level without grids
+    return rc;
 }
 
 // AoS FP64 host <-> SoA device (storage precision)
@@ -773,11 +840,7 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     a.reg_fine = rl.reg_fine;
     a.fine_patches = rl.fine_patches;
     a.sv = rl.sv;
-    auto launch = [&](const SweepArgs& x) {
-        if (g.mixed) return launch_sweep_mixed(g.cfg.solver, x, L.lds, g.stream, g.sweep_wgs);
-        return g.esize == 8 ? launch_sweep_f64(g.cfg.solver, x, L.lds, g.stream)
-                            : launch_sweep_f32(g.cfg.solver, x, L.lds, g.stream);
-    };
+    auto launch = [&](const SweepArgs& x) { return run_sweep(run_precision(), x, L.lds, g.sweep_wgs); };
     hipError_t e = hipSuccess;
     tbegin(AMRGPU_KT_SWEEP);
     if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH && !L.patch_tile_start.empty()) {
@@ -877,7 +940,7 @@ int launch_level_step(int level) {
         a.delt = L.possk;
         a.flux_form = g.mixed ? 1 : 0;
         tbegin(AMRGPU_KT_REGISTERS);
-        hipError_t e = launch_finereg(g.cfg.solver, (int)g.esize, a, g.meth.ph, g.stream);
+        hipError_t e = run_qad(a);
         tend(AMRGPU_KT_REGISTERS);
         if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "register fix-up: %s", hipGetErrorString(e));
     }
@@ -1108,7 +1171,7 @@ int richardson(int level, bool big_old, int mbuff, double tol, int iorder) {
     sa.dt = 2.0 * L.possk;
     sa.dx = 2.0 * L.hx;
     sa.dy = 2.0 * L.hy;
-    hipError_t e = launch_sweep_f64(g.cfg.solver, sa, L.rlds_c, g.stream);
+    hipError_t e = run_sweep(0, sa, L.rlds_c, 0);
     if (e == hipSuccess) {
         sa.patches = L.d_rfine;
         sa.tiles = L.d_rtiles_f;
@@ -1116,7 +1179,7 @@ int richardson(int level, bool big_old, int mbuff, double tol, int iorder) {
         sa.dt = L.possk;
         sa.dx = L.hx;
         sa.dy = L.hy;
-        e = launch_sweep_f64(g.cfg.solver, sa, L.rlds_f, g.stream);
+        e = run_sweep(0, sa, L.rlds_f, 0);
     }
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "Richardson steps: %s", hipGetErrorString(e));
     ErrfArgs ea{};
@@ -1241,13 +1304,25 @@ extern "C" {
 int amrgpu_init(const amrgpu_config* cfg) {
     if (!cfg) return fail(AMRGPU_ERR_ARG, "cfg is NULL");
     if (g.inited) amrgpu_finalize();
-    const int nv_expect[4] = {0, 1, 3, 4};
-    const int nw_expect[4] = {0, 1, 2, 4};
-    if (cfg->solver < 1 || cfg->solver > 3) return fail(AMRGPU_ERR_ARG, "unknown solver %d", cfg->solver);
-    if (cfg->nvar != nv_expect[cfg->solver])
-        return fail(AMRGPU_ERR_ARG, "solver %d needs nvar=%d", cfg->solver, nv_expect[cfg->solver]);
-    if (cfg->mwaves != nw_expect[cfg->solver])
-        return fail(AMRGPU_ERR_ARG, "solver %d needs mwaves=%d", cfg->solver, nw_expect[cfg->solver]);
+    const amrgpu_solver_ops* plugin = nullptr;
+    int nv_need, nw_need;
+    if (cfg->solver >= 1 && cfg->solver <= 3) {
+        const int nv_expect[4] = {0, 1, 3, 4};
+        const int nw_expect[4] = {0, 1, 2, 4};
+        nv_need = nv_expect[cfg->solver];
+        nw_need = nw_expect[cfg->solver];
+    } else if (cfg->solver >= AMRGPU_SOLVER_PLUGIN0 &&
+               cfg->solver < AMRGPU_SOLVER_PLUGIN0 + g_nplugins) {
+        plugin = &g_plugins[cfg->solver - AMRGPU_SOLVER_PLUGIN0];
+        nv_need = plugin->nvar;
+        nw_need = plugin->mwaves;
+    } else {
+        return fail(AMRGPU_ERR_ARG, "unknown solver %d", cfg->solver);
+    }
+    if (cfg->nvar != nv_need)
+        return fail(AMRGPU_ERR_ARG, "solver %d needs nvar=%d", cfg->solver, nv_need);
+    if (cfg->mwaves != nw_need)
+        return fail(AMRGPU_ERR_ARG, "solver %d needs mwaves=%d", cfg->solver, nw_need);
     if (cfg->nghost != NGHOST) return fail(AMRGPU_ERR_UNSUPPORTED, "nghost must be 2");
     if (cfg->naux != 0 || cfg->mcapa != 0)
         return fail(AMRGPU_ERR_UNSUPPORTED, "aux arrays / capacity are not supported");
@@ -1267,6 +1342,7 @@ int amrgpu_init(const amrgpu_config* cfg) {
             return fail(AMRGPU_ERR_ARG, "mthlim(%d)=%d invalid", mw + 1, cfg->mthlim[mw]);
     g = Context();
     g.cfg = *cfg;
+    g.plugin = plugin;
     g.esize = cfg->precision == AMRGPU_FP32 ? 4 : 8;
     g.mixed = cfg->precision == AMRGPU_MIXED;
     MethodArgs& M = g.meth;
@@ -1278,6 +1354,7 @@ int amrgpu_init(const amrgpu_config* cfg) {
     M.ph.zz = cfg->rho * M.ph.cc;                                     // zz = rho*cc
     M.ph.gamma = cfg->gamma;
     M.ph.efix = cfg->efix;
+    for (int k = 0; k < AMRGPU_SOLVER_PARAMS_MAX; k++) M.ph.user[k] = g_solver_params[k];
     M.order = cfg->method[1];
     M.transverse = cfg->method[2];
     M.use_fwaves = cfg->use_fwaves;
@@ -1291,14 +1368,46 @@ int amrgpu_init(const amrgpu_config* cfg) {
         SweepArgs none{};
         for (int lay : {0, 1, 2}) {   // run-time and fixed LDS layouts
             none.layout = lay;
-            HIPCHK(launch_sweep_f64(g.cfg.solver, none, 0, g.stream));
-            HIPCHK(launch_sweep_f32(g.cfg.solver, none, 0, g.stream));
-            for (int w : {5, 6, 7}) HIPCHK(launch_sweep_mixed(g.cfg.solver, none, 0, g.stream, w));
+            HIPCHK(run_sweep(0, none, 0, 0));
+            HIPCHK(run_sweep(1, none, 0, 0));
+            for (int w : {5, 6, 7}) HIPCHK(run_sweep(2, none, 0, w));
         }
     }
     g.txmax = 28;
     g.tymax = (g.esize == 8 && !g.mixed) ? 12 : 28;
     g.inited = true;
+    return AMRGPU_OK;
+}
+
+int amrgpu_register_solver(const amrgpu_solver_ops* ops, int* solver_id) {
+    if (!ops || !solver_id) return fail(AMRGPU_ERR_ARG, "NULL solver table or id");
+    if (ops->abi != AMRGPU_SOLVER_ABI)
+        return fail(AMRGPU_ERR_ARG, "solver table ABI %d, library %d", ops->abi, AMRGPU_SOLVER_ABI);
+    if (ops->sweep_args_bytes != (int)sizeof(SweepArgs) || ops->qad_args_bytes != (int)sizeof(QadArgs) ||
+        ops->physics_bytes != (int)sizeof(Physics))
+        return fail(AMRGPU_ERR_ARG, "solver table built against other headers (argument blocks "
+                    "%d/%d/%d bytes, library %d/%d/%d)", ops->sweep_args_bytes, ops->qad_args_bytes,
+                    ops->physics_bytes, (int)sizeof(SweepArgs), (int)sizeof(QadArgs),
+                    (int)sizeof(Physics));
+    if (ops->nvar < 1 || ops->mwaves < 1 || ops->mwaves > NWMAX)
+        return fail(AMRGPU_ERR_ARG, "solver table: nvar %d, mwaves %d (at most %d waves)", ops->nvar,
+                    ops->mwaves, NWMAX);
+    if (!ops->sweep_f64 || !ops->sweep_f32 || !ops->sweep_mixed || !ops->qad)
+        return fail(AMRGPU_ERR_ARG, "solver table has a NULL launcher");
+    if (g_nplugins >= NPLUGIN) return fail(AMRGPU_ERR_ARG, "at most %d plug-in solvers", NPLUGIN);
+    g_plugins[g_nplugins] = *ops;
+    *solver_id = AMRGPU_SOLVER_PLUGIN0 + g_nplugins++;
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_solver_params(int n, const double* params) {
+    if (n < 0 || n > AMRGPU_SOLVER_PARAMS_MAX || (n > 0 && !params))
+        return fail(AMRGPU_ERR_ARG, "solver params: n = %d (0..%d)", n, AMRGPU_SOLVER_PARAMS_MAX);
+    for (int k = 0; k < AMRGPU_SOLVER_PARAMS_MAX; k++) g_solver_params[k] = k < n ? params[k] : 0.0;
+    if (g.inited) {
+        for (int k = 0; k < AMRGPU_SOLVER_PARAMS_MAX; k++) g.meth.ph.user[k] = g_solver_params[k];
+        for (auto& L : g.levels) free_graphs(L);   // captured launches hold the old constants
+    }
     return AMRGPU_OK;
 }
 
@@ -1340,8 +1449,11 @@ int amrgpu_set_tiling(int txmax, int tymax) {
     if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
     if (txmax < 0 || tymax < 0 || txmax > 61 || tymax > 61)
         return fail(AMRGPU_ERR_ARG, "tile caps must be in [4, 61]");
-    if (txmax) g.txmax = std::max(4, txmax);
-    if (tymax) g.tymax = std::max(4, tymax);
+    const int tx = txmax ? std::max(4, txmax) : g.txmax, ty = tymax ? std::max(4, tymax) : g.tymax;
+    int rc0 = check_tiling(tx, ty, g.launch_mode);
+    if (rc0) return rc0;
+    g.txmax = tx;
+    g.tymax = ty;
     for (auto& L : g.levels)
         if (L.ngrids) {
             int rc = plan_level(L);
@@ -1468,8 +1580,14 @@ int amrgpu_set_geometry(int mxnest, const int* iregsz, const int* jregsz, const 
         g.hyposs[l] = hyposs[l - 1];
     }
     g.mxnest = mxnest;
-    for (auto& L : g.levels) free_plan(L);
+    // fill programs and registers are planned from the geometry, and captured
+    // level steps name their device tables: all of them are stale now
+    for (auto& L : g.levels) {
+        free_graphs(L);
+        free_plan(L);
+    }
     g.regs_valid = false;
+    g.layout_version++;
     return AMRGPU_OK;
 }
 
@@ -1551,6 +1669,8 @@ int amrgpu_set_launch_mode(int mode) {
         mode != AMRGPU_LAUNCH_PERSISTENT)
         return fail(AMRGPU_ERR_ARG, "unknown launch mode %d", mode);
     if (mode == g.launch_mode) return AMRGPU_OK;
+    int rc0 = check_tiling(g.txmax, g.tymax, mode);
+    if (rc0) return rc0;
     g.launch_mode = mode;
     for (auto& L : g.levels)
         if (L.ngrids) {
@@ -1730,3 +1850,4 @@ int amrgpu_synchronize(void) {
 }
 
 }  // extern "C"
+static_assert(AMRGPU_SOLVER_PARAMS_MAX == SOLVER_PARAMS_MAX, "plug-in constants");

@@ -3,75 +3,11 @@
 // cells onto the coarse cells they cover and the conservation total.  FP64 arithmetic in the
 // reference's order; built with -ffp-contract=off.
 #include <hip/hip_runtime.h>
+#include "qad.hpp"
 #include "regs.hpp"
 #include "sweep.hpp"
 
 namespace amrgpu {
-
-// The fine-side register work of one level step, one register slot per
-// thread, in the reference's order for that slot: qad.f:9-360 (the r Riemann
-// problems between the saved coarse value and the fine grid's first ghost
-// cells along one coarse segment, before the step) and then fluxad.f:36-92
-// (the r fine fluxes of the step through that segment).  `q` is the level's
-// time-t array with its ghost cells; the fluxes come from the perimeter
-// block the sweep wrote.
-template <class Sol, typename S>
-__global__ __launch_bounds__(256) void finereg_kernel(QadArgs a, Physics ph) {
-    constexpr int NV = Sol::NV, NW = Sol::NW;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nitems) return;
-    const QadItem it = a.items[t];
-    const PatchDesc pd = a.patches[it.grid];
-    const S* q = static_cast<const S*>(a.q) + pd.off;
-    double* sv = a.reg + pd.reg_off;
-    const double* qc = sv + (int64_t)NV * pd.lenbc;
-    const double* per = a.reg + perim_offset(pd, NV);
-    const int r = (it.side == 1 || it.side == 3) ? a.raty : a.ratx;
-    const double h = (it.side == 1 || it.side == 3) ? a.hy : a.hx;
-    const int ixy = (it.side == 1 || it.side == 3) ? 1 : 2;
-    double acc[NV];
-#pragma unroll
-    for (int m = 0; m < NV; m++) acc[m] = sv[(int64_t)(it.slot - 1) * NV + m];
-    double qcv[NV];
-#pragma unroll
-    for (int m = 0; m < NV; m++) qcv[m] = qc[(int64_t)(it.slot - 1) * NV + m];
-    for (int l = 1; l <= (a.flux_form ? 0 : r); l++) {
-        const int f = (it.seg - 1) * r + l;        // fine row (sides 1,3) or column (2,4)
-        int I, J;
-        if (it.side == 1) { I = 0; J = f; }
-        else if (it.side == 2) { I = f; J = pd.my + 1; }
-        else if (it.side == 3) { I = pd.mx + 1; J = f; }
-        else { I = f; J = 0; }
-        const int64_t li = (int64_t)(J + 1) * pd.mitot + (I + 1);
-        double qf[NV];
-#pragma unroll
-        for (int m = 0; m < NV; m++) qf[m] = (double)q[(int64_t)m * pd.plane + li];
-        // interface states: coarse is left on sides 1 and 4, right on 2 and 3
-        const double* ql = (it.side == 1 || it.side == 4) ? qcv : qf;
-        const double* qr = (it.side == 1 || it.side == 4) ? qf : qcv;
-        typename Sol::Cache cache = Sol::prep(ph, ixy, ql, qr);
-        double wave[NW][NV], s[NW], amdq[NV], apdq[NV];
-        Sol::rpn2(ph, ixy, ql, qr, nullptr, nullptr, cache, wave, s, amdq, apdq);
-#pragma unroll
-        for (int m = 0; m < NV; m++) {
-            if (it.side == 1 || it.side == 4)
-                acc[m] = acc[m] + amdq[m] * h * a.delt + apdq[m] * h * a.delt;
-            else
-                acc[m] = acc[m] - amdq[m] * h * a.delt - apdq[m] * h * a.delt;
-        }
-    }
-    // fluxad: perimeter side blocks 0 left, 1 top, 2 right, 3 bottom
-    const int64_t side_off[4] = {0, (int64_t)NV * pd.my, (int64_t)NV * (pd.my + pd.mx),
-                                 (int64_t)NV * (2 * pd.my + pd.mx)};
-    const double* ps = per + side_off[it.side - 1];
-    for (int l = 1; l <= r; l++) {
-        const int f = (it.seg - 1) * r + l;
-#pragma unroll
-        for (int m = 0; m < NV; m++) acc[m] = acc[m] + ps[(int64_t)(f - 1) * NV + m] * a.delt * h;
-    }
-#pragma unroll
-    for (int m = 0; m < NV; m++) sv[(int64_t)(it.slot - 1) * NV + m] = acc[m];
-}
 
 // upbnd.f:62-124: one coarse cell per thread, its register slots in list order
 template <typename S>
@@ -166,24 +102,13 @@ hipError_t launch_conck(int esize, const PatchDesc* patches, int npatch, const v
     return hipGetLastError();
 }
 
-template <class Sol>
-static void finereg_launch(int esize, dim3 grid, const QadArgs& a, const Physics& ph, hipStream_t st) {
-    if (esize == 8)
-        hipLaunchKernelGGL((finereg_kernel<Sol, double>), grid, dim3(256), 0, st, a, ph);
-    else
-        hipLaunchKernelGGL((finereg_kernel<Sol, float>), grid, dim3(256), 0, st, a, ph);
-}
-
 hipError_t launch_finereg(int solver, int esize, const QadArgs& a, const Physics& ph, hipStream_t st) {
-    if (a.nitems <= 0) return hipSuccess;
-    dim3 grid((a.nitems + 255) / 256);
     switch (solver) {
-    case SOLVER_ADVECTION: finereg_launch<Advection<double>>(esize, grid, a, ph, st); break;
-    case SOLVER_ACOUSTICS: finereg_launch<Acoustics<double>>(esize, grid, a, ph, st); break;
-    case SOLVER_EULER: finereg_launch<EulerRoe<double>>(esize, grid, a, ph, st); break;
+    case SOLVER_ADVECTION: return finereg_launch<Advection<double>>(esize, a, ph, st);
+    case SOLVER_ACOUSTICS: return finereg_launch<Acoustics<double>>(esize, a, ph, st);
+    case SOLVER_EULER: return finereg_launch<EulerRoe<double>>(esize, a, ph, st);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_update(int esize, const UpdateArgs& a, const int64_t* d_block_start,

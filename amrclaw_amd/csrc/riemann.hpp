@@ -30,12 +30,19 @@ namespace amrgpu {
 
 enum SolverId { SOLVER_ADVECTION = 1, SOLVER_ACOUSTICS = 2, SOLVER_EULER = 3 };
 
+// number of constants a plug-in solver may read (Physics::user)
+constexpr int SOLVER_PARAMS_MAX = 16;
+
 // physics constants (the reference's common /cparam/)
 struct Physics {
     double ubar, vbar;          // advection velocities
     double rho, bulk, cc, zz;   // acoustics: density, bulk modulus, c, Z
     double gamma;               // Euler
     int efix;                   // Euler entropy fix
+    // the constants of a plug-in solver, in the order its application sets
+    // them (amrgpu_set_solver_params): the /cparam/ block of a solver the
+    // library does not build in
+    double user[SOLVER_PARAMS_MAX];
 };
 
 template <typename T> __device__ __forceinline__ T dmin1(T a, T b) { return a < b ? a : b; }

@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256, WGS) void sweep_kernel(SweepArgs a) {
 // persistent launch: a grid sized to fill the device takes tiles from the
 // table in order (largest first) through the counter a.next, which starts
 // at zero; every workgroup leaves once the counter has passed the table
-template <class Sol, typename T, typename S, int WGS>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0>
 __global__ __launch_bounds__(256, WGS) void sweep_persistent_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int tile;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256, WGS) void sweep_persistent_kernel(SweepArgs a)
         __syncthreads();
         const int t = tile;
         if (t >= a.ntiles) break;   // the same value in every lane
-        sweep_tile<Sol, T, S>(a, t, smem_raw);
+        sweep_tile<Sol, T, S, LX, LY>(a, t, smem_raw);
         __syncthreads();            // LDS and `tile` are reused by the next tile
     }
 }

@@ -1,6 +1,8 @@
-// sweep_launch.hpp -- launchers of the 2-D sweep for one compute precision T,
-// arithmetic policy Mth and storage precision S (included by sweep_f64.hip
-// and sweep_f32.hip, which differ in their compile flags).
+// sweep_launch.hpp -- launchers of the 2-D sweep for one Riemann solver, one
+// compute precision T with arithmetic policy Mth and one storage precision S.
+// Included by sweep_f64.hip and sweep_f32.hip (the library's own solvers,
+// which differ in their compile flags) and by a plug-in solver's translation
+// unit (include/amrgpu_solver.hpp).
 #pragma once
 #include <algorithm>
 #include "kernels.hpp"
@@ -14,9 +16,8 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     if (!attr_set) {
         // allow tiles whose LDS footprint exceeds the 64 KiB default: the
         // CU's 160 KiB less the kernel's static LDS
-        const void* fns[3] = {(const void*)sweep_kernel<Sol, T, S, WGS>,
-                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS>,
-                              (const void*)sweep_kernel<Sol, T, S, WGS, LX, LY>};
+        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S, WGS, LX, LY>,
+                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS, LX, LY>};
         for (const void* f : fns) {
             hipFuncAttributes fa{};
             hipError_t e = hipFuncGetAttributes(&fa, f);
@@ -39,7 +40,7 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     if (grid == 0 || grid_lds != lds) {
         int per_cu = 0, dev = 0, ncu = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S, WGS>, 256, lds);
+            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S, WGS, LX, LY>, 256, lds);
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
@@ -47,44 +48,56 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
         grid_lds = lds;
     }
     const int g = std::min(grid, a.ntiles);
-    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S, WGS>), dim3(g), dim3(256), lds, st, a);
+    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S, WGS, LX, LY>), dim3(g), dim3(256), lds,
+                       st, a);
     return hipGetLastError();
 }
 
-template <typename T, class Mth, typename S, int WGS, int LX = 0, int LY = 0>
-static hipError_t sweep_dispatch_w(int solver, const SweepArgs& a, size_t lds, hipStream_t st) {
-    switch (solver) {
-    case SOLVER_ADVECTION: return sweep_go<Advection<T, Mth>, T, S, WGS>(a, lds, st);
-    case SOLVER_ACOUSTICS: return sweep_go<Acoustics<T, Mth>, T, S, WGS>(a, lds, st);
-    case SOLVER_EULER: return sweep_go<EulerRoe<T, Mth>, T, S, WGS, LX, LY>(a, lds, st);
-    default: return hipErrorInvalidValue;
+// The run-time layout or, when FIXED and every tile of the level fits the
+// layout's extents (a.layout: 1 = 28x12 for FP32 arithmetic, 2 = 12x12 for
+// FP64), the instantiation with compile-time LDS strides.
+template <class Sol, typename T, typename S, int WGS, bool FIXED>
+static hipError_t sweep_layout(const SweepArgs& a, size_t lds, hipStream_t st) {
+    if constexpr (FIXED) {
+        if (sizeof(T) == 4 && a.layout == 1) return sweep_go<Sol, T, S, WGS, 28, 12>(a, lds, st);
+        if (sizeof(T) == 8 && a.layout == 2) return sweep_go<Sol, T, S, WGS, 12, 12>(a, lds, st);
+    }
+    return sweep_go<Sol, T, S, WGS>(a, lds, st);
+}
+
+// The sweep of one solver family SolT<T, Mth>.  Occupancy bound: the mixed
+// path (FP32 arithmetic on FP64 storage, one edge array per direction, about
+// 20 KB of LDS per tile) comes in 5, 6 and 7 workgroups per CU (0: 7,
+// measured best on the batched Euler level); the FP32 path's two edge arrays
+// per direction keep it at 5 by LDS; the FP64 path is bounded for 4 (128
+// VGPRs for Euler with a 40-byte spill, 3 % faster than its unbounded 137
+// VGPRs at 3).  FIXED: instantiate the fixed-layout variants too (the Euler
+// solver, whose tuned caps they match).
+template <template <typename, class> class SolT, typename T, class Mth, typename S, bool FIXED>
+static hipError_t sweep_solver(const SweepArgs& a, size_t lds, hipStream_t st, int wgs) {
+    using Sol = SolT<T, Mth>;
+    if constexpr (sizeof(T) == 4 && sizeof(S) == 8) {
+        if (wgs == 5) return sweep_layout<Sol, T, S, 5, FIXED>(a, lds, st);
+        if (wgs == 6) return sweep_layout<Sol, T, S, 6, FIXED>(a, lds, st);
+        return sweep_layout<Sol, T, S, 7, FIXED>(a, lds, st);
+    } else if constexpr (sizeof(T) == 4) {
+        (void)wgs;
+        return sweep_layout<Sol, T, S, 5, FIXED>(a, lds, st);
+    } else {
+        (void)wgs;
+        return sweep_layout<Sol, T, S, 4, FIXED>(a, lds, st);
     }
 }
 
-// the occupancy bound: the mixed path (FP32 arithmetic on FP64 storage, one
-// edge array per direction, about 20 KB of LDS per tile) comes in 5, 6 and 7
-// workgroups per CU (0: 7, measured best on the batched Euler level); the
-// FP32 path's two edge arrays per direction keep it at 5 by LDS; the FP64
-// path is bounded for 4 (128 VGPRs for Euler with a 40-byte spill, 3 %
-// faster than its unbounded 137 VGPRs at 3)
+// the library's own solvers, by id
 template <typename T, class Mth, typename S>
 static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hipStream_t st,
                                  int wgs) {
-    if constexpr (sizeof(T) == 4 && sizeof(S) == 8) {
-        if (wgs == 5) return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
-        if (wgs == 6) return sweep_dispatch_w<T, Mth, S, 6>(solver, a, lds, st);
-        // the tuned tile caps of the Euler level: LDS strides fixed at
-        // compile time when every tile fits 28x12 (a.layout == 1)
-        if (a.layout == 1) return sweep_dispatch_w<T, Mth, S, 7, 28, 12>(solver, a, lds, st);
-        return sweep_dispatch_w<T, Mth, S, 7>(solver, a, lds, st);
-    } else if constexpr (sizeof(T) == 4) {
-        (void)wgs;
-        if (a.layout == 1) return sweep_dispatch_w<T, Mth, S, 5, 28, 12>(solver, a, lds, st);
-        return sweep_dispatch_w<T, Mth, S, 5>(solver, a, lds, st);
-    } else {
-        (void)wgs;
-        if (a.layout == 2) return sweep_dispatch_w<T, Mth, S, 4, 12, 12>(solver, a, lds, st);
-        return sweep_dispatch_w<T, Mth, S, 4>(solver, a, lds, st);
+    switch (solver) {
+    case SOLVER_ADVECTION: return sweep_solver<Advection, T, Mth, S, false>(a, lds, st, wgs);
+    case SOLVER_ACOUSTICS: return sweep_solver<Acoustics, T, Mth, S, false>(a, lds, st, wgs);
+    case SOLVER_EULER: return sweep_solver<EulerRoe, T, Mth, S, true>(a, lds, st, wgs);
+    default: return hipErrorInvalidValue;
     }
 }
 

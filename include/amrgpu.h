@@ -19,6 +19,7 @@
 #ifndef AMRGPU_H
 #define AMRGPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -46,7 +47,8 @@ enum { AMRGPU_FP64 = 0, AMRGPU_FP32 = 1, AMRGPU_MIXED = 2 };
  * by the shim from claw.data/amr.data (amr2.f90:152-392), plus the physics
  * constants the reference's solvers read from common /cparam/. */
 typedef struct amrgpu_config {
-    int solver;          /* AMRGPU_ADVECTION / _ACOUSTICS / _EULER          */
+    int solver;          /* AMRGPU_ADVECTION / _ACOUSTICS / _EULER, or the id
+                            of a registered plug-in solver                    */
     int nvar;            /* num_eqn                                           */
     int mwaves;          /* num_waves                                         */
     int naux;            /* num_aux (must be 0: no solver here reads aux)    */
@@ -272,6 +274,51 @@ int amrgpu_get_level(int level, int which, double* alloc_base,
  * advanc.f:124-126). */
 int amrgpu_sweep_level(int level, double dt, double* cfl_level,
                        double* dtlevnew);
+
+/* ---- Riemann-solver plug-ins ---------------------------------------------
+ * The reference passes the application's rpn2/rpt2 as external procedures
+ * (step2.f90:7,26; flux2.f:25-28,125,202,219) and calls rpn2 by name from
+ * qad.f:129, so an application supplies its own solvers at link time.  Here
+ * an application writes its solver as a device functor with the same
+ * argument meaning (include/amrgpu_solver.hpp documents the contract) in a
+ * HIP translation unit of its own; AMRGPU_DEFINE_SOLVER there instantiates
+ * the library's sweep and qad kernels for it and exports a function that
+ * fills this table.  The application registers the table and selects the
+ * solver by the id returned, in amrgpu_config.solver (with its nvar and
+ * mwaves).  Builds with -ffp-contract=off reproduce the oracle's FP64
+ * arithmetic bit for bit, as the library's own FP64 units do.
+ *
+ * The function pointers take the library's launch-argument blocks as
+ * opaque pointers and return a hipError_t value (0 = success).  The table
+ * records the sizes of those blocks as its translation unit saw them; the
+ * library refuses a table built against other headers. */
+#define AMRGPU_SOLVER_ABI 1
+#define AMRGPU_SOLVER_PARAMS_MAX 16
+#define AMRGPU_SOLVER_PLUGIN0 100      /* id of the first registered solver */
+typedef struct amrgpu_solver_ops {
+    int abi;                 /* AMRGPU_SOLVER_ABI                              */
+    int nvar, mwaves;        /* meqn and mwaves of the solver                  */
+    int sweep_args_bytes;    /* layout check of the launch-argument blocks     */
+    int qad_args_bytes;
+    int physics_bytes;
+    const char* name;
+    int (*sweep_f64)(const void* args, size_t lds, void* stream);
+    int (*sweep_f32)(const void* args, size_t lds, void* stream);
+    int (*sweep_mixed)(const void* args, size_t lds, void* stream, int wgs);
+    int (*qad)(int esize, const void* args, const void* physics, void* stream);
+} amrgpu_solver_ops;
+
+/* Register a plug-in solver (no counterpart call in the reference: there the
+ * linker binds rpn2/rpt2).  The table is copied; *solver_id receives
+ * AMRGPU_SOLVER_PLUGIN0 + k for the k-th registration.  At most 16 solvers;
+ * registration survives amrgpu_finalize. */
+int amrgpu_register_solver(const amrgpu_solver_ops* ops, int* solver_id);
+
+/* The constants a plug-in solver reads (Physics.user[0..n-1] on the device),
+ * the counterpart of the /cparam/ common block its setprob fills.  Taken by
+ * the next amrgpu_init, and at once by a library already initialised.
+ * n <= AMRGPU_SOLVER_PARAMS_MAX. */
+int amrgpu_set_solver_params(int n, const double* params);
 
 /* Device timing on the library's stream: record event `slot` (0..15) and
  * read the elapsed milliseconds between two recorded slots. */

@@ -29,7 +29,10 @@ extern "C" {
 #endif
 
 /* solver ids */
-enum { ORC_ADVECTION = 1, ORC_ACOUSTICS = 2, ORC_EULER = 3 };
+enum { ORC_ADVECTION = 1, ORC_ACOUSTICS = 2, ORC_EULER = 3,
+       /* a physics the library does not build in: the parity case of its
+        * solver plug-in contract */
+       ORC_SHALLOW = 4 };
 
 /* Method parameters: the fields of amr_module.f90 that the hot path reads
  * (method(7), mthlim(:), mwaves, mcapa, use_fwaves, cfl, cflv1) plus the
@@ -49,6 +52,7 @@ typedef struct {
     double gamma;                   /* Euler                */
     int efix;                       /* Euler entropy fix    */
     double wbar;                    /* 3-D advection, z velocity */
+    double grav;                    /* shallow water: gravity    */
 } orc_params;
 
 /* ---- Riemann solvers, Fortran calling convention over a 1-D slice -------

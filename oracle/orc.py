@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ADVECTION, ACOUSTICS, EULER = 1, 2, 3
+ADVECTION, ACOUSTICS, EULER, SHALLOW = 1, 2, 3, 4
 
 
 class OrcParams(C.Structure):
@@ -23,18 +23,18 @@ class OrcParams(C.Structure):
         ("ubar", C.c_double), ("vbar", C.c_double),
         ("rho", C.c_double), ("bulk", C.c_double), ("cc", C.c_double),
         ("zz", C.c_double), ("gamma", C.c_double), ("efix", C.c_int),
-        ("wbar", C.c_double),
+        ("wbar", C.c_double), ("grav", C.c_double),
     ]
 
 
 def make_params(solver, *, order=2, transverse=2, limiter=4, mwaves=None,
                 maux=0, mcapa=0, cfl=0.9, cflv1=1.0, ubar=0.5, vbar=1.0,
-                rho=1.0, bulk=4.0, gamma=1.4, efix=True, mbc=2):
+                rho=1.0, bulk=4.0, gamma=1.4, efix=True, mbc=2, grav=1.0):
     """Build the params of one configuration.  limiter: mthlim code applied
     to every wave (0 none, 1 minmod, 2 superbee, 3 van Leer, 4 MC, 5 BW)."""
     p = OrcParams()
     p.solver = solver
-    p.meqn, nw = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4)}[solver]
+    p.meqn, nw = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4), SHALLOW: (3, 3)}[solver]
     p.mwaves = nw if mwaves is None else mwaves
     p.maux, p.mbc = maux, mbc
     for k in range(8):
@@ -50,6 +50,7 @@ def make_params(solver, *, order=2, transverse=2, limiter=4, mwaves=None,
     p.cc = float(np.sqrt(bulk / rho))
     p.zz = rho * p.cc
     p.gamma, p.efix = gamma, int(bool(efix))
+    p.grav = grav
     return p
 
 

@@ -293,6 +293,82 @@ static void rpt2_euler(const orc_params* p, int ixy, int meqn, int mbc,
     }
 }
 
+/* ---- shallow water (h, hu, hv), Roe split without entropy fix ----------
+ * LeVeque 2002 sec. 15.3.3 (Roe averages: u, v weighted by sqrt(h), gravity
+ * wave speed from the mean depth) and sec. 21.6 (transverse split with the
+ * same averages).  Not one of the reference's example physics: the parity
+ * case of the library's solver plug-in contract. */
+typedef struct { double u, v, c; } sw_avg;
+
+static sw_avg sw_average(double grav, int mu, int mv, const double* L, const double* R) {
+    sw_avg r;
+    double hsl = sqrt(L[0]), hsr = sqrt(R[0]);
+    double hsq = hsl + hsr;
+    r.u = (L[mu - 1] / hsl + R[mu - 1] / hsr) / hsq;
+    r.v = (L[mv - 1] / hsl + R[mv - 1] / hsr) / hsq;
+    r.c = sqrt(grav * 0.5 * (L[0] + R[0]));
+    return r;
+}
+
+static void rpn2_shallow(const orc_params* p, int ixy, int meqn, int mwaves, int mbc, int mx,
+                         const double* ql, const double* qr, double* wave, double* s,
+                         double* amdq, double* apdq) {
+    int mu = (ixy == 1) ? 2 : 3, mv = (ixy == 1) ? 3 : 2;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        const double* L = &Q(qr, 1, i - 1);
+        const double* R = &Q(ql, 1, i);
+        sw_avg r = sw_average(p->grav, mu, mv, L, R);
+        double d1 = R[0] - L[0], d2 = R[mu - 1] - L[mu - 1], d3 = R[mv - 1] - L[mv - 1];
+        double a1 = ((r.u + r.c) * d1 - d2) / (2.0 * r.c);
+        double a2 = d3 - r.v * d1;
+        double a3 = (d2 - (r.u - r.c) * d1) / (2.0 * r.c);
+        W(1, 1, i) = a1; W(mu, 1, i) = a1 * (r.u - r.c); W(mv, 1, i) = a1 * r.v;
+        S(1, i) = r.u - r.c;
+        W(1, 2, i) = 0.0; W(mu, 2, i) = 0.0; W(mv, 2, i) = a2;
+        S(2, i) = r.u;
+        W(1, 3, i) = a3; W(mu, 3, i) = a3 * (r.u + r.c); W(mv, 3, i) = a3 * r.v;
+        S(3, i) = r.u + r.c;
+        for (int m = 1; m <= 3; m++) {
+            double am = 0.0, ap = 0.0;
+            for (int mw = 1; mw <= 3; mw++) {
+                am = am + dmin1(S(mw, i), 0.0) * W(m, mw, i);
+                ap = ap + dmax1(S(mw, i), 0.0) * W(m, mw, i);
+            }
+            Q(amdq, m, i) = am;
+            Q(apdq, m, i) = ap;
+        }
+    }
+    (void)meqn; (void)mwaves;
+}
+
+static void rpt2_shallow(const orc_params* p, int ixy, int meqn, int mbc, int mx,
+                         const double* ql, const double* qr, const double* asdq,
+                         double* bmasdq, double* bpasdq) {
+    int mu = (ixy == 1) ? 2 : 3, mv = (ixy == 1) ? 3 : 2;
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        const double* L = &Q(qr, 1, i - 1);
+        const double* R = &Q(ql, 1, i);
+        sw_avg r = sw_average(p->grav, mu, mv, L, R);
+        double A1 = Q(asdq, 1, i), Amu = Q(asdq, mu, i), Amv = Q(asdq, mv, i);
+        double a1 = ((r.v + r.c) * A1 - Amv) / (2.0 * r.c);
+        double a2 = Amu - r.u * A1;
+        double a3 = (Amv - (r.v - r.c) * A1) / (2.0 * r.c);
+        double wb[3][3], sb[3];
+        wb[0][0] = a1; wb[0][mu - 1] = a1 * r.u; wb[0][mv - 1] = a1 * (r.v - r.c); sb[0] = r.v - r.c;
+        wb[1][0] = 0.0; wb[1][mu - 1] = a2; wb[1][mv - 1] = 0.0; sb[1] = r.v;
+        wb[2][0] = a3; wb[2][mu - 1] = a3 * r.u; wb[2][mv - 1] = a3 * (r.v + r.c); sb[2] = r.v + r.c;
+        for (int m = 1; m <= 3; m++) {
+            double bm = 0.0, bp = 0.0;
+            for (int mw = 0; mw < 3; mw++) {
+                bm = bm + dmin1(sb[mw], 0.0) * wb[mw][m - 1];
+                bp = bp + dmax1(sb[mw], 0.0) * wb[mw][m - 1];
+            }
+            Q(bmasdq, m, i) = bm;
+            Q(bpasdq, m, i) = bp;
+        }
+    }
+}
+
 void orc_rpn2(const orc_params* p, int ixy, int maxm, int meqn, int mwaves,
               int maux, int mbc, int mx, const double* ql, const double* qr,
               const double* auxl, const double* auxr, double* wave, double* s,
@@ -310,6 +386,9 @@ void orc_rpn2(const orc_params* p, int ixy, int maxm, int meqn, int mwaves,
     case ORC_EULER:
         rpn2_euler(p, ixy, meqn, mwaves, mbc, mx, ql, qr, wave, s, amdq,
                    apdq);
+        break;
+    case ORC_SHALLOW:
+        rpn2_shallow(p, ixy, meqn, mwaves, mbc, mx, ql, qr, wave, s, amdq, apdq);
         break;
     }
 }
@@ -330,6 +409,9 @@ void orc_rpt2(const orc_params* p, int ixy, int imp, int maxm, int meqn,
         break;
     case ORC_EULER:
         rpt2_euler(p, ixy, meqn, mbc, mx, ql, qr, asdq, bmasdq, bpasdq);
+        break;
+    case ORC_SHALLOW:
+        rpt2_shallow(p, ixy, meqn, mbc, mx, ql, qr, asdq, bmasdq, bpasdq);
         break;
     }
 }
@@ -356,6 +438,14 @@ void orc_flux(const orc_params* p, int ixy, const double* q, double* f) {
         f[mu] = q[mu] * un + pr;
         f[mv] = q[mv] * un;
         f[3] = (q[3] + pr) * un;
+        break;
+    }
+    case ORC_SHALLOW: {
+        int mu = (ixy == 1) ? 1 : 2, mv = (ixy == 1) ? 2 : 1;
+        double un = q[mu] / q[0];
+        f[0] = q[mu];
+        f[mu] = q[mu] * un + 0.5 * p->grav * q[0] * q[0];
+        f[mv] = q[mv] * un;
         break;
     }
     }

@@ -466,3 +466,71 @@ def test_rpn3_harten_hyman(ixyz):
             assert abs(c / s0 + (lam1 - c) / s1 - 1.0) < 1e-10 * (1 + abs(1 / s0) + abs(1 / s1))
             n_tr += 1
     assert n_tr > 5 and n_skip < len(L) // 20, (n_tr, n_skip)
+
+
+# --------------------------------------------------------- shallow water
+# The physics of the solver plug-in parity case (not built into the library):
+# q = (h, hu, hv), f = (hu, hu^2 + g h^2 / 2, huv).  Its Roe matrix is the
+# flux Jacobian at the mean depth and the sqrt(h)-weighted velocities.
+GRAV = 1.7
+
+
+def sw_flux(q, k, g=GRAV):
+    h = q[0]
+    vel = q[1:3] / h
+    f = q * vel[k]
+    f[1 + k] = f[1 + k] + 0.5 * g * h * h
+    return f
+
+
+def sw_roe_matrix(L, R, k, g=GRAV):
+    wl, wr = np.sqrt(L[0]), np.sqrt(R[0])
+    vel = (L[1:3] / wl + R[1:3] / wr) / (wl + wr)
+    hbar = 0.5 * (L[0] + R[0])
+    q0 = np.array([hbar, hbar * vel[0], hbar * vel[1]], dtype=complex)
+    J = np.zeros((3, 3))
+    for j in range(3):
+        qc = q0.copy()
+        qc[j] += 1e-30j
+        J[:, j] = sw_flux(qc, k, g).imag / 1e-30
+    return J, vel, np.sqrt(g * hbar)
+
+
+def sw_pairs(rng, n):
+    h = 0.3 + 1.5 * rng.random((2, n))
+    vel = 1.2 * rng.normal(size=(2, n, 2))
+    return [np.column_stack([h[s], h[s][:, None] * vel[s]]) for s in (0, 1)]
+
+
+@pytest.mark.parametrize("ixy", [1, 2])
+def test_shallow_water_roe_split(ixy):
+    """rpn2 / rpt2 of the shallow-water restatement: waves are the spectral
+    parts of the jump for u - c, u, u + c of the Roe matrix, the Roe
+    property holds, A-dq / A+dq and B-asdq / B+asdq are spectral splits."""
+    rng = np.random.default_rng(101 + ixy)
+    mu, mv = normal_index_2d(ixy)
+    p = orc.make_params(orc.SHALLOW, grav=GRAV)
+    L, R = sw_pairs(rng, 60)
+    q = pairs_2d(L, R)
+    wave, s, amdq, apdq = orc.rpn2(p, ixy, q, q)
+    asdq = rng.normal(size=q.shape)
+    bm, bp = orc.rpt2(p, ixy, 1, q, q, asdq)
+    for k in range(len(L)):
+        i = 2 * k + 1
+        dq = R[k] - L[k]
+        A, vel, c = sw_roe_matrix(L[k], R[k], mu - 1)
+        un, vt = vel[mu - 1], vel[mv - 1]
+        lams = (un - c, un, un + c)
+        assert close(np.sort(np.linalg.eigvals(A).real), lams, 1 + abs(un) + c)
+        sc = np.abs(dq).max() * (1 + abs(un) + c) ** 2
+        assert close(s[i], lams, 1 + abs(un) + c)
+        for w, P in zip(wave[i], projectors(A, lams)):
+            assert close(w, P @ dq, sc)
+        df = sw_flux(R[k], mu - 1) - sw_flux(L[k], mu - 1)
+        assert close(amdq[i] + apdq[i], df, sc)
+        am, ap = split(A, lams, dq)
+        assert close(amdq[i], am, sc) and close(apdq[i], ap, sc)
+        B, _, _ = sw_roe_matrix(L[k], R[k], mv - 1)
+        m, pl = split(B, (vt - c, vt, vt + c), asdq[i])
+        sc2 = np.abs(asdq[i]).max() * (1 + abs(vt) + c) ** 2
+        assert close(bm[i], m, sc2) and close(bp[i], pl, sc2)
