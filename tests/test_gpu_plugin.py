@@ -44,9 +44,12 @@ def sw_states(rng, shape):
     return np.stack([h, h * u, h * v], -1)
 
 
-def configs(precision, **kw):
+def configs(precision, mthbc=(1, 1, 1, 1), **kw):
+    """Oracle params and library config of the plug-in solver; the boundary
+    conditions go to the library only (the oracle reads them from the
+    hierarchy)."""
     p = orc.make_params(orc.SHALLOW, grav=GRAV, **kw)
-    c = capi.make_config(solver_id(), nvar=3, mwaves=3, precision=precision, **kw)
+    c = capi.make_config(solver_id(), nvar=3, mwaves=3, precision=precision, mthbc=mthbc, **kw)
     return p, c
 
 
