@@ -31,6 +31,7 @@ UNITS = [
     ("fill.hip", ["-O3", "-ffp-contract=off"]),
     ("regs.hip", ["-O3", "-ffp-contract=off"]),
     ("flag.hip", ["-O3", "-ffp-contract=off"]),
+    ("gauges.hip", ["-O3", "-ffp-contract=off"]),
     ("fill_plan.cpp", ["-O2", "-x", "hip"]),
     ("regs_plan.cpp", ["-O2", "-x", "hip"]),
     ("capi.cpp", ["-O2", "-x", "hip"]),

@@ -290,7 +290,7 @@ def advance_level(level, want_cfl=True):
     return dt.value, cfl.value
 
 
-LAUNCH_BATCHED, LAUNCH_PER_PATCH, LAUNCH_PERSISTENT = 0, 1, 2
+LAUNCH_BATCHED, LAUNCH_PER_PATCH, LAUNCH_PERSISTENT, LAUNCH_MARCH = 0, 1, 2, 3
 KT_SWEEP, KT_FILL, KT_REGISTERS, KT_UPDATE, KT_FLAG = range(5)
 
 

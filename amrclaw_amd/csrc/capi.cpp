@@ -5,16 +5,20 @@
 // on one stream.  Mirrors the reference's level bookkeeping (node/rnode,
 // listOfGrids; amr_module.f90:30-151) for the data the level update needs.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstddef>
 #include <cstring>
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/amrgpu.h"
 #include "fill_plan.hpp"
+#include "gauges.hpp"
 #include "kernels.hpp"
 #include "regs_plan.hpp"
 
@@ -25,6 +29,7 @@ hipError_t launch_aos_soa(bool to_soa, int esize, double* stage, const int64_t* 
                           const int64_t* off, const int32_t* plane, const int64_t* cell_start,
                           int npatch, int nv, int64_t ncell, void* level, hipStream_t st);
 hipError_t launch_fill_stage(int esize, const FillArgs& a, hipStream_t st);
+hipError_t launch_gauges(int esize, const GaugeArgs& a, hipStream_t st);
 }
 
 using namespace amrgpu;
@@ -49,6 +54,19 @@ struct Level {
     unsigned long long* d_cfl = nullptr;
     size_t lds = 0;
     int layout = 0;   // fixed LDS layout of the Euler sweep every tile fits (1 28x12, 2 12x12)
+    // layout generation (a new value at every amrgpu_set_level of this
+    // level) and the generations the register arena d_reg was laid out for:
+    // this level's, the next coarser level's and the geometry's
+    uint64_t gen = 0;
+    uint64_t reg_gen = 0, reg_gen_coarse = 0, reg_geom = 0;
+    int64_t reg_elems = 0;
+    // listbc of every grid as the caller built it (amrgpu_set_listbc), kid
+    // as an index on the next finer level; empty: planned here
+    std::vector<std::vector<std::array<int, 5>>> listbc;
+    std::vector<char> listbc_set;
+    // march sweep (launch mode AMRGPU_LAUNCH_MARCH): its wave table
+    std::vector<MarchWave> mwaves;
+    MarchWave* d_mwaves = nullptr;
     // bulk transfer tables: patch element offsets, planes, first cell
     int64_t* d_off = nullptr;
     int32_t* d_plane = nullptr;
@@ -139,6 +157,7 @@ struct Context {
     hipEvent_t ev[NEVENTS] = {};
     int txmax = 28, tymax = 28;
     int sweep_wgs = 0;   // occupancy bound of the mixed sweep (0: its default)
+    int march_wgs = 0;   // occupancy bound of the march sweep (0: its default)
     Level levels[MAXLV];
     int mxnest = 0;
     int iregsz[MAXLV] = {}, jregsz[MAXLV] = {}, ratx[MAXLV] = {}, raty[MAXLV] = {};
@@ -153,6 +172,16 @@ struct Context {
     size_t tev_used[AMRGPU_KT_COUNT] = {};
     bool graphs = false;                        // replay advance_level as hipGraphs
     uint64_t layout_version = 1;                // bumped by every layout change
+    uint64_t gen_counter = 0;                   // source of Level::gen values
+    // gauges (amrgpu_set_gauges): their state on the device and the records
+    // sampled since the last amrgpu_get_gauge_records
+    std::vector<GaugeDev> gauges;
+    GaugeDev* d_gauges = nullptr;
+    int32_t* d_gmeta = nullptr;
+    double* d_gvals = nullptr;
+    unsigned int* d_gcount = nullptr;           // [0] records, [1] refused
+    int gcap = 0;
+    uint64_t geom_gen = 1;                      // bumped by every change of the geometry
 };
 
 Context g;
@@ -309,6 +338,7 @@ void free_level(Level& L) {
         if (p) hipFree(p);
     if (L.d_patches) hipFree(L.d_patches);
     if (L.d_tiles) hipFree(L.d_tiles);
+    if (L.d_mwaves) hipFree(L.d_mwaves);
     for (auto& b : L.buf)
         if (b) hipFree(b);
     if (L.d_cfl) hipFree(L.d_cfl);
@@ -319,6 +349,19 @@ void free_level(Level& L) {
     L = Level();
     L.time = t;
     L.possk = pk;
+}
+
+void free_gauges() {
+    quiesce();
+    void* ps[] = {g.d_gauges, g.d_gmeta, g.d_gvals, g.d_gcount};
+    for (void* p : ps)
+        if (p) hipFree(p);
+    g.d_gauges = nullptr;
+    g.d_gmeta = nullptr;
+    g.d_gvals = nullptr;
+    g.d_gcount = nullptr;
+    g.gauges.clear();
+    g.gcap = 0;
 }
 
 int seg_width(int n) { return n + 3 <= 16 ? 16 : (n + 3 <= 32 ? 32 : 64); }
@@ -392,6 +435,68 @@ Tiling make_tiling(const Level& L, int txmax, int tymax, int launch_mode) {
     return T;
 }
 
+// The march sweep's waves (sweep_march.hpp).  Each grid is cut into strips
+// of at most 60 columns; a strip of w columns takes a segment of 16, 32 or
+// 64 lanes (w + 4 lanes: the strip and its halo), and the cut of each grid
+// width minimises the lanes (fewest strips among equal cuts).  Segments of
+// grids of the same height and the same width are packed into waves (four
+// 16-lane, two 32-lane or one 64-lane segment per wave), which march
+// together; waves are ordered tallest first (the longest marches start
+// first, as the reference orders grids by cost, regrid.f:132-169).
+int march_lanes(int w) { return w + 4 <= 16 ? 16 : (w + 4 <= 32 ? 32 : 64); }
+
+std::vector<int> march_cut(int mx) {
+    static std::vector<std::vector<int>> memo;
+    if ((int)memo.size() <= mx) memo.resize(mx + 1);
+    if (!memo[mx].empty() || mx == 0) return memo[mx];
+    std::vector<int> cost(mx + 1, 0), nstrip(mx + 1, 0), last(mx + 1, 0);
+    for (int n = 1; n <= mx; n++) {
+        cost[n] = INT32_MAX;
+        for (int w = 1; w <= std::min(60, n); w++) {
+            const int c = cost[n - w] + march_lanes(w), ns = nstrip[n - w] + 1;
+            if (c < cost[n] || (c == cost[n] && ns < nstrip[n])) {
+                cost[n] = c;
+                nstrip[n] = ns;
+                last[n] = w;
+            }
+        }
+    }
+    std::vector<int> cut;
+    for (int n = mx; n > 0; n -= last[n]) cut.push_back(last[n]);
+    std::sort(cut.begin(), cut.end(), std::greater<int>());
+    memo[mx] = cut;
+    return cut;
+}
+
+std::vector<MarchWave> plan_march(const Level& L) {
+    // (my, segment lanes) -> strips
+    std::map<std::pair<int, int>, std::vector<MarchSeg>> groups;
+    for (int p = 0; p < L.ngrids; p++) {
+        const PatchDesc& pd = L.patches[p];
+        int x = 0;
+        for (int w : march_cut(pd.mx)) {
+            groups[{pd.my, march_lanes(w)}].push_back(MarchSeg{p, (int16_t)x, (int16_t)w});
+            x += w;
+        }
+    }
+    std::vector<MarchWave> waves;
+    for (auto& kv : groups) {
+        const int my = kv.first.first, sw = kv.first.second, per = 64 / sw;
+        const auto& segs = kv.second;
+        for (size_t i = 0; i < segs.size(); i += per) {
+            MarchWave mw{};
+            mw.segw = (int16_t)sw;
+            mw.my = (int16_t)my;
+            mw.nseg = (int16_t)std::min<size_t>(per, segs.size() - i);
+            for (int s = 0; s < mw.nseg; s++) mw.seg[s] = segs[i + s];
+            waves.push_back(mw);
+        }
+    }
+    std::stable_sort(waves.begin(), waves.end(),
+                     [](const MarchWave& a, const MarchWave& b) { return a.my > b.my; });
+    return waves;
+}
+
 constexpr size_t LDS_MAX = 160 * 1024;
 
 // Re-tile a level with the current caps and launch mode.  A tiling whose
@@ -414,6 +519,14 @@ int plan_level(Level& L) {
     if (!L.tiles.empty()) {
         HIPCHK(hipMalloc(&L.d_tiles, L.tiles.size() * sizeof(TileDesc)));
         HIPCHK(h2d(L.d_tiles, L.tiles.data(), L.tiles.size() * sizeof(TileDesc)));
+    }
+    if (L.d_mwaves) hipFree(L.d_mwaves);
+    L.d_mwaves = nullptr;
+    L.mwaves.clear();
+    if (g.launch_mode == AMRGPU_LAUNCH_MARCH && L.ngrids > 0) {
+        L.mwaves = plan_march(L);
+        HIPCHK(hipMalloc(&L.d_mwaves, L.mwaves.size() * sizeof(MarchWave)));
+        HIPCHK(h2d(L.d_mwaves, L.mwaves.data(), L.mwaves.size() * sizeof(MarchWave)));
     }
     return AMRGPU_OK;
 }
@@ -449,6 +562,15 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
     L.ngrids = ngrids;
     L.hx = hx;
     L.hy = hy;
+    L.gen = ++g.gen_counter;
+    L.listbc.assign(ngrids, {});
+    L.listbc_set.assign(ngrids, 0);
+    if (level >= 2) {
+        // the coarser level's lists name this level's grids by index
+        Level& C = g.levels[level - 1];
+        C.listbc.assign(C.ngrids, {});
+        C.listbc_set.assign(C.ngrids, 0);
+    }
     const int ng = NGHOST, nv = g.cfg.nvar;
     int64_t off = 0;
     for (int p = 0; p < ngrids; p++) {
@@ -608,6 +730,10 @@ HierGeom make_geom() {
         G.raty = g.raty[l];
         G.hx = g.hxposs[l];
         G.hy = g.hyposs[l];
+        const Level& L = g.levels[l];
+        bool any = false;
+        for (char c : L.listbc_set) any = any || c;
+        if (any) G.listbc = L.listbc;
     }
     H.xlower = g.cfg.xlower;
     H.xupper = g.cfg.xupper;
@@ -737,6 +863,14 @@ int upload(T** dst, const std::vector<T>& v) {
 int ensure_regs() {
     if (g.regs_valid) return AMRGPU_OK;
     if (g.mxnest < 1) return fail(AMRGPU_ERR_STATE, "amrgpu_set_geometry has not been called");
+    for (int l = 1; l < MAXLV; l++) {
+        const Level& L = g.levels[l];
+        int nset = 0;
+        for (char c : L.listbc_set) nset += c ? 1 : 0;
+        if (nset > 0 && nset < L.ngrids)
+            return fail(AMRGPU_ERR_STATE, "amrgpu_set_listbc: %d of the %d grids of level %d have "
+                        "a list", nset, L.ngrids, l);
+    }
     HierGeom H = make_geom();
     const int nv = g.cfg.nvar;
     std::vector<RegLevel> R(FILL_MAXLV);
@@ -744,7 +878,18 @@ int ensure_regs() {
     if (!plan_registers(H, nv, R.data(), err)) return fail(AMRGPU_ERR_ARG, "registers: %s", err.c_str());
     for (int l = 1; l <= H.nlev && l < MAXLV; l++) {
         Level& L = g.levels[l];
+        // a regrid replaces the levels above lbase only (regrid.f:72-118):
+        // the registers of a fine level whose grids, coarser grids and
+        // geometry are unchanged keep what they have accumulated so far in
+        // the current coarse step (the reference keeps node(ffluxptr) of
+        // such grids); the others start from zero (prepf.f)
+        const bool keep = l >= 2 && L.d_reg && L.reg_gen == L.gen &&
+                          L.reg_gen_coarse == g.levels[l - 1].gen && L.reg_geom == g.geom_gen &&
+                          L.reg_elems == R[l].reg_elems;
+        double* kept = keep ? L.d_reg : nullptr;
+        if (keep) L.d_reg = nullptr;
         free_regs(L);
+        L.reg_elems = 0;
         if (L.ngrids == 0) continue;
         const RegLevel& Rl = R[l];
         int rc;
@@ -753,8 +898,16 @@ int ensure_regs() {
                 L.patches[p].reg_off = Rl.reg_off[p];
                 L.patches[p].lenbc = Rl.lenbc[p];
             }
-            HIPCHK(hipMalloc(&L.d_reg, Rl.reg_elems * sizeof(double)));
-            HIPCHK(dzero(L.d_reg, Rl.reg_elems * sizeof(double)));
+            if (kept) {
+                L.d_reg = kept;
+            } else {
+                HIPCHK(hipMalloc(&L.d_reg, Rl.reg_elems * sizeof(double)));
+                HIPCHK(dzero(L.d_reg, Rl.reg_elems * sizeof(double)));
+            }
+            L.reg_elems = Rl.reg_elems;
+            L.reg_gen = L.gen;
+            L.reg_gen_coarse = g.levels[l - 1].gen;
+            L.reg_geom = g.geom_gen;
             if ((rc = upload(&L.d_qad, Rl.qad))) return rc;
             L.nqad = (int)Rl.qad.size();
             if (!plan_saveqc(H, l, nv, L.sq, err))
@@ -843,7 +996,15 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     auto launch = [&](const SweepArgs& x) { return run_sweep(run_precision(), x, L.lds, g.sweep_wgs); };
     hipError_t e = hipSuccess;
     tbegin(AMRGPU_KT_SWEEP);
-    if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH && !L.patch_tile_start.empty()) {
+    if (g.launch_mode == AMRGPU_LAUNCH_MARCH && L.d_mwaves && !flux_out_dev && !rl.sv) {
+        // the march sweep writes no per-edge flux arrays and no fluxsv
+        // entries: a level with a finer level, or a call asking for the
+        // flux arrays, takes the tile sweep
+        SweepArgs x = a;
+        x.waves = L.d_mwaves;
+        x.nwaves = (int)L.mwaves.size();
+        e = run_sweep(run_precision(), x, 0, g.march_wgs);
+    } else if (g.launch_mode == AMRGPU_LAUNCH_PER_PATCH && !L.patch_tile_start.empty()) {
         for (int p = 0; p < L.ngrids && e == hipSuccess; p++) {
             SweepArgs x = a;
             x.tiles = L.d_tiles + L.patch_tile_start[p];
@@ -1036,6 +1197,25 @@ int advance_level(int level, double* dtlevnew, double* cfl_level) {
     if (!(L.possk > 0)) return fail(AMRGPU_ERR_STATE, "level %d has no time step (amrgpu_set_time)", level);
     rc = g.graphs ? graph_level_step(level) : launch_level_step(level);
     if (rc) return rc;
+    if (!g.gauges.empty()) {
+        // advanc.f:249-253: the gauges of this level's grids sample the
+        // time-t values after their ghost cells are filled (buf[cur], which
+        // the sweep only read)
+        GaugeArgs a{};
+        a.gauges = g.d_gauges;
+        a.n = (int)g.gauges.size();
+        a.level = level;
+        a.tgrid = L.time;
+        a.patches = L.d_patches;
+        a.q = L.buf[L.cur];
+        a.nv = g.cfg.nvar;
+        a.meta = g.d_gmeta;
+        a.vals = g.d_gvals;
+        a.count = g.d_gcount;
+        a.cap = g.gcap;
+        hipError_t e = launch_gauges((int)g.esize, a, g.stream);
+        if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "gauges: %s", hipGetErrorString(e));
+    }
     L.cur = 1 - L.cur;                                           // new <-> old roles
     L.time = L.time + L.possk;                                   // advanc.f:289
     if (cfl_level || dtlevnew) return level_cfl(L, L.possk, cfl_level, dtlevnew);
@@ -1414,6 +1594,7 @@ int amrgpu_set_solver_params(int n, const double* params) {
 int amrgpu_finalize(void) {
     if (!g.inited) return AMRGPU_OK;
     hipStreamSynchronize(g.stream);
+    free_gauges();
     for (auto& L : g.levels) free_level(L);
     for (auto& v : g.tev)
         for (auto& p : v) {
@@ -1437,10 +1618,18 @@ int amrgpu_last_error(char* buf, int len) {
 
 int amrgpu_set_occupancy(int wgs) {
     if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
-    if (wgs != 0 && (wgs < 5 || wgs > 7))
-        return fail(AMRGPU_ERR_ARG, "occupancy bound must be 5, 6 or 7 workgroups per CU (0: default)");
-    if (wgs == g.sweep_wgs) return AMRGPU_OK;
-    g.sweep_wgs = wgs;
+    if (wgs != 0 && wgs != 2 && wgs != 3 && (wgs < 5 || wgs > 7))
+        return fail(AMRGPU_ERR_ARG,
+                    "occupancy bound must be 5, 6 or 7 workgroups per CU for the tile sweep, "
+                    "2 or 3 waves per SIMD for the march sweep (0: defaults)");
+    if (wgs == 0) {
+        g.sweep_wgs = 0;
+        g.march_wgs = 0;
+    } else if (wgs <= 3) {
+        g.march_wgs = wgs;
+    } else {
+        g.sweep_wgs = wgs;
+    }
     for (auto& L : g.levels) free_graphs(L);   // captured launches name the kernel
     return AMRGPU_OK;
 }
@@ -1507,6 +1696,115 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out, double dx
     return AMRGPU_OK;
 }
 
+int amrgpu_set_listbc(int level, int grid, int n, const int* list5) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    Level& L = g.levels[level];
+    if (level < 1 || grid < 0 || grid >= L.ngrids || n < 0 || (n > 0 && !list5))
+        return fail(AMRGPU_ERR_ARG, "bad set_listbc arguments");
+    if ((int)L.listbc.size() != L.ngrids) {
+        L.listbc.assign(L.ngrids, {});
+        L.listbc_set.assign(L.ngrids, 0);
+    }
+    auto& v = L.listbc[grid];
+    v.resize(n);
+    for (int e = 0; e < n; e++)
+        for (int c = 0; c < 5; c++) v[e][c] = list5[5 * e + c];
+    L.listbc_set[grid] = 1;
+    // the fluxsv and upbnd programs come from the lists; captured level
+    // steps name their tables
+    g.regs_valid = false;
+    g.layout_version++;
+    for (auto& Lv : g.levels) free_graphs(Lv);
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_gauges(int n, const double* x, const double* y, const double* t_start,
+                      const double* t_end, const double* min_dt) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (n < 0 || (n > 0 && (!x || !y || !t_start || !t_end || !min_dt)))
+        return fail(AMRGPU_ERR_ARG, "bad gauge arrays");
+    free_gauges();
+    if (n == 0) return AMRGPU_OK;
+    g.gauges.resize(n);
+    for (int i = 0; i < n; i++) {
+        GaugeDev& G = g.gauges[i];
+        G = GaugeDev{};
+        G.x = x[i];
+        G.y = y[i];
+        G.t_start = t_start[i];
+        G.t_end = t_end[i];
+        G.min_dt = min_dt[i];
+        G.last_time = t_start[i];      // gauges_module.f90:131
+        G.level = 0;
+        G.grid = -1;
+    }
+    g.gcap = std::max(4096, 64 * n);
+    HIPCHK(hipMalloc(&g.d_gauges, n * sizeof(GaugeDev)));
+    HIPCHK(hipMalloc(&g.d_gmeta, (size_t)g.gcap * 2 * sizeof(int32_t)));
+    HIPCHK(hipMalloc(&g.d_gvals, (size_t)g.gcap * (g.cfg.nvar + 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&g.d_gcount, 2 * sizeof(unsigned int)));
+    HIPCHK(h2d(g.d_gauges, g.gauges.data(), n * sizeof(GaugeDev)));
+    HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long*>(g.d_gcount), 0ull, 1, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return AMRGPU_OK;
+}
+
+int amrgpu_set_gauge_sources(const int* level, const int* grid) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    const int n = (int)g.gauges.size();
+    if (n == 0) return AMRGPU_OK;
+    if (!level || !grid) return fail(AMRGPU_ERR_ARG, "bad gauge source arrays");
+    // the device holds each gauge's last record time: read it back first
+    HIPCHK(d2h(g.gauges.data(), g.d_gauges, n * sizeof(GaugeDev)));
+    for (int i = 0; i < n; i++) {
+        GaugeDev& G = g.gauges[i];
+        const int l = level[i], k = grid[i];
+        if (l < 1 || l >= MAXLV || k < 0 || k >= g.levels[l].ngrids) {
+            G.level = 0;
+            G.grid = -1;
+            continue;
+        }
+        const Level& L = g.levels[l];
+        G.level = l;
+        G.grid = k;
+        G.hx = L.hx;
+        G.hy = L.hy;
+        G.xlow = L.cxlo[k] - NGHOST * L.hx;     // advanc.f:229-230
+        G.ylow = L.cylo[k] - NGHOST * L.hy;
+    }
+    HIPCHK(h2d(g.d_gauges, g.gauges.data(), n * sizeof(GaugeDev)));
+    return AMRGPU_OK;
+}
+
+int amrgpu_get_gauge_records(int maxrec, int* nrec, int* gauge, int* level, double* vals) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (!nrec) return fail(AMRGPU_ERR_ARG, "nrec is NULL");
+    *nrec = 0;
+    if (g.gauges.empty()) return AMRGPU_OK;
+    unsigned int cnt[2] = {0, 0};
+    HIPCHK(d2h(cnt, g.d_gcount, sizeof cnt));
+    if (cnt[1] > 0)
+        return fail(AMRGPU_ERR_STATE, "%u gauge records were refused (record buffer of %d full, "
+                    "or a gauge outside its source grid)", cnt[1], g.gcap);
+    const int n = (int)cnt[0];
+    if (n > maxrec) return fail(AMRGPU_ERR_ARG, "%d gauge records waiting, room for %d", n, maxrec);
+    if (n > 0) {
+        if (!gauge || !level || !vals) return fail(AMRGPU_ERR_ARG, "bad gauge record arrays");
+        std::vector<int32_t> meta(2 * (size_t)n);
+        HIPCHK(d2h(meta.data(), g.d_gmeta, meta.size() * sizeof(int32_t)));
+        HIPCHK(d2h(vals, g.d_gvals, (size_t)n * (g.cfg.nvar + 1) * sizeof(double)));
+        for (int r = 0; r < n; r++) {
+            gauge[r] = meta[2 * r];
+            level[r] = meta[2 * r + 1];
+        }
+    }
+    HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long*>(g.d_gcount), 0ull, 1, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    *nrec = n;
+    return AMRGPU_OK;
+}
+
 int amrgpu_set_level(int level, int ngrids, const int* ilo, const int* ihi, const int* jlo,
                      const int* jhi, double hx, double hy) {
     int rc = check_level(level);
@@ -1568,10 +1866,19 @@ int amrgpu_set_geometry(int mxnest, const int* iregsz, const int* jregsz, const 
     if (mxnest < 1 || mxnest >= MAXLV || !iregsz || !jregsz || !intratx || !intraty || !hxposs ||
         !hyposs)
         return fail(AMRGPU_ERR_ARG, "bad geometry arguments");
+    bool same = mxnest == g.mxnest;
     for (int l = 1; l <= mxnest; l++) {
         if (iregsz[l - 1] < 1 || jregsz[l - 1] < 1 || intratx[l - 1] < 1 || intraty[l - 1] < 1 ||
             !(hxposs[l - 1] > 0) || !(hyposs[l - 1] > 0))
             return fail(AMRGPU_ERR_ARG, "bad geometry of level %d", l);
+        same = same && g.iregsz[l] == iregsz[l - 1] && g.jregsz[l] == jregsz[l - 1] &&
+               g.ratx[l] == intratx[l - 1] && g.raty[l] == intraty[l - 1] &&
+               g.hxposs[l] == hxposs[l - 1] && g.hyposs[l] == hyposs[l - 1];
+    }
+    // the same geometry again (a driver hands it over after every regrid):
+    // nothing planned from it is stale
+    if (same) return AMRGPU_OK;
+    for (int l = 1; l <= mxnest; l++) {
         g.iregsz[l] = iregsz[l - 1];
         g.jregsz[l] = jregsz[l - 1];
         g.ratx[l] = intratx[l - 1];
@@ -1587,6 +1894,7 @@ int amrgpu_set_geometry(int mxnest, const int* iregsz, const int* jregsz, const 
         free_plan(L);
     }
     g.regs_valid = false;
+    g.geom_gen++;
     g.layout_version++;
     return AMRGPU_OK;
 }
@@ -1666,7 +1974,7 @@ int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richa
 int amrgpu_set_launch_mode(int mode) {
     if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
     if (mode != AMRGPU_LAUNCH_BATCHED && mode != AMRGPU_LAUNCH_PER_PATCH &&
-        mode != AMRGPU_LAUNCH_PERSISTENT)
+        mode != AMRGPU_LAUNCH_PERSISTENT && mode != AMRGPU_LAUNCH_MARCH)
         return fail(AMRGPU_ERR_ARG, "unknown launch mode %d", mode);
     if (mode == g.launch_mode) return AMRGPU_OK;
     int rc0 = check_tiling(g.txmax, g.tymax, mode);

@@ -65,6 +65,11 @@ struct Lev3 {
     Reg3Level reg;
     RegGrid3* d_regs = nullptr;
     double* d_reg = nullptr;
+    // layout generation (a new value at every amrgpu3_set_level of this
+    // level) and the generations d_reg was laid out for (this level's and
+    // the next coarser level's), with its size
+    uint64_t gen = 0, reg_gen = 0, reg_gen_coarse = 0;
+    int64_t reg_elems = 0;
     Qad3Item* d_qad = nullptr;
     Sv3Entry* d_sv = nullptr;
     Upbnd3Cell* d_ucells = nullptr;
@@ -87,6 +92,7 @@ struct Amr3 {
     int64_t scratch_cap = 0;
     bool planned = false;        // ghost-fill programs
     bool planned_regs = false;   // saveqc programs and registers
+    uint64_t gen_counter = 0;    // source of Lev3::gen values
     int plane_block = 0;         // sweep plane block of the levels (0: per level)
 };
 
@@ -218,10 +224,22 @@ int ensure_plans() {
         if (!rc) rc = upload(L.d_uents, L.reg.uents);
         if (!rc) rc = upload(L.d_avg, L.reg.avg);
         if (rc) return rc;
-        dfree(L.d_reg);
-        const int64_t n = std::max<int64_t>(1, L.reg.reg_elems);
-        HIPCHKA(hipMalloc((void**)&L.d_reg, n * sizeof(double)));
-        HIPCHKA(hipMemsetAsync(L.d_reg, 0, n * sizeof(double), ctx3_stream()));   // prepf.f
+        // a regrid replaces the levels above lbase only (regrid.f): a fine
+        // level whose grids and coarser grids are unchanged keeps what its
+        // registers have accumulated in the current coarse step, as the
+        // reference keeps node(ffluxptr) of such grids; the others start
+        // from zero (prepf.f)
+        const bool keep = l >= 2 && L.d_reg && L.reg_gen == L.gen &&
+                          L.reg_gen_coarse == A.lev[l - 1].gen && L.reg_elems == L.reg.reg_elems;
+        if (!keep) {
+            dfree(L.d_reg);
+            const int64_t n = std::max<int64_t>(1, L.reg.reg_elems);
+            HIPCHKA(hipMalloc((void**)&L.d_reg, n * sizeof(double)));
+            HIPCHKA(hipMemsetAsync(L.d_reg, 0, n * sizeof(double), ctx3_stream()));   // prepf.f
+        }
+        L.reg_gen = L.gen;
+        L.reg_gen_coarse = l >= 2 ? A.lev[l - 1].gen : 0;
+        L.reg_elems = L.reg.reg_elems;
     }
     HIPCHKA(hipStreamSynchronize(ctx3_stream()));
     A.planned_regs = true;
@@ -385,6 +403,23 @@ int amrgpu3_set_geometry(int nlev, const int* iregsz, const int* jregsz, const i
     for (int s = 0; s < 6; s++)
         if (mthbc[s] == 3 && nv < s / 2 + 2)
             return ctx3_fail(AMRGPU_ERR_UNSUPPORTED, "wall boundaries need a momentum component");
+    {
+        // the same geometry again (a driver hands it over after every
+        // regrid): keep the levels, their plans and their registers
+        const Hier3Geom& O = A.H;
+        bool same = A.geom && O.nlev == nlev;
+        for (int l = 1; same && l <= nlev; l++) {
+            const Level3Geom& G = O.lev[l];
+            same = G.iregsz == iregsz[l - 1] && G.jregsz == jregsz[l - 1] &&
+                   G.kregsz == kregsz[l - 1] && G.ratx == ratx[l - 1] && G.raty == raty[l - 1] &&
+                   G.ratz == ratz[l - 1] && G.hx == hx[l - 1] && G.hy == hy[l - 1] &&
+                   G.hz == hz[l - 1];
+        }
+        same = same && O.xlower == domain[0] && O.xupper == domain[1] && O.ylower == domain[2] &&
+               O.yupper == domain[3] && O.zlower == domain[4] && O.zupper == domain[5];
+        for (int k = 0; same && k < 6; k++) same = O.mthbc[k] == mthbc[k];
+        if (same) return AMRGPU_OK;
+    }
     amr3_release();
     Hier3Geom& H = A.H;
     H.nlev = nlev;
@@ -435,6 +470,7 @@ int amrgpu3_set_level(int level, int ngrids, const int* boxes) {
     free_level(L);
     L.time = t;
     L.possk = pk;
+    L.gen = ++A.gen_counter;
     G.boxes.clear();
     const int nv = ctx3_cfg().nvar;
     int64_t qo = 0, fo = 0;

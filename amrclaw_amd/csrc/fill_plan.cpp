@@ -19,6 +19,58 @@
 
 namespace amrgpu {
 
+void BoxIndex::build(const std::vector<std::array<int, 4>>& boxes, int iregsz, int jregsz) {
+    const int n = (int)boxes.size();
+    start.clear();
+    items.clear();
+    nbx = nby = 0;
+    if (n == 0 || iregsz < 1 || jregsz < 1) return;
+    double area = 0;
+    for (const auto& b : boxes) area += double(b[1] - b[0] + 1) * double(b[3] - b[2] + 1);
+    bs = std::max(8, (int)std::ceil(std::sqrt(area / n)));
+    nbx = (iregsz + bs - 1) / bs;
+    nby = (jregsz + bs - 1) / bs;
+    std::vector<int> count((size_t)nbx * nby + 1, 0);
+    auto range = [&](const std::array<int, 4>& b, int& bx0, int& bx1, int& by0, int& by1) {
+        bx0 = std::min(std::max(b[0], 0) / bs, nbx - 1);
+        bx1 = std::min(std::max(b[1], 0) / bs, nbx - 1);
+        by0 = std::min(std::max(b[2], 0) / bs, nby - 1);
+        by1 = std::min(std::max(b[3], 0) / bs, nby - 1);
+    };
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            start.assign((size_t)nbx * nby + 1, 0);
+            for (size_t b = 0; b + 1 < count.size(); b++) start[b + 1] = start[b] + count[b];
+            items.assign(start.back(), 0);
+            std::fill(count.begin(), count.end(), 0);
+        }
+        for (int k = 0; k < n; k++) {
+            int bx0, bx1, by0, by1;
+            range(boxes[k], bx0, bx1, by0, by1);
+            for (int by = by0; by <= by1; by++)
+                for (int bx = bx0; bx <= bx1; bx++) {
+                    const size_t b = (size_t)by * nbx + bx;
+                    if (pass == 0) count[b]++;
+                    else items[start[b] + count[b]++] = k;
+                }
+        }
+    }
+}
+
+void BoxIndex::query(int ilo, int ihi, int jlo, int jhi, std::vector<int>& out) const {
+    out.clear();
+    if (nbx == 0 || ihi < ilo || jhi < jlo) return;
+    const int bx0 = std::min(std::max(ilo, 0) / bs, nbx - 1), bx1 = std::min(std::max(ihi, 0) / bs, nbx - 1);
+    const int by0 = std::min(std::max(jlo, 0) / bs, nby - 1), by1 = std::min(std::max(jhi, 0) / bs, nby - 1);
+    for (int by = by0; by <= by1; by++)
+        for (int bx = bx0; bx <= bx1; bx++) {
+            const size_t b = (size_t)by * nbx + bx;
+            out.insert(out.end(), items.begin() + start[b], items.begin() + start[b + 1]);
+        }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+}
+
 namespace {
 
 struct Planner {
@@ -27,10 +79,23 @@ struct Planner {
     int nv;
     std::string err;
     std::vector<std::vector<int>> patch_buf;   // [level][grid] -> buffer id
+    std::vector<BoxIndex> index;               // [level]: the grids a rectangle meets
+    std::vector<int> cand;
 
     Planner(const HierGeom& h, FillPlan& p, int nvar) : H(h), P(p), nv(nvar) {
         patch_buf.resize(H.nlev + 1);
-        for (int l = 1; l <= H.nlev; l++) patch_buf[l].assign(H.lev[l].boxes.size(), -1);
+        index.resize(H.nlev + 1);
+        for (int l = 1; l <= H.nlev; l++) {
+            patch_buf[l].assign(H.lev[l].boxes.size(), -1);
+            index[l].build(H.lev[l].boxes, H.lev[l].iregsz, H.lev[l].jregsz);
+        }
+    }
+
+    // the grids of `level` that can meet [ilo..ihi] x [jlo..jhi], ascending
+    // (a superset: callers intersect each box themselves)
+    const std::vector<int>& candidates(int level, int ilo, int ihi, int jlo, int jhi) {
+        index[level].query(ilo, ihi, jlo, jhi, cand);
+        return cand;
     }
 
     int pbuf(int level, int grid) {
@@ -148,8 +213,9 @@ struct Planner {
             return -1;
         }
         std::vector<unsigned char> flag((size_t)ni * nj, 0);
-        // intfil: same-level grids intersecting the rectangle
-        for (int k = 0; k < (int)G.boxes.size(); k++) {
+        // intfil: same-level grids intersecting the rectangle, in grid order
+        const std::vector<int> ks = candidates(level, ilo, ihi, jlo, jhi);
+        for (int k : ks) {
             const auto& b = G.boxes[k];
             if (msrc >= 0) {
                 // a grid's own strips take same-level values only from its
@@ -337,7 +403,8 @@ struct Planner {
                std::vector<unsigned char>& covered) {
         const LevelGeom& G = H.lev[level];
         const FillBuf fb = P.bufs[buf];
-        for (int k = 0; k < (int)G.boxes.size(); k++) {
+        const std::vector<int> ks = candidates(level, ilo, ihi, jlo, jhi);
+        for (int k : ks) {
             const auto& b = G.boxes[k];
             int ixlo = std::max(b[0], ilo), ixhi = std::min(b[1], ihi);
             int jxlo = std::max(b[2], jlo), jxhi = std::min(b[3], jhi);

@@ -16,6 +16,25 @@ struct LevelGeom {
     int iregsz = 0, jregsz = 0;              // level size in cells
     int ratx = 2, raty = 2;                  // ratio to the next finer level
     double hx = 0, hy = 0;
+    // listbc of every grid as the caller's prepc.f built it (i, j counted
+    // from the first ghost cell, side, kid as its 0-based index on the next
+    // finer level, slot); empty: the planner builds its own (setuse.f order
+    // over the finer grids in their index order)
+    std::vector<std::vector<std::array<int, 5>>> listbc;
+};
+
+// A bucket index over the boxes of one level: the grids a rectangle can
+// meet, in ascending grid order, without scanning every grid of the level
+// (the reference scans them all: intfil.f90:153-156 with msrc = -1,
+// init_bndryList.f:50-107 for the neighbour lists).  Buckets are squares of
+// about the mean box size over the level's index space.
+struct BoxIndex {
+    int bs = 1, nbx = 0, nby = 0;
+    std::vector<int> start, items;   // CSR: grids of bucket b in items[start[b]..start[b+1])
+    void build(const std::vector<std::array<int, 4>>& boxes, int iregsz, int jregsz);
+    // grids whose box meets [ilo..ihi] x [jlo..jhi], ascending
+    void query(int ilo, int ihi, int jlo, int jhi, std::vector<int>& out) const;
+    bool empty() const { return nbx == 0; }
 };
 
 struct HierGeom {

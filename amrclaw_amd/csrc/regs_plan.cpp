@@ -83,9 +83,31 @@ bool plan_registers(const HierGeom& H, int nvar, RegLevel* R, std::string& err) 
         const int rx = C.ratx, ry = C.raty;
         const int imax = C.iregsz - 1, jmax = C.jregsz - 1;
         Rl.sv_grid_start.push_back(0);
+        const bool given = !C.listbc.empty();
+        if (given && C.listbc.size() != C.boxes.size()) {
+            err = "listbc given for some grids of a level only";
+            return false;
+        }
         for (size_t m = 0; m < C.boxes.size(); m++) {
             const auto& cb = C.boxes[m];
             const int ilo = cb[0], ihi = cb[1], jlo = cb[2], jhi = cb[3];
+            if (given) {
+                // the caller's list, in its order: upbnd adds a coarse cell's
+                // entries in list order (upbnd.f:59-113), so the order of the
+                // kids prepc visited fixes the FP64 sums
+                for (const auto& e : C.listbc[m]) {
+                    const int I = e[0] - NGHOST, J = e[1] - NGHOST, side = e[2], kid = e[3];
+                    if (I < 1 || I > ihi - ilo + 1 || J < 1 || J > jhi - jlo + 1 || side < 1 ||
+                        side > 4 || kid < 0 || kid >= (int)F.boxes.size() || e[4] < 1 ||
+                        e[4] > R[l + 1].lenbc[kid]) {
+                        err = "listbc entry outside its grids";
+                        return false;
+                    }
+                    Rl.sv.push_back({I, J, side, kid, e[4], 0});
+                }
+                Rl.sv_grid_start.push_back((int32_t)Rl.sv.size());
+                continue;
+            }
             for (size_t kid = 0; kid < F.boxes.size(); kid++) {
                 const auto& fb = F.boxes[kid];
                 int iclo = fb[0] / rx, jclo = fb[2] / ry, ichi = fb[1] / rx, jchi = fb[3] / ry;

@@ -51,6 +51,10 @@ struct SweepArgs {
     // persistent launch when non-null: the tile counter, zeroed before the
     // launch
     unsigned long long* next;
+    // march sweep (sweep_march.hpp) when non-null: its wave table; the tile
+    // fields are then unused
+    const MarchWave* waves;
+    int32_t nwaves;
 };
 
 // Perimeter flux block of a fine grid (FP64, after svdflx and qc1d):
@@ -125,7 +129,11 @@ template <class Sol, typename T> struct LaneResult {
 // updates telescope and the totals are conserved to FP64 rounding although
 // the waves are FP32, and F carries only G's FP32 error, which scales with
 // the jump, not with f.
-template <class Sol, typename T, bool FLUXFORM = false>
+//
+// STD: the method is known to be the standard one (std_method: second
+// order, both transverse corrections, MC limiter on every wave, no
+// f-waves), so its switches are compile-time constants; same arithmetic.
+template <class Sol, typename T, bool FLUXFORM = false, bool STD = false>
 __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T* ql,
                                            const T* qr, bool inner, int w, T dtdx,
                                            LaneResult<Sol, T>& o) {
@@ -161,7 +169,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
     T cqxx[NV];
 #pragma unroll
     for (int m = 0; m < NV; m++) cqxx[m] = T(0);
-    if (M.order != 1) {
+    if (STD || M.order != 1) {
 #pragma unroll
         for (int mw = 0; mw < NW; mw++) {
             T wn2 = T(0), dotr = T(0);
@@ -173,7 +181,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
                 dotr = dotr + wave[mw][m] * wnext;
             }
             T dotl = lane_up(dotr, w);
-            const int mth = M.mthlim[mw];
+            const int mth = STD ? 4 : M.mthlim[mw];
             if (mth != 0 && inner && wn2 != T(0)) {
                 using Mth = typename Sol::Math;
                 T r = Mth::ratio((s[mw] > T(0)) ? dotl : dotr, wn2);
@@ -190,7 +198,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
 #pragma unroll
             for (int mw = 0; mw < NW; mw++) {
                 if (SKIPZ && Sol::wave_zero(ixy, mw, m)) continue;
-                T abs_sign = M.use_fwaves ? copysign(T(1), s[mw]) : fabs(s[mw]);
+                T abs_sign = (!STD && M.use_fwaves) ? copysign(T(1), s[mw]) : fabs(s[mw]);
                 c = c + abs_sign * (T(1) - fabs(s[mw]) * dtdxave) * wave[mw][m];
             }
             cqxx[m] = c;
@@ -199,8 +207,8 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
         }
     }
 
-    if (M.transverse != 0) {
-        if (M.order > 1 && M.transverse == 2) {
+    if (STD || M.transverse != 0) {
+        if (STD || (M.order > 1 && M.transverse == 2)) {
 #pragma unroll
             for (int m = 0; m < NV; m++) {
                 amdq[m] = amdq[m] + cqxx[m];
@@ -235,6 +243,16 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+// The standard method of the run configurations (method(2) = 2,
+// method(3) = 2, MC limiter on every wave, no f-waves): the sweeps have
+// instantiations with these switches fixed at compile time.
+__host__ __device__ inline bool std_method(const MethodArgs& M) {
+    if (M.order != 2 || M.transverse != 2 || M.use_fwaves) return false;
+    for (int mw = 0; mw < M.mwaves && mw < NWMAX; mw++)
+        if (M.mthlim[mw] != 4) return false;
+    return true;
+}
+
 // LDS bytes needed for a tile of tx * ty cells (compute type T)
 __host__ __device__ inline int sweep_qs_stride(int tx) { return (tx + 4) | 1; }
 // LDS of one tile: the staged states and the edge increments; `merged` (the
@@ -252,7 +270,7 @@ __host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty, bool m
 // one tile of the sweep, run by one workgroup.  LX, LY: compile-time extents
 // of the LDS layout (the tile caps; every tile fits in them), so that the
 // LDS strides are constants; 0: the tile's own extents
-template <class Sol, typename T, typename S, int LX = 0, int LY = 0>
+template <class Sol, typename T, typename S, int LX = 0, int LY = 0, bool STD = false>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                                            unsigned char* smem_raw) {
     constexpr int NV = Sol::NV;
@@ -324,7 +342,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             const bool inner = live && c >= 1 && c <= tx + 1;
             LaneResult<Sol, T> o;
-            flux2_lane<Sol, T, MIXED>(a.meth, 1, ql, qr, inner, w, dtdx, o);
+            flux2_lane<Sol, T, MIXED, STD>(a.meth, 1, ql, qr, inner, w, dtdx, o);
             cfl = dmax1(cfl, o.cfl);
             if (inner && r >= 1 && r <= ty) {
 #pragma unroll
@@ -335,7 +353,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             const bool cell = live && c >= 1 && c <= tx;       // cell k = c
             // step A: gadd(.,2) of row r goes to edge r+1
-            if (a.meth.transverse != 0 && cell && r <= ty) {
+            if ((STD || a.meth.transverse != 0) && cell && r <= ty) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
                     T* g = &gm[m * gpl + r * gsx + (c - 1)];
@@ -344,7 +362,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             __syncthreads();
             // step B: gadd(.,1) of row r goes to edge r
-            if (a.meth.transverse != 0 && cell && r >= 1) {
+            if ((STD || a.meth.transverse != 0) && cell && r >= 1) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
                     T* g = &gm[m * gpl + (r - 1) * gsx + (c - 1)];
@@ -373,7 +391,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             const bool inner = live && e >= 1 && e <= ty + 1;
             LaneResult<Sol, T> o;
-            flux2_lane<Sol, T, MIXED>(a.meth, 2, ql, qr, inner, w, dtdy, o);
+            flux2_lane<Sol, T, MIXED, STD>(a.meth, 2, ql, qr, inner, w, dtdy, o);
             cfl = dmax1(cfl, o.cfl);
             if (inner && c >= 1 && c <= tx) {
 #pragma unroll
@@ -386,7 +404,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             const bool cell = live && e >= 1 && e <= ty;       // cell j = e
             // step A: gadd(.,2) of column c goes to x-edge c+1
-            if (a.meth.transverse != 0 && cell && c <= tx) {
+            if ((STD || a.meth.transverse != 0) && cell && c <= tx) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
                     const int fi = m * fpl + (e - 1) * fstr + c;
@@ -396,7 +414,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             __syncthreads();
             // step B: gadd(.,1) of column c goes to x-edge c
-            if (a.meth.transverse != 0 && cell && c >= 1) {
+            if ((STD || a.meth.transverse != 0) && cell && c >= 1) {
 #pragma unroll
                 for (int m = 0; m < NV; m++) {
                     const int fi = m * fpl + (e - 1) * fstr + (c - 1);
@@ -560,16 +578,16 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
 // batched launch: one workgroup per tile of the tile table.  WGS: the
 // workgroups per CU the register allocation is bounded for (a tuning knob:
 // the mixed path is instantiated for 5, 6 and 7)
-template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0, bool STD = false>
 __global__ __launch_bounds__(256, WGS) void sweep_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    sweep_tile<Sol, T, S, LX, LY>(a, blockIdx.x, smem_raw);
+    sweep_tile<Sol, T, S, LX, LY, STD>(a, blockIdx.x, smem_raw);
 }
 
 // persistent launch: a grid sized to fill the device takes tiles from the
 // table in order (largest first) through the counter a.next, which starts
 // at zero; every workgroup leaves once the counter has passed the table
-template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0, bool STD = false>
 __global__ __launch_bounds__(256, WGS) void sweep_persistent_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int tile;
@@ -578,7 +596,7 @@ __global__ __launch_bounds__(256, WGS) void sweep_persistent_kernel(SweepArgs a)
         __syncthreads();
         const int t = tile;
         if (t >= a.ntiles) break;   // the same value in every lane
-        sweep_tile<Sol, T, S, LX, LY>(a, t, smem_raw);
+        sweep_tile<Sol, T, S, LX, LY, STD>(a, t, smem_raw);
         __syncthreads();            // LDS and `tile` are reused by the next tile
     }
 }

@@ -7,17 +7,18 @@
 #include <algorithm>
 #include "kernels.hpp"
 #include "sweep.hpp"
+#include "sweep_march.hpp"
 
 namespace amrgpu {
 
-template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0, bool STD = false>
 static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     static bool attr_set = false;
     if (!attr_set) {
         // allow tiles whose LDS footprint exceeds the 64 KiB default: the
         // CU's 160 KiB less the kernel's static LDS
-        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S, WGS, LX, LY>,
-                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS, LX, LY>};
+        const void* fns[2] = {(const void*)sweep_kernel<Sol, T, S, WGS, LX, LY, STD>,
+                              (const void*)sweep_persistent_kernel<Sol, T, S, WGS, LX, LY, STD>};
         for (const void* f : fns) {
             hipFuncAttributes fa{};
             hipError_t e = hipFuncGetAttributes(&fa, f);
@@ -30,8 +31,8 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     }
     if (a.ntiles == 0) return hipSuccess;   // (a call with no tiles only sets the attributes)
     if (!a.next) {
-        hipLaunchKernelGGL((sweep_kernel<Sol, T, S, WGS, LX, LY>), dim3(a.ntiles), dim3(256), lds,
-                           st, a);
+        hipLaunchKernelGGL((sweep_kernel<Sol, T, S, WGS, LX, LY, STD>), dim3(a.ntiles), dim3(256),
+                           lds, st, a);
         return hipGetLastError();
     }
     // persistent: as many workgroups as fit on the device at this LDS size
@@ -40,7 +41,7 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
     if (grid == 0 || grid_lds != lds) {
         int per_cu = 0, dev = 0, ncu = 0;
         hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S, WGS, LX, LY>, 256, lds);
+            &per_cu, (const void*)sweep_persistent_kernel<Sol, T, S, WGS, LX, LY, STD>, 256, lds);
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (e != hipSuccess) return e;
@@ -48,8 +49,8 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
         grid_lds = lds;
     }
     const int g = std::min(grid, a.ntiles);
-    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S, WGS, LX, LY>), dim3(g), dim3(256), lds,
-                       st, a);
+    hipLaunchKernelGGL((sweep_persistent_kernel<Sol, T, S, WGS, LX, LY, STD>), dim3(g), dim3(256),
+                       lds, st, a);
     return hipGetLastError();
 }
 
@@ -59,7 +60,16 @@ static hipError_t sweep_go(const SweepArgs& a, size_t lds, hipStream_t st) {
 template <class Sol, typename T, typename S, int WGS, bool FIXED>
 static hipError_t sweep_layout(const SweepArgs& a, size_t lds, hipStream_t st) {
     if constexpr (FIXED) {
-        if (sizeof(T) == 4 && a.layout == 1) return sweep_go<Sol, T, S, WGS, 28, 12>(a, lds, st);
+        if (sizeof(T) == 4 && a.layout == 1) {
+            // the standard method's instantiation too (its attributes are
+            // set by the same attributes-only call)
+            if (a.ntiles == 0 && !a.waves) {
+                hipError_t e = sweep_go<Sol, T, S, WGS, 28, 12, true>(a, lds, st);
+                if (e != hipSuccess) return e;
+            }
+            if (std_method(a.meth)) return sweep_go<Sol, T, S, WGS, 28, 12, true>(a, lds, st);
+            return sweep_go<Sol, T, S, WGS, 28, 12>(a, lds, st);
+        }
         if (sizeof(T) == 8 && a.layout == 2) return sweep_go<Sol, T, S, WGS, 12, 12>(a, lds, st);
     }
     return sweep_go<Sol, T, S, WGS>(a, lds, st);
@@ -73,9 +83,30 @@ static hipError_t sweep_layout(const SweepArgs& a, size_t lds, hipStream_t st) {
 // VGPRs for Euler with a 40-byte spill, 3 % faster than its unbounded 137
 // VGPRs at 3).  FIXED: instantiate the fixed-layout variants too (the Euler
 // solver, whose tuned caps they match).
+// The march sweep (a.waves set): four independent waves per workgroup, no
+// LDS.  Its register bound: 2 waves per SIMD (256 VGPRs, the default), or 3
+// with FP32 arithmetic (wgs == 3: 168 VGPRs, part of the state spilled).
+template <class Sol, typename T, typename S, int MWGS, bool STD = false>
+static hipError_t march_go(const SweepArgs& a, hipStream_t st) {
+    if (a.nwaves == 0) return hipSuccess;
+    hipLaunchKernelGGL((sweep_march_kernel<Sol, T, S, MWGS, STD>), dim3((a.nwaves + 3) / 4),
+                       dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 template <template <typename, class> class SolT, typename T, class Mth, typename S, bool FIXED>
 static hipError_t sweep_solver(const SweepArgs& a, size_t lds, hipStream_t st, int wgs) {
     using Sol = SolT<T, Mth>;
+    if (a.waves) {
+        if constexpr (sizeof(T) == 4) {
+            if constexpr (FIXED) {
+                if (std_method(a.meth))
+                    return wgs == 3 ? march_go<Sol, T, S, 3, true>(a, st) : march_go<Sol, T, S, 2, true>(a, st);
+            }
+            if (wgs == 3) return march_go<Sol, T, S, 3>(a, st);
+        }
+        return march_go<Sol, T, S, 2>(a, st);
+    }
     if constexpr (sizeof(T) == 4 && sizeof(S) == 8) {
         if (wgs == 5) return sweep_layout<Sol, T, S, 5, FIXED>(a, lds, st);
         if (wgs == 6) return sweep_layout<Sol, T, S, 6, FIXED>(a, lds, st);

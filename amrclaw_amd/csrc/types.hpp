@@ -45,4 +45,17 @@ struct TileDesc {
 };
 enum { TILE_FLUXAD_L = 1, TILE_FLUXAD_R = 2, TILE_FLUXAD_B = 4, TILE_FLUXAD_T = 8 };
 
+// The march sweep's work unit (sweep_march.hpp): one 64-lane wave holding
+// nseg strips of segw lanes each (16, 32 or 64), all of grids my rows high.
+// Strip s covers the interior columns x0+1 .. x0+w (1-based, w <= segw - 4)
+// of grid `patch`.
+struct MarchSeg {
+    int32_t patch;
+    int16_t x0, w;
+};
+struct MarchWave {
+    MarchSeg seg[4];
+    int16_t segw, nseg, my, pad;
+};
+
 }  // namespace amrgpu

@@ -94,9 +94,16 @@ subroutine amrgpu3_upload_hierarchy(nvar)
      isz(level) = iregsz(level)
      jsz(level) = jregsz(level)
      ksz(level) = kregsz(level)
-     rx(level) = intratx(level)
-     ry(level) = intraty(level)
-     rz(level) = intratz(level)
+     ! the finest level has no ratio to a finer one (amr3.f90 reads the
+     ! ratios of levels 1..mxnest-1 only)
+     rx(level) = 1
+     ry(level) = 1
+     rz(level) = 1
+     if (level .lt. mxnest) then
+        rx(level) = intratx(level)
+        ry(level) = intraty(level)
+        rz(level) = intratz(level)
+     end if
   end do
   dom = (/ xlower, xupper, ylower, yupper, zlower, zupper /)
   bc = mthbc(1:6)

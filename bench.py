@@ -72,6 +72,8 @@ def parse():
                     help="time the one-launch-per-patch sweep instead")
     ap.add_argument("--persistent", action="store_true",
                     help="persistent sweep launch (workgroups take tiles from a counter)")
+    ap.add_argument("--march", action="store_true",
+                    help="march sweep (column strips marching up the rows in registers)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay the level steps as hipGraphs in the timed region")
     ap.add_argument("--no-compare", action="store_true",
@@ -121,12 +123,17 @@ def install(capi, H, precision, launch, graphs, txmax, tymax, occupancy=0):
             txmax, tymax = tuned
     if txmax or tymax:
         capi.set_tiling(txmax, tymax)
-    wgs = occupancy or autotune.load_occupancy(capi.EULER, key)
-    if wgs and precision == capi.MIXED:
-        capi.set_occupancy(wgs)
+    if launch == "march":
+        if occupancy:
+            capi.set_occupancy(occupancy)
+    else:
+        wgs = occupancy or autotune.load_occupancy(capi.EULER, key)
+        if wgs and precision == capi.MIXED:
+            capi.set_occupancy(wgs)
     if launch != "batched":
         capi.set_launch_mode({"per-patch": capi.LAUNCH_PER_PATCH,
-                              "persistent": capi.LAUNCH_PERSISTENT}[launch])
+                              "persistent": capi.LAUNCH_PERSISTENT,
+                              "march": capi.LAUNCH_MARCH}[launch])
     capi.load_hierarchy(H)
     dt2 = H.levels[1]["possk"]
     capi.set_time(1, 0.0, 2.0 * dt2)
@@ -161,7 +168,7 @@ def run(capi, dt2, steps, warmup, sync):
     return out
 
 
-def conservation_drift(capi, precision, npatch=256, coarse_steps=4):
+def conservation_drift(capi, precision, npatch=256, coarse_steps=4, launch="batched"):
     """Composite conservation on a periodic domain: the two-level batched
     hierarchy with npatch level-2 patches and periodic boundaries, level 1
     first averaged from level 2, then coarse_steps Berger-Oliger steps
@@ -173,6 +180,8 @@ def conservation_drift(capi, precision, npatch=256, coarse_steps=4):
                            gamma=1.4, efix=True, mthbc=(2, 2, 2, 2), precision=precision)
     capi.init(cfg)
     try:
+        if launch == "march":
+            capi.set_launch_mode(capi.LAUNCH_MARCH)
         capi.load_hierarchy(H)
         dt2 = H.levels[1]["possk"]
         capi.set_time(1, 0.0, 2.0 * dt2)
@@ -675,7 +684,8 @@ def main():
     bytes_per_cell = 2 * 4 * esize            # (2 nvar + maux) * s
 
     print("bench: timed run", a.precision, file=sys.stderr, flush=True)
-    launch = "per-patch" if a.per_patch else ("persistent" if a.persistent else "batched")
+    launch = ("per-patch" if a.per_patch else "persistent" if a.persistent else
+              "march" if a.march else "batched")
     dt2 = install(capi, H, prec, launch, a.graphs, a.txmax, a.tymax, a.occupancy)
     r = run(capi, dt2, a.steps, a.warmup, sync)
     q_main = interiors(capi, H) if not a.no_compare else None
@@ -701,7 +711,7 @@ def main():
                                 for k in ("sweep", "fill", "registers")},
     }
     if sweep_n:
-        tuned_default = not (a.txmax or a.tymax or a.per_patch or a.persistent)
+        tuned_default = not (a.txmax or a.tymax or a.per_patch or a.persistent or a.march)
         tr = MEASURED_TRAFFIC.get(("euler2d", a.precision, npatch)) if tuned_default else None
         kern_ms = sweep_ms / sweep_n        # one timed record per step's sweep
         achieved = cells * bytes_per_cell / (kern_ms / 1e3) / 1e9
@@ -709,7 +719,7 @@ def main():
                               "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                               "traffic": tr[0] if tr else None,
                               "traffic_source": tr[1] if tr else None,
-                              "kernel": "sweep_kernel<EulerRoe>",
+                              "kernel": "sweep_march_kernel<EulerRoe>" if launch == "march" else "sweep_kernel<EulerRoe>",
                               "kernel_ms": kern_ms,
                               "bytes_per_cell_update": bytes_per_cell,
                               "algorithmic_bytes_per_launch": cells * bytes_per_cell}
@@ -772,9 +782,10 @@ def main():
             "setup": "periodic two-level hierarchy, 256 level-2 patches, 4 coarse steps "
                      "(each: level 1, two level-2 steps, update with refluxing); "
                      "|conck(1) - conck0(1)| / conck0(1)",
-            "mixed": conservation_drift(capi, capi.MIXED),
-            "fp32": conservation_drift(capi, capi.FP32),
-            "fp64": conservation_drift(capi, capi.FP64)}
+            "launch": launch,
+            "mixed": conservation_drift(capi, capi.MIXED, launch=launch),
+            "fp32": conservation_drift(capi, capi.FP32, launch=launch),
+            "fp64": conservation_drift(capi, capi.FP64, launch=launch)}
     if a.cpu_seconds > 0:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = _quiet_stdout(lambda: cpu_baseline(H, dt2, a.cpu_seconds))

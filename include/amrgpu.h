@@ -81,10 +81,12 @@ int amrgpu_last_error(char* buf, int len);
 
 /* Tile-size caps of the sweep (autotuner knob); 0 keeps the current value. */
 int amrgpu_set_tiling(int txmax, int tymax);
-/* Occupancy bound of the AMRGPU_MIXED sweep (autotuner knob, no counterpart
- * in the reference): the workgroups per CU its register allocation is
- * bounded for, 5, 6 or 7 (0: the default, 7).  The other precisions ignore
- * it. */
+/* Occupancy bounds (autotuner knobs, no counterpart in the reference):
+ * 5, 6 or 7 set the workgroups per CU the register allocation of the
+ * AMRGPU_MIXED tile sweep is bounded for (default 7; the other precisions
+ * ignore it); 2 or 3 set the waves per SIMD of the march sweep with FP32
+ * arithmetic (AMRGPU_LAUNCH_MARCH; default 2, no spill; 3 spills part of
+ * its state); 0 restores both defaults. */
 int amrgpu_set_occupancy(int wgs);
 
 /* Single-patch time step: stepgrid.f:24 (step2 + conservative update + dtnew)
@@ -174,9 +176,14 @@ int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richa
  * AMRGPU_LAUNCH_PERSISTENT launches as many workgroups as the device holds
  * and lets them take the level's tiles (largest first) from a device
  * counter; AMRGPU_LAUNCH_PER_PATCH issues one launch per grid, the plain
- * per-patch GPU path, kept as the comparison baseline.  Results are
- * identical.  No reference counterpart. */
-enum { AMRGPU_LAUNCH_BATCHED = 0, AMRGPU_LAUNCH_PER_PATCH = 1, AMRGPU_LAUNCH_PERSISTENT = 2 };
+ * per-patch GPU path, kept as the comparison baseline;
+ * AMRGPU_LAUNCH_MARCH runs the sweep of a level that has no finer level as
+ * column strips marching up the rows with every quantity in registers (one
+ * Riemann solve per interface instead of the tiles' halo re-solves; levels
+ * with a finer level keep the tile sweep).  Results are identical (FP64) or
+ * agree to FP32 rounding.  No reference counterpart. */
+enum { AMRGPU_LAUNCH_BATCHED = 0, AMRGPU_LAUNCH_PER_PATCH = 1, AMRGPU_LAUNCH_PERSISTENT = 2,
+       AMRGPU_LAUNCH_MARCH = 3 };
 int amrgpu_set_launch_mode(int mode);
 
 /* With on != 0, amrgpu_advance_level captures its launches (ghost fill,
@@ -245,6 +252,20 @@ int amrgpu_plan_listbc(int nlev, const int* ngrids, const int* boxes,
                        const int* intraty, const int* mthbc, int nvar, int level,
                        int grid, int* list5, int maxn, int* n);
 
+/* The coarse side of the flux registers of grid `grid` of `level` as the
+ * caller's prepc.f built it (node(cfluxptr), prepc.f:47-154): n entries of
+ * five ints, (i, j) of the coarse cell counted from the first ghost cell,
+ * the side 1..4 facing the fine grid, the fine grid as its 0-based index on
+ * level+1 (amrgpu_set_level order) and its register slot.  upbnd adds the
+ * entries of a coarse cell in list order (upbnd.f:59-113), so handing the
+ * reference's lists over keeps FP64 sums bit for bit when the reference's
+ * kid order differs from its grid list (regrid.f builds the lists before
+ * arrangeGrids sorts the grids).  Without this call the library builds the
+ * lists itself, visiting the finer grids in their index order.  Either all
+ * grids of a level have a list (possibly empty) or none; amrgpu_set_level
+ * of the level or of the next finer level discards them. */
+int amrgpu_set_listbc(int level, int grid, int n, const int* list5);
+
 /* Define the grids of one level (after regrid): global index boxes
  * [ilo..ihi] x [jlo..jhi] and mesh widths hx, hy (hxposs/hyposs).
  * Replaces the layout bookkeeping of regrid.f:72-118 on the device side;
@@ -274,6 +295,30 @@ int amrgpu_get_level(int level, int which, double* alloc_base,
  * advanc.f:124-126). */
 int amrgpu_sweep_level(int level, double dt, double* cfl_level,
                        double* dtlevnew);
+
+/* ---- Gauges ----------------------------------------------------------------
+ * The reference samples every gauge from its best source grid (setbestsrc:
+ * the finest grid containing it, gauges_module.f90:255-302) before that
+ * grid's step, by bilinear interpolation of the time-t values with the
+ * ghost cells filled (update_gauges, gauges_module.f90:313-452, called from
+ * par_advanc, advanc.f:249-253).  Here amrgpu_advance_level samples the
+ * gauges whose source lies on the level it advances, on the device, with
+ * the same arithmetic (FP64 records equal the reference's bit for bit),
+ * and keeps the records until they are fetched.
+ *
+ * amrgpu_set_gauges: locations, time windows and minimum time increments
+ * (gauges.data); replaces any previous set.  amrgpu_set_gauge_sources: per
+ * gauge the level and the grid (index within amrgpu_set_level's order) of
+ * its source, level 0 for none; call it after every layout change, as the
+ * reference calls setbestsrc after regrid (tick.f:203).
+ * amrgpu_get_gauge_records: the records since the last call, oldest first
+ * per gauge: gauge[r] (0-based), level[r], and vals[r*(nvar+1) ...] = the
+ * grid time then the nvar interpolated components; fails if more than
+ * maxrec are waiting or records were lost. */
+int amrgpu_set_gauges(int n, const double* x, const double* y, const double* t_start,
+                      const double* t_end, const double* min_dt);
+int amrgpu_set_gauge_sources(const int* level, const int* grid);
+int amrgpu_get_gauge_records(int maxrec, int* nrec, int* gauge, int* level, double* vals);
 
 /* ---- Riemann-solver plug-ins ---------------------------------------------
  * The reference passes the application's rpn2/rpt2 as external procedures

@@ -8,11 +8,38 @@
  * functions; the constants come from the params installed with
  * orc_set_plugin(), the analogue of filling /cparam/ in setprob.
  */
+#include <math.h>
+
 #include "orc.h"
 
 static orc_params g_plugin;
 
 void orc_set_plugin(const orc_params* p) { g_plugin = *p; }
+
+/* The physics of a run of the reference's own driver (oracle/driver/app.f90
+ * calls this from its setprob): the solver and the /cparam/ constants, with
+ * the acoustic impedance and sound speed derived as the library derives
+ * them (cc = sqrt(K / rho), zz = rho cc). */
+void orc_driver_physics(const int* solver, const double* ubar, const double* vbar,
+                        const double* rho, const double* bulk, const double* gamma,
+                        const int* efix) {
+    orc_params p = {0};
+    static const int nv[5] = {0, 1, 3, 4, 3}, nw[5] = {0, 1, 2, 4, 3};
+    p.solver = *solver;
+    p.meqn = (*solver >= 1 && *solver <= 4) ? nv[*solver] : 0;
+    p.mwaves = (*solver >= 1 && *solver <= 4) ? nw[*solver] : 0;
+    p.mbc = 2;
+    p.ubar = *ubar;
+    p.vbar = *vbar;
+    p.rho = *rho;
+    p.bulk = *bulk;
+    p.cc = sqrt(*bulk / *rho);
+    p.zz = *rho * p.cc;
+    p.gamma = *gamma;
+    p.efix = *efix;
+    p.grav = 1.0;
+    g_plugin = p;
+}
 
 void orc_rpn2_byref(const int* ixy, const int* maxm, const int* meqn,
                     const int* mwaves, const int* maux, const int* mbc,

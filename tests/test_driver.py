@@ -1,0 +1,117 @@
+"""The reference's own 2-D driver, built from its unmodified sources with
+the application of oracle/driver, reproduces the reference's regression
+data: tests/advection_2d_square (the reference's setrun.py parameters,
+three levels, regridding every 2 steps, gauges 1 and 2) gives the gauge
+files of tests/advection_2d_square/regression_data (copied to
+tests/golden/advection_2d_square) value for value.  This pins the data
+writer (amrclaw_amd.rundata), the driver build and the oracle's advection
+solver against the reference's own test; tests/test_gpu_driver.py then
+runs the same program with the device library linked in."""
+import os
+
+import numpy as np
+import pytest
+
+from tests import driver_runs as D
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "advection_2d_square")
+
+
+@pytest.mark.skipif(not os.path.exists(D.EXE["ref"]), reason="driver not built (make -C oracle driver)")
+def test_reference_driver_reproduces_regression_gauges(tmp_path):
+    r = D.run("ref", D.square(), str(tmp_path), threads=2)
+    assert r.returncode == 0, open(tmp_path / "run.log").read()[-2000:]
+    for g in (1, 2):
+        got = np.loadtxt(tmp_path / ("gauge%05d.txt" % g), comments="#")
+        want = np.loadtxt(os.path.join(GOLD, "gauge%05d.txt" % g), comments="#")
+        assert got.shape == want.shape
+        assert np.array_equal(got, want), g          # level, time, q at e15.7
+
+
+def test_rundata_round_trip_order(tmp_path):
+    """The claw.data fields in amr2.f90's read order (src/2d/amr2.f90:
+    152-327): ndim in column 1, then the domain, sizes, counts, output and
+    algorithm parameters, mthbc(1), mthbc(3) then mthbc(2), mthbc(4)."""
+    rd = D.config("C2")
+    rd.write(str(tmp_path))
+    lines = [l for l in open(tmp_path / "claw.data").read().split("\n")]
+    assert lines[0].startswith("#") and lines[1] == ""
+    body = lines[2:]
+    assert body[0] == "2"
+    assert body[1].split() == ["-1.0", "-1.0"] and body[3].split() == ["50", "50"]
+    assert body[4:8] == ["3", "2", "0", "0.0"]
+    body = [l for l in body if l]
+    # mthlim, num_ghost, mthbc(1) mthbc(3), mthbc(2) mthbc(4), restart,
+    # restart file, checkpt_style
+    assert body[-7:-4] == ["4 4", "2", "1 1"] and body[-4] == "1 1"
+    assert body[-3:] == ["F", "fort.chk00000", "0"]
+    reg = open(tmp_path / "regions.data").read().split("\n")
+    assert reg[2][:2] == "0 "                                  # read as (i2)
+
+
+@pytest.mark.skipif(not os.path.exists(D.EXE["ref"]), reason="driver not built (make -C oracle driver)")
+def test_checkpoint_comparison_ignores_only_free_blocks(tmp_path):
+    """checkpoint_records zeroes alloc's free blocks and nothing else: on a
+    reference checkpoint of C1, a word changed inside a free block leaves
+    the records equal, a word changed in a grid's solution (store1 of the
+    first level-1 grid, node table of record 4) makes alloc differ."""
+    import struct
+    rd = D.config("C1", checkpt_style=3, checkpt_interval=40, tfinal=0.8)
+    r = D.run("ref", rd, str(tmp_path), threads=2)
+    assert r.returncode == 0
+    path = str(tmp_path / "fort.chk00040")
+    raw = bytearray(open(path, "rb").read())
+    recs, pos, i = [], [], 0
+    while i < len(raw):
+        n = struct.unpack("<i", raw[i:i + 4])[0]
+        recs.append(bytes(raw[i + 4:i + 4 + n]))
+        pos.append(i + 4)
+        i += 8 + n
+    lenmax, lendim, memsize, maxgr = struct.unpack("<4i", recs[0][:16])
+    lfree = np.frombuffer(recs[3][:8 * D.LFDIM], "<i4").reshape(2, D.LFDIM)
+    lenf = struct.unpack("<i", recs[3][8 * D.LFDIM:8 * D.LFDIM + 4])[0]
+    free = [(int(lfree[0, k]), int(lfree[1, k])) for k in range(lenf)
+            if lfree[1, k] > 0 and lfree[0, k] + lfree[1, k] - 1 <= lendim]
+    assert free, "no free block inside alloc(1:lendim)"
+    node = np.frombuffer(recs[4][8 * 5 * maxgr:8 * 5 * maxgr + 4 * 19 * maxgr],
+                         "<i4").reshape(maxgr, 19)
+    lstart1 = struct.unpack("<i", recs[4][8 * 5 * maxgr + 4 * 19 * maxgr:][:4])[0]
+    store1 = int(node[lstart1 - 1, 6])                # node(store1, lstart(1))
+    base = D.checkpoint_records(path)
+
+    def planted(word):
+        b = bytearray(raw)
+        at = pos[1] + 8 * (word - 1)
+        b[at] ^= 0x40
+        p = str(tmp_path / "planted")
+        open(p, "wb").write(bytes(b))
+        return D.checkpoint_records(p)
+    in_free = planted(free[0][0] + free[0][1] // 2)
+    assert all(x == y for x, y in zip(base, in_free))
+    in_grid = planted(store1 + 10)
+    assert in_grid[1] != base[1]
+    assert all(x == y for k, (x, y) in enumerate(zip(base, in_grid)) if k != 1)
+
+
+@pytest.mark.skipif(not os.path.exists(D.EXE["ref"]), reason="driver not built (make -C oracle driver)")
+def test_reference_restart_reproduces_uninterrupted_run(tmp_path):
+    """The restart protocol the device-driven restart test follows, on the
+    reference alone: C1 checkpointed at step 40, restarted from it in a
+    fresh directory, ends with the uninterrupted run's final frame byte
+    for byte (the restarted run numbers its frames on from the restart
+    frame, so its fort.*0003 is the uninterrupted fort.*0002)."""
+    import shutil
+    rd = D.config("C1", checkpt_style=3, checkpt_interval=40)
+    full = tmp_path / "full"
+    assert D.run("ref", rd, str(full), threads=2).returncode == 0
+    chk = "fort.chk00040"
+    rdir = tmp_path / "restart"
+    rdir.mkdir()
+    for f in D.RESTART_FILES(chk):
+        shutil.copy(full / f, rdir / f)
+    rest = D.config("C1", checkpt_style=3, checkpt_interval=40, restart=True, restart_file=chk)
+    r = D.run("ref", rest, str(rdir), threads=2)
+    assert r.returncode == 0, open(rdir / "run.log").read()[-2000:]
+    for kind in ("q", "b", "t"):
+        assert open(rdir / ("fort.%s0003" % kind), "rb").read() == \
+            open(full / ("fort.%s0002" % kind), "rb").read(), kind

@@ -108,10 +108,10 @@ def test_level_update_fp64_multi_tile(gpu, tiling):
 
 @pytest.mark.parametrize("graphs", [False, True])
 @pytest.mark.parametrize("mode", [capi.LAUNCH_BATCHED, capi.LAUNCH_PER_PATCH,
-                                  capi.LAUNCH_PERSISTENT])
+                                  capi.LAUNCH_PERSISTENT, capi.LAUNCH_MARCH])
 def test_level_update_launch_variants(gpu, mode, graphs):
-    """One launch per patch and hipGraph replay of the level steps give the
-    oracle's results bit for bit (graphs: every replay after the first of a
+    """One launch per patch, the persistent and the march sweep, and hipGraph
+    replay of the level steps give the oracle's results bit for bit (graphs: every replay after the first of a
     key is checked through the final state after three coarse steps, with
     the program's own synchronising work between the replays)."""
     print("amr variants", mode, graphs, flush=True)

@@ -12,7 +12,7 @@ before update: the registers then hold the whole coarse-fine flux
 difference, which the fluctuation form (FP64, the reference's) and the
 flux form (mixed) compute as the same quantity.
 
-FP64, batched and one launch per patch: the whole of both levels, the dt /
+FP64, batched, one launch per patch and the march sweep: the whole of both levels, the dt /
 CFL results and the registers bit for bit against the oracle.  Mixed (FP32
 waves on FP64 storage, the benchmark's headline path) and FP32 storage:
 relative L1 <= 1e-5 against the oracle on both levels, level-2 mass within
@@ -71,7 +71,7 @@ def _mass(H, q, l):
     return tot * L["hx"] * L["hy"]
 
 
-@pytest.mark.parametrize("launch", [None, capi.LAUNCH_PER_PATCH])
+@pytest.mark.parametrize("launch", [None, capi.LAUNCH_PER_PATCH, capi.LAUNCH_MARCH])
 def test_c4_fp64_bit_exact(c4, launch):
     H, O, olog = c4
     print("c4 fp64", launch, flush=True)
@@ -89,12 +89,13 @@ def test_c4_fp64_bit_exact(c4, launch):
         capi.finalize()
 
 
+@pytest.mark.parametrize("launch", [None, capi.LAUNCH_MARCH])
 @pytest.mark.parametrize("precision", [capi.MIXED, capi.FP32])
-def test_c4_fp32_within_tolerance(c4, precision):
+def test_c4_fp32_within_tolerance(c4, precision, launch):
     H, O, olog = c4
-    print("c4 fp32-arithmetic", precision, flush=True)
+    print("c4 fp32-arithmetic", precision, launch, flush=True)
     try:
-        dlog = _device_run(H, precision)
+        dlog = _device_run(H, precision, launch)
         for l in (1, 2):
             got = capi.read_level(H, l, 0)
             want = O.H.levels[l - 1]["qnew"]

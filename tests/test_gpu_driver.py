@@ -1,0 +1,93 @@
+"""The drop-in boundary driven by the reference's own program: amr2_gpu is
+the reference's 2-D driver (amr2, tick, setgrd, regrid, flagging, valout,
+conck, gauges; src/2d compiled unmodified) with advanc.f and update.f
+replaced by amrclaw_amd/fortran/amrgpu_shim.f90 over libamrgpu.so, and
+amr2_ref the same program with the reference's own advanc / update
+(tests/driver_runs.py).  Both run the same data files; in FP64 every file
+that depends on the solution must be the same byte for byte: the frames
+(fort.q headers with every grid's level, box and corner, so the patch
+layouts that regridding produced; fort.b binary values; fort.t), the gauge
+files, and fort.amr's per-level-step CFL / dt lines and conservation
+lines.  The square advection run also reproduces the reference's
+regression gauges (tests/golden/advection_2d_square)."""
+import os
+
+import numpy as np
+import pytest
+
+from tests import driver_runs as D
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "advection_2d_square")
+built = pytest.mark.skipif(not (os.path.exists(D.EXE["ref"]) and os.path.exists(D.EXE["gpu"])),
+                           reason="driver not built (make -C oracle driver)")
+
+
+@built
+@pytest.mark.parametrize("name", ["square", "C1", "C2", "C3"])
+def test_device_driven_run_matches_reference_run(gpu, tmp_path, name):
+    rd = D.square(output_format="binary", tprint=True) if name == "square" else D.config(name)
+    print("driver run", name, flush=True)
+    outs = {}
+    for which in ("ref", "gpu"):
+        d = str(tmp_path / which)
+        r = D.run(which, rd, d, timeout=300)
+        log = open(os.path.join(d, "run.log")).read()
+        assert r.returncode == 0 and "***" not in log, (which, log[-3000:])
+        outs[which] = D.outputs(d)
+        print("  ", which, "done:", len(outs[which]), "files", flush=True)
+    ref, dev = outs["ref"], outs["gpu"]
+    assert sorted(ref) == sorted(dev)
+    assert ref["fort.amr:steps"].count("AMRCLAW") > 10
+    for f in sorted(ref):
+        assert ref[f] == dev[f], (f, D.first_difference(ref[f], dev[f]))
+    if name == "square":
+        for g in (1, 2):
+            got = np.loadtxt(tmp_path / "gpu" / ("gauge%05d.txt" % g), comments="#")
+            want = np.loadtxt(os.path.join(GOLD, "gauge%05d.txt" % g), comments="#")
+            assert np.array_equal(got, want), g
+
+
+@built
+def test_checkpoints_interchangeable(gpu, tmp_path):
+    """Checkpoint / restart across the boundary (check.f:66-82,
+    restrt.f:39-60, both the reference's, unmodified): C1 checkpointed
+    every 40 coarse steps.  The device-driven run writes the same
+    fort.chk00040 as the reference's run, record for record (every
+    allocated block of alloc: the solutions at both times, the coarse-fine
+    registers and boundary lists; the free list, the node tables, the run
+    state and counters; only the wall and CPU timers differ, as they do
+    between two runs of the reference, and the free blocks of alloc, which
+    hold stale temporaries), and a device-driven run restarted
+    from the reference's checkpoint ends with the reference's
+    uninterrupted final frame byte for byte."""
+    import shutil
+    rd = D.config("C1", checkpt_style=3, checkpt_interval=40)
+    for which in ("ref", "gpu"):
+        d = str(tmp_path / which)
+        r = D.run(which, rd, d, timeout=300)
+        assert r.returncode == 0, which
+    ref, dev = D.outputs(str(tmp_path / "ref")), D.outputs(str(tmp_path / "gpu"))
+    assert sorted(ref) == sorted(dev)
+    for f in sorted(ref):
+        assert ref[f] == dev[f], (f, D.first_difference(ref[f], dev[f]))
+    chk = "fort.chk00040"
+    a = D.checkpoint_records(str(tmp_path / "ref" / chk))
+    b = D.checkpoint_records(str(tmp_path / "gpu" / chk))
+    assert len(a) == len(b) == 6
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert x == y, k
+    rdir = tmp_path / "restart"
+    rdir.mkdir()
+    # a restart appends to the gauge files (gauges_module.f90:198 opens
+    # them old) and the timing table: both must be there
+    for f in D.RESTART_FILES(chk):
+        shutil.copy(tmp_path / "ref" / f, rdir / f)
+    rest = D.config("C1", checkpt_style=3, checkpt_interval=40, restart=True, restart_file=chk)
+    r = D.run("gpu", rest, str(rdir), timeout=300)
+    assert r.returncode == 0, open(rdir / "run.log").read()[-2000:]
+    # the restarted run writes a frame at the restart time (2), then t = 1 (3)
+    for kind in ("q", "b", "t"):
+        assert open(rdir / ("fort.%s0003" % kind), "rb").read() == \
+            open(tmp_path / "ref" / ("fort.%s0002" % kind), "rb").read(), kind
