@@ -28,6 +28,7 @@ UNITS = [
     ("sweep_f64.hip", ["-O3", "-ffp-contract=off"]),
     ("sweep_f32.hip", ["-O3"]),
     ("util.hip", ["-O2"]),
+    ("noalloc.hip", ["-O2"]),
     ("fill.hip", ["-O3", "-ffp-contract=off"]),
     ("regs.hip", ["-O3", "-ffp-contract=off"]),
     ("flag.hip", ["-O3", "-ffp-contract=off"]),

@@ -88,6 +88,8 @@ def lib():
                                       C.c_double, C.c_double, C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p,
                                       dpo, dpo]
+        L.amrgpu_stepgrid_noalloc.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                              C.c_double, C.c_double, C.c_double, dpo, dpo]
         L.amrgpu_set_level.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, _ip,
                                        C.c_double, C.c_double]
         L.amrgpu_put_grid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
@@ -216,6 +218,16 @@ def stepgrid(q, dx, dy, dt, want_fluxes=False):
     _check(lib().amrgpu_stepgrid(mx, my, q, out, dx, dy, dt, *ptrs,
                                  C.byref(cfl), C.byref(dtn)))
     return out, fl, cfl.value, dtn.value
+
+
+def stepgrid_noalloc(mx, my, q_in_dev, q_out_dev, dx, dy, dt):
+    """amrgpu_stepgrid_noalloc: q_in_dev / q_out_dev are device addresses
+    of (my+4, mx+4, nvar) FP64 arrays (the reference's layout).  Returns
+    (cflgrid, dtnew)."""
+    cfl, dtn = C.c_double(), C.c_double()
+    _check(lib().amrgpu_stepgrid_noalloc(mx, my, C.c_void_p(q_in_dev), C.c_void_p(q_out_dev),
+                                         dx, dy, dt, C.byref(cfl), C.byref(dtn)))
+    return cfl.value, dtn.value
 
 
 def set_level(level, boxes, hx, hy):

@@ -20,6 +20,7 @@
 #include "fill_plan.hpp"
 #include "gauges.hpp"
 #include "kernels.hpp"
+#include "noalloc.hpp"
 #include "regs_plan.hpp"
 
 namespace amrgpu {
@@ -1553,6 +1554,9 @@ int amrgpu_init(const amrgpu_config* cfg) {
             for (int w : {5, 6, 7}) HIPCHK(run_sweep(2, none, 0, w));
         }
     }
+    // the device address of amrgpu_stepgrid_noalloc's static work space on
+    // this GPU
+    HIPCHK(noalloc_init());
     g.txmax = 28;
     g.tymax = (g.esize == 8 && !g.mixed) ? 12 : 28;
     g.inited = true;
@@ -1693,6 +1697,69 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out, double dx
         hipFree(d_off);
     }
     free_level(L);
+    return AMRGPU_OK;
+}
+
+int amrgpu_stepgrid_noalloc(int mx, int my, const double* q_in, double* q_out, double dx,
+                            double dy, double dt, double* cflgrid, double* dtnew) {
+    if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
+    if (!q_in || !q_out || mx < 1 || my < 1 || !(dx > 0) || !(dy > 0))
+        return fail(AMRGPU_ERR_ARG, "bad stepgrid_noalloc arguments");
+    unsigned char* base = noalloc_scratch();
+    if (!base)
+        return fail(AMRGPU_ERR_STATE, "no work space on this device: call amrgpu_init on it first");
+    const int nv = g.cfg.nvar;
+    const int64_t plane = (int64_t)(mx + 2 * NGHOST) * (my + 2 * NGHOST);
+    const int64_t bytes = ((int64_t)nv * plane * (int64_t)g.esize + 255) / 256 * 256;
+    if (mx > 30000 || my > 30000 || 2 * bytes > (int64_t)NOALLOC_DATA_BYTES)
+        return fail(AMRGPU_ERR_ARG,
+                    "grid %dx%d needs %lld bytes of work space; amrgpu_stepgrid_noalloc holds %lld",
+                    mx, my, (long long)(2 * bytes), (long long)NOALLOC_DATA_BYTES);
+    // the tiling amrgpu_stepgrid's level planner gives one grid
+    const auto xs = split_axis(mx, g.txmax), ys = split_axis(my, g.tymax);
+    const int ntx = (int)xs.size(), nty = (int)ys.size(), tw = xs[0].second, th = ys[0].second;
+    if ((int64_t)ntx * nty > NOALLOC_MAX_TILES)
+        return fail(AMRGPU_ERR_ARG, "grid %dx%d has %d tiles at caps %dx%d; at most %d", mx, my,
+                    ntx * nty, g.txmax, g.tymax, NOALLOC_MAX_TILES);
+    const bool f32 = g.mixed || g.esize == 4;
+    size_t lds = f32 ? sweep_lds_bytes<float>(nv, tw, th, g.mixed) : sweep_lds_bytes<double>(nv, tw, th);
+    const int lx = f32 ? 28 : 12, ly = 12;
+    int layout = 0;
+    if (g.cfg.solver == SOLVER_EULER && tw <= lx && th <= ly) {
+        layout = f32 ? 1 : 2;
+        lds = std::max(lds, f32 ? sweep_lds_bytes<float>(nv, lx, ly, g.mixed)
+                                : sweep_lds_bytes<double>(nv, lx, ly));
+    }
+    if (lds > LDS_MAX)
+        return fail(AMRGPU_ERR_ARG, "tile caps %dx%d need %zu bytes of LDS", g.txmax, g.tymax, lds);
+    unsigned char* qa = base + NOALLOC_DATA_OFF;
+    unsigned char* qb = qa + bytes;
+    HIPCHK(noalloc_setup(base, mx, my, tw, th, ntx, nty, g.stream));
+    HIPCHK(noalloc_transpose(true, (int)g.esize, q_in, nullptr, qa, nv, (int)plane, g.stream));
+    // ghost cells of the output keep the input values, as in stepgrid.f
+    HIPCHK(noalloc_transpose(true, (int)g.esize, q_in, nullptr, qb, nv, (int)plane, g.stream));
+    SweepArgs a{};
+    a.qin = qa;
+    a.qout = qb;
+    a.patches = reinterpret_cast<const PatchDesc*>(base + NOALLOC_PATCH_OFF);
+    a.tiles = reinterpret_cast<const TileDesc*>(base + NOALLOC_TILES_OFF);
+    a.ntiles = ntx * nty;
+    a.layout = layout;
+    a.dt = dt;
+    a.dx = dx;
+    a.dy = dy;
+    a.meth = g.meth;
+    a.cfl_patch = reinterpret_cast<unsigned long long*>(base + NOALLOC_CFL_OFF);
+    hipError_t e = run_sweep(run_precision(), a, lds, g.sweep_wgs);
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "sweep launch: %s", hipGetErrorString(e));
+    HIPCHK(noalloc_transpose(false, (int)g.esize, nullptr, q_out, qb, nv, (int)plane, g.stream));
+    unsigned long long cb = 0;
+    HIPCHK(hipMemcpyAsync(&cb, base + NOALLOC_CFL_OFF, sizeof cb, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    double c;
+    std::memcpy(&c, &cb, 8);
+    if (cflgrid) *cflgrid = c;
+    if (dtnew) *dtnew = c > 0.0 ? dt * g.cfg.cfl / c : RINFINITY;   // stepgrid.f:206-212
     return AMRGPU_OK;
 }
 

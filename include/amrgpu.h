@@ -98,6 +98,29 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out,
                     double dx, double dy, double dt, double* fm, double* fp,
                     double* gm, double* gp, double* cflgrid, double* dtnew);
 
+/* amrgpu_stepgrid on device-resident arrays without allocating: no
+ * counterpart in the reference.  For callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  q_in / q_out are DEVICE pointers to the reference's layout,
+ * AoS FP64 (nvar, mx+4, my+4), ghost cells of q_in filled; q_out receives
+ * the updated interior and q_in's ghost cells, exactly as amrgpu_stepgrid's
+ * q_out (same kernels, same arithmetic).  cflgrid / dtnew are host
+ * pointers, returned after the stream has drained.  The work buffers (the
+ * grid's descriptor and tile table, its Courant maximum and two SoA copies
+ * of the grid) live in one static device array of
+ * AMRGPU_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: grids
+ * whose two copies need more than that (in FP64 storage with nvar = 4,
+ * about (mx+4)(my+4) <= 260,000 cells) or more than AMRGPU_NOALLOC_MAX_TILES
+ * tiles are refused with AMRGPU_ERR_ARG.  Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other host threads, other
+ * streams) race; the array keeps the last call's contents.  Call amrgpu_init
+ * on the device first (it looks up the array's address there). */
+#define AMRGPU_NOALLOC_SCRATCH_BYTES (16u << 20)
+#define AMRGPU_NOALLOC_MAX_TILES 4096
+int amrgpu_stepgrid_noalloc(int mx, int my, const double* q_in, double* q_out,
+                            double dx, double dy, double dt, double* cflgrid,
+                            double* dtnew);
+
 /* Level geometry of the whole run (domain.f:92-106, amr2.f90:455): for
  * levels 1..mxnest the index-space size iregsz/jregsz, the refinement ratio
  * to the next level intratx/intraty and the mesh widths hxposs/hyposs
