@@ -1,5 +1,5 @@
 // sweep_march.hpp -- the level sweep as column strips marching up the rows,
-// every quantity in registers (no LDS, no barriers).
+// every quantity in registers (no barriers; LDS only as per-lane storage).
 //
 // The work unit is one 64-lane wave holding up to four strips (segments of
 // 16, 32 or 64 lanes) of grids of the same height my.  A strip covers the
@@ -20,7 +20,8 @@
 // Every interface of a strip is solved once (the tile sweep re-solves the
 // rows and columns of each tile's halo), the transverse increments of both
 // directions meet in registers, and each cell of the grid is read from
-// memory once.
+// memory once (the mixed update's FP64 rows come back from a per-lane ring
+// in LDS).
 //
 // The fluxes are accumulated in the reference's order (step2.f90:117-166,
 // flux2.f:202-233), the same sums as the tile sweep (sweep.hpp):
@@ -137,13 +138,15 @@ __global__ __launch_bounds__(256, WGS) void sweep_march_kernel(SweepArgs a) {
     using Cache = typename Sol::Cache;
     const int wv = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (wv >= a.nwaves) return;   // the whole wave leaves
-    const MarchWave D = a.waves[wv];
-    const int lane = threadIdx.x & 63, segw = D.segw;
+    // fields read from the table directly (a local copy of the wave's
+    // record, indexed by segment, would live in scratch memory)
+    const MarchWave* D = a.waves + wv;
+    const int lane = threadIdx.x & 63, segw = D->segw;
     const int sidx = lane / segw, l = lane - sidx * segw;
-    const bool seg_live = sidx < D.nseg;
-    const MarchSeg sg = D.seg[seg_live ? sidx : 0];
+    const bool seg_live = sidx < D->nseg;
+    const MarchSeg sg = D->seg[seg_live ? sidx : 0];
     const PatchDesc pd = a.patches[sg.patch];
-    const int W = sg.w, x0 = sg.x0, my = D.my;
+    const int W = sg.w, x0 = sg.x0, my = D->my;
     const int k = min(l - 1, W + 2);                 // strip cell of this lane
     const bool live = seg_live && l <= W + 3;
     const int col = x0 + k;                          // patch-local 1-based column
@@ -160,29 +163,41 @@ __global__ __launch_bounds__(256, WGS) void sweep_march_kernel(SweepArgs a) {
     const T hdy = T(0.5) * dtdy;
     const bool trans = STD || a.meth.transverse != 0;
 
-    // row R of this lane's column, in storage and compute precision; the
-    // ghost cells of the output keep their time-t values (stepgrid.f
-    // updates the interior only), each written by one lane
-    auto load_row = [&](int R, T q[NV]) {
+    // The mixed update needs rows in storage precision two steps after they
+    // were loaded: each lane keeps its last three rows in a ring in LDS
+    // (written and read by the same lane only, so no barrier), which costs
+    // LDS instructions instead of a second read of the row from memory.
+    __shared__ S ring[MIXED ? 3 : 1][NV][256];
+    const int tid = threadIdx.x;
+    auto slot = [](int R) { return (R + 3) % 3; };   // R >= -1
+
+    // row R of this lane's column in storage precision: issued a step
+    // before the row is needed, so the load's latency overlaps a step's work
+    auto fetch_row = [&](int R, S p[NV]) {
         const int64_t li = (int64_t)(R + 1) * mitot + (col + 1);
-        S p[NV];
 #pragma unroll
-        for (int m = 0; m < NV; m++) {
-            p[m] = qin[m * plane + li];
-            q[m] = T(p[m]);
+        for (int m = 0; m < NV; m++) p[m] = qin[m * plane + li];
+    };
+    // row R, fetched, in compute precision; the ghost cells of the output
+    // keep their time-t values (stepgrid.f updates the interior only), each
+    // written by one lane
+    auto accept_row = [&](int R, const S p[NV], T q[NV]) {
+        const int64_t li = (int64_t)(R + 1) * mitot + (col + 1);
+#pragma unroll
+        for (int m = 0; m < NV; m++) q[m] = T(p[m]);
+        if constexpr (MIXED) {
+#pragma unroll
+            for (int m = 0; m < NV; m++) ring[slot(R)][m][tid] = p[m];
         }
         if (ghost_cols || (x_cell && (R < 1 || R > my))) {
 #pragma unroll
             for (int m = 0; m < NV; m++) qout[m * plane + li] = MIXED ? p[m] : S(q[m]);
         }
     };
-    // row R of this lane's column in storage precision, read again where the
-    // mixed update needs it (a recent read: the cache holds it) instead of
-    // carrying it in registers down the march
+    // row R (loaded at most two steps before) in storage precision
     auto stored_row = [&](int R, S p[NV]) {
-        const int64_t li = (int64_t)(R + 1) * mitot + (col + 1);
 #pragma unroll
-        for (int m = 0; m < NV; m++) p[m] = qin[m * plane + li];
+        for (int m = 0; m < NV; m++) p[m] = ring[slot(R)][m][tid];
     };
     // FP64 physical flux of a stored state (the mixed path's flux form)
     auto phys = [&](int ixy, const S p[NV], double f[NV]) {
@@ -212,12 +227,24 @@ __global__ __launch_bounds__(256, WGS) void sweep_march_kernel(SweepArgs a) {
     T cfl = T(0);
     double* per = a.reg ? a.reg + perim_offset(pd, NV) : nullptr;
 
-    load_row(-1, qb);
-    // unrolled by two: the state carried from one step to the next changes
-    // registers instead of being copied
-#pragma unroll 2
+    S pn[NV];                   // the next row, in flight
+    fetch_row(-1, pn);
+    accept_row(-1, pn, qb);
+    fetch_row(0, pn);
     for (int J = -1; J <= my + 1; J++) {
-        load_row(J + 1, qc);
+        accept_row(J + 1, pn, qc);
+        if (J + 2 <= my + 2) fetch_row(J + 2, pn);   // rows -1 .. my+2 exist
+
+        // 4. (first, while the y-sweep's values of this step are not yet
+        // live) x-row J (J >= 0): the whole flux2 slice, neighbours by DPP
+        LaneResult<Sol, T> o;
+        if (J >= 0) {
+            T ql[NV];
+#pragma unroll
+            for (int m = 0; m < NV; m++) ql[m] = lane_up(qb[m], 0);   // cell k-1
+            flux2_lane<Sol, T, MIXED, STD>(a.meth, 1, ql, qb, xs_inner, 0, dtdx, o);
+            cfl = dmax1(cfl, o.cfl);
+        }
 
         // 2. y-interface J+1 (rows J, J+1): Roe cache and rpn2
         Cache cN = Sol::prep(a.meth.ph, 2, qb, qc);
@@ -242,16 +269,6 @@ __global__ __launch_bounds__(256, WGS) void sweep_march_kernel(SweepArgs a) {
                 gy1[m] = (SKIPZ ? -(hdy * bm1[m]) : T(0) - hdy * bm1[m]) - hdy * bm2p[m];
                 gy2[m] = (SKIPZ ? -(hdy * bp1[m]) : T(0) - hdy * bp1[m]) - hdy * bp2p[m];
             }
-        }
-
-        // 4. x-row J (J >= 0): the whole flux2 slice, neighbours by DPP
-        LaneResult<Sol, T> o;
-        if (J >= 0) {
-            T ql[NV];
-#pragma unroll
-            for (int m = 0; m < NV; m++) ql[m] = lane_up(qb[m], 0);   // cell k-1
-            flux2_lane<Sol, T, MIXED, STD>(a.meth, 1, ql, qb, xs_inner, 0, dtdx, o);
-            cfl = dmax1(cfl, o.cfl);
         }
 
         // 5a. the x-fluxes of row J-1 (rows 1..my are needed)

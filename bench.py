@@ -10,9 +10,14 @@ corrections (method(2) = method(3) = 2), extrapolation boundaries; the
 four-quadrant initial state with a seeded +-1e-3 perturbation.
 
 One step = one level update of level 2 (amrgpu_advance_level: ghost fill
-with time-interpolated coarse values, the sweep of all 4096 patches with
-the fine-side register work qad + fluxad), always at the same point of the
-coarse step (level-2 time dt2, between the coarse times 0 and 2 dt2).
+with time-interpolated coarse values, the sweep of all 4096 patches and the
+fine-side register work), always at the same point of the coarse step
+(level-2 time dt2, between the coarse times 0 and 2 dt2).  The register
+work differs by precision: the FP64 and FP32 runs do qad and fluxad as the
+reference does; the mixed run works in flux form, so its registers take
+the fluxes the sweep computed at the perimeter and qad, which turns the
+fine side's fluctuations into fluxes, is skipped: its speed-up over FP64
+batched is therefore a little larger than a like-for-like one.
 Inputs are resident in HBM before the timed region; the CFL readback is
 not part of the step.
 
@@ -44,14 +49,27 @@ HBM_PEAK_GBS = 8000.0
 # plain-copy calibration: FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB); the
 # summaries and commands are the files named here.
 MEASURED_TRAFFIC = {
-    ("euler2d", "mixed", 4096): (1200800 * 2048 + 783700 * 1024,
-                                 "profiles/level_update_euler_mixed_4096.md"),
-    ("euler2d", "fp32", 4096): (459126 * 2048 + 450853 * 1024,
-                                "profiles/level_update_euler_fp32_4096.md"),
+    ("euler2d", "mixed", 4096, "batched"): (1200800 * 2048 + 783700 * 1024,
+                                            "profiles/level_update_euler_mixed_4096.md"),
+    ("euler2d", "fp32", 4096, "batched"): (459126 * 2048 + 450853 * 1024,
+                                           "profiles/level_update_euler_fp32_4096.md"),
+    ("euler2d", "mixed", 4096, "march"): (397129 * 2048 + 782660 * 1024,
+                                          "profiles/level_update_euler_mixed_4096_march.md"),
     ("euler3d", "fp32", 512): (
         (271400 + 875200 + 1415000 + 330700) * 2048 + (801800 + 1136000 + 1143000 + 98580) * 1024,
         "profiles/step3_euler_fp32_512.md"),
 }
+# Vector-ALU wave instructions per launch of the same kernels (rocprofv3
+# --pmc SQ_INSTS_VALU of the default workloads; the summaries named here).
+MEASURED_VALU = {
+    ("euler2d", "mixed", 4096, "march"): (423823602,
+                                          "profiles/level_update_euler_mixed_4096_march.md"),
+    ("euler2d", "mixed", 4096, "batched"): (757577540,
+                                            "profiles/level_update_euler_mixed_4096.md"),
+}
+SIMD_COUNT = 1024                 # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9                  # peak engine clock
+VALU_ISSUE_CYCLES = 2             # a SIMD issues one wave64 VALU instruction per 2 cycles
 METRIC = "cell-updates/s, FP32-batched vs FP64 per-patch at fixed L1 err; % HBM peak"
 
 
@@ -72,8 +90,12 @@ def parse():
                     help="time the one-launch-per-patch sweep instead")
     ap.add_argument("--persistent", action="store_true",
                     help="persistent sweep launch (workgroups take tiles from a counter)")
+    ap.add_argument("--tile", action="store_true",
+                    help="time the batched tile sweep (one workgroup per LDS tile) instead "
+                         "of the default march sweep")
     ap.add_argument("--march", action="store_true",
-                    help="march sweep (column strips marching up the rows in registers)")
+                    help="march sweep (the default: column strips marching up the rows in "
+                         "registers)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay the level steps as hipGraphs in the timed region")
     ap.add_argument("--no-compare", action="store_true",
@@ -685,7 +707,7 @@ def main():
 
     print("bench: timed run", a.precision, file=sys.stderr, flush=True)
     launch = ("per-patch" if a.per_patch else "persistent" if a.persistent else
-              "march" if a.march else "batched")
+              "batched" if a.tile else "march")
     dt2 = install(capi, H, prec, launch, a.graphs, a.txmax, a.tymax, a.occupancy)
     r = run(capi, dt2, a.steps, a.warmup, sync)
     q_main = interiors(capi, H) if not a.no_compare else None
@@ -711,18 +733,31 @@ def main():
                                 for k in ("sweep", "fill", "registers")},
     }
     if sweep_n:
-        tuned_default = not (a.txmax or a.tymax or a.per_patch or a.persistent or a.march)
-        tr = MEASURED_TRAFFIC.get(("euler2d", a.precision, npatch)) if tuned_default else None
+        tuned_default = not (a.txmax or a.tymax or a.occupancy) and launch in ("march", "batched")
+        key = ("euler2d", a.precision, npatch, launch)
+        tr = MEASURED_TRAFFIC.get(key) if tuned_default else None
+        vr = MEASURED_VALU.get(key) if tuned_default else None
         kern_ms = sweep_ms / sweep_n        # one timed record per step's sweep
         achieved = cells * bytes_per_cell / (kern_ms / 1e3) / 1e9
         result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                               "traffic": tr[0] if tr else None,
                               "traffic_source": tr[1] if tr else None,
-                              "kernel": "sweep_march_kernel<EulerRoe>" if launch == "march" else "sweep_kernel<EulerRoe>",
+                              "kernel": ("sweep_march_kernel<EulerRoe>" if launch == "march"
+                                         else "sweep_kernel<EulerRoe>"),
                               "kernel_ms": kern_ms,
                               "bytes_per_cell_update": bytes_per_cell,
                               "algorithmic_bytes_per_launch": cells * bytes_per_cell}
+        if vr:
+            # the kernel issues VALU work, it does not saturate HBM: the
+            # vector instructions it executes per launch (SQ_INSTS_VALU, one
+            # count per wave instruction) per cell update, and the issue rate
+            # they imply at this run's kernel time
+            result["roofline"]["valu"] = {
+                "wave_instructions_per_launch": vr[0],
+                "lane_instructions_per_cell": vr[0] * 64 / cells,
+                "issue_frac": vr[0] * VALU_ISSUE_CYCLES / (SIMD_COUNT * kern_ms * 1e-3 * CLOCK_HZ),
+                "source": vr[1]}
     if not a.no_compare and prec != capi.FP64:
         h2 = H.levels[1]["hx"]
 
