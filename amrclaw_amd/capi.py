@@ -424,6 +424,37 @@ def plan_fill(H, level, domain=None, mthbc=None):
     return list(ops)[:nops.value], list(bufs)[:nbufs.value], bytes(masks)[:nmask.value]
 
 
+def regrid_level(level, boxes, time):
+    """amrgpu_regrid_level: replace the grids of `level` by boxes ((n, 4)
+    ilo, ihi, jlo, jhi) filled on the device as filval fills new grids."""
+    b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 4))
+    cols = [np.ascontiguousarray(b[:, k]) for k in range(4)]
+    L = lib()
+    L.amrgpu_regrid_level.argtypes = [C.c_int, C.c_int, _ip, _ip, _ip, _ip, C.c_double]
+    _check(L.amrgpu_regrid_level(level, len(b), *cols, time))
+
+
+def bndry_lists(boxes, iregsz, jregsz):
+    """amrgpu_bndry_lists (host only): every grid's neighbour list as
+    makeBndryList builds it, as a list of lists of 0-based grid indices."""
+    b = np.ascontiguousarray(np.asarray(boxes, dtype=np.int32).reshape(-1, 4))
+    n = len(b)
+    counts = np.zeros(max(n, 1), dtype=np.int32)
+    total = C.c_int64()
+    L = lib()
+    L.amrgpu_bndry_lists.argtypes = [C.c_int, _ip, C.c_int, C.c_int, _ip, C.c_void_p, C.c_int64,
+                                     C.POINTER(C.c_int64)]
+    L.amrgpu_bndry_lists(n, b, iregsz, jregsz, counts, None, 0, C.byref(total))
+    items = np.zeros(max(total.value, 1), dtype=np.int32)
+    _check(L.amrgpu_bndry_lists(n, b, iregsz, jregsz, counts, items.ctypes.data_as(C.c_void_p),
+                                total.value, C.byref(total)))
+    out, pos = [], 0
+    for k in range(n):
+        out.append([int(x) for x in items[pos:pos + counts[k]]])
+        pos += counts[k]
+    return out
+
+
 def plan_listbc(H, level, grid):
     """Host-planned listbc entries of one coarse grid (no device)."""
     lv = H.levels

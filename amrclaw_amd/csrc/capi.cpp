@@ -2109,6 +2109,143 @@ static_assert(offsetof(amrgpu_fill_buf, off) == offsetof(FillBuf, off), "fill bu
 
 static FillPlan g_inspect;
 
+int amrgpu_regrid_level(int level, int ngrids, const int* ilo, const int* ihi, const int* jlo,
+                        const int* jhi, double time) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (level < 2 || level >= FILL_OLDSLOT || level > g.mxnest)
+        return fail(AMRGPU_ERR_ARG, "regrid: level %d outside 2..mxnest", level);
+    if (ngrids < 0 || (ngrids > 0 && (!ilo || !ihi || !jlo || !jhi)))
+        return fail(AMRGPU_ERR_ARG, "bad grid arrays");
+    if (g.levels[level - 1].ngrids == 0)
+        return fail(AMRGPU_ERR_STATE, "regrid: level %d has no grids to interpolate from", level - 1);
+    if (!(g.hxposs[level] > 0) || g.iregsz[level] < 1)
+        return fail(AMRGPU_ERR_STATE, "regrid: the geometry of level %d is not set", level);
+    Level& L = g.levels[level];
+    // keep the old grids' new-time values and descriptors: define_level
+    // frees the rest of the old level
+    std::vector<std::array<int, 4>> old_boxes;
+    for (const auto& pd : L.patches) old_boxes.push_back({pd.ilo, pd.ihi, pd.jlo, pd.jhi});
+    void* old_buf = L.buf[L.cur];
+    PatchDesc* old_d = L.d_patches;
+    L.buf[L.cur] = nullptr;
+    L.d_patches = nullptr;
+    auto drop_old = [&]() {
+        if (old_buf) hipFree(old_buf);
+        if (old_d) hipFree(old_d);
+        old_buf = nullptr;
+        old_d = nullptr;
+    };
+    rc = define_level(level, ngrids, ilo, ihi, jlo, jhi, g.hxposs[level], g.hyposs[level]);
+    if (rc) {
+        drop_old();
+        return rc;
+    }
+    L.time = time;
+    if (ngrids == 0) {
+        drop_old();
+        return AMRGPU_OK;
+    }
+    HierGeom H = make_geom();
+    LevelGeom& O = H.lev[FILL_OLDSLOT];
+    O = H.lev[level];
+    O.boxes = old_boxes;
+    O.listbc.clear();
+    FillPlan P;
+    std::string err;
+    if (!plan_filval(H, level, g.cfg.nvar, P, err)) {
+        drop_old();
+        return fail(AMRGPU_ERR_ARG, "level %d regrid: %s", level, err.c_str());
+    }
+    FillOp* d_ops = nullptr;
+    FillBuf* d_bufs = nullptr;
+    unsigned char* d_masks = nullptr;
+    double* d_scratch = nullptr;
+    auto drop_plan = [&]() {
+        void* ps[] = {d_ops, d_bufs, d_masks, d_scratch};
+        for (void* p : ps)
+            if (p) hipFree(p);
+    };
+    hipError_t e = hipSuccess;
+    if (!P.ops.empty()) e = hipMalloc(&d_ops, P.ops.size() * sizeof(FillOp));
+    if (e == hipSuccess && !P.ops.empty()) e = h2d(d_ops, P.ops.data(), P.ops.size() * sizeof(FillOp));
+    if (e == hipSuccess && !P.bufs.empty()) e = hipMalloc(&d_bufs, P.bufs.size() * sizeof(FillBuf));
+    if (e == hipSuccess && !P.bufs.empty()) e = h2d(d_bufs, P.bufs.data(), P.bufs.size() * sizeof(FillBuf));
+    if (e == hipSuccess && !P.masks.empty()) e = hipMalloc(&d_masks, P.masks.size());
+    if (e == hipSuccess && !P.masks.empty()) e = h2d(d_masks, P.masks.data(), P.masks.size());
+    if (e == hipSuccess && P.scratch_elems > 0)
+        e = hipMalloc(&d_scratch, (size_t)P.scratch_elems * sizeof(double));
+    FillArgs a{};
+    a.bufs = d_bufs;
+    a.scratch = d_scratch;
+    a.masks = d_masks;
+    for (int l = 1; l < MAXLV && l < FILL_MAXLV; l++) {
+        Level& S = g.levels[l];
+        a.lvl_new[l] = S.buf[S.cur];
+        a.lvl_old[l] = S.buf[1 - S.cur];
+        a.lvl_patches[l] = S.d_patches;
+        a.lvl_reg[l] = S.d_reg;
+        a.hx[l] = g.hxposs[l];
+        a.hy[l] = g.hyposs[l];
+        a.tmode[l] = TIME_NEW;
+        a.alphai[l] = 1.0;
+        a.alphac[l] = 0.0;
+    }
+    a.lvl_new[FILL_OLDSLOT] = old_buf;
+    a.lvl_old[FILL_OLDSLOT] = old_buf;
+    a.lvl_patches[FILL_OLDSLOT] = old_d;
+    a.xlower = g.cfg.xlower;
+    a.ylower = g.cfg.ylower;
+    for (int k = 0; k < 4; k++) a.mthbc[k] = g.cfg.mthbc[k];
+    a.nv = g.cfg.nvar;
+    for (int s = 0; e == hipSuccess && s < P.nstages; s++) {
+        a.ops = d_ops + P.stage_start[s];
+        a.nops = P.stage_start[s + 1] - P.stage_start[s];
+        e = launch_fill_stage((int)g.esize, a, g.stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    drop_plan();
+    drop_old();
+    if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "level %d regrid: %s", level, hipGetErrorString(e));
+    return AMRGPU_OK;
+}
+
+int amrgpu_bndry_lists(int n, const int* boxes, int iregsz, int jregsz, int* counts, int* items,
+                       int64_t cap, int64_t* total) {
+    if (n < 0 || (n > 0 && (!boxes || !counts)) || !total || cap < 0 || iregsz < 1 || jregsz < 1)
+        return fail(AMRGPU_ERR_ARG, "bad amrgpu_bndry_lists arguments");
+    std::vector<std::array<int, 4>> bx(n);
+    for (int k = 0; k < n; k++)
+        bx[k] = {boxes[4 * k], boxes[4 * k + 1], boxes[4 * k + 2], boxes[4 * k + 3]};
+    BoxIndex index;
+    index.build(bx, iregsz, jregsz);
+    std::vector<int> cand;
+    int64_t pos = 0;
+    for (int k = 0; k < n; k++) {
+        const int ilo = bx[k][0] - NGHOST, ihi = bx[k][1] + NGHOST;
+        const int jlo = bx[k][2] - NGHOST, jhi = bx[k][3] + NGHOST;
+        index.query(ilo, ihi, jlo, jhi, cand);
+        int cnt = 0;
+        // each grid found goes to the front of the reference's list, so the
+        // list runs from the highest index down
+        for (auto it = cand.rbegin(); it != cand.rend(); ++it) {
+            const int s = *it;
+            if (s == k) continue;
+            const auto& b = bx[s];
+            if (std::max(b[0], ilo) <= std::min(b[1], ihi) && std::max(b[2], jlo) <= std::min(b[3], jhi)) {
+                if (items && pos < cap) items[pos] = s;
+                pos++;
+                cnt++;
+            }
+        }
+        counts[k] = cnt;
+    }
+    *total = pos;
+    if (pos > cap) return fail(AMRGPU_ERR_ARG, "amrgpu_bndry_lists: %lld entries, room for %lld",
+                               (long long)pos, (long long)cap);
+    return AMRGPU_OK;
+}
+
 int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes, const int* iregsz,
                      const int* jregsz, const int* intratx, const int* intraty,
                      const double* hxposs, const double* hyposs, const double* domain,

@@ -108,6 +108,38 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
                 dst.store(n, op.di0 + if_ - 1, op.dj0 + jf - 1, valint);
             }
         }
+    } else if (op.type == FOP_FILVAL) {
+        // src: the coarse patch grown by one coarse cell (ni x nj); dst: the
+        // new grid; mask: its interior cells, row by row, 1 where an old
+        // grid supplied the value (the reference's setflags test)
+        const BufRef<S> crs = bufref<S>(a, op.src);
+        const int rx = op.ratx, ry = op.raty, mic = op.ni, mjc = op.nj;
+        const int ncx = mic - 2, ncc = ncx * (mjc - 2), nx = ncx * rx;
+        const unsigned char* mask = a.masks + op.mask_off;
+        for (int t = tid; t < ncc * nv; t += 256) {
+            const int m = t / ncc, c = t - m * ncc;
+            const int jc = c / ncx + 1, ic = c - (jc - 1) * ncx + 1;   // 0-based, 1..mic-2
+            const double vc = crs.load(m, ic, jc);
+            const double vxp = crs.load(m, ic + 1, jc), vxm = crs.load(m, ic - 1, jc);
+            const double vyp = crs.load(m, ic, jc + 1), vym = crs.load(m, ic, jc - 1);
+            double s1p = vxp - vc, s1m = vc - vxm;
+            double slopex = fmin(fabs(s1p), fabs(s1m)) * copysign(1.0, vxp - vxm);
+            if (s1m * s1p <= 0.0) slopex = 0.0;
+            s1p = vyp - vc;
+            s1m = vc - vym;
+            double slopey = fmin(fabs(s1p), fabs(s1m)) * copysign(1.0, vyp - vym);
+            if (s1m * s1p <= 0.0) slopey = 0.0;
+            for (int jco = 1; jco <= ry; jco++) {
+                const double yoff = ((double)jco - 0.5) / (double)ry - 0.5;
+                const int jf = (jc - 1) * ry + jco - 1;                    // interior, 0-based
+                for (int ico = 1; ico <= rx; ico++) {
+                    const double xoff = ((double)ico - 0.5) / (double)rx - 0.5;
+                    const int if_ = (ic - 1) * rx + ico - 1;
+                    if (!mask[(int64_t)jf * nx + if_])
+                        dst.store(m, NGHOST + if_, NGHOST + jf, vc + xoff * slopex + yoff * slopey);
+                }
+            }
+        }
     } else if (op.type == FOP_COPYBACK) {
         const BufRef<S> src = bufref<S>(a, op.src);
         for (int t = tid; t < ncell * nv; t += 256) {

@@ -22,8 +22,13 @@ enum FillOpType : int32_t {
     FOP_BC = 2,        // bc2amr on a whole buffer (bc2amr.f90:84-296)
     FOP_COPYBACK = 3,  // prefilrecur: patchCopyOut (prefilp.f90:218-245)
     FOP_RING = 4,      // cstore: coarse perimeter ring of a buffer -> qc1d (cstore.f:4-63)
-    FOP_COARSEN = 5    // coarsen.f: 2x2 means of a buffer into another (prepbigstep.f:58)
+    FOP_COARSEN = 5,   // coarsen.f: 2x2 means of a buffer into another (prepbigstep.f:58)
+    FOP_FILVAL = 6     // filval.f90:154-188: min-mod interpolation of a coarse patch into
+                       // the interior cells of a new grid no old grid covered
 };
+// the FillArgs level slot that holds the old grids of a level being
+// regridded (gfixup.f fills the new grids while the old ones still exist)
+constexpr int FILL_OLDSLOT = FILL_MAXLV - 1;
 // FillOp.flags bits (COPY)
 enum : int32_t {
     FOPF_FORCE_NEW = 1,   // icall copies store1, no time blend (icall.f)

@@ -573,6 +573,104 @@ struct Planner {
         int ready = std::max(std::max(r[0], r[1]), std::max(r[2], r[3]));
         return bc_op(buf, hx, hy, xl, xr, yb, yt, ready + 1);
     }
+
+    // preintcopy.f: icall over [ilo..ihi] x [jlo..jhi] split into the
+    // pieces inside the domain and their periodic images
+    void preicall(int level, int buf, int ilo, int ihi, int jlo, int jhi,
+                  std::vector<unsigned char>& covered) {
+        const LevelGeom& G = H.lev[level];
+        int ist[3], iend[3], ishift[3], jst[3], jend[3], jshift[3];
+        if (H.xperdom) {
+            ist[0] = ilo; ist[1] = 0; ist[2] = G.iregsz;
+            iend[0] = -1; iend[1] = G.iregsz - 1; iend[2] = ihi;
+            ishift[0] = G.iregsz; ishift[1] = 0; ishift[2] = -G.iregsz;
+        } else {
+            ist[0] = G.iregsz; ist[1] = ilo; ist[2] = G.iregsz;
+            iend[0] = -1; iend[1] = ihi; iend[2] = -1;
+            ishift[0] = ishift[1] = ishift[2] = 0;
+        }
+        if (H.yperdom) {
+            jst[0] = jlo; jst[1] = 0; jst[2] = G.jregsz;
+            jend[0] = -1; jend[1] = G.jregsz - 1; jend[2] = jhi;
+            jshift[0] = G.jregsz; jshift[1] = 0; jshift[2] = -G.jregsz;
+        } else {
+            jst[0] = G.jregsz; jst[1] = jlo; jst[2] = G.jregsz;
+            jend[0] = -1; jend[1] = jhi; jend[2] = -1;
+            jshift[0] = jshift[1] = jshift[2] = 0;
+        }
+        for (int a = 0; a < 3; a++) {
+            const int i1 = std::max(ilo, ist[a]), i2 = std::min(ihi, iend[a]);
+            if (i1 > i2) continue;
+            for (int c = 0; c < 3; c++) {
+                const int j1 = std::max(jlo, jst[c]), j2 = std::min(jhi, jend[c]);
+                if (j1 > j2) continue;
+                icall(level, buf, i1 + ishift[a], i2 + ishift[a], j1 + jshift[c], j2 + jshift[c],
+                      i1 - ilo + 1, j1 - jlo + 1, covered);
+            }
+        }
+    }
+
+    // filval.f90:3-188 (no aux) for new grid k of `level`: the coarse patch
+    // grown by one coarse cell from level-1's grids (intcopy / preintcopy,
+    // stage 0) with bc2amr (stage 1); the old grids of the level (slot
+    // FILL_OLDSLOT) copied over the grid and its ghost ring (stage 0); the
+    // min-mod interpolation into the interior cells no old grid reached
+    // (stage 2)
+    bool filval(int level, int k) {
+        const LevelGeom& F = H.lev[level];
+        const LevelGeom& C = H.lev[level - 1];
+        const int levc = level - 1, ng = NGHOST;
+        const auto& b = F.boxes[k];
+        const int rx = C.ratx, ry = C.raty;
+        const int nx = b[1] - b[0] + 1, ny = b[3] - b[2] + 1;
+        if (nx % rx || ny % ry || b[0] % rx || b[2] % ry)
+            return fail("filval: a new grid is not aligned to the coarse cells");
+        const int mic = nx / rx + 2, mjc = ny / ry + 2;                     // gfixup.f:83-84
+        const int iclo = b[0] / rx - 1, jclo = b[2] / ry - 1;              // filval.f90:59-62
+        const int ichi = (b[1] + 1) / rx, jchi = (b[3] + 1) / ry;
+        const int S = scratch(mic, mjc);
+        std::vector<unsigned char> cov_c((size_t)mic * mjc, 0);
+        const bool sxc = iclo < 0 || ichi >= C.iregsz, syc = jclo < 0 || jchi >= C.jregsz;
+        if ((H.xperdom && sxc) || (H.yperdom && syc))
+            preicall(levc, S, iclo, ichi, jclo, jchi, cov_c);
+        else
+            icall(levc, S, iclo, ichi, jclo, jchi, 1, 1, cov_c);
+        if (!err.empty()) return false;
+        const double dxc = F.hx * rx, dyc = F.hy * ry;                     // filval.f90:46-47
+        const double xl = (b[0] * F.hx + H.xlower) - dxc, xr = ((b[1] + 1) * F.hx + H.xlower) + dxc;
+        const double yb = (b[2] * F.hy + H.ylower) - dyc, yt = ((b[3] + 1) * F.hy + H.ylower) + dyc;
+        if (!bc_op(S, dxc, dyc, xl, xr, yb, yt, 1)) return false;
+        const int D = pbuf(level, k);
+        const int mitot = nx + 2 * ng, mjtot = ny + 2 * ng;
+        std::vector<unsigned char> cov((size_t)mitot * mjtot, 0);
+        const bool sxf = b[0] < 0 || b[1] >= F.iregsz, syf = b[2] < 0 || b[3] >= F.jregsz;
+        if ((H.xperdom && sxf) || (H.yperdom && syf))
+            preicall(FILL_OLDSLOT, D, b[0] - ng, b[1] + ng, b[2] - ng, b[3] + ng, cov);
+        else
+            icall(FILL_OLDSLOT, D, b[0] - ng, b[1] + ng, b[2] - ng, b[3] + ng, 1, 1, cov);
+        if (!err.empty()) return false;
+        FillOp op{};
+        op.type = FOP_FILVAL;
+        op.dst = D;
+        op.src = S;
+        op.ni = mic;
+        op.nj = mjc;
+        op.ratx = rx;
+        op.raty = ry;
+        op.level = level;
+        op.stage = 2;
+        if (P.masks.size() + (size_t)nx * ny > (size_t)INT32_MAX) return fail("filval: masks too large");
+        op.mask_off = (int32_t)P.masks.size();
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) P.masks.push_back(cov[(size_t)(j + ng) * mitot + (i + ng)]);
+        push(op);
+        return true;
+    }
+
+    void index_slot(int slot) {
+        if ((int)index.size() <= slot) index.resize(slot + 1);
+        index[slot].build(H.lev[slot].boxes, H.lev[slot].iregsz, H.lev[slot].jregsz);
+    }
 };
 
 }  // namespace
@@ -584,6 +682,23 @@ static void finish_plan(FillPlan& P) {
     P.stage_start.assign(P.nstages + 1, 0);
     for (const auto& op : P.ops) P.stage_start[op.stage + 1]++;
     for (int s = 0; s < P.nstages; s++) P.stage_start[s + 1] += P.stage_start[s];
+}
+
+bool plan_filval(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err) {
+    P = FillPlan();
+    if (level < 2 || level > H.nlev || level >= FILL_OLDSLOT) {
+        err = "filval: level out of range";
+        return false;
+    }
+    Planner pl(H, P, nvar);
+    pl.index_slot(FILL_OLDSLOT);
+    for (int k = 0; k < (int)H.lev[level].boxes.size(); k++)
+        if (!pl.filval(level, k)) {
+            err = pl.err.empty() ? "filval planning failed" : pl.err;
+            return false;
+        }
+    finish_plan(P);
+    return true;
 }
 
 bool plan_saveqc(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err) {

@@ -69,6 +69,9 @@ struct RichPlan {
     int64_t arena_elems = 0;
 };
 bool plan_richardson(const HierGeom& H, int level, int nvar, RichPlan& R, std::string& err);
+// filval.f90 (gfixup.f:60-98) for the new grids of `level` (H.lev[level]),
+// the old grids of the level described in H.lev[FILL_OLDSLOT]
+bool plan_filval(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
 // saveqc.f for the grids of fine level `level` (coarse level `level`-1)
 bool plan_saveqc(const HierGeom& H, int level, int nvar, FillPlan& P, std::string& err);
 

@@ -102,12 +102,14 @@ def parse():
                     help="skip the FP64 comparison runs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="CPU baseline budget (0 disables it)")
-    ap.add_argument("--workload", choices=["euler2d", "euler3d", "amr3d", "C1", "C2", "C3"],
+    ap.add_argument("--workload", choices=["euler2d", "euler3d", "amr3d", "C1", "C2", "C3",
+                                           "regrid"],
                     default="euler2d",
                     help="euler3d: the batched 3-D Euler step (512 grids); amr3d: coarse "
                          "Berger-Oliger steps of the 3-D radial run configuration (three "
                          "levels, the whole 3-D level update); C1, C2, C3: the 2-D run "
-                         "configurations to their end times; default: the headline 2-D "
+                         "configurations to their end times; regrid: the device regrid "
+                         "(filval) of the headline level; default: the headline 2-D "
                          "level update")
     ap.add_argument("--plane-block", type=int, default=0,
                     help="3-D: planes per workgroup of a sweep (0: library default)")
@@ -559,6 +561,121 @@ def cpu_baseline_amr3(H, cells, budget):
                       "updates each) in %.1f s" % (n, cells, wall)}
 
 
+def regrid_layout(H, shift=2):
+    """A new level-2 layout for the regrid workload: every level-2 box
+    shifted by `shift` fine cells in x, kept where it still nests in one
+    level-1 grid (its coarse patch grown by one coarse cell inside that
+    grid), so the new grids take part of their values from the old grids
+    and the rest from the coarse level."""
+    C, F = H.levels[0], H.levels[1]
+    cb = np.asarray(C["boxes"])
+    out = []
+    for b in np.asarray(F["boxes"]):
+        n = (b[0] + shift, b[1] + shift, b[2], b[3])
+        if n[1] >= F["iregsz"]:
+            continue
+        ic0, ic1, jc0, jc1 = n[0] // 2 - 1, (n[1] + 1) // 2, n[2] // 2 - 1, (n[3] + 1) // 2
+        inside = ((cb[:, 0] <= max(ic0, 0)) & (min(ic1, C["iregsz"] - 1) <= cb[:, 1]) &
+                  (cb[:, 2] <= max(jc0, 0)) & (min(jc1, C["jregsz"] - 1) <= cb[:, 3]))
+        if inside.any():
+            out.append(n)
+    return np.asarray(out, dtype=np.int32)
+
+
+def bench_regrid(a):
+    """The regrid front end at the headline's patch count: level 2 of the
+    batched hierarchy regridded on the device (amrgpu_regrid_level: filval
+    for every new grid, old grids copied, coarse interpolation elsewhere,
+    planning included) back and forth between its layout and a shifted
+    one; the neighbour lists (makeBndryList) of the new layout through the
+    library's bucket index against the oracle's all-pairs restatement; CPU
+    baseline: the reference's own filval.f90 on a sample of the new grids."""
+    import torch
+    from amrclaw_amd import capi, workload
+    print("bench: building the regrid workload", file=sys.stderr, flush=True)
+    H, _ = workload.batched_euler_hierarchy(a.npatch)
+    old = np.asarray(H.levels[1]["boxes"], dtype=np.int32)
+    new = regrid_layout(H)
+    cells_new = int(((new[:, 1] - new[:, 0] + 1) * (new[:, 3] - new[:, 2] + 1)).sum())
+    cells_old = int(((old[:, 1] - old[:, 0] + 1) * (old[:, 3] - old[:, 2] + 1)).sum())
+    cfg = capi.make_config(capi.EULER, order=2, transverse=2, limiter=4, cfl=0.9, cflv1=1.0,
+                           gamma=1.4, efix=True, precision=capi.FP64)
+    capi.init(cfg)
+    capi.load_hierarchy(H)
+    t2 = H.levels[1]["time"]
+
+    def sync():
+        capi.synchronize()
+        torch.cuda.synchronize()
+
+    layouts = [new, old]
+    for w in range(max(a.warmup, 1) * 2):
+        capi.regrid_level(2, layouts[w % 2], t2)
+    sync()
+    steps = a.steps * 2
+    t0 = time.perf_counter()
+    for s in range(steps):
+        capi.regrid_level(2, layouts[s % 2], t2)
+    sync()
+    wall = time.perf_counter() - t0
+    capi.finalize()
+    cells_avg = (cells_new + cells_old) / 2
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        capi.bndry_lists(new, H.levels[1]["iregsz"], H.levels[1]["jregsz"])
+    t_lists = (time.perf_counter() - t0) / reps
+    result = {"metric": "new-grid cells filled per second (device regrid: filval)",
+              "value": cells_avg * steps / wall, "unit": "cell-fills/s", "n_gpus": 1,
+              "steps": steps, "warmup": a.warmup, "ms_per_step": wall / steps * 1e3,
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+              "data": "synthetic",
+              "config": {"workload": "regrid of the batched 2-D Euler level 2 (%d grids) "
+                                     "to a layout shifted by 2 cells (%d grids) and back" %
+                                     (len(old), len(new)),
+                         "cells_new": cells_new, "cells_old": cells_old},
+              "neighbour_lists_ms": {"bucket_index": t_lists * 1e3}}
+    try:
+        from oracle import orc
+        p = orc.make_params(orc.EULER)
+        h1 = orc.hier_struct(H)
+        L = orc.lib()
+        import ctypes as C
+        L.orc_bndry_list.argtypes = [C.POINTER(orc.OrcParams), C.POINTER(orc.OrcHier), C.c_int,
+                                     C.c_int, C.POINTER(C.c_int)]
+        out = (C.c_int * len(old))()
+        t0 = time.perf_counter()
+        for k in range(len(old)):
+            L.orc_bndry_list(C.byref(p), C.byref(h1), 2, k, out)
+        result["neighbour_lists_ms"]["all_pairs_oracle"] = (time.perf_counter() - t0) * 1e3
+        if a.cpu_seconds > 0 and orc.ref_available():
+            print("bench: CPU baseline (reference filval)", file=sys.stderr, flush=True)
+
+            def ref_sample():
+                R = orc.ref_load_hier(p, H)
+                R.ref_filval.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_double, C.c_int,
+                                         C.c_void_p]
+                done, ncell = 0, 0
+                t0 = time.perf_counter()
+                while time.perf_counter() - t0 < a.cpu_seconds and done < len(new):
+                    b = new[done]
+                    val = np.zeros((b[3] - b[2] + 5, b[1] - b[0] + 5, 4))
+                    R.ref_filval(2, (C.c_int * 4)(*[int(x) for x in b]), t2, 4,
+                                 val.ctypes.data_as(C.c_void_p))
+                    done += 1
+                    ncell += (b[1] - b[0] + 1) * (b[3] - b[2] + 1)
+                el = time.perf_counter() - t0
+                return {"value": ncell / el, "unit": "cell-fills/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": "the reference's filval.f90 (intcopy, bc2amr, min-mod "
+                                  "interpolation) on the first %d of the %d new grids "
+                                  "(%d cells) in %.1f s" % (done, len(new), ncell, el)}
+            result["cpu_baseline"] = _quiet_stdout(ref_sample)
+    except OSError:
+        pass
+    print(json.dumps(result))
+
+
 def bench_config(a):
     """A 2-D run configuration (amrclaw_amd.workload.config_hierarchy: C1
     advection, periodic, 50^2 + two levels at ratio 2; C2 acoustics, 50^2 +
@@ -684,6 +801,9 @@ def main():
         return
     if a.workload in ("C1", "C2", "C3"):
         bench_config(a)
+        return
+    if a.workload == "regrid":
+        bench_regrid(a)
         return
     import torch
     from amrclaw_amd import capi, workload

@@ -264,6 +264,33 @@ int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes,
 int amrgpu_plan_fill_get(amrgpu_fill_op* ops, amrgpu_fill_buf* bufs,
                          unsigned char* masks);
 
+/* Regrid of one level on the device (gfixup.f:33-98 with filval.f90:3-188,
+ * naux = 0): the grids of `level` are replaced by the ngrids new boxes, and
+ * the new grids are filled as filval fills them: the old grids' new-time
+ * values copied over each new grid and its ghost ring (intcopy.f /
+ * preintcopy.f), and the min-mod interpolation of the coarse patch grown
+ * by one coarse cell (from level-1's current grids, with bc2amr) into the
+ * interior cells no old grid covers.  Level-1 must hold the (already
+ * regridded) coarse grids the new ones nest in, as regrid builds levels
+ * coarse to fine.  The level's time becomes `time` (rnode(timemult) of the
+ * new grids); ghost cells no old grid covers are left for the next bound.
+ * The old grids are released; registers and fill programs are planned anew
+ * at the next step, as after amrgpu_set_level. */
+int amrgpu_regrid_level(int level, int ngrids, const int* ilo, const int* ihi, const int* jlo,
+                        const int* jhi, double time);
+
+/* The neighbour lists of makeBndryList (init_bndryList.f:50-107; host
+ * only, needs no device and no init): for each of the n grids of one level
+ * (boxes as (ilo, ihi, jlo, jhi)), the other grids whose box meets its box
+ * grown by two ghost cells, in the order the reference's linked list holds
+ * them (descending grid index), found through a bucket index over the
+ * level's index space (iregsz x jregsz) instead of the reference's
+ * all-pairs scan.  counts[k] receives grid k's count and items (room for
+ * cap entries) the lists one after another; *total is the full number of
+ * entries, and AMRGPU_ERR_ARG is returned when it exceeds cap. */
+int amrgpu_bndry_lists(int n, const int* boxes, int iregsz, int jregsz, int* counts, int* items,
+                       int64_t cap, int64_t* total);
+
 /* Register-list inspection (host only; needs no device and no init): the
  * listbc entries (prepc.f:47-154 with setuse.f:61-213) of grid `grid` of
  * coarse level `level` for the hierarchy described as in amrgpu_plan_fill,

@@ -157,6 +157,12 @@ int orc_prep_regs(const orc_params* p, const orc_hier* h, orc_regs* r);
 void orc_free_regs(orc_regs* r);
 /* saveqc.f:13-109 for the grids of fine level `level` */
 int orc_saveqc(const orc_params* p, const orc_hier* h, orc_regs* r, int level);
+/* filval.f90:3-187 (naux = 0) for a new grid of `level` with box (ilo, ihi,
+ * jlo, jhi); the grids of `level` in h are the old ones */
+int orc_filval(const orc_params* p, const orc_hier* h, int level, const int* box, double time,
+               double* val);
+/* init_bndryList.f:50-107: grid k's neighbour list in its linked-list order */
+int orc_bndry_list(const orc_params* p, const orc_hier* h, int level, int k, int* out);
 /* qad.f:9-360 for grid `grid` of fine level `level` (uses its new array) */
 int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid);
 /* advanc.f:10-142 with par_advanc :170-292 (no gauges): bound every grid,

@@ -378,6 +378,58 @@ def ref_bound(p, H, level, grid, time):
     return ref_get_grid(H, level, grid)
 
 
+def filval(p, H, level, box, time, val):
+    """orc_filval: the new grid with box (ilo, ihi, jlo, jhi) of `level`
+    (H's grids of `level` are the old ones), into val (my+4, mx+4, nvar),
+    updated in place and returned."""
+    L = lib()
+    L.orc_filval.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcHier), C.c_int,
+                             C.POINTER(C.c_int), C.c_double, C.c_void_p]
+    h = hier_struct(H)
+    b = (C.c_int * 4)(*[int(x) for x in box])
+    assert val.flags.c_contiguous and val.dtype == np.float64
+    rc = L.orc_filval(C.byref(p), C.byref(h), level, b, time, val.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError("orc_filval error %d" % rc)
+    return val
+
+
+def ref_filval(p, H, level, box, time, val):
+    """The reference's filval.f90 on a freshly installed copy of H."""
+    R = ref_load_hier(p, H)
+    R.ref_filval.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_double, C.c_int, C.c_void_p]
+    b = (C.c_int * 4)(*[int(x) for x in box])
+    assert val.flags.c_contiguous and val.dtype == np.float64
+    R.ref_filval(level, b, time, H.nvar, val.ctypes.data_as(C.c_void_p))
+    return val
+
+
+def bndry_list(p, H, level, k):
+    """orc_bndry_list: grid k's neighbours (0-based) in makeBndryList's order."""
+    L = lib()
+    L.orc_bndry_list.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcHier), C.c_int, C.c_int,
+                                 C.POINTER(C.c_int)]
+    h = hier_struct(H)
+    out = (C.c_int * max(len(H.levels[level - 1]["boxes"]), 1))()
+    n = L.orc_bndry_list(C.byref(p), C.byref(h), level, k, out)
+    return list(out)[:n]
+
+
+def ref_bndry_lists(p, H, level):
+    """Every grid's neighbour list as the reference's makeBndryList built it
+    (on a freshly installed copy of H)."""
+    R = ref_load_hier(p, H)
+    R.ref_bndry_list.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int)]
+    R.ref_bndry_list.restype = C.c_int
+    n = len(H.levels[level - 1]["boxes"])
+    lists = []
+    for k in range(n):
+        out = (C.c_int * max(n, 1))()
+        c = R.ref_bndry_list(level, k, out)
+        lists.append(list(out)[:c])
+    return lists
+
+
 # ------------------------------------------------------------ registers ---
 class OrcRegs(C.Structure):
     _fields_ = [

@@ -678,6 +678,110 @@ int orc_saveqc(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
     return 0;
 }
 
+/* filval.f90:3-187 (naux = 0, mcapa = 0) with intcopy.f and preintcopy.f:
+ * the values of a new grid of `level` whose box is (ilo, ihi, jlo, jhi).
+ * The grids of `level` in h are the OLD ones (gfixup.f:60-98 fills the new
+ * grids before it frees them); level-1 holds the coarse grids the new grid
+ * nests in.  val is the new grid's AoS array (nvar, nx + 2ng, ny + 2ng).
+ * Component 1 of every cell is first NEEDS_TO_BE_SET (filval.f90:140),
+ * then the old grids are copied over the grid and its ghost ring, and the
+ * interior cells still flagged get the min-mod interpolation of the coarse
+ * patch grown by one coarse cell (intcopy of store1, bc2amr).  Cells of the
+ * ghost ring no old grid covers keep the flag in component 1 and what val
+ * held before in the others. */
+int orc_filval(const orc_params* p, const orc_hier* h, int level, const int* box, double time,
+               double* val) {
+    (void)time;                                                   /* bc2amr's user BCs only */
+    const int nv = p->meqn, ng = p->mbc;
+    const double needs = (double)10.e33f;                         /* amr_module.f90:170 */
+    const int levc = level - 1;
+    const int rx = h->intratx[levc], ry = h->intraty[levc];       /* :44-45 */
+    const double dx = h->hx[level], dy = h->hy[level];
+    const double dxc = dx * rx, dyc = dy * ry;                    /* :46-47 */
+    const int ilo = box[0], ihi = box[1], jlo = box[2], jhi = box[3];
+    const int nx = ihi - ilo + 1, ny = jhi - jlo + 1;
+    const int mitot = nx + 2 * ng, mjtot = ny + 2 * ng;
+    const int mic = nx / rx + 2, mjc = ny / ry + 2;               /* gfixup.f:83-84 */
+    const double xleft = ilo * dx + h->xlower, xright = (ihi + 1) * dx + h->xlower; /* grdfit2.f:98-101 */
+    const double ybot = jlo * dy + h->ylower, ytop = (jhi + 1) * dy + h->ylower;
+    const double xl = xleft - dxc, xr = xright + dxc, yb = ybot - dyc, yt = ytop + dyc;  /* :48-51 */
+    const int iclo = ilo / rx - 1, jclo = jlo / ry - 1;           /* :59-62 */
+    const int ichi = (ihi + 1) / rx - 1 + 1, jchi = (jhi + 1) / ry - 1 + 1;
+    const int sxf = ilo < 0 || ihi >= h->iregsz[level], syf = jlo < 0 || jhi >= h->jregsz[level];
+    const int sxc = iclo < 0 || ichi >= h->iregsz[levc], syc = jclo < 0 || jchi >= h->jregsz[levc];
+    if (h->spheredom) return ORC_E_SPHERE;
+    double* valc = malloc((size_t)nv * mic * mjc * sizeof(double));
+    if (!valc) return ORC_E_ALLOC;
+    for (size_t t = 0; t < (size_t)nv * mic * mjc; t++) valc[t] = NAN;   /* unset local */
+    if ((h->xperdom && sxc) || (h->yperdom && syc))               /* :69-74 */
+        preicall(p, h, valc, mic, iclo, ichi, jclo, jchi, levc);
+    else
+        icall(p, h, valc, mic, iclo, ichi, jclo, jchi, levc, 1, 1);
+    int rc = orc_bc2amr(p, h, valc, mic, mjc, dxc, dyc, xl, xr, yb, yt);   /* :98-99 */
+    if (rc) { free(valc); return rc; }
+    for (int j = 1; j <= mjtot; j++)                              /* :140 */
+        for (int i = 1; i <= mitot; i++) VAL(val, nv, mitot, 1, i, j) = needs;
+    if ((h->xperdom && sxf) || (h->yperdom && syf))               /* :141-147 */
+        preicall(p, h, val, mitot, ilo - ng, ihi + ng, jlo - ng, jhi + ng, level);
+    else
+        icall(p, h, val, mitot, ilo - ng, ihi + ng, jlo - ng, jhi + ng, level, 1, 1);
+    unsigned char* unset = malloc((size_t)mitot * mjtot);         /* setflags, :148 */
+    if (!unset) { free(valc); return ORC_E_ALLOC; }
+    for (int j = 1; j <= mjtot; j++)
+        for (int i = 1; i <= mitot; i++)
+            unset[(size_t)(j - 1) * mitot + (i - 1)] = VAL(val, nv, mitot, 1, i, j) == needs;
+    for (int j = 2; j <= mjc - 1; j++)                            /* :154-188 */
+        for (int i = 2; i <= mic - 1; i++)
+            for (int m = 1; m <= nv; m++) {
+                const double vc = VAL(valc, nv, mic, m, i, j);
+                double s1p = VAL(valc, nv, mic, m, i + 1, j) - vc;
+                double s1m = vc - VAL(valc, nv, mic, m, i - 1, j);
+                double slopex = fmin(fabs(s1p), fabs(s1m)) *
+                                copysign(1.0, VAL(valc, nv, mic, m, i + 1, j) - VAL(valc, nv, mic, m, i - 1, j));
+                if (s1m * s1p <= 0.0) slopex = 0.0;
+                s1p = VAL(valc, nv, mic, m, i, j + 1) - vc;
+                s1m = vc - VAL(valc, nv, mic, m, i, j - 1);
+                double slopey = fmin(fabs(s1p), fabs(s1m)) *
+                                copysign(1.0, VAL(valc, nv, mic, m, i, j + 1) - VAL(valc, nv, mic, m, i, j - 1));
+                if (s1m * s1p <= 0.0) slopey = 0.0;
+                for (int jco = 1; jco <= ry; jco++) {
+                    const double yoff = ((double)jco - 0.5) / ry - 0.5;
+                    const int jfine = (j - 2) * ry + ng + jco;
+                    for (int ico = 1; ico <= rx; ico++) {
+                        const double xoff = ((double)ico - 0.5) / rx - 0.5;
+                        const int ifine = (i - 2) * rx + ng + ico;
+                        if (unset[(size_t)(jfine - 1) * mitot + (ifine - 1)])
+                            VAL(val, nv, mitot, m, ifine, jfine) = vc + xoff * slopex + yoff * slopey;
+                    }
+                }
+            }
+    free(unset);
+    free(valc);
+    return 0;
+}
+
+/* init_bndryList.f:50-107 (makeBndryList) for grid k of `level`: the other
+ * grids of the level whose box meets grid k's box grown by nghost, in the
+ * order its linked list holds them (each found grid is inserted at the
+ * front, so the last one found comes first).  Writes the 0-based grid
+ * indices into out (room for ngrids - 1) and returns their count. */
+int orc_bndry_list(const orc_params* p, const orc_hier* h, int level, int k, int* out) {
+    const int ng = p->mbc;
+    const int* b = h->boxes[level] + 4 * k;
+    const int ilo = b[0] - ng, ihi = b[1] + ng, jlo = b[2] - ng, jhi = b[3] + ng;
+    int n = 0;
+    for (int s = 0; s < h->ngrids[level]; s++) {
+        if (s == k) continue;
+        const int* c = h->boxes[level] + 4 * s;
+        if (imax(c[0], ilo) <= imin(c[1], ihi) && imax(c[2], jlo) <= imin(c[3], jhi)) {
+            for (int t = n; t > 0; t--) out[t] = out[t - 1];
+            out[0] = s;
+            n++;
+        }
+    }
+    return n;
+}
+
 /* qad.f:9-360 (no aux, no source terms) */
 int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid) {
     const int nv = p->meqn, ng = p->mbc, mw = p->mwaves;

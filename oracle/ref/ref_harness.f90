@@ -494,6 +494,50 @@ contains
     end do
   end subroutine ref_flag_level
 
+  ! filval.f90:3 for a new grid of `level` with box (ilo, ihi, jlo, jhi):
+  ! the installed grids of `level` are the old ones, those of level-1 the
+  ! coarse ones; corners as grdfit2.f:98-101 sets them, mic / mjc as
+  ! gfixup.f:83-84.  val (nvar, mitot, mjtot) is updated in place.
+  subroutine ref_filval(level, box, t, nvar, val) bind(C, name="ref_filval")
+    integer(c_int), value :: level, nvar
+    integer(c_int), intent(in) :: box(4)
+    real(c_double), value :: t
+    real(c_double) :: val(*)
+    integer :: ilo, ihi, jlo, jhi, nx, ny, mitot, mjtot, mic, mjc
+    real(kind=8) :: xl, xr, yb, yt, aux(1)
+    ilo = box(1)
+    ihi = box(2)
+    jlo = box(3)
+    jhi = box(4)
+    nx = ihi - ilo + 1
+    ny = jhi - jlo + 1
+    mitot = nx + 2*nghost
+    mjtot = ny + 2*nghost
+    mic = nx/intratx(level-1) + 2
+    mjc = ny/intraty(level-1) + 2
+    xl = ilo*hxposs(level) + xlower
+    xr = (ihi+1)*hxposs(level) + xlower
+    yb = jlo*hyposs(level) + ylower
+    yt = (jhi+1)*hyposs(level) + ylower
+    call filval(val, mitot, mjtot, hxposs(level), hyposs(level), level, t, mic, mjc, &
+         xl, xr, yb, yt, nvar, 0, ilo, ihi, jlo, jhi, aux, 0)
+  end subroutine ref_filval
+
+  ! the neighbour list makeBndryList built for grid k of `level`
+  ! (init_bndryList.f:50-107), in its linked-list order, as 0-based indices
+  integer(c_int) function ref_bndry_list(level, k, out) bind(C, name="ref_bndry_list")
+    integer(c_int), value :: level, k
+    integer(c_int) :: out(*)
+    integer :: mptr, i, spot
+    mptr = listOfGrids(listStart(level) + k)
+    spot = node(bndListSt, mptr)
+    do i = 1, node(bndListNum, mptr)
+       out(i) = bndList(spot, gridNbor) - listStart(level)
+       spot = bndList(spot, nextfree)
+    end do
+    ref_bndry_list = node(bndListNum, mptr)
+  end function ref_bndry_list
+
 end module ref_harness
 
 ! listbc lives in the double heap as integers (prepc.f:55-59, setuse.f:95);

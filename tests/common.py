@@ -154,6 +154,28 @@ def nested_hierarchy(rng, solver_states, nvar, nx=24, ny=20, nlev=3, ratio=2, mt
     return H
 
 
+def nested_boxes(rng, H, level, nbox, lo=4, hi=None):
+    """Random disjoint boxes for `level` of H, properly nested in H's level
+    level-1 the way regrid makes new grids (each, coarsened and grown by
+    one coarse cell, inside one level-1 grid or outside the domain), as a
+    new layout for a regrid of `level`."""
+    C = H.levels[level - 2]
+    F = H.levels[level - 1]
+    rx, ry = C["ratx"], C["raty"]
+    isz, jsz = C["iregsz"], C["jregsz"]
+    full = level - 1 == 1
+    xe, ye = full or not H.xper, full or not H.yper
+    within = []
+    for (a, b, c, d) in C["boxes"]:
+        within.append((0 if (a == 0 and xe) else (a + 1) * rx,
+                       isz * rx - 1 if (b == isz - 1 and xe) else b * rx - 1,
+                       0 if (c == 0 and ye) else (c + 1) * ry,
+                       jsz * ry - 1 if (d == jsz - 1 and ye) else d * ry - 1))
+    return disjoint_boxes(rng, F["iregsz"], F["jregsz"], nbox, lo=lo,
+                          hi=hi or max(8, F["iregsz"] // 3), align=max(rx, ry),
+                          within=within, tries=4000)
+
+
 def smooth_euler(rng, shape):
     """Positive, smoothly varying Euler states (for hierarchies)."""
     ny, nx = shape

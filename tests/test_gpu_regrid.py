@@ -14,8 +14,8 @@ import pytest
 from amrclaw_amd import capi
 from amrclaw_amd.hierarchy import Hier
 from oracle import orc
-from tests.common import (berger_oliger, both_params, disjoint_boxes, nested_hierarchy,
-                          smooth_euler)
+from tests.common import (berger_oliger, both_params, disjoint_boxes, nested_boxes,
+                          nested_hierarchy, smooth_euler)
 
 pytestmark = pytest.mark.gpu
 
@@ -146,5 +146,106 @@ def test_refused_tiling_leaves_level_intact(gpu):
         assert lo == ld
         for l in range(1, H.nlev + 1):
             assert np.array_equal(capi.read_level(H, l, 0), O.H.levels[l - 1]["qnew"]), l
+    finally:
+        capi.finalize()
+
+
+def _filval_level(p, Hc, level, boxes, t):
+    """The oracle's filval for new boxes of `level` of Hc (level-1 already
+    in its new layout, `level` still the old one): the new grids' arrays."""
+    out = []
+    for b in boxes:
+        val = np.zeros((b[3] - b[2] + 5, b[1] - b[0] + 5, Hc.nvar))
+        out.append(orc.filval(p, Hc, level, b, t, val))
+    return out
+
+
+@pytest.mark.parametrize("bc", [((1, 1, 1, 1), False, False), ((2, 2, 2, 2), True, True),
+                                ((2, 2, 3, 3), True, False)])
+@pytest.mark.parametrize("first_new", [2, 3])
+def test_device_regrid_matches_filval(gpu, bc, first_new):
+    """amrgpu_regrid_level replaces a level's grids and fills them on the
+    device as gfixup/filval do (old grids copied over the new grid and its
+    ghost ring, min-mod interpolation from the coarse level elsewhere):
+    levels first_new.. regridded coarse to fine after a coarse step, the
+    new grids' interiors and every ghost cell an old grid covered bit-exact
+    against the oracle's filval (pinned to the reference's filval.f90), and
+    two more coarse steps on the new layout bit-exact against the oracle."""
+    print("device regrid", bc, first_new, flush=True)
+    mthbc, xp, yp = bc
+    rng = np.random.default_rng(700 + first_new + 10 * xp + 20 * yp)
+    H = nested_hierarchy(rng, smooth_euler, 4, mthbc=mthbc, xper=xp, yper=yp, dt1=0.006)
+    p, _ = both_params(orc.EULER, precision=capi.FP64)
+    cfg = capi.make_config(capi.EULER, precision=capi.FP64, mthbc=mthbc)
+    O = orc.OracleAMR(p, H.copy())
+    capi.init(cfg)
+    try:
+        capi.load_hierarchy(H)
+        D = DeviceAMR()
+        lo, ld = [], []
+        berger_oliger(O, 1, H.nlev, 2, lo)
+        berger_oliger(D, 1, H.nlev, 2, ld)
+        assert lo == ld
+        Hc = O.H.copy()
+        needs = float(np.float32(10.e33))
+        for l in range(first_new, H.nlev + 1):
+            t = Hc.levels[l - 2]["time"]
+            boxes = []
+            for _ in range(20):                # small periodic levels leave little room
+                boxes = boxes or nested_boxes(rng, Hc, l, 4)
+            assert boxes, "no nested boxes drawn"
+            vals = _filval_level(p, Hc, l, boxes, t)
+            capi.regrid_level(l, boxes, t)
+            # Hc's level l becomes the new layout with the filval values
+            L = Hc.levels[l - 1]
+            H2 = Hier(Hc.nvar, domain=Hc.domain, mthbc=Hc.mthbc, xper=Hc.xper, yper=Hc.yper)
+            for m, M in enumerate(Hc.levels, start=1):
+                if m == l:
+                    H2.add_level(boxes, M["hx"], M["hy"], M["possk"], t, M["iregsz"],
+                                 M["jregsz"], M["ratx"], M["raty"])
+                    for k, v in enumerate(vals):
+                        H2.grid(l, k, 0)[:] = v
+                        H2.grid(l, k, 1)[:] = v
+                else:
+                    H2.add_level(M["boxes"], M["hx"], M["hy"], M["possk"], M["time"],
+                                 M["iregsz"], M["jregsz"], M["ratx"], M["raty"])
+                    H2.levels[-1]["qnew"][:] = M["qnew"]
+                    H2.levels[-1]["qold"][:] = M["qold"]
+            got = capi.read_level(H2, l, 0)
+            for k, v in enumerate(vals):
+                shape = v.shape
+                g = got[H2.levels[l - 1]["off"][k]:H2.levels[l - 1]["off"][k] + v.size].reshape(shape)
+                assert np.array_equal(g[2:-2, 2:-2], v[2:-2, 2:-2]), ("interior", l, k)
+                covered = v[..., 0] != needs
+                assert np.array_equal(g[covered], v[covered]), ("ghosts", l, k)
+            Hc = H2
+        O2 = orc.OracleAMR(p, Hc.copy())
+        lo, ld = [], []
+        for _ in range(2):
+            berger_oliger(O2, 1, Hc.nlev, 2, lo)
+            berger_oliger(D, 1, Hc.nlev, 2, ld)
+        assert lo == ld, "dtlevnew / cfl after the device regrid"
+        for l in range(1, Hc.nlev + 1):
+            assert np.array_equal(capi.read_level(Hc, l, 0), O2.H.levels[l - 1]["qnew"]), l
+    finally:
+        capi.finalize()
+
+
+def test_device_regrid_refusals(gpu):
+    """Level 1, a level above mxnest and a coarse level without grids are
+    refused before anything changes."""
+    print("device regrid refusals", flush=True)
+    rng = np.random.default_rng(3)
+    H = nested_hierarchy(rng, smooth_euler, 4, mthbc=(1, 1, 1, 1), dt1=0.006, nlev=2)
+    p, cfg = both_params(orc.EULER, precision=capi.FP64)
+    capi.init(cfg)
+    try:
+        capi.load_hierarchy(H)
+        with pytest.raises(capi.AmrgpuError):
+            capi.regrid_level(1, [(0, 3, 0, 3)], 0.0)
+        with pytest.raises(capi.AmrgpuError):
+            capi.regrid_level(5, [(0, 3, 0, 3)], 0.0)
+        before = capi.read_level(H, 2, 0)
+        assert np.array_equal(before, H.levels[1]["qnew"])
     finally:
         capi.finalize()
