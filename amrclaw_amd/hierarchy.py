@@ -16,8 +16,9 @@ class Hier:
     (my+4, mx+4, nvar) block per grid, like the reference's alloc)."""
 
     def __init__(self, nvar, domain=(0.0, 1.0, 0.0, 1.0), mthbc=(1, 1, 1, 1),
-                 xper=False, yper=False, ng=2):
+                 xper=False, yper=False, ng=2, naux=0):
         self.nvar, self.ng = nvar, ng
+        self.naux = naux
         self.domain = domain
         self.mthbc = tuple(mthbc)
         self.xper, self.yper = xper, yper
@@ -34,6 +35,13 @@ class Hier:
                                 qnew=np.zeros(tot), qold=np.zeros(tot), hx=hx, hy=hy,
                                 possk=possk, time=time, iregsz=iregsz, jregsz=jregsz,
                                 ratx=ratx, raty=raty))
+        if self.naux > 0:
+            # aux arrays (alloc(node(storeaux))): naux values per cell, the
+            # same cells as q
+            asz = (sizes // self.nvar) * self.naux
+            L = self.levels[-1]
+            L["auxoff"] = np.concatenate([[0], np.cumsum(asz)[:-1]]).astype(np.int64)
+            L["aux"] = np.zeros(max(int(asz.sum()), 1))
 
     @property
     def nlev(self):
@@ -45,6 +53,13 @@ class Hier:
         a = L["qnew"] if which == 0 else L["qold"]
         n = int(np.prod(L["shapes"][k]))
         return a[L["off"][k]:L["off"][k] + n].reshape(L["shapes"][k])
+
+    def aux(self, l, k):
+        """View of grid k's aux array on level l as (my+4, mx+4, naux)."""
+        L = self.levels[l - 1]
+        sh = L["shapes"][k][:2] + (self.naux,)
+        n = int(np.prod(sh))
+        return L["aux"][L["auxoff"][k]:L["auxoff"][k] + n].reshape(sh)
 
     def copy(self):
         import copy

@@ -32,7 +32,10 @@ extern "C" {
 enum { ORC_ADVECTION = 1, ORC_ACOUSTICS = 2, ORC_EULER = 3,
        /* a physics the library does not build in: the parity case of its
         * solver plug-in contract */
-       ORC_SHALLOW = 4 };
+       ORC_SHALLOW = 4,
+       /* variable-coefficient advection with edge velocities in aux(1:2)
+        * (the annulus test's rpn2_vc_advection / rpt2_vc_advection) */
+       ORC_VCADV = 5 };
 
 /* Method parameters: the fields of amr_module.f90 that the hot path reads
  * (method(7), mthlim(:), mwaves, mcapa, use_fwaves, cfl, cflv1) plus the
@@ -53,6 +56,10 @@ typedef struct {
     int efix;                       /* Euler entropy fix    */
     double wbar;                    /* 3-D advection, z velocity */
     double grav;                    /* shallow water: gravity    */
+    /* amr_module auxtype(1:maux): 0 "center", 1 "xleft", 2 "yleft",
+     * 3 "capacity" (qad.f:81-91 reads the fine grid's face values for
+     * xleft / yleft) */
+    int auxtype[8];
 } orc_params;
 
 /* ---- Riemann solvers, Fortran calling convention over a 1-D slice -------
@@ -121,6 +128,10 @@ typedef struct {
     double xlower, xupper, ylower, yupper;
     int xperdom, yperdom, spheredom;
     int mthbc[4];
+    /* aux arrays (maux > 0): grid k of level l has (maux, mx+2ng, my+2ng)
+     * at aux[l] + auxoff[l][k] (alloc(node(storeaux))); NULL: none */
+    double* aux[ORC_MAXLV];
+    const int64_t* auxoff[ORC_MAXLV];
 } orc_hier;
 
 /* ghost filling, bound.f90:51-158 (with filrecur, intfil, trimbd,

@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ADVECTION, ACOUSTICS, EULER, SHALLOW = 1, 2, 3, 4
+ADVECTION, ACOUSTICS, EULER, SHALLOW, VCADV = 1, 2, 3, 4, 5
 
 
 class OrcParams(C.Structure):
@@ -24,17 +24,22 @@ class OrcParams(C.Structure):
         ("rho", C.c_double), ("bulk", C.c_double), ("cc", C.c_double),
         ("zz", C.c_double), ("gamma", C.c_double), ("efix", C.c_int),
         ("wbar", C.c_double), ("grav", C.c_double),
+        ("auxtype", C.c_int * 8),
     ]
+
+
+AUX_CENTER, AUX_XLEFT, AUX_YLEFT, AUX_CAPACITY = 0, 1, 2, 3
 
 
 def make_params(solver, *, order=2, transverse=2, limiter=4, mwaves=None,
                 maux=0, mcapa=0, cfl=0.9, cflv1=1.0, ubar=0.5, vbar=1.0,
-                rho=1.0, bulk=4.0, gamma=1.4, efix=True, mbc=2, grav=1.0):
+                rho=1.0, bulk=4.0, gamma=1.4, efix=True, mbc=2, grav=1.0, auxtype=()):
     """Build the params of one configuration.  limiter: mthlim code applied
     to every wave (0 none, 1 minmod, 2 superbee, 3 van Leer, 4 MC, 5 BW)."""
     p = OrcParams()
     p.solver = solver
-    p.meqn, nw = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4), SHALLOW: (3, 3)}[solver]
+    p.meqn, nw = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4), SHALLOW: (3, 3),
+                  VCADV: (1, 1)}[solver]
     p.mwaves = nw if mwaves is None else mwaves
     p.maux, p.mbc = maux, mbc
     for k in range(8):
@@ -51,6 +56,8 @@ def make_params(solver, *, order=2, transverse=2, limiter=4, mwaves=None,
     p.zz = rho * p.cc
     p.gamma, p.efix = gamma, int(bool(efix))
     p.grav = grav
+    for k in range(8):
+        p.auxtype[k] = auxtype[k] if k < len(auxtype) else AUX_CENTER
     return p
 
 
@@ -299,6 +306,8 @@ class OrcHier(C.Structure):
         ("ylower", C.c_double), ("yupper", C.c_double),
         ("xperdom", C.c_int), ("yperdom", C.c_int), ("spheredom", C.c_int),
         ("mthbc", C.c_int * 4),
+        ("aux", C.POINTER(C.c_double) * MAXLV),
+        ("auxoff", C.POINTER(C.c_int64) * MAXLV),
     ]
 
 
@@ -320,6 +329,10 @@ def hier_struct(H):
     h.xperdom, h.yperdom, h.spheredom = int(H.xper), int(H.yper), 0
     for k in range(4):
         h.mthbc[k] = H.mthbc[k]
+    if getattr(H, "naux", 0) > 0:
+        for l, L in enumerate(H.levels, start=1):
+            h.aux[l] = L["aux"].ctypes.data_as(C.POINTER(C.c_double))
+            h.auxoff[l] = L["auxoff"].ctypes.data_as(C.POINTER(C.c_int64))
     return h
 
 
@@ -359,6 +372,15 @@ def ref_load_hier(p, H):
             for which in (0, 1):
                 g = np.ascontiguousarray(H.grid(l, k, which))
                 R.ref_hier_xfer(l, k, which, H.nvar, g.ctypes.data_as(C.c_void_p), 1)
+    naux = getattr(H, "naux", 0)
+    R.ref_hier_set_aux.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    R.ref_hier_set_aux(naux, (C.c_int * 8)(*list(p.auxtype)))
+    if naux > 0:
+        R.ref_hier_put_aux.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        for l in range(1, H.nlev + 1):
+            for k in range(len(lv[l - 1]["boxes"])):
+                a = np.ascontiguousarray(H.aux(l, k))
+                R.ref_hier_put_aux(l, k, a.ctypes.data_as(C.c_void_p))
     return R
 
 

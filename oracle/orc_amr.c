@@ -535,7 +535,7 @@ int orc_prep_regs(const orc_params* p, const orc_hier* h, orc_regs* r) {
             int nx = b[1] - b[0] + 1, ny = b[3] - b[2] + 1;
             r->lenbc[l][k] = 2 * (nx / h->intratx[l - 1] + ny / h->intraty[l - 1]);
             r->roff[l][k] = tot;
-            tot += 2 * (int64_t)nv * r->lenbc[l][k];
+            tot += (2 * (int64_t)nv + p->maux) * r->lenbc[l][k];   /* prepf.f:33-48 */
         }
         r->nreg[l] = tot;
         r->reg[l] = calloc((size_t)tot + 1, sizeof(double));
@@ -584,10 +584,13 @@ int orc_prep_regs(const orc_params* p, const orc_hier* h, orc_regs* r) {
     return 0;
 }
 
-/* icall.f (copy from the level's grids, new time) into val at (iputst,jputst) */
-static void icall(const orc_params* p, const orc_hier* h, double* val, int nrow, int ilo,
-                  int ihi, int jlo, int jhi, int level, int iputst, int jputst) {
-    const int nv = p->meqn, ng = p->mbc;
+/* icall.f (copy from the level's grids, new time) into val at (iputst,jputst),
+ * and their aux values into auxv (maux per cell) when auxv and the level's
+ * aux arrays are given */
+static void icall_aux(const orc_params* p, const orc_hier* h, double* val, double* auxv,
+                      int nrow, int ilo, int ihi, int jlo, int jhi, int level, int iputst,
+                      int jputst) {
+    const int nv = p->meqn, ng = p->mbc, na = p->maux;
     for (int k = 0; k < h->ngrids[level]; k++) {
         const int* b = h->boxes[level] + 4 * k;
         int ixlo = imax(b[0], ilo), ixhi = imin(b[1], ihi);
@@ -600,12 +603,25 @@ static void icall(const orc_params* p, const orc_hier* h, double* val, int nrow,
                 for (int m = 1; m <= nv; m++)
                     VAL(val, nv, nrow, m, i - ilo + iputst, j - jlo + jputst) =
                         VAL(q, nv, mitot, m, i - b[0] + ng + 1, j - b[2] + ng + 1);
+        if (auxv && na > 0 && h->aux[level]) {
+            const double* a = h->aux[level] + h->auxoff[level][k];
+            for (int j = jxlo; j <= jxhi; j++)
+                for (int i = ixlo; i <= ixhi; i++)
+                    for (int m = 1; m <= na; m++)
+                        VAL(auxv, na, nrow, m, i - ilo + iputst, j - jlo + jputst) =
+                            VAL(a, na, mitot, m, i - b[0] + ng + 1, j - b[2] + ng + 1);
+        }
     }
 }
 
+static void icall(const orc_params* p, const orc_hier* h, double* val, int nrow, int ilo,
+                  int ihi, int jlo, int jhi, int level, int iputst, int jputst) {
+    icall_aux(p, h, val, NULL, nrow, ilo, ihi, jlo, jhi, level, iputst, jputst);
+}
+
 /* preicall.f (periodic wrap of icall) */
-static void preicall(const orc_params* p, const orc_hier* h, double* val, int nrow, int ilo,
-                     int ihi, int jlo, int jhi, int level) {
+static void preicall_aux(const orc_params* p, const orc_hier* h, double* val, double* auxv,
+                         int nrow, int ilo, int ihi, int jlo, int jhi, int level) {
     const int ng = p->mbc;
     int ist[3], iend[3], ishift[3], jst[3], jend[3], jshift[3];
     if (h->xperdom) {
@@ -632,10 +648,15 @@ static void preicall(const orc_params* p, const orc_hier* h, double* val, int nr
         for (int b = 0; b < 3; b++) {
             int j1 = imax(jlo, jst[b]), j2 = imin(jhi, jend[b]);
             if (j1 > j2) continue;
-            icall(p, h, val, nrow, i1 + ishift[a], i2 + ishift[a], j1 + jshift[b], j2 + jshift[b],
-                  level, i1 - ilo + 1, j1 - jlo + 1);
+            icall_aux(p, h, val, auxv, nrow, i1 + ishift[a], i2 + ishift[a], j1 + jshift[b],
+                      j2 + jshift[b], level, i1 - ilo + 1, j1 - jlo + 1);
         }
     }
+}
+
+static void preicall(const orc_params* p, const orc_hier* h, double* val, int nrow, int ilo,
+                     int ihi, int jlo, int jhi, int level) {
+    preicall_aux(p, h, val, NULL, nrow, ilo, ihi, jlo, jhi, level);
 }
 
 /* saveqc.f:13-109 and cstore.f:4-63 */
@@ -658,13 +679,26 @@ int orc_saveqc(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
         double* val = malloc((size_t)nrow * ncol * nv * sizeof(double));
         if (!val) return ORC_E_ALLOC;
         for (size_t t = 0; t < (size_t)nrow * ncol * nv; t++) val[t] = NAN;   /* igetsp: unset */
+        /* the aux part of the temporary, NEEDS_TO_BE_SET until copied
+         * (saveqc.f:62-64); where the patch sticks out of a non-periodic
+         * domain the reference then calls the application's setaux
+         * (saveqc.f:88-90), which the oracle does not restate: those ring
+         * cells lie outside the domain and their register slots are never
+         * read back (upbnd only corrects coarse cells of the domain) */
+        const int na = p->maux;
+        double* auxv = NULL;
+        if (na > 0) {
+            auxv = malloc((size_t)nrow * ncol * na * sizeof(double));
+            if (!auxv) { free(val); return ORC_E_ALLOC; }
+            for (size_t t = 0; t < (size_t)nrow * ncol * na; t++) auxv[t] = (double)10.e33f;
+        }
         int sticks = iclo < 0 || ichi == h->iregsz[levc] || jclo < 0 || jchi == h->jregsz[levc];
         if (sticks && (h->xperdom || h->yperdom || h->spheredom))
-            preicall(p, h, val, nrow, iclo, ichi, jclo, jchi, levc);
+            preicall_aux(p, h, val, auxv, nrow, iclo, ichi, jclo, jchi, levc);
         else
-            icall(p, h, val, nrow, iclo, ichi, jclo, jchi, levc, 1, 1);
+            icall_aux(p, h, val, auxv, nrow, iclo, ichi, jclo, jchi, levc, 1, 1);
         int rc = orc_bc2amr(p, h, val, nrow, ncol, hxc, hyc, xl, xr, yb, yt);
-        if (rc) { free(val); return rc; }
+        if (rc) { free(val); free(auxv); return rc; }
         double* qc1d = r->reg[level] + r->roff[level][k] + (size_t)nv * lenbc;
         int index = 0;
 #define QC(m, ix) qc1d[((m) - 1) + (size_t)nv * ((ix) - 1)]
@@ -673,7 +707,18 @@ int orc_saveqc(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
         for (int j = 2; j <= ncol - 1; j++) { index++; for (int m = 1; m <= nv; m++) QC(m, index) = VAL(val, nv, nrow, m, nrow, j); }
         for (int i = 2; i <= nrow - 1; i++) { index++; for (int m = 1; m <= nv; m++) QC(m, index) = VAL(val, nv, nrow, m, i, 1); }
 #undef QC
+        if (na > 0) {                                            /* cstore.f, auxc1d */
+            double* auxc1d = r->reg[level] + r->roff[level][k] + 2 * (size_t)nv * lenbc;
+            int ix = 0;
+#define AC(m, i) auxc1d[((m) - 1) + (size_t)na * ((i) - 1)]
+            for (int j = 2; j <= ncol - 1; j++) { ix++; for (int m = 1; m <= na; m++) AC(m, ix) = VAL(auxv, na, nrow, m, 1, j); }
+            for (int i = 2; i <= nrow - 1; i++) { ix++; for (int m = 1; m <= na; m++) AC(m, ix) = VAL(auxv, na, nrow, m, i, ncol); }
+            for (int j = 2; j <= ncol - 1; j++) { ix++; for (int m = 1; m <= na; m++) AC(m, ix) = VAL(auxv, na, nrow, m, nrow, j); }
+            for (int i = 2; i <= nrow - 1; i++) { ix++; for (int m = 1; m <= na; m++) AC(m, ix) = VAL(auxv, na, nrow, m, i, 1); }
+#undef AC
+        }
         free(val);
+        free(auxv);
     }
     return 0;
 }
@@ -784,7 +829,7 @@ int orc_bndry_list(const orc_params* p, const orc_hier* h, int level, int k, int
 
 /* qad.f:9-360 (no aux, no source terms) */
 int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int grid) {
-    const int nv = p->meqn, ng = p->mbc, mw = p->mwaves;
+    const int nv = p->meqn, ng = p->mbc, mw = p->mwaves, na = p->maux;
     const int* b = h->boxes[level] + 4 * grid;
     const int nr = b[1] - b[0] + 1, nc = b[3] - b[2] + 1;
     const int mitot = nr + 2 * ng, mjtot = nc + 2 * ng;
@@ -793,34 +838,52 @@ int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int 
     const int lenbc = r->lenbc[level][grid];
     double* svdflx = r->reg[level] + r->roff[level][grid];
     const double* qc1d = svdflx + (size_t)nv * lenbc;
+    const double* auxc1d = svdflx + 2 * (size_t)nv * lenbc;
     const double* valbig = h->qnew[level] + h->qoff[level][grid];
+    const double* aux = (na > 0 && h->aux[level]) ? h->aux[level] + h->auxoff[level][grid] : NULL;
     const int npos = (nr > nc ? nr : nc) + 2;          /* positions 1..npos */
+    const int nab = na > 0 ? na : 1;
     double* ql = calloc((size_t)nv * npos, sizeof(double));
     double* qr = calloc((size_t)nv * npos, sizeof(double));
     double* wave = calloc((size_t)nv * mw * npos, sizeof(double));
     double* s = calloc((size_t)mw * npos, sizeof(double));
     double* amdq = calloc((size_t)nv * npos, sizeof(double));
     double* apdq = calloc((size_t)nv * npos, sizeof(double));
-    double aux1[1] = {0};
-    if (!ql || !qr || !wave || !s || !amdq || !apdq) return ORC_E_ALLOC;
+    double* auxl = calloc((size_t)nab * npos, sizeof(double));
+    double* auxr = calloc((size_t)nab * npos, sizeof(double));
+    if (!ql || !qr || !wave || !s || !amdq || !apdq || !auxl || !auxr) return ORC_E_ALLOC;
 #define P1(a, m, pos) (a)[((m) - 1) + (size_t)nv * ((pos) - 1)]
+#define PA(a, m, pos) (a)[((m) - 1) + (size_t)na * ((pos) - 1)]
 #define QB(m, i, j) VAL(valbig, nv, mitot, m, i, j)
+#define AB(m, i, j) VAL(aux, na, mitot, m, i, j)
 #define QC(m, ix) qc1d[((m) - 1) + (size_t)nv * ((ix) - 1)]
+#define AC(m, ix) auxc1d[((m) - 1) + (size_t)na * ((ix) - 1)]
 #define SV(m, ix) svdflx[((m) - 1) + (size_t)nv * ((ix) - 1)]
     /* rpn2 is called with mbc = nghost and mx = n+1-2*nghost, arrays indexed
-     * from position 1 (= Fortran index 1-mbc): interfaces at positions 2..n+1 */
+     * from position 1 (= Fortran index 1-mbc): interfaces at positions 2..n+1;
+     * aux as qad.f:81-91 (side 1), :160-185 (side 2), :228-255 (side 3),
+     * :300-330 (side 4): the fine grid's face value for xleft / yleft, else
+     * the cell's, and the coarse ring's auxc1d on the coarse side */
     int index = 0, influx = 0;
     /* side 1 */
-    for (int j = ng + 1; j <= mjtot - ng; j++)
+    for (int j = ng + 1; j <= mjtot - ng; j++) {
+        if (aux)
+            for (int m = 1; m <= na; m++)
+                PA(auxl, m, j - ng + 1) = p->auxtype[m - 1] == 1 ? AB(m, ng + 1, j) : AB(m, ng, j);
         for (int m = 1; m <= nv; m++) P1(ql, m, j - ng + 1) = QB(m, ng, j);
+    }
     {
         int lind = 0;
         for (int jc = 1; jc <= nc / lry; jc++) {
             index++;
-            for (int l = 1; l <= lry; l++) { lind++; for (int m = 1; m <= nv; m++) P1(qr, m, lind) = QC(m, index); }
+            for (int l = 1; l <= lry; l++) {
+                lind++;
+                if (aux) for (int m = 1; m <= na; m++) PA(auxr, m, lind) = AC(m, index);
+                for (int m = 1; m <= nv; m++) P1(qr, m, lind) = QC(m, index);
+            }
         }
     }
-    orc_rpn2(p, 1, npos - 2 * ng, nv, mw, 0, ng, nc + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+    orc_rpn2(p, 1, npos - 2 * ng, nv, mw, na, ng, nc + 1 - 2 * ng, ql, qr, auxl, auxr, wave, s, amdq, apdq);
     for (int j = 1; j <= nc / lry; j++) {
         influx++;
         int jfine = (j - 1) * lry;
@@ -831,14 +894,23 @@ int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int 
     }
     if (mjtot != 2 * ng + 1) {
         /* side 2 */
-        for (int i = ng + 1; i <= mitot - ng; i++)
+        for (int i = ng + 1; i <= mitot - ng; i++) {
+            if (aux) for (int m = 1; m <= na; m++) PA(auxr, m, i - ng) = AB(m, i, mjtot - ng + 1);
             for (int m = 1; m <= nv; m++) P1(qr, m, i - ng) = QB(m, i, mjtot - ng + 1);
+        }
         int lind = 0;
         for (int ic = 1; ic <= nr / lrx; ic++) {
             index++;
-            for (int l = 1; l <= lrx; l++) { lind++; for (int m = 1; m <= nv; m++) P1(ql, m, lind + 1) = QC(m, index); }
+            for (int l = 1; l <= lrx; l++) {
+                lind++;
+                if (aux)
+                    for (int m = 1; m <= na; m++)
+                        PA(auxl, m, lind + 1) = p->auxtype[m - 1] == 2
+                            ? AB(m, (ic - 1) * lrx + ng + l, mjtot - ng + 1) : AC(m, index);
+                for (int m = 1; m <= nv; m++) P1(ql, m, lind + 1) = QC(m, index);
+            }
         }
-        orc_rpn2(p, 2, npos - 2 * ng, nv, mw, 0, ng, nr + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+        orc_rpn2(p, 2, npos - 2 * ng, nv, mw, na, ng, nr + 1 - 2 * ng, ql, qr, auxl, auxr, wave, s, amdq, apdq);
         for (int i = 1; i <= nr / lrx; i++) {
             influx++;
             int ifine = (i - 1) * lrx;
@@ -849,16 +921,25 @@ int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int 
         }
     }
     /* side 3 */
-    for (int j = ng + 1; j <= mjtot - ng; j++)
+    for (int j = ng + 1; j <= mjtot - ng; j++) {
+        if (aux) for (int m = 1; m <= na; m++) PA(auxr, m, j - ng) = AB(m, mitot - ng + 1, j);
         for (int m = 1; m <= nv; m++) P1(qr, m, j - ng) = QB(m, mitot - ng + 1, j);
+    }
     {
         int lind = 0;
         for (int jc = 1; jc <= nc / lry; jc++) {
             index++;
-            for (int l = 1; l <= lry; l++) { lind++; for (int m = 1; m <= nv; m++) P1(ql, m, lind + 1) = QC(m, index); }
+            for (int l = 1; l <= lry; l++) {
+                lind++;
+                if (aux)
+                    for (int m = 1; m <= na; m++)
+                        PA(auxl, m, lind + 1) = p->auxtype[m - 1] == 1
+                            ? AB(m, mitot - ng + 1, (jc - 1) * lry + ng + l) : AC(m, index);
+                for (int m = 1; m <= nv; m++) P1(ql, m, lind + 1) = QC(m, index);
+            }
         }
     }
-    orc_rpn2(p, 1, npos - 2 * ng, nv, mw, 0, ng, nc + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+    orc_rpn2(p, 1, npos - 2 * ng, nv, mw, na, ng, nc + 1 - 2 * ng, ql, qr, auxl, auxr, wave, s, amdq, apdq);
     for (int j = 1; j <= nc / lry; j++) {
         influx++;
         int jfine = (j - 1) * lry;
@@ -869,14 +950,22 @@ int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int 
     }
     if (mjtot != 2 * ng + 1) {
         /* side 4 */
-        for (int i = ng + 1; i <= mitot - ng; i++)
+        for (int i = ng + 1; i <= mitot - ng; i++) {
+            if (aux)
+                for (int m = 1; m <= na; m++)
+                    PA(auxl, m, i - ng + 1) = p->auxtype[m - 1] == 2 ? AB(m, i, ng + 1) : AB(m, i, ng);
             for (int m = 1; m <= nv; m++) P1(ql, m, i - ng + 1) = QB(m, i, ng);
+        }
         int lind = 0;
         for (int ic = 1; ic <= nr / lrx; ic++) {
             index++;
-            for (int l = 1; l <= lrx; l++) { lind++; for (int m = 1; m <= nv; m++) P1(qr, m, lind) = QC(m, index); }
+            for (int l = 1; l <= lrx; l++) {
+                lind++;
+                if (aux) for (int m = 1; m <= na; m++) PA(auxr, m, lind) = AC(m, index);
+                for (int m = 1; m <= nv; m++) P1(qr, m, lind) = QC(m, index);
+            }
         }
-        orc_rpn2(p, 2, npos - 2 * ng, nv, mw, 0, ng, nr + 1 - 2 * ng, ql, qr, aux1, aux1, wave, s, amdq, apdq);
+        orc_rpn2(p, 2, npos - 2 * ng, nv, mw, na, ng, nr + 1 - 2 * ng, ql, qr, auxl, auxr, wave, s, amdq, apdq);
         for (int i = 1; i <= nr / lrx; i++) {
             influx++;
             int ifine = (i - 1) * lrx;
@@ -887,10 +976,13 @@ int orc_qad(const orc_params* p, const orc_hier* h, orc_regs* r, int level, int 
         }
     }
 #undef P1
+#undef PA
 #undef QB
+#undef AB
 #undef QC
+#undef AC
 #undef SV
-    free(ql); free(qr); free(wave); free(s); free(amdq); free(apdq);
+    free(ql); free(qr); free(wave); free(s); free(amdq); free(apdq); free(auxl); free(auxr);
     return 0;
 }
 
@@ -982,7 +1074,8 @@ int orc_advanc(const orc_params* p, orc_hier* h, orc_regs* r, int level, double*
         double* f = malloc(4 * ntot * sizeof(double));
         if (!f) return ORC_E_ALLOC;
         double cflgrid, dtnew;
-        orc_stepgrid(p, mx, my, q, NULL, h->hx[level], h->hy[level], h->possk[level], f, f + ntot,
+        const double* aux = (p->maux > 0 && h->aux[level]) ? h->aux[level] + h->auxoff[level][k] : NULL;
+        orc_stepgrid(p, mx, my, q, aux, h->hx[level], h->hy[level], h->possk[level], f, f + ntot,
                      f + 2 * ntot, f + 3 * ntot, &cflgrid, &dtnew);
         if (has_fine) fluxsv(p, h, r, level, k, f, f + ntot, f + 2 * ntot, f + 3 * ntot);
         if (level > 1) fluxad(p, h, r, level, k, f, f + ntot, f + 2 * ntot, f + 3 * ntot);
@@ -1006,6 +1099,8 @@ int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
         int ilo = cb[0], ihi = cb[1], jlo = cb[2], jhi = cb[3];
         int mitot = ihi - ilo + 1 + 2 * ng, mjtot = jhi - jlo + 1 + 2 * ng;
         double* val = h->qnew[level] + h->qoff[level][m0];
+        const int capa = p->mcapa > 0 && h->aux[level];
+        const double* caux = capa ? h->aux[level] + h->auxoff[level][m0] : NULL;
         if (r->listbc[level]) {                                    /* upbnd */
             int* iused = calloc((size_t)mitot * mjtot, sizeof(int));
             if (!iused) return ORC_E_ALLOC;
@@ -1016,6 +1111,7 @@ int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
                 int* u = &iused[(icrse - 1) + (size_t)mitot * (jcrse - 1)];
                 if ((*u / norm) % 2 == 1) continue;                 /* :79-83 */
                 double sgnm = (iside == 2 || iside == 3 || iside == 6) ? -1.0 : 1.0;
+                if (capa) sgnm = sgnm / VAL(caux, p->maux, mitot, p->mcapa, icrse, jcrse);   /* :104-108 */
                 const double* svd = r->reg[level + 1] + r->roff[level + 1][kid] + (size_t)nv * (lkid - 1);
                 for (int m = 1; m <= nv; m++)
                     VAL(val, nv, mitot, m, icrse, jcrse) = VAL(val, nv, mitot, m, icrse, jcrse) + sgnm * svd[m - 1] / area;
@@ -1029,6 +1125,7 @@ int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
             int iclo = fb[0] / rx, jclo = fb[2] / ry, ichi = fb[1] / rx, jchi = fb[3] / ry;
             int mi = fb[1] - fb[0] + 1 + 2 * ng;
             const double* qf = h->qnew[level + 1] + h->qoff[level + 1][kid];
+            const double* faux = capa && h->aux[level + 1] ? h->aux[level + 1] + h->auxoff[level + 1][kid] : NULL;
             int iplo = imax(ilo, iclo), jplo = imax(jlo, jclo), iphi = imin(ihi, ichi), jphi = imin(jhi, jchi);
             if (iplo > iphi || jplo > jphi) continue;
             double totrat = (double)(rx * ry);
@@ -1037,13 +1134,26 @@ int orc_update(const orc_params* p, const orc_hier* h, orc_regs* r, int level) {
                 for (int j = jplo - jlo + ng + 1; j <= jphi - jlo + ng + 1; j++) {
                     int jff = jplo * ry - fb[2] + ng + 1 + (j - (jplo - jlo + ng + 1)) * ry;
                     for (int m = 1; m <= nv; m++) VAL(val, nv, mitot, m, i, j) = 0.0;
-                    for (int jco = 1; jco <= ry; jco++)
-                        for (int ico = 1; ico <= rx; ico++)
-                            for (int m = 1; m <= nv; m++)
-                                VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j)
-                                    + VAL(qf, nv, mi, m, iff + ico - 1, jff + jco - 1);
-                    for (int m = 1; m <= nv; m++)
-                        VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j) / totrat;
+                    if (!faux) {
+                        for (int jco = 1; jco <= ry; jco++)
+                            for (int ico = 1; ico <= rx; ico++)
+                                for (int m = 1; m <= nv; m++)
+                                    VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j)
+                                        + VAL(qf, nv, mi, m, iff + ico - 1, jff + jco - 1);
+                        for (int m = 1; m <= nv; m++)
+                            VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j) / totrat;
+                    } else {                                       /* update.f:147-161 */
+                        for (int jco = 1; jco <= ry; jco++)
+                            for (int ico = 1; ico <= rx; ico++) {
+                                const double cf = VAL(faux, p->maux, mi, p->mcapa, iff + ico - 1, jff + jco - 1);
+                                for (int m = 1; m <= nv; m++)
+                                    VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j)
+                                        + VAL(qf, nv, mi, m, iff + ico - 1, jff + jco - 1) * cf;
+                            }
+                        for (int m = 1; m <= nv; m++)
+                            VAL(val, nv, mitot, m, i, j) = VAL(val, nv, mitot, m, i, j) /
+                                (totrat * VAL(caux, p->maux, mitot, p->mcapa, i, j));
+                    }
                 }
             }
         }
@@ -1058,8 +1168,15 @@ double orc_conck(const orc_params* p, const orc_hier* h, int level) {
         const int* b = h->boxes[level] + 4 * k;
         int mitot = b[1] - b[0] + 1 + 2 * ng, mjtot = b[3] - b[2] + 1 + 2 * ng;
         const double* q = h->qnew[level] + h->qoff[level][k];
-        for (int j = ng + 1; j <= mjtot - ng; j++)
-            for (int i = ng + 1; i <= mitot - ng; i++) totmass = totmass + VAL(q, nv, mitot, 1, i, j);
+        if (p->mcapa > 0 && h->aux[level]) {                       /* conck.f:56-60 */
+            const double* a = h->aux[level] + h->auxoff[level][k];
+            for (int j = ng + 1; j <= mjtot - ng; j++)
+                for (int i = ng + 1; i <= mitot - ng; i++)
+                    totmass = totmass + VAL(q, nv, mitot, 1, i, j) * VAL(a, p->maux, mitot, p->mcapa, i, j);
+        } else {
+            for (int j = ng + 1; j <= mjtot - ng; j++)
+                for (int i = ng + 1; i <= mitot - ng; i++) totmass = totmass + VAL(q, nv, mitot, 1, i, j);
+        }
     }
     return totmass * h->hx[level] * h->hy[level];
 }

@@ -58,6 +58,38 @@ static void rpt2_advection(const orc_params* p, int ixy, int meqn, int mbc,
     }
 }
 
+/* ---- variable-coefficient advection q_t + (u q)_x + (v q)_y = 0 ----------
+ * clawpack/riemann's rpn2_vc_advection / rpt2_vc_advection (the solvers of
+ * the reference's tests/advection_2d_annulus, Makefile:31-32): the edge
+ * velocities are aux fields, aux(1) the x-velocity at the left edge of a
+ * cell and aux(2) the y-velocity at its bottom edge.  One wave, Δq, moving
+ * at the velocity of the interface; each fluctuation goes downwind
+ * (LeVeque 2002 sec. 9.5 for the normal solve, sec. 21.2 for the transverse
+ * split with the velocities of the edges the fluctuation crosses). */
+#define AUX(a, ma, i) (a)[((ma) - 1) + maux * POS(i)]
+static void rpn2_vcadv(int ixy, int meqn, int mwaves, int maux, int mbc, int mx,
+                       const double* ql, const double* qr, const double* auxl, double* wave,
+                       double* s, double* amdq, double* apdq) {
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        W(1, 1, i) = Q(ql, 1, i) - Q(qr, 1, i - 1);
+        S(1, i) = AUX(auxl, ixy, i);            /* the velocity at the edge i - 1/2 */
+        Q(amdq, 1, i) = dmin1(AUX(auxl, ixy, i), 0.0) * W(1, 1, i);
+        Q(apdq, 1, i) = dmax1(AUX(auxl, ixy, i), 0.0) * W(1, 1, i);
+    }
+}
+
+static void rpt2_vcadv(int ixy, int imp, int meqn, int maux, int mbc, int mx,
+                       const double* aux2, const double* aux3, const double* asdq,
+                       double* bmasdq, double* bpasdq) {
+    const int kv = 3 - ixy;                    /* the transverse velocity */
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        const int i1 = i - 2 + imp;            /* the cell the fluctuation entered */
+        Q(bmasdq, 1, i) = dmin1(AUX(aux2, kv, i1), 0.0) * Q(asdq, 1, i);
+        Q(bpasdq, 1, i) = dmax1(AUX(aux3, kv, i1), 0.0) * Q(asdq, 1, i);
+    }
+}
+#undef AUX
+
 /* ---- linear acoustics, q = (p, u, v), density rho, bulk modulus K -------
  * LeVeque 2002 sec. 3.10 / 21.2: waves r1 = (-Z,1,0) at -c, r2 = (Z,1,0)
  * at +c (normal velocity component mu), impedance Z = rho c. */
@@ -373,7 +405,7 @@ void orc_rpn2(const orc_params* p, int ixy, int maxm, int meqn, int mwaves,
               int maux, int mbc, int mx, const double* ql, const double* qr,
               const double* auxl, const double* auxr, double* wave, double* s,
               double* amdq, double* apdq) {
-    (void)maxm; (void)maux; (void)auxl; (void)auxr;
+    (void)maxm; (void)auxr;
     switch (p->solver) {
     case ORC_ADVECTION:
         rpn2_advection(p, ixy, meqn, mwaves, mbc, mx, ql, qr, wave, s, amdq,
@@ -390,6 +422,9 @@ void orc_rpn2(const orc_params* p, int ixy, int maxm, int meqn, int mwaves,
     case ORC_SHALLOW:
         rpn2_shallow(p, ixy, meqn, mwaves, mbc, mx, ql, qr, wave, s, amdq, apdq);
         break;
+    case ORC_VCADV:
+        rpn2_vcadv(ixy, meqn, mwaves, maux, mbc, mx, ql, qr, auxl, wave, s, amdq, apdq);
+        break;
     }
 }
 
@@ -398,8 +433,7 @@ void orc_rpt2(const orc_params* p, int ixy, int imp, int maxm, int meqn,
               const double* qr, const double* aux1, const double* aux2,
               const double* aux3, const double* asdq, double* bmasdq,
               double* bpasdq) {
-    (void)imp; (void)maxm; (void)mwaves; (void)maux;
-    (void)aux1; (void)aux2; (void)aux3;
+    (void)maxm; (void)mwaves; (void)aux1;
     switch (p->solver) {
     case ORC_ADVECTION:
         rpt2_advection(p, ixy, meqn, mbc, mx, asdq, bmasdq, bpasdq);
@@ -412,6 +446,9 @@ void orc_rpt2(const orc_params* p, int ixy, int imp, int maxm, int meqn,
         break;
     case ORC_SHALLOW:
         rpt2_shallow(p, ixy, meqn, mbc, mx, ql, qr, asdq, bmasdq, bpasdq);
+        break;
+    case ORC_VCADV:
+        rpt2_vcadv(ixy, imp, meqn, maux, mbc, mx, aux2, aux3, asdq, bmasdq, bpasdq);
         break;
     }
 }

@@ -45,6 +45,8 @@ module ref_harness
   use amr_module
   implicit none
   logical :: harness_parallel = .false.
+  ! aux values per cell of the installed grids (ref_hier_set_aux)
+  integer :: hnaux = 0
 contains
 
   ! method(1:7), mthlim(1:mwaves), mcapa, use_fwaves, cfl_desired, cfl_max
@@ -259,7 +261,7 @@ contains
     mitot = node(ndihi,mptr)-node(ndilo,mptr)+1+2*nghost
     mjtot = node(ndjhi,mptr)-node(ndjlo,mptr)+1+2*nghost
     call bound(t, nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, mptr, &
-         alloc(node(storeaux,mptr)), 0)
+         alloc(node(storeaux,mptr)), hnaux)
   end subroutine ref_bound
 
   ! prepf (fine levels) and prepc (coarse levels), as regrid.f:81-82 does
@@ -267,7 +269,7 @@ contains
     integer(c_int), value :: nlev, nvar
     integer :: l
     do l = 2, nlev
-       call prepf(l, nvar, 0)
+       call prepf(l, nvar, hnaux)
     end do
     do l = 1, nlev - 1
        call prepc(l, nvar)
@@ -295,11 +297,11 @@ contains
        mitot = node(ndihi,mptr) - node(ndilo,mptr) + 1 + 2*nghost
        mjtot = node(ndjhi,mptr) - node(ndjlo,mptr) + 1 + 2*nghost
        call bound(rnode(timemult,mptr), nvar, nghost, alloc(node(store1,mptr)), mitot, mjtot, &
-            mptr, alloc(node(storeaux,mptr)), 0)
+            mptr, alloc(node(storeaux,mptr)), hnaux)
     end do
     !$omp end parallel do
     if (level+1 .le. mxnest) then
-       if (lstart(level+1) .ne. null) call saveqc(level+1, nvar, 0)
+       if (lstart(level+1) .ne. null) call saveqc(level+1, nvar, hnaux)
     endif
     dtlevnew = rinfinity
     cfl_level = 0.d0
@@ -348,10 +350,10 @@ contains
        locsvq = locsvf + nvar*lenbc
        locx1d = locsvq + nvar*lenbc
        call qad(alloc(locnew),mitot,mjtot,nvar,alloc(locsvf),alloc(locsvq),lenbc, &
-            intratx(level-1),intraty(level-1),hx,hy,0,alloc(locaux),alloc(locx1d),delt,mptr)
+            intratx(level-1),intraty(level-1),hx,hy,hnaux,alloc(locaux),alloc(locx1d),delt,mptr)
     endif
     call stepgrid(alloc(locnew),fm,fp,gm,gp,mitot,mjtot,nghost,delt,dtnew,hx,hy,nvar, &
-         xlow,ylow,time,mptr,0,alloc(locaux))
+         xlow,ylow,time,mptr,hnaux,alloc(locaux))
     if (node(cfluxptr,mptr) .ne. 0) call fluxsv(mptr,fm,fp,gm,gp,alloc(node(cfluxptr,mptr)), &
          mitot,mjtot,nvar,listsp(level),delt,hx,hy)
     if (node(ffluxptr,mptr) .ne. 0) then
@@ -366,7 +368,7 @@ contains
 
   subroutine ref_update(level, nvar) bind(C, name="ref_update")
     integer(c_int), value :: level, nvar
-    call update(level, nvar, 0)
+    call update(level, nvar, hnaux)
   end subroutine ref_update
 
   ! register block (svdflx then qc1d) of grid k (0-based) of fine level l
@@ -493,6 +495,52 @@ contains
        off = off + n
     end do
   end subroutine ref_flag_level
+
+  ! aux arrays of every installed grid (node(storeaux), naux values per
+  ! cell like store1) and their types (amr_module auxtype: 0 "center",
+  ! 1 "xleft", 2 "yleft", 3 "capacity"); call after ref_hier_setup and
+  ! before ref_prep_regs
+  subroutine ref_hier_set_aux(naux_in, types) bind(C, name="ref_hier_set_aux")
+    integer(c_int), value :: naux_in
+    integer(c_int), intent(in) :: types(*)
+    integer :: ma, l, mptr, mitot, mjtot
+    integer, external :: igetsp
+    hnaux = naux_in
+    if (allocated(auxtype)) deallocate(auxtype)
+    allocate(auxtype(max(naux_in, 1)))
+    auxtype = "center"
+    do ma = 1, naux_in
+       select case (types(ma))
+       case (1)
+          auxtype(ma) = "xleft"
+       case (2)
+          auxtype(ma) = "yleft"
+       case (3)
+          auxtype(ma) = "capacity"
+       end select
+    end do
+    if (naux_in == 0) return
+    do l = 1, maxlv
+       if (numgrids(l) == 0) cycle
+       mptr = lstart(l)
+       do while (mptr /= 0)
+          mitot = node(ndihi, mptr) - node(ndilo, mptr) + 1 + 2*nghost
+          mjtot = node(ndjhi, mptr) - node(ndjlo, mptr) + 1 + 2*nghost
+          node(storeaux, mptr) = igetsp(naux_in*mitot*mjtot)
+          mptr = node(levelptr, mptr)
+       end do
+    end do
+  end subroutine ref_hier_set_aux
+
+  ! copy the aux array (naux, mitot, mjtot) of grid k (from 0) of level l in
+  subroutine ref_hier_put_aux(l, k, aux) bind(C, name="ref_hier_put_aux")
+    integer(c_int), value :: l, k
+    real(c_double), intent(in) :: aux(*)
+    integer :: mptr, n
+    mptr = listOfGrids(listStart(l) + k)
+    n = hnaux*(node(ndihi,mptr)-node(ndilo,mptr)+1+2*nghost)*(node(ndjhi,mptr)-node(ndjlo,mptr)+1+2*nghost)
+    alloc(node(storeaux,mptr):node(storeaux,mptr)+n-1) = aux(1:n)
+  end subroutine ref_hier_put_aux
 
   ! filval.f90:3 for a new grid of `level` with box (ilo, ihi, jlo, jhi):
   ! the installed grids of `level` are the old ones, those of level-1 the

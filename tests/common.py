@@ -112,13 +112,13 @@ def random_hierarchy(rng, nvar=3, nx=24, ny=20, nlev=3, ratio=2, mthbc=(1, 1, 1,
 
 
 def nested_hierarchy(rng, solver_states, nvar, nx=24, ny=20, nlev=3, ratio=2, mthbc=(1, 1, 1, 1),
-                     xper=False, yper=False, dt1=None, nfine=5, level1_split=(2, 2)):
+                     xper=False, yper=False, dt1=None, nfine=5, level1_split=(2, 2), naux=0):
     """Properly nested hierarchy at one time (t = 0, old = new) for time
     stepping: every level-(l+1) grid, coarsened and grown by one coarse
     cell, lies inside one level-l grid or outside the domain (the nesting
     saveqc and the coarse-fine registers assume).  solver_states(rng, shape)
     returns physical states of shape shape + (nvar,)."""
-    H = Hier(nvar, mthbc=mthbc, xper=xper, yper=yper)
+    H = Hier(nvar, mthbc=mthbc, xper=xper, yper=yper, naux=naux)
     hx, hy = 1.0 / nx, 1.0 / ny
     isz, jsz = nx, ny
     cuts_i = np.linspace(0, nx, level1_split[0] + 1).astype(int)
@@ -174,6 +174,37 @@ def nested_boxes(rng, H, level, nbox, lo=4, hi=None):
     return disjoint_boxes(rng, F["iregsz"], F["jregsz"], nbox, lo=lo,
                           hi=hi or max(8, F["iregsz"] // 3), align=max(rx, ry),
                           within=within, tries=4000)
+
+
+def vc_annulus_aux(H, amp=0.3):
+    """Fill the aux arrays of H (naux = 3) as setaux does for
+    variable-coefficient advection with capacity on every grid and ghost
+    cell: aux(1) the x-velocity at the cell's left edge and aux(2) the
+    y-velocity at its bottom edge, differenced from the stream function
+    amp sin(pi x) sin(pi y) (discretely divergence-free, zero normal flow
+    through the unit square's edges), aux(3) the capacity 1 + 0.3 x + 0.2 y
+    at the cell centre (its exact cell average, so a coarse cell's capacity
+    is the mean of its children's)."""
+    psi = lambda x, y: amp * np.sin(np.pi * x) * np.sin(np.pi * y)
+    x0, _, y0, _ = H.domain
+    for l, L in enumerate(H.levels, start=1):
+        hx, hy = L["hx"], L["hy"]
+        for k, b in enumerate(L["boxes"]):
+            a = H.aux(l, k)
+            jj, ii = np.meshgrid(np.arange(b[2] - 2, b[3] + 3), np.arange(b[0] - 2, b[1] + 3),
+                                 indexing="ij")
+            xl, yb = x0 + ii * hx, y0 + jj * hy
+            a[..., 0] = (psi(xl, yb + hy) - psi(xl, yb)) / hy
+            a[..., 1] = -(psi(xl + hx, yb) - psi(xl, yb)) / hx
+            a[..., 2] = 1.0 + 0.3 * (xl + 0.5 * hx) + 0.2 * (yb + 0.5 * hy)
+
+
+def positive_states(rng, shape):
+    """A smooth positive scalar field with noise (one component)."""
+    ny, nx = shape
+    y, x = np.meshgrid(np.linspace(0, 1, ny), np.linspace(0, 1, nx), indexing="ij")
+    return (1.0 + 0.5 * np.sin(5 * x + rng.random() * 6) * np.cos(4 * y)
+            + 0.05 * rng.random(shape))[..., None]
 
 
 def smooth_euler(rng, shape):
