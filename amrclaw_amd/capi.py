@@ -13,7 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libamrgpu.so")
 
-ADVECTION, ACOUSTICS, EULER = 1, 2, 3
+ADVECTION, ACOUSTICS, EULER, VCADV = 1, 2, 3, 4
+AUX_CENTER, AUX_XLEFT, AUX_YLEFT, AUX_CAPACITY = 0, 1, 2, 3
 FP64, FP32, MIXED = 0, 1, 2
 
 
@@ -30,7 +31,7 @@ class AmrgpuConfig(C.Structure):
         ("ubar", C.c_double), ("vbar", C.c_double),
         ("rho", C.c_double), ("bulk", C.c_double),
         ("gamma", C.c_double), ("efix", C.c_int),
-        ("precision", C.c_int),
+        ("precision", C.c_int), ("auxtype", C.c_int * 8),
     ]
 
 
@@ -96,6 +97,7 @@ def lib():
         L.amrgpu_get_grid.argtypes = [C.c_int, C.c_int, C.c_int, _dp]
         _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
         L.amrgpu_put_level.argtypes = [C.c_int, C.c_int, _dp, _lp]
+        L.amrgpu_put_level_aux.argtypes = [C.c_int, _dp, _lp]
         L.amrgpu_get_level.argtypes = [C.c_int, C.c_int, _dp, _lp]
         L.amrgpu_sweep_level.argtypes = [C.c_int, C.c_double, C.c_void_p,
                                          C.c_void_p]
@@ -132,24 +134,29 @@ def _check(rc):
 def make_config(solver, *, order=2, transverse=2, limiter=4, cfl=0.9,
                 cflv1=1.0, mthbc=(1, 1, 1, 1), domain=(0.0, 1.0, 0.0, 1.0),
                 ubar=0.5, vbar=1.0, rho=1.0, bulk=4.0, gamma=1.4, efix=True,
-                precision=FP64, nvar=None, mwaves=None):
+                precision=FP64, nvar=None, mwaves=None, naux=0, mcapa=0, auxtype=()):
     """Configuration of one run (claw.data / amr.data fields, setprob
     constants).  limiter: one mthlim code for every wave, or a list.
-    A plug-in solver (id from register_solver) needs nvar and mwaves."""
+    A plug-in solver (id from register_solver) needs nvar and mwaves.
+    naux / mcapa / auxtype: num_aux, capa_index (1-based, 0 for none) and
+    the AUX_* code of each aux component (amr.data aux_type)."""
     c = AmrgpuConfig()
     c.solver = solver
-    if solver in (ADVECTION, ACOUSTICS, EULER):
-        c.nvar, c.mwaves = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4)}[solver]
+    if solver in (ADVECTION, ACOUSTICS, EULER, VCADV):
+        c.nvar, c.mwaves = {ADVECTION: (1, 1), ACOUSTICS: (3, 2), EULER: (4, 4),
+                            VCADV: (1, 1)}[solver]
     else:
         c.nvar, c.mwaves = nvar, mwaves
-    c.naux, c.nghost = 0, 2
+    c.naux, c.nghost = naux, 2
+    for k in range(8):
+        c.auxtype[k] = auxtype[k] if k < len(auxtype) else AUX_CENTER
     meth = [0, order, transverse, 0, 0, 0, 0]
     for k in range(7):
         c.method[k] = meth[k]
     lim = limiter if isinstance(limiter, (list, tuple)) else [limiter] * c.mwaves
     for k in range(8):
         c.mthlim[k] = lim[k] if k < len(lim) else 0
-    c.mcapa, c.use_fwaves = 0, 0
+    c.mcapa, c.use_fwaves = mcapa, 0
     c.cfl, c.cflv1 = cfl, cflv1
     for k in range(4):
         c.mthbc[k] = mthbc[k]
@@ -255,6 +262,13 @@ def put_level(level, qs, which=0):
     offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     packed = np.concatenate([np.ascontiguousarray(q, dtype=np.float64).ravel() for q in qs])
     _check(lib().amrgpu_put_level(level, which, packed, offs))
+
+
+def put_level_aux(level, aux, offsets):
+    """amrgpu_put_level_aux: every grid's aux array (naux values per cell,
+    component fastest, ghost cells included) at element offsets of aux."""
+    _check(lib().amrgpu_put_level_aux(level, np.ascontiguousarray(aux, dtype=np.float64),
+                                      np.ascontiguousarray(offsets, dtype=np.int64)))
 
 
 def get_level(level, shapes, which=0):
@@ -384,6 +398,8 @@ def load_hierarchy(H):
         if len(L["boxes"]):
             _check(lib().amrgpu_put_level(l, 0, L["qnew"], L["off"]))
             _check(lib().amrgpu_put_level(l, 1, L["qold"], L["off"]))
+            if getattr(H, "naux", 0) > 0:
+                put_level_aux(l, L["aux"], L["auxoff"])
         set_time(l, L["time"], L["possk"])
 
 

@@ -100,6 +100,10 @@ struct Level {
     FillOp* d_sq_ops = nullptr;
     FillBuf* d_sq_bufs = nullptr;
     double* d_sq_scratch = nullptr;
+    double* d_sq_aux = nullptr;          // the saveqc program's aux scratch (aux mode)
+    // aux arrays of the grids (FP64, see Context::auxk), set by
+    // amrgpu_put_level_aux
+    double* d_aux = nullptr;
     double* d_partial = nullptr;        // conck total (one double; sized one per grid)
     // grid corners rnode(cornxlo/cornylo) (default ilo*hx + xlower)
     std::vector<double> cxlo, cylo;
@@ -154,6 +158,10 @@ struct Context {
     MethodArgs meth{};
     size_t esize = 8;
     bool mixed = false;   // AMRGPU_MIXED: FP32 sweep on FP64 storage, flux form
+    // aux arrays: a level's FP64 aux array holds grid p's component c at
+    // patches[p].off * auxk + c * plane (auxk = ceil(naux / nvar) keeps the
+    // grids' blocks apart); 0 without aux
+    int auxk = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[NEVENTS] = {};
     int txmax = 28, tymax = 28;
@@ -301,9 +309,10 @@ void free_plan(Level& L) {
 void free_regs(Level& L) {
     quiesce();
     void* ps[] = {L.d_reg, L.d_sv, L.d_qad, L.d_ucells, L.d_uents, L.d_avg, L.d_avg_start,
-                  L.d_sq_ops, L.d_sq_bufs, L.d_sq_scratch};
+                  L.d_sq_ops, L.d_sq_bufs, L.d_sq_scratch, L.d_sq_aux};
     for (void* p : ps)
         if (p) hipFree(p);
+    L.d_sq_aux = nullptr;
     L.d_reg = nullptr;
     L.d_sv = nullptr;
     L.d_qad = nullptr;
@@ -343,6 +352,7 @@ void free_level(Level& L) {
     for (auto& b : L.buf)
         if (b) hipFree(b);
     if (L.d_cfl) hipFree(L.d_cfl);
+    if (L.d_aux) hipFree(L.d_aux);
     if (L.d_off) hipFree(L.d_off);
     if (L.d_plane) hipFree(L.d_plane);
     if (L.d_cell_start) hipFree(L.d_cell_start);
@@ -604,6 +614,10 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
         }
         HIPCHK(hipMalloc(&L.d_cfl, ngrids * sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&L.d_partial, ngrids * sizeof(double)));
+        if (g.auxk > 0) {
+            HIPCHK(hipMalloc(&L.d_aux, (size_t)L.nelem * g.auxk * sizeof(double)));
+            HIPCHK(dzero(L.d_aux, (size_t)L.nelem * g.auxk * sizeof(double)));
+        }
         std::vector<int64_t> off(ngrids), cs(ngrids);
         std::vector<int32_t> pl(ngrids);
         int64_t c = 0;
@@ -744,6 +758,7 @@ HierGeom make_geom() {
     H.yperdom = g.cfg.mthbc[2] == 2 && g.cfg.mthbc[3] == 2;
     H.spheredom = false;
     for (int k = 0; k < 4; k++) H.mthbc[k] = g.cfg.mthbc[k];
+    H.naux = g.cfg.naux;
     return H;
 }
 
@@ -994,10 +1009,16 @@ int launch_sweep(Level& L, double dt, double* flux_out_dev, const int64_t* flux_
     a.reg_fine = rl.reg_fine;
     a.fine_patches = rl.fine_patches;
     a.sv = rl.sv;
+    if (g.auxk > 0) {
+        a.aux = L.d_aux;
+        a.naux = g.cfg.naux;
+        a.auxk = g.auxk;
+        a.mcapa = g.cfg.mcapa;
+    }
     auto launch = [&](const SweepArgs& x) { return run_sweep(run_precision(), x, L.lds, g.sweep_wgs); };
     hipError_t e = hipSuccess;
     tbegin(AMRGPU_KT_SWEEP);
-    if (g.launch_mode == AMRGPU_LAUNCH_MARCH && L.d_mwaves && !flux_out_dev && !rl.sv) {
+    if (g.launch_mode == AMRGPU_LAUNCH_MARCH && L.d_mwaves && !flux_out_dev && !rl.sv && !a.aux) {
         // the march sweep writes no per-edge flux arrays and no fluxsv
         // entries: a level with a finer level, or a call asking for the
         // flux arrays, takes the tile sweep
@@ -1062,6 +1083,45 @@ int level_cfl(Level& L, double dt, double* cfl_level, double* dtlevnew) {
     return AMRGPU_OK;
 }
 
+// The aux part of saveqc.f / cstore.f for fine level `level`: the saveqc
+// program run again in aux mode, the coarse grids' aux values copied into
+// the coarse patches (NEEDS_TO_BE_SET where none reaches: the reference
+// then calls the application's setaux, amr_module's default of which sets
+// nothing) and each patch's ring stored in the grid's auxc1d.
+int run_saveqc_aux(int level) {
+    Level& F = g.levels[level];
+    const FillPlan& P = F.sq;
+    if (P.ops.empty()) return AMRGPU_OK;
+    const int64_t n = P.scratch_elems * g.auxk;
+    if (!F.d_sq_aux && n > 0) HIPCHK(hipMalloc(&F.d_sq_aux, (size_t)n * sizeof(double)));
+    if (n > 0) {
+        const double needs = (double)10.e33f;                  // amr_module.f90:170
+        unsigned long long bits;
+        std::memcpy(&bits, &needs, sizeof bits);
+        HIPCHK(launch_fill_u64(reinterpret_cast<unsigned long long*>(F.d_sq_aux), bits, n, g.stream));
+    }
+    FillArgs a{};
+    a.bufs = F.d_sq_bufs;
+    a.scratch = F.d_sq_aux;
+    for (int l = 1; l < MAXLV && l < FILL_MAXLV; l++) {
+        Level& S = g.levels[l];
+        a.lvl_patches[l] = S.d_patches;
+        a.lvl_reg[l] = S.d_reg;
+        a.lvl_aux[l] = S.d_aux;
+    }
+    a.nv = g.cfg.nvar;
+    a.aux_mode = 1;
+    a.naux = g.cfg.naux;
+    a.auxk = g.auxk;
+    for (int s = 0; s < P.nstages; s++) {
+        a.ops = F.d_sq_ops + P.stage_start[s];
+        a.nops = P.stage_start[s + 1] - P.stage_start[s];
+        hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
+        if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "saveqc aux: %s", hipGetErrorString(e));
+    }
+    return AMRGPU_OK;
+}
+
 // advanc.f:16-305 for one level: ghost fill, saveqc of the finer level,
 // the sweep with fluxsv / fluxad's perimeter fluxes, qad + fluxad, time
 // the launches of one level step (bound, saveqc of the finer level, sweep,
@@ -1080,6 +1140,10 @@ int launch_level_step(int level) {
         Level& F = g.levels[level + 1];
         rc = run_fill(F.sq, F.d_sq_ops, F.d_sq_bufs, nullptr, F.d_sq_scratch, L.time);
         if (rc) return rc;
+        if (g.auxk > 0) {
+            rc = run_saveqc_aux(level + 1);
+            if (rc) return rc;
+        }
         rl.reg_fine = F.d_reg;
         rl.fine_patches = F.d_patches;
         rl.sv = L.d_sv;
@@ -1101,6 +1165,12 @@ int launch_level_step(int level) {
         a.hy = L.hy;
         a.delt = L.possk;
         a.flux_form = g.mixed ? 1 : 0;
+        if (g.auxk > 0) {
+            a.naux = g.cfg.naux;
+            a.aux = L.d_aux;
+            a.auxk = g.auxk;
+            for (int c = 0; c < 8; c++) a.auxtype[c] = g.cfg.auxtype[c];
+        }
         tbegin(AMRGPU_KT_REGISTERS);
         hipError_t e = run_qad(a);
         tend(AMRGPU_KT_REGISTERS);
@@ -1246,6 +1316,12 @@ int update_level(int level) {
     a.qf = F.buf[F.cur];
     a.reg = F.d_reg;
     a.area = C.hx * C.hy;
+    if (g.auxk > 0 && g.cfg.mcapa > 0) {
+        a.aux_c = C.d_aux;
+        a.aux_f = F.d_aux;
+        a.mcapa = g.cfg.mcapa;
+        a.auxk = g.auxk;
+    }
     tbegin(AMRGPU_KT_UPDATE);
     hipError_t e = launch_update((int)g.esize, a, C.d_avg_start, C.navg_cells, g.stream);
     tend(AMRGPU_KT_UPDATE);
@@ -1487,9 +1563,9 @@ int amrgpu_init(const amrgpu_config* cfg) {
     if (g.inited) amrgpu_finalize();
     const amrgpu_solver_ops* plugin = nullptr;
     int nv_need, nw_need;
-    if (cfg->solver >= 1 && cfg->solver <= 3) {
-        const int nv_expect[4] = {0, 1, 3, 4};
-        const int nw_expect[4] = {0, 1, 2, 4};
+    if (cfg->solver >= 1 && cfg->solver <= 4) {
+        const int nv_expect[5] = {0, 1, 3, 4, 1};
+        const int nw_expect[5] = {0, 1, 2, 4, 1};
         nv_need = nv_expect[cfg->solver];
         nw_need = nw_expect[cfg->solver];
     } else if (cfg->solver >= AMRGPU_SOLVER_PLUGIN0 &&
@@ -1505,8 +1581,26 @@ int amrgpu_init(const amrgpu_config* cfg) {
     if (cfg->mwaves != nw_need)
         return fail(AMRGPU_ERR_ARG, "solver %d needs mwaves=%d", cfg->solver, nw_need);
     if (cfg->nghost != NGHOST) return fail(AMRGPU_ERR_UNSUPPORTED, "nghost must be 2");
-    if (cfg->naux != 0 || cfg->mcapa != 0)
-        return fail(AMRGPU_ERR_UNSUPPORTED, "aux arrays / capacity are not supported");
+    if (cfg->naux < 0 || cfg->naux > 8 || cfg->mcapa < 0 || cfg->mcapa > cfg->naux)
+        return fail(AMRGPU_ERR_ARG, "naux=%d mcapa=%d: need 0 <= mcapa <= naux <= 8", cfg->naux,
+                    cfg->mcapa);
+    if (cfg->solver == SOLVER_VCADV && cfg->naux < 2)
+        return fail(AMRGPU_ERR_ARG, "the variable-coefficient advection solver reads aux(1:2)");
+    if (plugin && cfg->naux > 0)
+        return fail(AMRGPU_ERR_UNSUPPORTED, "aux arrays with a plug-in solver");
+    if (cfg->naux > 0 && cfg->precision == AMRGPU_MIXED)
+        return fail(AMRGPU_ERR_UNSUPPORTED,
+                    "aux arrays / capacity run in FP64 or FP32 (the mixed flux form needs a "
+                    "state-only flux)");
+    for (int k = 0; k < cfg->naux; k++)
+        if (cfg->auxtype[k] < AMRGPU_AUX_CENTER || cfg->auxtype[k] > AMRGPU_AUX_CAPACITY)
+            return fail(AMRGPU_ERR_ARG, "auxtype(%d)=%d invalid", k + 1, cfg->auxtype[k]);
+    // a solid wall negates q(2) on the x-sides and q(3) on the y-sides
+    // (bc2amr.f90:145, 190, 236, 283)
+    if (((cfg->mthbc[0] == 3 || cfg->mthbc[1] == 3) && cfg->nvar < 2) ||
+        ((cfg->mthbc[2] == 3 || cfg->mthbc[3] == 3) && cfg->nvar < 3))
+        return fail(AMRGPU_ERR_ARG, "a solid wall negates q(2) (x-sides) or q(3) (y-sides): "
+                                    "nvar=%d has no such component", cfg->nvar);
     if ((cfg->mthbc[0] == 2) != (cfg->mthbc[1] == 2) || (cfg->mthbc[2] == 2) != (cfg->mthbc[3] == 2))
         return fail(AMRGPU_ERR_ARG, "periodic boundaries must be periodic on both sides");
     for (int k = 0; k < 4; k++)
@@ -1526,6 +1620,7 @@ int amrgpu_init(const amrgpu_config* cfg) {
     g.plugin = plugin;
     g.esize = cfg->precision == AMRGPU_FP32 ? 4 : 8;
     g.mixed = cfg->precision == AMRGPU_MIXED;
+    g.auxk = cfg->naux > 0 ? (cfg->naux + cfg->nvar - 1) / cfg->nvar : 0;
     MethodArgs& M = g.meth;
     M.ph.ubar = cfg->ubar;
     M.ph.vbar = cfg->vbar;
@@ -1660,6 +1755,8 @@ int amrgpu_stepgrid(int mx, int my, const double* q_in, double* q_out, double dx
                     double* dtnew) {
     if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
     if (!q_in || !q_out || mx < 1 || my < 1) return fail(AMRGPU_ERR_ARG, "bad stepgrid arguments");
+    if (g.auxk > 0)
+        return fail(AMRGPU_ERR_UNSUPPORTED, "stepgrid takes no aux array: use a level (put_level_aux)");
     int ilo = 0, ihi = mx - 1, jlo = 0, jhi = my - 1;
     int rc = define_level(0, 1, &ilo, &ihi, &jlo, &jhi, dx, dy);
     if (rc) return rc;
@@ -1705,6 +1802,8 @@ int amrgpu_stepgrid_noalloc(int mx, int my, const double* q_in, double* q_out, d
     if (!g.inited) return fail(AMRGPU_ERR_STATE, "amrgpu_init has not been called");
     if (!q_in || !q_out || mx < 1 || my < 1 || !(dx > 0) || !(dy > 0))
         return fail(AMRGPU_ERR_ARG, "bad stepgrid_noalloc arguments");
+    if (g.auxk > 0)
+        return fail(AMRGPU_ERR_UNSUPPORTED, "stepgrid_noalloc takes no aux array");
     unsigned char* base = noalloc_scratch();
     if (!base)
         return fail(AMRGPU_ERR_STATE, "no work space on this device: call amrgpu_init on it first");
@@ -1917,6 +2016,26 @@ int amrgpu_get_level(int level, int which, double* alloc_base, const int64_t* of
     return xfer_level(g.levels[level], false, which, alloc_base, offsets);
 }
 
+int amrgpu_put_level_aux(int level, const double* alloc_base, const int64_t* offsets) {
+    int rc = check_level(level);
+    if (rc) return rc;
+    if (!alloc_base || !offsets) return fail(AMRGPU_ERR_ARG, "bad put_level_aux arguments");
+    if (g.auxk == 0) return fail(AMRGPU_ERR_STATE, "the run has no aux arrays (naux = 0)");
+    Level& L = g.levels[level];
+    if (L.ngrids == 0) return AMRGPU_OK;
+    const int na = g.cfg.naux;
+    std::vector<double> h((size_t)L.nelem * g.auxk, 0.0);
+    for (int p = 0; p < L.ngrids; p++) {
+        const PatchDesc& pd = L.patches[p];
+        const double* src = alloc_base + offsets[p];
+        double* dst = h.data() + pd.off * g.auxk;
+        for (int64_t c = 0; c < pd.plane; c++)
+            for (int m = 0; m < na; m++) dst[(int64_t)m * pd.plane + c] = src[c * na + m];
+    }
+    HIPCHK(h2d(L.d_aux, h.data(), h.size() * sizeof(double)));
+    return AMRGPU_OK;
+}
+
 int amrgpu_sweep_level(int level, double dt, double* cfl_level, double* dtlevnew) {
     int rc = check_level(level);
     if (rc) return rc;
@@ -2002,7 +2121,8 @@ int amrgpu_conck(int level, double* totmass) {
     Level& L = g.levels[level];
     *totmass = 0.0;
     if (L.ngrids == 0) return AMRGPU_OK;
-    hipError_t e = launch_conck((int)g.esize, L.d_patches, L.ngrids, L.buf[L.cur], L.d_partial, g.stream);
+    hipError_t e = launch_conck((int)g.esize, L.d_patches, L.ngrids, L.buf[L.cur], L.d_partial, g.stream,
+                                L.d_aux, L.d_aux ? g.cfg.mcapa : 0, g.auxk);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "conck: %s", hipGetErrorString(e));
     double tot = 0.0;   // the sequential sum over the level (conck.f:36-60)
     HIPCHK(hipMemcpyAsync(&tot, L.d_partial, sizeof(double), hipMemcpyDeviceToHost, g.stream));
@@ -2031,6 +2151,8 @@ int amrgpu_flag_level(int level, int flag_gradient, double tolsp, int flag_richa
     if (nregions < 0 || (nregions > 0 && !regions) || mbuff < NGHOST || mbuff > 64 ||
         (flags_base && !offsets) || iorder < 1 || iorder > 8)
         return fail(AMRGPU_ERR_ARG, "bad flag_level arguments");
+    if (flag_richardson && g.auxk > 0)
+        return fail(AMRGPU_ERR_UNSUPPORTED, "Richardson estimation with aux arrays");
     if (flag_richardson && level == g.mxnest)
         return fail(AMRGPU_ERR_ARG, "Richardson estimation needs old values, which level %d "
                                     "(mxnest) does not keep", level);
@@ -2119,6 +2241,9 @@ int amrgpu_regrid_level(int level, int ngrids, const int* ilo, const int* ihi, c
         return fail(AMRGPU_ERR_ARG, "bad grid arrays");
     if (g.levels[level - 1].ngrids == 0)
         return fail(AMRGPU_ERR_STATE, "regrid: level %d has no grids to interpolate from", level - 1);
+    if (g.auxk > 0)
+        return fail(AMRGPU_ERR_UNSUPPORTED,
+                    "regrid with aux arrays (filval.f90 then calls setaux and fixcapaq)");
     if (!(g.hxposs[level] > 0) || g.iregsz[level] < 1)
         return fail(AMRGPU_ERR_STATE, "regrid: the geometry of level %d is not set", level);
     Level& L = g.levels[level];

@@ -3,6 +3,7 @@
 // interpolation are evaluated exactly as the reference's FP64 code.
 #include <hip/hip_runtime.h>
 #include "fill.hpp"
+#include "sweep.hpp"
 
 namespace amrgpu {
 
@@ -40,12 +41,53 @@ __device__ BufRef<S> bufref(const FillArgs& a, int b) {
     return r;
 }
 
+// the aux-mode copy of a saveqc program: aux values of the coarse grids
+// into the coarse patch (icall.f's aux part) and the patch's ring into
+// auxc1d (cstore.f); NEEDS_TO_BE_SET stays where no coarse grid reached
+__device__ void fill_aux_op(const FillArgs& a, const FillOp& op, int tid) {
+    const int na = a.naux;
+    if (op.type == FOP_COPY) {
+        const PatchDesc sp = a.lvl_patches[op.src_level][op.src];
+        const double* sa = a.lvl_aux[op.src_level] + sp.off * a.auxk;
+        const FillBuf fb = a.bufs[op.dst];
+        double* da = a.scratch + fb.off * a.auxk;
+        const int ncell = op.ni * op.nj;
+        for (int t = tid; t < ncell * na; t += 256) {
+            const int c = t / ncell, r = t - c * ncell;
+            const int dj = r / op.ni, di = r - dj * op.ni;
+            da[(int64_t)c * fb.plane + (int64_t)(op.dj0 + dj) * fb.nrow + (op.di0 + di)] =
+                sa[(int64_t)c * sp.plane + (int64_t)(op.sj0 + dj) * sp.mitot + (op.si0 + di)];
+        }
+    } else if (op.type == FOP_RING) {
+        const FillBuf fb = a.bufs[op.src];
+        const double* sa = a.scratch + fb.off * a.auxk;
+        const PatchDesc pf = a.lvl_patches[op.level][op.si0];
+        double* ac1d = a.lvl_reg[op.level] + aux_ring_offset(pf, a.nv);
+        const int nrow = op.ni, ncol = op.nj;
+        const int n1 = ncol - 2, n2 = nrow - 2;
+        const int len = 2 * (n1 + n2);
+        for (int t = tid; t < len * na; t += 256) {
+            const int idx = t / na, c = t - idx * na;
+            int i, j;
+            if (idx < n1) { i = 0; j = 1 + idx; }
+            else if (idx < n1 + n2) { i = 1 + idx - n1; j = ncol - 1; }
+            else if (idx < 2 * n1 + n2) { i = nrow - 1; j = 1 + idx - n1 - n2; }
+            else { i = 1 + idx - 2 * n1 - n2; j = 0; }
+            ac1d[(int64_t)idx * na + c] = sa[(int64_t)c * fb.plane + (int64_t)j * nrow + i];
+        }
+    }
+}
+
 template <typename S>
 __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
     const FillOp op = a.ops[blockIdx.x];
     const int tid = threadIdx.x, nv = a.nv;
     if (a.clear && blockIdx.x == 0)
         for (int64_t t = tid; t < a.nclear; t += 256) a.clear[t] = 0ull;
+    if (a.aux_mode) {
+        fill_aux_op(a, op, tid);
+        return;
+    }
     const BufRef<S> dst = bufref<S>(a, op.dst);
     const int ncell = op.ni * op.nj;
 

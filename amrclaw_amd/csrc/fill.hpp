@@ -91,6 +91,13 @@ struct FillArgs {
     // maxima, cleared here for the sweep that follows: one launch fewer)
     unsigned long long* clear;
     int64_t nclear;
+    // aux mode (a saveqc program run a second time for the aux values,
+    // saveqc.f:60-90 and cstore.f): copies read the levels' aux arrays
+    // (FP64, naux components, patch p at off * auxk), scratch buffer b's
+    // component c starts at scratch + bufs[b].off * auxk + c * plane, rings
+    // write auxc1d; boundary conditions leave aux alone
+    const double* lvl_aux[FILL_MAXLV];
+    int32_t aux_mode, naux, auxk, pad2;
 };
 
 }  // namespace amrgpu

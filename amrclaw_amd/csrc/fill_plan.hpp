@@ -43,6 +43,7 @@ struct HierGeom {
     double xlower = 0, xupper = 1, ylower = 0, yupper = 1;
     bool xperdom = false, yperdom = false, spheredom = false;
     int mthbc[4] = {1, 1, 1, 1};
+    int naux = 0;                            // aux values per cell (register arenas hold auxc1d)
 };
 
 struct FillPlan {

@@ -18,5 +18,6 @@ hipError_t launch_errf1(const ErrfArgs& a, int ngrids, hipStream_t st);
 // conck.f: the level's sum of q(1) over the interiors, in the reference's
 // sequential order, into out[0]
 hipError_t launch_conck(int esize, const PatchDesc* patches, int npatch, const void* q,
-                        double* out, hipStream_t st);
+                        double* out, hipStream_t st, const double* aux = nullptr, int mcapa = 0,
+                        int auxk = 0);
 }  // namespace amrgpu

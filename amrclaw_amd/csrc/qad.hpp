@@ -51,9 +51,39 @@ __global__ __launch_bounds__(256) void finereg_kernel(QadArgs a, Physics ph) {
         // interface states: coarse is left on sides 1 and 4, right on 2 and 3
         const double* ql = (it.side == 1 || it.side == 4) ? qcv : qf;
         const double* qr = (it.side == 1 || it.side == 4) ? qf : qcv;
+        // aux of the two cells (qad.f:81-91, 160-185, 228-255, 300-330): the
+        // coarse ring's value on the coarse side, on the fine side the ghost
+        // cell's, or for an xleft / yleft component the fine face between
+        // them (sides 1 and 4 take the first interior cell's; on sides 2
+        // and 3 the face value is the ghost cell's, and it replaces the
+        // coarse ring's value)
+        constexpr int SNA = sol_naux<Sol>::value, NA = SNA > 0 ? SNA : 1;
+        double al[NA], ar[NA];
+        if constexpr (SNA > 0) {
+            const double* fa = a.aux + pd.off * a.auxk;
+            const double* ac = a.reg + aux_ring_offset(pd, NV) + (int64_t)(it.slot - 1) * a.naux;
+            const int64_t lin = (int64_t)(J + 1) * pd.mitot + (I + 1);
+            // the face cell on sides 1 (I = 1) and 4 (J = 1)
+            const int64_t lface = it.side == 1 ? lin + 1 : (it.side == 4 ? lin + pd.mitot : lin);
+#pragma unroll
+            for (int c = 0; c < SNA; c++) {
+                const double vf = fa[(int64_t)c * pd.plane + lin];
+                const double vface = fa[(int64_t)c * pd.plane + lface];
+                const bool face = (a.auxtype[c] == 1 && (it.side == 1 || it.side == 3)) ||
+                                  (a.auxtype[c] == 2 && (it.side == 2 || it.side == 4));
+                if (it.side == 1 || it.side == 4) {
+                    al[c] = ac[c];
+                    ar[c] = face ? vface : vf;
+                } else {
+                    al[c] = vf;
+                    ar[c] = face ? vf : ac[c];
+                }
+            }
+        }
         typename Sol::Cache cache = Sol::prep(ph, ixy, ql, qr);
         double wave[NW][NV], s[NW], amdq[NV], apdq[NV];
-        Sol::rpn2(ph, ixy, ql, qr, nullptr, nullptr, cache, wave, s, amdq, apdq);
+        Sol::rpn2(ph, ixy, ql, qr, SNA > 0 ? al : nullptr, SNA > 0 ? ar : nullptr, cache, wave, s,
+                  amdq, apdq);
 #pragma unroll
         for (int m = 0; m < NV; m++) {
             if (it.side == 1 || it.side == 4)

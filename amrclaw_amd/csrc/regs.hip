@@ -18,12 +18,15 @@ __global__ __launch_bounds__(256) void upbnd_kernel(UpdateArgs a) {
     const PatchDesc pd = a.patches[c.grid];
     S* q = static_cast<S*>(a.q) + pd.off;
     const int64_t li = (int64_t)c.j * pd.mitot + c.i;
+    // capacity: the sign divided by the coarse cell's capacity (upbnd.f:104-108)
+    const double capa = a.mcapa > 0 ? a.aux_c[pd.off * a.auxk + (int64_t)(a.mcapa - 1) * pd.plane + li] : 1.0;
     for (int m = 0; m < a.nv; m++) {
         double v = (double)q[(int64_t)m * pd.plane + li];
         for (int e = c.e0; e < c.e1; e++) {
             const UpbndEntry en = a.entries[e];
             const double* svd = a.reg + a.fine_patches[en.kid].reg_off + (int64_t)a.nv * (en.lkid - 1);
-            v = v + en.sgnm * svd[m] / a.area;
+            const double sgnm = a.mcapa > 0 ? en.sgnm / capa : en.sgnm;
+            v = v + sgnm * svd[m] / a.area;
         }
         q[(int64_t)m * pd.plane + li] = (S)v;
     }
@@ -50,6 +53,21 @@ __global__ __launch_bounds__(256) void average_kernel(UpdateArgs a, const int64_
     const int64_t lic = (int64_t)(b.cj0 + cj) * pc.mitot + (b.ci0 + ci);
     const int fi = b.fi0 + ci * a.ratx, fj = b.fj0 + cj * a.raty;
     const double totrat = (double)(a.ratx * a.raty);
+    if (a.mcapa > 0) {
+        // update.f:147-161: the capacity-weighted mean
+        const double* capf = a.aux_f + pf.off * a.auxk + (int64_t)(a.mcapa - 1) * pf.plane;
+        const double capc = a.aux_c[pc.off * a.auxk + (int64_t)(a.mcapa - 1) * pc.plane + lic];
+        for (int m = 0; m < a.nv; m++) {
+            double v = 0.0;
+            for (int jco = 0; jco < a.raty; jco++)
+                for (int ico = 0; ico < a.ratx; ico++) {
+                    const int64_t lf = (int64_t)(fj + jco) * pf.mitot + (fi + ico);
+                    v = v + (double)qf[(int64_t)m * pf.plane + lf] * capf[lf];
+                }
+            q[(int64_t)m * pc.plane + lic] = (S)(v / (totrat * capc));
+        }
+        return;
+    }
     for (int m = 0; m < a.nv; m++) {
         double v = 0.0;
         for (int jco = 0; jco < a.raty; jco++)
@@ -74,17 +92,22 @@ __device__ __forceinline__ double lane_value(double v, int k) {
 // sequential FP64 sum bit for bit.  A diagnostic: one wave, latency-bound.
 template <typename S>
 __global__ __launch_bounds__(64) void conck_kernel(const PatchDesc* patches, int npatch,
-                                                   const void* qv, double* out) {
+                                                   const void* qv, double* out, const double* aux,
+                                                   int mcapa, int auxk) {
     const int lane = threadIdx.x;
     double tot = 0.0;
     for (int p = 0; p < npatch; p++) {
         const PatchDesc pd = patches[p];
         const S* q = static_cast<const S*>(qv) + pd.off;
+        // capacity: q(1) * capa (conck.f:56-60)
+        const double* ca = mcapa > 0 ? aux + pd.off * auxk + (int64_t)(mcapa - 1) * pd.plane : nullptr;
         for (int j = 0; j < pd.my; j++) {
             const S* row = q + (int64_t)(j + NGHOST) * pd.mitot + NGHOST;
+            const double* crow = ca ? ca + (int64_t)(j + NGHOST) * pd.mitot + NGHOST : nullptr;
             for (int i0 = 0; i0 < pd.mx; i0 += 64) {
                 const int n = min(64, pd.mx - i0);
-                const double v = lane < n ? (double)row[i0 + lane] : 0.0;
+                const double v = lane < n ? (crow ? (double)row[i0 + lane] * crow[i0 + lane]
+                                                  : (double)row[i0 + lane]) : 0.0;
                 for (int k = 0; k < n; k++) tot = tot + lane_value(v, k);
             }
         }
@@ -93,12 +116,14 @@ __global__ __launch_bounds__(64) void conck_kernel(const PatchDesc* patches, int
 }
 
 hipError_t launch_conck(int esize, const PatchDesc* patches, int npatch, const void* q,
-                        double* out, hipStream_t st) {
+                        double* out, hipStream_t st, const double* aux, int mcapa, int auxk) {
     if (npatch <= 0) return hipSuccess;
     if (esize == 8)
-        hipLaunchKernelGGL(conck_kernel<double>, dim3(1), dim3(64), 0, st, patches, npatch, q, out);
+        hipLaunchKernelGGL(conck_kernel<double>, dim3(1), dim3(64), 0, st, patches, npatch, q, out,
+                           aux, mcapa, auxk);
     else
-        hipLaunchKernelGGL(conck_kernel<float>, dim3(1), dim3(64), 0, st, patches, npatch, q, out);
+        hipLaunchKernelGGL(conck_kernel<float>, dim3(1), dim3(64), 0, st, patches, npatch, q, out,
+                           aux, mcapa, auxk);
     return hipGetLastError();
 }
 
@@ -107,6 +132,7 @@ hipError_t launch_finereg(int solver, int esize, const QadArgs& a, const Physics
     case SOLVER_ADVECTION: return finereg_launch<Advection<double>>(esize, a, ph, st);
     case SOLVER_ACOUSTICS: return finereg_launch<Acoustics<double>>(esize, a, ph, st);
     case SOLVER_EULER: return finereg_launch<EulerRoe<double>>(esize, a, ph, st);
+    case SOLVER_VCADV: return finereg_launch<VcAdvection<double>>(esize, a, ph, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -46,6 +46,12 @@ struct QadArgs {
     // the grid edges are the fluxes themselves and qad's Riemann problems,
     // which turn fluctuations into fluxes, are skipped
     int32_t flux_form;
+    // aux arrays of the level (FP64, naux components, patch p at
+    // patches[p].off * auxk) and their types (AMRGPU_AUX_*); naux 0: none
+    int32_t naux;
+    const double* aux;
+    int32_t auxk;
+    int32_t auxtype[8];
     int32_t pad;
 };
 
@@ -63,6 +69,11 @@ struct UpdateArgs {
     const void* qf;                 // fine level, new time
     const double* reg;              // fine level register arena
     double area;                    // hxposs*hyposs of the coarse level
+    // capacity (mcapa > 0): the coarse and fine levels' aux arrays (FP64,
+    // patch p at off * auxk), the capacity's component (1-based)
+    const double* aux_c;
+    const double* aux_f;
+    int32_t mcapa, auxk;
 };
 
 }  // namespace amrgpu

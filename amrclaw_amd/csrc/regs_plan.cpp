@@ -65,8 +65,10 @@ bool plan_registers(const HierGeom& H, int nvar, RegLevel* R, std::string& err) 
             int lenbc = 2 * (nx / C.ratx + ny / C.raty);
             Rl.reg_off.push_back(Rl.reg_elems);
             Rl.lenbc.push_back(lenbc);
-            // svdflx(nvar, lenbc), qc1d(nvar, lenbc), perimeter fluxes
-            Rl.reg_elems += 2 * (int64_t)nvar * lenbc + 2 * (int64_t)nvar * (nx + ny);
+            // svdflx(nvar, lenbc), qc1d(nvar, lenbc), perimeter fluxes and,
+            // with aux arrays, auxc1d(naux, lenbc) (prepf.f:33-48)
+            Rl.reg_elems += 2 * (int64_t)nvar * lenbc + 2 * (int64_t)nvar * (nx + ny) +
+                            (int64_t)H.naux * lenbc;
             Rl.reg_elems = (Rl.reg_elems + 31) / 32 * 32;
             const int nyc = ny / C.raty, nxc = nx / C.ratx;
             for (int s = 1; s <= nyc; s++) Rl.qad.push_back({(int)k, 1, s, s});

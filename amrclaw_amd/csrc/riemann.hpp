@@ -9,8 +9,13 @@
 //   ql, qr   the reference's qr(:,i-1) and ql(:,i): the states left and
 //            right of interface i (AMRClaw passes one slice for both,
 //            flux2.f:125-126)
-//   auxl/auxr, aux1..3   aux rows (step2.f90:98-102); none of the solvers
-//            here reads aux, the parameters keep the contract's shape
+//   auxl/auxr the aux values (naux each) of the cells left and right of
+//            the interface; aux1..3 those of the cell a transverse
+//            fluctuation entered and of its neighbours below and above it
+//            (x-slice; left and right of it for a y-slice), the entries
+//            step2.f90:98-102 hands to rpt2 at index i-2+imp.  Solvers
+//            that read aux declare NAUX (the components they read);
+//            the others receive null pointers
 //   outputs  wave[mw][m], s[mw], amdq[m], apdq[m]; bmasdq[m], bpasdq[m]
 // A solver also defines Cache/prep(): quantities both rpn2 and the two rpt2
 // calls at one interface derive from the same two states (Roe averages).
@@ -28,7 +33,13 @@
 
 namespace amrgpu {
 
-enum SolverId { SOLVER_ADVECTION = 1, SOLVER_ACOUSTICS = 2, SOLVER_EULER = 3 };
+enum SolverId { SOLVER_ADVECTION = 1, SOLVER_ACOUSTICS = 2, SOLVER_EULER = 3, SOLVER_VCADV = 4 };
+
+// the aux components a solver reads (Sol::NAUX when it declares one, else 0)
+template <class S, class = void> struct sol_naux { static constexpr int value = 0; };
+template <class S> struct sol_naux<S, decltype((void)S::NAUX, void())> {
+    static constexpr int value = S::NAUX;
+};
 
 // number of constants a plug-in solver may read (Physics::user)
 constexpr int SOLVER_PARAMS_MAX = 16;
@@ -106,6 +117,42 @@ template <typename T, class Mth = ExactMath<T>> struct Advection {
         T stran = (ixy == 1) ? T(ph.vbar) : T(ph.ubar);
         bm[0] = dmin1(stran, T(0)) * asdq[0];
         bp[0] = dmax1(stran, T(0)) * asdq[0];
+    }
+};
+
+// ----------------------------------------------------------------------------
+// Variable-coefficient advection q_t + u q_x + v q_y = 0 with the edge
+// velocities as aux fields (clawpack/riemann's rpn2_vc_advection /
+// rpt2_vc_advection, the solvers of the reference's
+// tests/advection_2d_annulus): aux(1) the x-velocity at a cell's left edge,
+// aux(2) the y-velocity at its bottom edge.  The wave is the jump, moving at
+// the velocity of the interface (the right cell's left / bottom edge); a
+// transverse fluctuation splits by the velocities of the edges below and
+// above the cell it entered (LeVeque 2002 sec. 9.5, 21.2).
+template <typename T, class Mth = ExactMath<T>> struct VcAdvection {
+    template <typename U> using Rebind = VcAdvection<U, ExactMath<U>>;
+    static constexpr int NV = 1, NW = 1, NAUX = 2;
+    using Math = Mth;
+    struct Cache {};
+    __device__ static constexpr bool wave_zero(int, int, int) { return false; }
+    __device__ static Cache prep(const Physics&, int, const T*, const T*) { return {}; }
+    __device__ static void rpn2(const Physics&, int ixy, const T* ql, const T* qr, const T*,
+                                const T* auxr, const Cache&, T wave[NW][NV], T s[NW],
+                                T amdq[NV], T apdq[NV]) {
+        wave[0][0] = qr[0] - ql[0];
+        s[0] = auxr[ixy - 1];
+        amdq[0] = dmin1(auxr[ixy - 1], T(0)) * wave[0][0];
+        apdq[0] = dmax1(auxr[ixy - 1], T(0)) * wave[0][0];
+    }
+    // no state-only flux (the velocities live on the edges): this solver runs
+    // in the fluctuation form only (FP64 and FP32 storage)
+    __device__ static void flux(const Physics&, int, const T*, T f[NV]) { f[0] = T(0); }
+    __device__ static void rpt2(const Physics&, int ixy, int, const T*, const T*, const T*,
+                                const T* aux2, const T* aux3, const Cache&, const T asdq[NV],
+                                T bm[NV], T bp[NV]) {
+        const int kv = 3 - ixy;
+        bm[0] = dmin1(aux2[kv - 1], T(0)) * asdq[0];
+        bp[0] = dmax1(aux3[kv - 1], T(0)) * asdq[0];
     }
 };
 

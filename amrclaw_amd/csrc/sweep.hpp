@@ -55,6 +55,11 @@ struct SweepArgs {
     // fields are then unused
     const MarchWave* waves;
     int32_t nwaves;
+    // aux arrays of the level (FP64, naux components per cell; patch p's
+    // component a of cell li at aux[patches[p].off * auxk + a * plane + li])
+    // and the capacity component (1-based, 0: none, step2.f90:91-95)
+    const double* aux;
+    int32_t naux, auxk, mcapa, pad2;
 };
 
 // Perimeter flux block of a fine grid (FP64, after svdflx and qc1d):
@@ -63,6 +68,11 @@ struct SweepArgs {
 // y-edge 1; component fastest.  These are the fluxes fluxad.f:36-92 adds.
 __host__ __device__ inline int64_t perim_offset(const PatchDesc& pd, int nv) {
     return pd.reg_off + 2 * (int64_t)nv * pd.lenbc;
+}
+// auxc1d(naux, lenbc) of a fine grid when its level has aux arrays: the
+// coarse ring's aux values cstore.f saves, after the perimeter block
+__host__ __device__ inline int64_t aux_ring_offset(const PatchDesc& pd, int nv) {
+    return perim_offset(pd, nv) + 2 * (int64_t)nv * (pd.mx + pd.my);
 }
 
 // Neighbour exchange between the lanes of a segment (segments of 16, 32 or 64
@@ -110,6 +120,21 @@ __device__ __forceinline__ T phi_limiter(int mth, T r) {
 }
 
 // Results of one interface lane of a segment.
+// The aux values one lane's interface needs when its level has aux arrays
+// (a capacity, or a solver that reads aux): the solver's components of the
+// cells left and right of the interface and of each one's neighbours across
+// the slice (below / above for an x-slice, left / right for a y-slice: the
+// aux1 / aux3 entries rpt2 reads at i-2+imp, step2.f90:98-102), and the
+// capacity-scaled dt/dx of the two cells (dtdx1d, step2.f90:91-95).
+template <typename T, int NA> struct LaneAux {
+    T l[NA], r[NA];
+    T lm[NA], lp[NA], rm[NA], rp[NA];
+    T dtl, dtr;
+};
+template <class Sol> constexpr int lane_naux() {
+    return sol_naux<Sol>::value > 0 ? sol_naux<Sol>::value : 1;
+}
+
 template <class Sol, typename T> struct LaneResult {
     T fadm[Sol::NV], fadp[Sol::NV];   // faddm / faddp at this interface
     T g1[Sol::NV], g2[Sol::NV];       // gadd(.,1) / gadd(.,2) of the cell right of it
@@ -133,18 +158,25 @@ template <class Sol, typename T> struct LaneResult {
 // STD: the method is known to be the standard one (std_method: second
 // order, both transverse corrections, MC limiter on every wave, no
 // f-waves), so its switches are compile-time constants; same arithmetic.
-template <class Sol, typename T, bool FLUXFORM = false, bool STD = false>
+//
+// AUX: the level has aux arrays; ax holds the lane's values (dtdx is then
+// unused: the two cells' dtdx1d replace it where flux2.f reads them).
+template <class Sol, typename T, bool FLUXFORM = false, bool STD = false, bool AUX = false>
 __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T* ql,
                                            const T* qr, bool inner, int w, T dtdx,
-                                           LaneResult<Sol, T>& o) {
+                                           LaneResult<Sol, T>& o,
+                                           const LaneAux<T, lane_naux<Sol>()>* ax = nullptr) {
     constexpr int NV = Sol::NV, NW = Sol::NW;
+    constexpr bool SAUX = AUX && sol_naux<Sol>::value > 0;   // the solver reads aux
+    const T dtl = AUX ? ax->dtl : dtdx, dtr = AUX ? ax->dtr : dtdx;
     // FP32 arithmetic (no bit-exactness requirement) leaves out the terms of
     // the wave components that are zero by construction and the additions
     // of zero that the reference's sums start from
     constexpr bool SKIPZ = sizeof(T) == 4;
     typename Sol::Cache cache = Sol::prep(M.ph, ixy, ql, qr);
     T wave[NW][NV], s[NW], amdq[NV], apdq[NV];
-    Sol::rpn2(M.ph, ixy, ql, qr, nullptr, nullptr, cache, wave, s, amdq, apdq);
+    Sol::rpn2(M.ph, ixy, ql, qr, SAUX ? ax->l : nullptr, SAUX ? ax->r : nullptr, cache, wave, s,
+              amdq, apdq);
 
     if (FLUXFORM) {
 #pragma unroll
@@ -162,7 +194,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
     T cfl = T(0);
     if (inner) {
 #pragma unroll
-        for (int mw = 0; mw < NW; mw++) cfl = dmax1(dmax1(cfl, dtdx * s[mw]), -dtdx * s[mw]);
+        for (int mw = 0; mw < NW; mw++) cfl = dmax1(dmax1(cfl, dtr * s[mw]), -dtl * s[mw]);
     }
     o.cfl = cfl;
 
@@ -191,7 +223,7 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
                     if (!(SKIPZ && Sol::wave_zero(ixy, mw, m))) wave[mw][m] = wl * wave[mw][m];
             }
         }
-        const T dtdxave = T(0.5) * (dtdx + dtdx);
+        const T dtdxave = T(0.5) * (dtl + dtr);
 #pragma unroll
         for (int m = 0; m < NV; m++) {
             T c = T(0);
@@ -216,9 +248,12 @@ __device__ __forceinline__ void flux2_lane(const MethodArgs& M, int ixy, const T
             }
         }
         T bm1[NV], bp1[NV], bm2[NV], bp2[NV];
-        Sol::rpt2(M.ph, ixy, 1, ql, qr, nullptr, nullptr, nullptr, cache, amdq, bm1, bp1);
-        Sol::rpt2(M.ph, ixy, 2, ql, qr, nullptr, nullptr, nullptr, cache, apdq, bm2, bp2);
-        const T hdt = T(0.5) * dtdx;
+        Sol::rpt2(M.ph, ixy, 1, ql, qr, SAUX ? ax->lm : nullptr, SAUX ? ax->l : nullptr,
+                  SAUX ? ax->lp : nullptr, cache, amdq, bm1, bp1);
+        Sol::rpt2(M.ph, ixy, 2, ql, qr, SAUX ? ax->rm : nullptr, SAUX ? ax->r : nullptr,
+                  SAUX ? ax->rp : nullptr, cache, apdq, bm2, bp2);
+        // the increments land in the cell right of the interface: its dtdx1d
+        const T hdt = T(0.5) * dtr;
 #pragma unroll
         for (int m = 0; m < NV; m++) {
             T nbm = lane_down(bm1[m], w);   // imp=1 result of interface k+1
@@ -270,7 +305,8 @@ __host__ __device__ inline size_t sweep_lds_bytes(int nv, int tx, int ty, bool m
 // one tile of the sweep, run by one workgroup.  LX, LY: compile-time extents
 // of the LDS layout (the tile caps; every tile fits in them), so that the
 // LDS strides are constants; 0: the tile's own extents
-template <class Sol, typename T, typename S, int LX = 0, int LY = 0, bool STD = false>
+template <class Sol, typename T, typename S, int LX = 0, int LY = 0, bool STD = false,
+          bool AUX = false>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                                            unsigned char* smem_raw) {
     constexpr int NV = Sol::NV;
@@ -297,6 +333,28 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
 
     const int tid = threadIdx.x;
     const S* qin = static_cast<const S*>(a.qin) + pd.off;
+    // aux (AUX): component c of the patch's ghost-origin cell (J, I), and a
+    // lane's LaneAux for the interface between cells (JL, IL) and (JR, IR)
+    // of a slice in direction ixy
+    constexpr int NA = lane_naux<Sol>(), SNA = sol_naux<Sol>::value;
+    const double* auxp = AUX ? a.aux + pd.off * a.auxk : nullptr;
+    auto auxv = [&](int c, int J, int I) -> T {
+        return T(auxp[(int64_t)c * pd.plane + (int64_t)J * pd.mitot + I]);
+    };
+    auto fill_aux = [&](LaneAux<T, NA>& ax, int ixy, int JL, int IL, int JR, int IR, T dt) {
+        const int dJ = ixy == 1 ? 1 : 0, dI = ixy == 1 ? 0 : 1;
+#pragma unroll
+        for (int c = 0; c < SNA; c++) {
+            ax.l[c] = auxv(c, JL, IL);
+            ax.r[c] = auxv(c, JR, IR);
+            ax.lm[c] = auxv(c, JL - dJ, IL - dI);
+            ax.lp[c] = auxv(c, JL + dJ, IL + dI);
+            ax.rm[c] = auxv(c, JR - dJ, IR - dI);
+            ax.rp[c] = auxv(c, JR + dJ, IR + dI);
+        }
+        ax.dtl = a.mcapa > 0 ? dt / auxv(a.mcapa - 1, JL, IL) : dt;   // step2.f90:91-95
+        ax.dtr = a.mcapa > 0 ? dt / auxv(a.mcapa - 1, JR, IR) : dt;
+    };
     // mixed path: FP64 physical flux of the stored state at patch-local
     // linear index li (x-flux for ixy = 1, y-flux for 2)
     auto phys_flux = [&](int ixy, int64_t li, double* f) {
@@ -342,7 +400,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             const bool inner = live && c >= 1 && c <= tx + 1;
             LaneResult<Sol, T> o;
-            flux2_lane<Sol, T, MIXED, STD>(a.meth, 1, ql, qr, inner, w, dtdx, o);
+            LaneAux<T, NA> ax;
+            if constexpr (AUX)
+                fill_aux(ax, 1, td.j0 + rc + 1, td.i0 + cc, td.j0 + rc + 1, td.i0 + cc + 1, dtdx);
+            flux2_lane<Sol, T, MIXED, STD, AUX>(a.meth, 1, ql, qr, inner, w, dtdx, o, &ax);
             cfl = dmax1(cfl, o.cfl);
             if (inner && r >= 1 && r <= ty) {
 #pragma unroll
@@ -391,7 +452,10 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
             }
             const bool inner = live && e >= 1 && e <= ty + 1;
             LaneResult<Sol, T> o;
-            flux2_lane<Sol, T, MIXED, STD>(a.meth, 2, ql, qr, inner, w, dtdy, o);
+            LaneAux<T, NA> ax;
+            if constexpr (AUX)
+                fill_aux(ax, 2, td.j0 + ee, td.i0 + cc + 1, td.j0 + ee + 1, td.i0 + cc + 1, dtdy);
+            flux2_lane<Sol, T, MIXED, STD, AUX>(a.meth, 2, ql, qr, inner, w, dtdy, o, &ax);
             cfl = dmax1(cfl, o.cfl);
             if (inner && c >= 1 && c <= tx) {
 #pragma unroll
@@ -455,7 +519,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
                     T q = qs[(m * qsy + j + 2) * qsx + k + 2];
                     T dx_ = fm[m * fpl + j * fstr + (k + 1)] - fp[m * fpl + j * fstr + k];
                     T dy_ = gm[m * gpl + (j + 1) * gsx + k] - gp[m * gpl + j * gsx + k];
-                    qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
+                    if (AUX && a.mcapa > 0)                 // stepgrid.f:166-172
+                        qout[(int64_t)m * pd.plane + li] =
+                            S(q - (dtdx * dx_ + dtdy * dy_) / auxv(a.mcapa - 1, td.j0 + j + 2, td.i0 + k + 2));
+                    else
+                        qout[(int64_t)m * pd.plane + li] = S(q - dtdx * dx_ - dtdy * dy_);
                 }
             }
         }
@@ -578,10 +646,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& a, const int tile,
 // batched launch: one workgroup per tile of the tile table.  WGS: the
 // workgroups per CU the register allocation is bounded for (a tuning knob:
 // the mixed path is instantiated for 5, 6 and 7)
-template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0, bool STD = false>
+template <class Sol, typename T, typename S, int WGS, int LX = 0, int LY = 0, bool STD = false,
+          bool AUX = false>
 __global__ __launch_bounds__(256, WGS) void sweep_kernel(SweepArgs a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    sweep_tile<Sol, T, S, LX, LY, STD>(a, blockIdx.x, smem_raw);
+    sweep_tile<Sol, T, S, LX, LY, STD, AUX>(a, blockIdx.x, smem_raw);
 }
 
 // persistent launch: a grid sized to fill the device takes tiles from the

@@ -86,6 +86,28 @@ static hipError_t sweep_layout(const SweepArgs& a, size_t lds, hipStream_t st) {
 // The march sweep (a.waves set): four independent waves per workgroup, no
 // LDS.  Its register bound: 2 waves per SIMD (256 VGPRs, the default), or 3
 // with FP32 arithmetic (wgs == 3: 168 VGPRs, part of the state spilled).
+// A level with aux arrays (a capacity or a solver that reads aux): the tile
+// sweep with the run-time layout, per-lane aux loads and the capacity
+// update (FP64 or FP32 storage; the mixed path has no aux variant).
+template <class Sol, typename T, typename S, int WGS>
+static hipError_t sweep_aux_go(const SweepArgs& a, size_t lds, hipStream_t st) {
+    static bool attr_set = false;
+    const void* f = (const void*)sweep_kernel<Sol, T, S, WGS, 0, 0, false, true>;
+    if (!attr_set) {
+        hipFuncAttributes fa{};
+        hipError_t e = hipFuncGetAttributes(&fa, f);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024 - (int)fa.sharedSizeBytes);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    if (a.ntiles == 0) return hipSuccess;
+    hipLaunchKernelGGL((sweep_kernel<Sol, T, S, WGS, 0, 0, false, true>), dim3(a.ntiles), dim3(256),
+                       lds, st, a);
+    return hipGetLastError();
+}
+
 template <class Sol, typename T, typename S, int MWGS, bool STD = false>
 static hipError_t march_go(const SweepArgs& a, hipStream_t st) {
     if (a.nwaves == 0) return hipSuccess;
@@ -97,6 +119,19 @@ static hipError_t march_go(const SweepArgs& a, hipStream_t st) {
 template <template <typename, class> class SolT, typename T, class Mth, typename S, bool FIXED>
 static hipError_t sweep_solver(const SweepArgs& a, size_t lds, hipStream_t st, int wgs) {
     using Sol = SolT<T, Mth>;
+    if constexpr (sizeof(T) == sizeof(S)) {
+        if (a.aux) return sweep_aux_go<Sol, T, S, sizeof(T) == 8 ? 4 : 5>(a, lds, st);
+        if (a.ntiles == 0 && !a.waves) {   // attributes only: the aux variant's too
+            hipError_t e = sweep_aux_go<Sol, T, S, sizeof(T) == 8 ? 4 : 5>(a, lds, st);
+            if (e != hipSuccess) return e;
+        }
+    }
+    if constexpr (sol_naux<Sol>::value > 0) {
+        // a solver that reads aux runs only in the aux sweep
+        (void)lds; (void)st; (void)wgs;
+        return a.ntiles == 0 && !a.waves ? hipSuccess : hipErrorInvalidValue;
+    } else {
+    if (a.aux) return hipErrorInvalidValue;
     if (a.waves) {
         if constexpr (sizeof(T) == 4) {
             if constexpr (FIXED) {
@@ -118,6 +153,7 @@ static hipError_t sweep_solver(const SweepArgs& a, size_t lds, hipStream_t st, i
         (void)wgs;
         return sweep_layout<Sol, T, S, 4, FIXED>(a, lds, st);
     }
+    }
 }
 
 // the library's own solvers, by id
@@ -128,6 +164,7 @@ static hipError_t sweep_dispatch(int solver, const SweepArgs& a, size_t lds, hip
     case SOLVER_ADVECTION: return sweep_solver<Advection, T, Mth, S, false>(a, lds, st, wgs);
     case SOLVER_ACOUSTICS: return sweep_solver<Acoustics, T, Mth, S, false>(a, lds, st, wgs);
     case SOLVER_EULER: return sweep_solver<EulerRoe, T, Mth, S, true>(a, lds, st, wgs);
+    case SOLVER_VCADV: return sweep_solver<VcAdvection, T, Mth, S, false>(a, lds, st, wgs);
     default: return hipErrorInvalidValue;
     }
 }

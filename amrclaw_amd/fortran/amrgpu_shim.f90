@@ -28,6 +28,7 @@ module amrgpu_iface
      real(c_double) :: xlower, xupper, ylower, yupper
      real(c_double) :: ubar, vbar, rho, bulk, gamma
      integer(c_int) :: efix, precision
+     integer(c_int) :: auxtype(8)
   end type amrgpu_config
 
   interface
@@ -63,6 +64,13 @@ module amrgpu_iface
           bind(C, name="amrgpu_put_level")
        import :: c_int, c_double, c_int64_t
        integer(c_int), value :: level, which
+       real(c_double), intent(in) :: alloc_base(*)
+       integer(c_int64_t), intent(in) :: offsets(*)
+     end function
+     integer(c_int) function amrgpu_put_level_aux(level, alloc_base, offsets) &
+          bind(C, name="amrgpu_put_level_aux")
+       import :: c_int, c_double, c_int64_t
+       integer(c_int), value :: level
        real(c_double), intent(in) :: alloc_base(*)
        integer(c_int64_t), intent(in) :: offsets(*)
      end function
@@ -287,7 +295,7 @@ subroutine amrgpu_sync_hierarchy(nvar, naux)
   integer :: level, j, mptr, n, mw
   integer, allocatable :: sig(:)
   integer(c_int), allocatable :: ilo(:), ihi(:), jlo(:), jhi(:)
-  integer(c_int64_t), allocatable :: off1(:), off2(:)
+  integer(c_int64_t), allocatable :: off1(:), off2(:), offa(:)
   real(c_double), allocatable :: cx(:), cy(:)
   integer(c_int) :: isz(maxlv), jsz(maxlv), rx(maxlv), ry(maxlv)
   if (.not. dev_ready) then
@@ -301,6 +309,18 @@ subroutine amrgpu_sync_hierarchy(nvar, naux)
         cfg%mthlim(mw) = mthlim(mw)
      end do
      cfg%mcapa = mcapa
+     ! amr.data's aux_type strings (amr2.f90:346-352) as the library's codes
+     cfg%auxtype = 0
+     do mw = 1, min(naux, 8)
+        select case (trim(auxtype(mw)))
+        case ('xleft')
+           cfg%auxtype(mw) = 1
+        case ('yleft')
+           cfg%auxtype(mw) = 2
+        case ('capacity')
+           cfg%auxtype(mw) = 3
+        end select
+     end do
      cfg%use_fwaves = 0
      if (use_fwaves) cfg%use_fwaves = 1
      cfg%cfl = cfl
@@ -338,7 +358,7 @@ subroutine amrgpu_sync_hierarchy(nvar, naux)
         end if
      end if
      allocate(ilo(max(n,1)), ihi(max(n,1)), jlo(max(n,1)), jhi(max(n,1)))
-     allocate(off1(max(n,1)), off2(max(n,1)), cx(max(n,1)), cy(max(n,1)))
+     allocate(off1(max(n,1)), off2(max(n,1)), offa(max(n,1)), cx(max(n,1)), cy(max(n,1)))
      do j = 1, n
         mptr = listOfGrids(listStart(level) + j - 1)
         ilo(j) = node(ndilo, mptr)
@@ -349,6 +369,7 @@ subroutine amrgpu_sync_hierarchy(nvar, naux)
         cy(j) = rnode(cornylo, mptr)
         off1(j) = node(store1, mptr) - 1          ! 0-based offsets into alloc
         off2(j) = node(store2, mptr) - 1
+        if (naux .gt. 0) offa(j) = node(storeaux, mptr) - 1
      end do
      call amrgpu_check(amrgpu_set_level(level, n, ilo, ihi, jlo, jhi, hxposs(level), &
           hyposs(level)), 'set_level')
@@ -357,10 +378,12 @@ subroutine amrgpu_sync_hierarchy(nvar, naux)
         call amrgpu_check(amrgpu_put_level(level, 0, alloc, off1), 'put_level new')
         if (level .lt. mxnest) call amrgpu_check(amrgpu_put_level(level, 1, alloc, off2), &
              'put_level old')
+        if (naux .gt. 0) call amrgpu_check(amrgpu_put_level_aux(level, alloc, offa), &
+             'put_level_aux')
         call amrgpu_check(amrgpu_set_time(level, rnode(timemult, listOfGrids(listStart(level))), &
              possk(level)), 'set_time')
      end if
-     deallocate(ilo, ihi, jlo, jhi, off1, off2, cx, cy)
+     deallocate(ilo, ihi, jlo, jhi, off1, off2, offa, cx, cy)
      call move_alloc(sig, held(level)%v)
   end do
   ! the coarse side of the registers as prepc.f built it (its kid order is

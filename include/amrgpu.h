@@ -34,7 +34,11 @@ enum {
     AMRGPU_ERR_UNSUPPORTED = 4
 };
 
-enum { AMRGPU_ADVECTION = 1, AMRGPU_ACOUSTICS = 2, AMRGPU_EULER = 3 };
+/* AMRGPU_VCADV: variable-coefficient advection q_t + u q_x + v q_y = 0 with
+ * the edge velocities in aux(1) (x-velocity at a cell's left edge) and aux(2)
+ * (y-velocity at its bottom edge) -- the rpn2/rpt2 of the reference's
+ * tests/advection_2d_annulus; needs naux >= 2 and FP64 or FP32 precision. */
+enum { AMRGPU_ADVECTION = 1, AMRGPU_ACOUSTICS = 2, AMRGPU_EULER = 3, AMRGPU_VCADV = 4 };
 /* Precision of a run.  FP64: the reference's arithmetic, bit for bit.
  * FP32: storage and wave/flux arithmetic in FP32; registers, coarse
  * interpolation, averaging and totals FP64.  MIXED: wave/flux arithmetic in
@@ -51,12 +55,13 @@ typedef struct amrgpu_config {
                             of a registered plug-in solver                    */
     int nvar;            /* num_eqn                                           */
     int mwaves;          /* num_waves                                         */
-    int naux;            /* num_aux (must be 0: no solver here reads aux)    */
+    int naux;            /* num_aux: aux values per cell (0..8)                */
     int nghost;          /* num_ghost (must be 2)                             */
     int method[7];       /* method(1:7): [1]=order, [2]=transverse            */
     int mthlim[8];       /* limiter per wave: 0 none 1 minmod 2 superbee
                             3 van Leer 4 MC 5 Beam-Warming                    */
-    int mcapa;           /* capa_index (must be 0)                            */
+    int mcapa;           /* capa_index: the capacity's aux component (1-based),
+                            0 for none                                        */
     int use_fwaves;
     double cfl;          /* cfl_desired                                       */
     double cflv1;        /* cfl_max                                           */
@@ -67,7 +72,12 @@ typedef struct amrgpu_config {
     double gamma;        /* Euler                                             */
     int efix;            /* Euler entropy fix                                 */
     int precision;       /* AMRGPU_FP64 or AMRGPU_FP32 (bulk arithmetic)     */
+    int auxtype[8];      /* aux_type per component (amr_module auxtype):
+                            AMRGPU_AUX_CENTER, _XLEFT, _YLEFT, _CAPACITY; qad.f
+                            takes the fine grid's face value for xleft /
+                            yleft components                                  */
 } amrgpu_config;
+enum { AMRGPU_AUX_CENTER = 0, AMRGPU_AUX_XLEFT = 1, AMRGPU_AUX_YLEFT = 2, AMRGPU_AUX_CAPACITY = 3 };
 
 /* Initialise (or re-initialise) the library on the current HIP device.
  * Replaces the parameter set-up the reference does in amr2.f90:148-392. */
@@ -263,6 +273,16 @@ int amrgpu_plan_fill(int nlev, const int* ngrids, const int* boxes,
                      int* nbufs, int64_t* nmask, int64_t* nscratch);
 int amrgpu_plan_fill_get(amrgpu_fill_op* ops, amrgpu_fill_buf* bufs,
                          unsigned char* masks);
+
+/* The aux arrays of every grid of `level` (alloc(node(storeaux)): naux
+ * values per cell, component fastest, ghost cells included, as setaux
+ * leaves them), at 0-based element offsets offsets[k] of alloc_base.  The
+ * library keeps them until the level's layout changes (set them again after
+ * every amrgpu_set_level of the level).  The capacity (mcapa) scales dtdx
+ * in the sweep and the update (step2.f90:91-95, stepgrid.f:166-172), the
+ * reflux (upbnd.f:104-108), the averaging (update.f:147-161) and conck
+ * (conck.f:56-60); solvers that read aux get the interface's cells' values. */
+int amrgpu_put_level_aux(int level, const double* alloc_base, const int64_t* offsets);
 
 /* Regrid of one level on the device (gfixup.f:33-98 with filval.f90:3-188,
  * naux = 0): the grids of `level` are replaced by the ngrids new boxes, and

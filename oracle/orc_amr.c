@@ -8,7 +8,8 @@
  *   ORC_E_LEVEL1  filrecur: cells of a level-1 patch left unset (filpatch.f90:168)
  *   ORC_E_SCRATCH filrecur: coarse scratch too small (filpatch.f90:220-227)
  *   ORC_E_TIME    intfil: time not available from a source grid (intfil.f90:197-233)
- *   ORC_E_BC      bc2amr: user (0) or unknown boundary condition (bc2amr.f90:122-154)
+ *   ORC_E_BC      bc2amr: user (0) or unknown boundary condition (bc2amr.f90:122-154), or a
+ *                 solid wall on a system without the velocity component it negates
  *   ORC_E_SPHERE  spherical domains are not restated
  *
  * One guard departs from the literal Fortran: bc2amr on an array lying wholly
@@ -37,6 +38,12 @@ int orc_bc2amr(const orc_params* p, const orc_hier* h, double* val, int nrow,
                int ncol, double hx, double hy, double xlo_patch,
                double xhi_patch, double ylo_patch, double yhi_patch) {
     const int nv = p->meqn;
+    /* a solid wall negates q(2) on the x-sides and q(3) on the y-sides
+     * (:145, :190, :236, :283): a system with fewer components has no wall
+     * condition (the reference's loops would write past the patch) */
+    if (((h->mthbc[0] == 3 || h->mthbc[1] == 3) && nv < 2) ||
+        ((h->mthbc[2] == 3 || h->mthbc[3] == 3) && nv < 3))
+        return ORC_E_BC;
     double hxmarg = hx * .01, hymarg = hy * .01;                 /* :104-105 */
     if (h->xperdom && (h->yperdom || h->spheredom)) return 0;    /* :109 */
     if (xlo_patch < h->xlower - hxmarg) {                         /* :118 */

@@ -34,8 +34,9 @@ def test_config_struct_layout():
     import subprocess
     import tempfile
     src = '#include "%s"\n#include <stdio.h>\n#include <stddef.h>\n' \
-          'int main(void){printf("%%zu %%zu %%zu\\n", sizeof(amrgpu_config),' \
-          ' offsetof(amrgpu_config, gamma), offsetof(amrgpu_config, precision));return 0;}\n' % HDR
+          'int main(void){printf("%%zu %%zu %%zu %%zu\\n", sizeof(amrgpu_config),' \
+          ' offsetof(amrgpu_config, gamma), offsetof(amrgpu_config, precision),' \
+          ' offsetof(amrgpu_config, auxtype));return 0;}\n' % HDR
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "t.c")
         open(c, "w").write(src)
@@ -45,6 +46,7 @@ def test_config_struct_layout():
     assert int(out[0]) == C.sizeof(capi.AmrgpuConfig)
     assert int(out[1]) == capi.AmrgpuConfig.gamma.offset
     assert int(out[2]) == capi.AmrgpuConfig.precision.offset
+    assert int(out[3]) == capi.AmrgpuConfig.auxtype.offset
 
 
 @pytest.mark.parametrize("field,value", [("solver", 9), ("nvar", 7), ("nghost", 3),
@@ -57,6 +59,26 @@ def test_init_rejects_bad_config(field, value):
     buf = C.create_string_buffer(256)
     capi.lib().amrgpu_last_error(buf, 256)
     assert buf.value
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(solver=capi.VCADV, naux=1), "aux(1:2)"),
+    (dict(solver=capi.VCADV, naux=3, mcapa=3, precision=capi.MIXED), "mixed"),
+    (dict(solver=capi.VCADV, naux=3, mcapa=4), "mcapa"),
+    (dict(solver=capi.VCADV, naux=3, auxtype=(1, 2, 7)), "auxtype"),
+    (dict(solver=capi.ADVECTION, naux=9), "naux"),
+    (dict(solver=capi.ADVECTION, mthbc=(3, 3, 1, 1)), "wall"),
+    (dict(solver=capi.VCADV, naux=2, mthbc=(1, 1, 3, 3)), "wall")])
+def test_init_rejects_bad_aux_config(kw, msg):
+    """Aux and capacity settings the library does not run are refused at
+    init with a message (the mixed precision has no aux sweep)."""
+    solver = kw.pop("solver")
+    cfg = capi.make_config(solver, **kw)
+    rc = capi.lib().amrgpu_init(C.byref(cfg))
+    assert rc != 0
+    buf = C.create_string_buffer(256)
+    capi.lib().amrgpu_last_error(buf, 256)
+    assert msg in buf.value.decode().lower()
 
 
 def test_calls_before_init_fail_cleanly():

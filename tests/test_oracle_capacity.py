@@ -182,3 +182,18 @@ def test_capacity_composite_mass_is_conserved():
         berger_oliger(O, 1, H.nlev, 2)
         m = O.conck(1)
         assert abs(m - m0) <= 1e-13 * abs(m0), (m, m0)
+
+
+def test_wall_needs_the_velocity_component():
+    """bc2amr's solid wall negates q(2) on the x-sides and q(3) on the
+    y-sides: the oracle refuses it for the one-equation solver instead of
+    writing past the patch (the reference's loops would)."""
+    from tests.common import berger_oliger, nested_hierarchy, positive_states, vc_annulus_aux
+    rng = np.random.default_rng(3)
+    H = nested_hierarchy(rng, positive_states, 1, mthbc=(3, 3, 3, 3), dt1=0.008, naux=3)
+    vc_annulus_aux(H)
+    p = orc.make_params(orc.VCADV, limiter=3, maux=3, mcapa=3,
+                        auxtype=(orc.AUX_XLEFT, orc.AUX_YLEFT, orc.AUX_CAPACITY))
+    O = orc.OracleAMR(p, H)
+    with pytest.raises(RuntimeError):
+        berger_oliger(O, 1, H.nlev, 2)
