@@ -414,6 +414,16 @@ subroutine amrgpu_install_gauges()
   ! install them at the first step that samples them
   if (gauges_ready .or. num_gauges .eq. 0) return
   gauges_ready = .true.
+  ! the device samples q only (gauges_module.f90:409-417 samples aux
+  ! components too when gauges.data asks for them)
+  do i = 1, num_gauges
+     if (allocated(gauges(i)%aux_out_vars)) then
+        if (any(gauges(i)%aux_out_vars)) then
+           print *, 'amrgpu: gauge output of aux components is not sampled on the device'
+           stop 1
+        end if
+     end if
+  end do
   allocate(x(num_gauges), y(num_gauges), t0(num_gauges), t1(num_gauges), dtm(num_gauges))
   do i = 1, num_gauges
      x(i) = gauges(i)%x

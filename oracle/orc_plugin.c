@@ -24,10 +24,10 @@ void orc_driver_physics(const int* solver, const double* ubar, const double* vba
                         const double* rho, const double* bulk, const double* gamma,
                         const int* efix) {
     orc_params p = {0};
-    static const int nv[5] = {0, 1, 3, 4, 3}, nw[5] = {0, 1, 2, 4, 3};
+    static const int nv[6] = {0, 1, 3, 4, 3, 1}, nw[6] = {0, 1, 2, 4, 3, 1};
     p.solver = *solver;
-    p.meqn = (*solver >= 1 && *solver <= 4) ? nv[*solver] : 0;
-    p.mwaves = (*solver >= 1 && *solver <= 4) ? nw[*solver] : 0;
+    p.meqn = (*solver >= 1 && *solver <= 5) ? nv[*solver] : 0;
+    p.mwaves = (*solver >= 1 && *solver <= 5) ? nw[*solver] : 0;
     p.mbc = 2;
     p.ubar = *ubar;
     p.vbar = *vbar;

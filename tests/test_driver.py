@@ -115,3 +115,35 @@ def test_reference_restart_reproduces_uninterrupted_run(tmp_path):
     for kind in ("q", "b", "t"):
         assert open(rdir / ("fort.%s0003" % kind), "rb").read() == \
             open(full / ("fort.%s0002" % kind), "rb").read(), kind
+
+
+ANNULUS_GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                            "advection_2d_annulus")
+
+
+@pytest.mark.skipif(not os.path.exists(D.EXE["annulus_ref"]),
+                    reason="annulus driver not built (make -C oracle annulus)")
+def test_annulus_reference_driver_against_regression_gauges(tmp_path):
+    """tests/advection_2d_annulus (capacity, aux xleft / yleft, periodic
+    theta) run by the reference's program with its own setaux / qinit and
+    the oracle's variable-coefficient Riemann solver.  The regression data
+    (tests/golden/advection_2d_annulus, written in 2016 by an earlier
+    amrclaw with gfortran) are matched within clawpack's gauge tolerance
+    (rtol 1e-14, atol 1e-8) where the runs share their grids: gauge 2 for
+    its first 92 records (t <= 0.384, levels 1 and 2), after which a
+    regrid decision differs and the gauge moves to another level; gauge 1
+    (at the corner of the level-3 region) for its first 17 records and to
+    within 1e-3 (values up to 0.99) for the whole run, on the same
+    record times and levels."""
+    r = D.run("annulus_ref", D.annulus(), str(tmp_path), threads=2)
+    assert r.returncode == 0, open(tmp_path / "run.log").read()[-2000:]
+    got = {g: np.loadtxt(tmp_path / ("gauge%05d.txt" % g), comments="#") for g in (1, 2)}
+    want = {g: np.loadtxt(os.path.join(ANNULUS_GOLD, "gauge%05d.txt" % g), comments="#")
+            for g in (1, 2)}
+    a, b = got[2][:92], want[2][:92]
+    assert np.array_equal(a[:, :2], b[:, :2])
+    assert np.allclose(a[:, 2], b[:, 2], rtol=1e-14, atol=1e-8)
+    a, b = got[1], want[1]
+    assert a.shape == b.shape and np.array_equal(a[:, :2], b[:, :2])
+    assert np.allclose(a[:17, 2], b[:17, 2], rtol=1e-14, atol=1e-8)
+    assert np.abs(a[:, 2] - b[:, 2]).max() < 1e-3
