@@ -46,6 +46,7 @@ def lib():
         L.amrgpu3_put_patch.argtypes = [C.c_int, _dp]
         L.amrgpu3_get_patch.argtypes = [C.c_int, _dp]
         L.amrgpu3_step_patches.argtypes = [C.c_double] * 4 + [C.c_void_p]
+        L.amrgpu3_set_step_mode.argtypes = [C.c_int]
         L.amrgpu3_get_fluxes.argtypes = [C.c_int] + [C.c_void_p] * 6
         L.amrgpu3_stepgrid.argtypes = [C.c_int] * 3 + [_dp, _dp] + [C.c_double] * 4 + \
                                       [C.c_void_p] * 6 + [dpo, dpo]
@@ -138,6 +139,15 @@ def step_patches3(dx, dy, dz, dt, npatch=None):
     cfl = np.zeros(npatch, dtype=np.float64)
     _check(lib().amrgpu3_step_patches(dx, dy, dz, dt, cfl.ctypes.data_as(C.c_void_p)))
     return cfl
+
+
+STEP_PHASES, STEP_FUSED = 0, 1
+
+
+def set_step_mode3(mode):
+    """amrgpu3_set_step_mode: STEP_PHASES (flux arrays in HBM, FP64 bit-exact)
+    or STEP_FUSED (FP32, every face on chip; the FP32 default)."""
+    _check(lib().amrgpu3_set_step_mode(int(mode)))
 
 
 def get_fluxes3(p, shape):

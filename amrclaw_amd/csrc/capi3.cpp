@@ -41,6 +41,15 @@ struct Context3 {
     void* d_q = nullptr;
     void* d_face = nullptr;
     int64_t q_elems = 0, face_elems = 0;
+    // the fused FP32 step (sweep3_tile.hpp): its tiles, the buffer it writes
+    // (swapped with d_q after each step; both hold the ghost cells put_patch
+    // wrote, which the step never changes) and its LDS size
+    int step_mode = 0;                  // 0 phase kernels, 1 fused tiles
+    bool fluxes_valid = false;          // the last step kept the flux arrays
+    std::vector<Tile3> tiles;
+    Tile3* d_tiles = nullptr;
+    void* d_q2 = nullptr;
+    size_t tile_lds = 0;
     unsigned long long* d_cfl = nullptr;
     std::vector<unsigned long long> h_cfl;
     // flagging (flag2refine) and conck state
@@ -105,6 +114,13 @@ void free_batch() {
     hipFree(g3.d_patches);
     hipFree(g3.d_q);
     hipFree(g3.d_face);
+    hipFree(g3.d_q2);
+    hipFree(g3.d_tiles);
+    g3.d_q2 = nullptr;
+    g3.d_tiles = nullptr;
+    g3.tiles.clear();
+    g3.tile_lds = 0;
+    g3.fluxes_valid = false;
     hipFree(g3.d_cfl);
     hipFree(g3.d_flags);
     hipFree(g3.d_foff);
@@ -208,6 +224,51 @@ void plan_blocks_of(const std::vector<Patch3>& patches, int pbk, std::vector<Blo
 
 void plan_blocks() { plan_blocks_of(g3.patches, g3.plane_block, g3.blocks); }
 
+// n cells as near-equal parts of at most TILE3_MAX: (start, length)
+void split_tile3(int n, std::vector<std::pair<int, int>>& out) {
+    out.clear();
+    const int k = (n + TILE3_MAX - 1) / TILE3_MAX;
+    int s = 0;
+    for (int i = 0; i < k; i++) {
+        const int len = n / k + (i < n % k ? 1 : 0);
+        out.emplace_back(s, len);
+        s += len;
+    }
+}
+
+// the fused step's tiles, largest first, and the LDS the largest needs
+void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>& tiles,
+                 size_t& lds) {
+    tiles.clear();
+    lds = 0;
+    std::vector<std::pair<int, int>> sx, sy, sz;
+    for (int p = 0; p < (int)patches.size(); p++) {
+        const Patch3& pt = patches[p];
+        split_tile3(pt.mx, sx);
+        split_tile3(pt.my, sy);
+        split_tile3(pt.mz, sz);
+        for (auto& z : sz)
+            for (auto& y : sy)
+                for (auto& x : sx) {
+                    Tile3 t{};
+                    t.patch = p;
+                    t.i0 = (int16_t)x.first;
+                    t.j0 = (int16_t)y.first;
+                    t.k0 = (int16_t)z.first;
+                    t.tx = (int16_t)x.second;
+                    t.ty = (int16_t)y.second;
+                    t.tz = (int16_t)z.second;
+                    tiles.push_back(t);
+                    const size_t b = (size_t)nv * ((size_t)(t.tx + 4) * (t.ty + 4) * (t.tz + 4) +
+                                                   (size_t)t.tx * t.ty * t.tz) * sizeof(float);
+                    lds = std::max(lds, b);
+                }
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [](const Tile3& a, const Tile3& b) {
+        return a.tx * a.ty * a.tz > b.tx * b.ty * b.tz;
+    });
+}
+
 int define_batch(int npatch, const int* mx, const int* my, const int* mz) {
     free_batch();
     const int nv = g3.cfg.nvar;
@@ -235,6 +296,15 @@ int define_batch(int npatch, const int* mx, const int* my, const int* mz) {
     HIPCHK3(hipMalloc(&g3.d_face, std::max<int64_t>(1, fo) * g3.esize));
     HIPCHK3(hipMemsetAsync(g3.d_q, 0, std::max<int64_t>(1, qo) * g3.esize, g3.stream));
     HIPCHK3(hipMemsetAsync(g3.d_face, 0, std::max<int64_t>(1, fo) * g3.esize, g3.stream));
+    if (g3.esize == 4) {
+        // the fused step writes the second buffer
+        plan_tiles3(g3.patches, nv, g3.tiles, g3.tile_lds);
+        HIPCHK3(hipMalloc(&g3.d_q2, std::max<int64_t>(1, qo) * g3.esize));
+        HIPCHK3(hipMemsetAsync(g3.d_q2, 0, std::max<int64_t>(1, qo) * g3.esize, g3.stream));
+        HIPCHK3(hipMalloc(&g3.d_tiles, std::max<size_t>(1, g3.tiles.size()) * sizeof(Tile3)));
+        if (!g3.tiles.empty())
+            HIPCHK3(h2d3(g3.d_tiles, g3.tiles.data(), g3.tiles.size() * sizeof(Tile3)));
+    }
     HIPCHK3(hipMalloc(&g3.d_cfl, std::max(1, npatch) * sizeof(unsigned long long)));
     for (int k = 0; k < NKIND; k++) {
         const auto& b = g3.blocks[k];
@@ -261,6 +331,8 @@ int put_q(int p, const double* q) {
         for (int64_t c = 0; c < cells; c++)
             for (int m = 0; m < nv; m++) h[m * cells + c] = (float)q[c * nv + m];
         HIPCHK3(h2d3(static_cast<float*>(g3.d_q) + pt.qoff, h.data(), h.size() * 4));
+        if (g3.d_q2)   // the fused step's output buffer keeps the same ghost cells
+            HIPCHK3(h2d3(static_cast<float*>(g3.d_q2) + pt.qoff, h.data(), h.size() * 4));
     }
     return AMRGPU_OK;
 }
@@ -333,7 +405,51 @@ int step_io(const Step3IO& io, double dx, double dy, double dz, double dt, int e
     return AMRGPU_OK;
 }
 
-int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
+// the fused FP32 step of the batch: d_q -> d_q2 tile by tile, then swap
+int step_fused(double dx, double dy, double dz, double dt) {
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !(dt >= 0))
+        return fail3(AMRGPU_ERR_ARG, "bad mesh widths or time step");
+    if (g3.tile_lds > 160 * 1024)
+        return fail3(AMRGPU_ERR_UNSUPPORTED, "tile LDS %zu B exceeds a CU's", g3.tile_lds);
+    HIPCHK3(hipMemsetAsync(g3.d_cfl, 0, std::max(1, g3.npatch) * sizeof(unsigned long long),
+                           g3.stream));
+    Tile3Args a{};
+    a.qin = g3.d_q;
+    a.qout = g3.d_q2;
+    a.patches = g3.d_patches;
+    a.tiles = g3.d_tiles;
+    a.ntiles = (int)g3.tiles.size();
+    a.nv = g3.cfg.nvar;
+    a.dtd[0] = dt / dx;
+    a.dtd[1] = dt / dy;
+    a.dtd[2] = dt / dz;
+    a.meth = g3.meth;
+    a.cfl_patch = g3.d_cfl;
+    hipEvent_t stop;
+    int rc = tbegin(0, &stop);
+    if (rc) return rc;
+    HIPCHK3(launch_sweep3_tile_f32(g3.cfg.solver, a, g3.tile_lds, g3.stream));
+    rc = tend(stop);
+    if (rc) return rc;
+    std::swap(g3.d_q, g3.d_q2);
+    g3.fluxes_valid = false;
+    return AMRGPU_OK;
+}
+
+int step_batch(double dx, double dy, double dz, double dt, double* cflgrid, bool fused) {
+    if (fused) {
+        int rc = step_fused(dx, dy, dz, dt);
+        if (rc) return rc;
+        if (cflgrid) {
+            HIPCHK3(d2h3(g3.h_cfl.data(), g3.d_cfl, g3.npatch * sizeof(unsigned long long)));
+            for (int p = 0; p < g3.npatch; p++) {
+                double v;
+                std::memcpy(&v, &g3.h_cfl[p], sizeof v);
+                cflgrid[p] = v;
+            }
+        }
+        return AMRGPU_OK;
+    }
     Step3IO io{};
     io.q = g3.d_q;
     io.face = g3.d_face;
@@ -346,6 +462,7 @@ int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
     io.d_cfl = g3.d_cfl;
     int rc = step_io(io, dx, dy, dz, dt);
     if (rc) return rc;
+    g3.fluxes_valid = true;
     if (cflgrid) {
         HIPCHK3(d2h3(g3.h_cfl.data(), g3.d_cfl, g3.npatch * sizeof(unsigned long long)));
         for (int p = 0; p < g3.npatch; p++) {
@@ -358,6 +475,9 @@ int step_batch(double dx, double dy, double dz, double dt, double* cflgrid) {
 }
 
 int get_fluxes(int p, double* const out[6]) {
+    if (!g3.fluxes_valid)
+        return fail3(AMRGPU_ERR_STATE, "no flux arrays: the last step was the fused one, which "
+                                       "keeps every face on chip (amrgpu3_set_step_mode(0))");
     const Patch3& pt = g3.patches[p];
     const int nv = g3.cfg.nvar;
     const Faces3 F = faces3(pt, nv);
@@ -527,6 +647,7 @@ int amrgpu3_init(const amrgpu3_config* cfg) {
     if (g3.inited) amrgpu3_finalize();
     g3.cfg = c;
     g3.esize = c.precision == AMRGPU_FP64 ? 8 : 4;
+    g3.step_mode = g3.esize == 4 ? 1 : 0;   // FP32: the fused step by default
     Method3 M{};
     M.ph.ubar = c.ubar;
     M.ph.vbar = c.vbar;
@@ -593,7 +714,18 @@ int amrgpu3_get_patch(int patch, double* q) {
 int amrgpu3_step_patches(double dx, double dy, double dz, double dt, double* cflgrid) {
     int rc = need_init();
     if (rc) return rc;
-    return step_batch(dx, dy, dz, dt, cflgrid);
+    return step_batch(dx, dy, dz, dt, cflgrid, g3.esize == 4 && g3.step_mode == 1);
+}
+
+int amrgpu3_set_step_mode(int mode) {
+    int rc = need_init();
+    if (rc) return rc;
+    if (mode != 0 && mode != 1) return fail3(AMRGPU_ERR_ARG, "step mode must be 0 or 1");
+    if (mode == 1 && g3.esize != 4)
+        return fail3(AMRGPU_ERR_UNSUPPORTED,
+                     "the fused step runs in FP32; FP64 keeps the reference's face order");
+    g3.step_mode = mode;
+    return AMRGPU_OK;
 }
 
 int amrgpu3_get_fluxes(int patch, double* fm, double* fp, double* gm, double* gp, double* hm,
@@ -615,7 +747,9 @@ int amrgpu3_stepgrid(int mx, int my, int mz, const double* q_in, double* q_out, 
     rc = put_q(0, q_in);
     if (rc) return rc;
     double cfl = 0.0;
-    rc = step_batch(dx, dy, dz, dt, &cfl);
+    // flux outputs come from the phase kernels' arrays
+    const bool want = fm && fp && gm && gp && hm && hp;
+    rc = step_batch(dx, dy, dz, dt, &cfl, g3.esize == 4 && g3.step_mode == 1 && !want);
     if (rc) return rc;
     rc = get_q(0, q_out);
     if (rc) return rc;

@@ -55,7 +55,7 @@ MEASURED_TRAFFIC = {
                                            "profiles/level_update_euler_fp32_4096.md"),
     ("euler2d", "mixed", 4096, "march"): (397129 * 2048 + 782660 * 1024,
                                           "profiles/level_update_euler_mixed_4096_march.md"),
-    ("euler3d", "fp32", 512): (
+    ("euler3d", "fp32", 512, "phases"): (
         (271400 + 875200 + 1415000 + 330700) * 2048 + (801800 + 1136000 + 1143000 + 98580) * 1024,
         "profiles/step3_euler_fp32_512.md"),
 }
@@ -113,6 +113,10 @@ def parse():
                          "level update")
     ap.add_argument("--plane-block", type=int, default=0,
                     help="3-D: planes per workgroup of a sweep (0: library default)")
+    ap.add_argument("--step3", choices=["fused", "phases"], default="fused",
+                    help="3-D FP32 step: fused (one kernel per tile, faces on chip; the "
+                         "library's FP32 default) or phases (x, y, z sweep kernels with the "
+                         "flux arrays in HBM, then the update); FP64 always runs phases")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB "
                          "(reference kernel for scaling FETCH_SIZE/WRITE_SIZE)")
@@ -337,9 +341,13 @@ def bench3(a):
     from amrclaw_amd import autotune
     plane_block = a.plane_block or autotune.load_plane_block(p3) or 0
 
+    fused = prec == capi3.FP32 and a.step3 == "fused"
+
     def install(precision):
         capi3.init3(capi3.make_config3(capi3.EULER, precision=precision))
         capi3.set_patches3(sizes, plane_block)
+        if precision == capi3.FP32:
+            capi3.set_step_mode3(capi3.STEP_FUSED if fused else capi3.STEP_PHASES)
         for k, q in enumerate(qs):
             capi3.put_patch3(k, q)
 
@@ -368,7 +376,8 @@ def bench3(a):
     wall, kt = run(a.steps, a.warmup)
     q_main = interiors() if not a.no_compare else None
     capi3.finalize3()
-    names = ("x_sweep", "y_sweep", "z_sweep", "update")
+    # the fused step is one launch, timed in the x-sweep's slot
+    names = ("fused_step",) if fused else ("x_sweep", "y_sweep", "z_sweep", "update")
     kms = {nm: t[0] / max(t[1], 1) for nm, t in zip(names, kt)}
     result = {
         "metric": METRIC,
@@ -379,20 +388,23 @@ def bench3(a):
         "config": {"workload": "batched 3-D Euler step: 512 grids, mx,my,mz even U[8,32] "
                                "seed 20261203, radial bump, Roe+efix, MC, method(3)=22",
                    "patches": len(qs), "cells_per_step": cells, "nghost": 2,
-                   "precision": p3, "launch": "batched",
-                   "plane_block": plane_block or 4},
+                   "precision": p3, "launch": "fused tiles" if fused else "batched",
+                   "step3": "fused" if fused else "phases",
+                   "plane_block": None if fused else (plane_block or 4)},
         "kernels_ms_per_step": kms,
     }
     step_kern = sum(kms.values())
     dom = max(names[:3], key=lambda nm: kms[nm])
     bpc = 2 * 5 * esize
     achieved = cells * bpc / (step_kern / 1e3) / 1e9
-    tr = MEASURED_TRAFFIC.get(("euler3d", p3, len(qs))) if not a.plane_block else None
+    tkey = ("euler3d", p3, len(qs), "fused" if fused else "phases")
+    tr = MEASURED_TRAFFIC.get(tkey) if not a.plane_block else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": tr[0] if tr else None,
                           "traffic_source": tr[1] if tr else None,
-                          "kernel": "sweep3_phase_kernel (x+y+z) + update3_kernel",
+                          "kernel": "sweep3_tile_kernel" if fused else
+                                    "sweep3_phase_kernel (x+y+z) + update3_kernel",
                           "kernel_ms": step_kern, "dominant": dom,
                           "bytes_per_cell_update": bpc,
                           "algorithmic_bytes_per_launch": cells * bpc}

@@ -70,6 +70,16 @@ int amrgpu3_get_patch(int patch, double* q);
  * cflgrid[npatch] receives each grid's Courant number (may be NULL). */
 int amrgpu3_step_patches(double dx, double dy, double dz, double dt, double* cflgrid);
 
+/* How amrgpu3_step_patches runs an FP32 batch.  0: the x-, y- and z-sweep
+ * kernels accumulate the six flux arrays in HBM in step3.f's order, then the
+ * update (the flux arrays stay readable through amrgpu3_get_fluxes).  1 (the
+ * FP32 default): one fused kernel per tile of at most 10 cells per
+ * direction, every face kept on chip and each face value added straight to
+ * the two cells it separates; the same interface arithmetic, summed in
+ * another order (agrees with mode 0 to FP32 rounding); no flux arrays.  FP64
+ * runs mode 0 only (bit for bit the reference). No reference counterpart. */
+int amrgpu3_set_step_mode(int mode);
+
 /* The six flux arrays of the last step of one grid, in stepgrid's shape
  * (nvar, mx+4, my+4, mz+4).  Only faces of interior cells are written:
  * fm/fp at x-faces 1..mx+1 of rows 1..my, 1..mz; gm/gp, hm/hp likewise. */

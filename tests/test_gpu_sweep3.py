@@ -171,3 +171,84 @@ def test_flag_patches3_matches_oracle(gpu, precision):
     m = capi3.conck3(h)
     assert m == orc.mass3(qs, h)   # the sequential sum of the same doubles, bit for bit
     capi3.finalize3()
+
+
+FUSED_SIZES = [(32, 32, 32), (1, 2, 3), (10, 10, 10), (11, 21, 9), (20, 8, 26), (3, 30, 4),
+               (13, 6, 9), (16, 12, 22)]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fused_batch_fp32_within_tolerance(gpu, case):
+    """The fused FP32 step (amrgpu3_set_step_mode(1): one workgroup per tile
+    of at most 10^3 cells, every face on chip, each face value added to the
+    two cells it separates) on a batch of uneven grids, including grids
+    thinner than a tile and grids cut into several tiles per direction:
+    each grid within relative L1 1e-5 of the FP64 oracle for each of two
+    steps (ghost cells kept), and its Courant numbers those of the phase
+    kernels (the same FP32 wave speeds over the same interfaces)."""
+    p, c = params(case, precision=capi3.FP32)
+    capi3.init3(c)
+    rng = np.random.default_rng(700 + CASES.index(case))
+    qs = [patch(rng, case[0], *s) for s in FUSED_SIZES]
+    dx, dy, dz, dt = 1.0 / 40, 1.0 / 36, 1.0 / 44, 0.06 / 44
+    cfls = {}
+    for mode in (capi3.STEP_PHASES, capi3.STEP_FUSED):
+        capi3.set_patches3(FUSED_SIZES)
+        capi3.set_step_mode3(mode)
+        for k, q in enumerate(qs):
+            capi3.put_patch3(k, q)
+        ref = [q.copy() for q in qs]
+        cfls[mode] = []
+        for step in range(2):
+            print("fused3", case, "mode", mode, "step", step, flush=True)
+            cfls[mode].append(capi3.step_patches3(dx, dy, dz, dt, npatch=len(qs)))
+            for k, s in enumerate(FUSED_SIZES):
+                ref[k], _, co, _ = orc.stepgrid3(p, ref[k], dx, dy, dz, dt)
+                got = capi3.get_patch3(k, ref[k].shape)
+                inner = (slice(2, -2),) * 3
+                assert rel_l1(got[inner], ref[k][inner]) <= 1e-5, (mode, step, k, s)
+                # ghost cells are the ones put_patch wrote (stored in FP32)
+                ghost = np.ones(got.shape[:3], dtype=bool)
+                ghost[inner] = False
+                want = qs[k][ghost].astype(np.float32).astype(np.float64)
+                assert np.array_equal(got[ghost], want), (mode, step, k)
+                # after the first step the FP32 states differ at rounding level
+                if step == 0:
+                    assert cfls[mode][0][k] == pytest.approx(co, rel=1e-5), (mode, k)
+    # (the two kernels' FP32 code may contract multiply-adds differently)
+    assert np.allclose(cfls[0][0], cfls[1][0], rtol=1e-6, atol=0)
+    capi3.finalize3()
+
+
+def test_fused_step_keeps_no_flux_arrays(gpu):
+    p, c = params((orc.EULER, 2, 22, 4), precision=capi3.FP32)
+    capi3.init3(c)
+    rng = np.random.default_rng(5)
+    q = patch(rng, orc.EULER, 6, 5, 4)
+    capi3.set_patches3([(6, 5, 4)])
+    capi3.put_patch3(0, q)
+    capi3.step_patches3(0.1, 0.1, 0.1, 0.002)
+    with pytest.raises(capi3.AmrgpuError):
+        capi3.get_fluxes3(0, q.shape)
+    capi3.set_step_mode3(capi3.STEP_PHASES)
+    capi3.step_patches3(0.1, 0.1, 0.1, 0.002)
+    capi3.get_fluxes3(0, q.shape)
+    capi3.finalize3()
+
+
+def test_fused_step_is_deterministic(gpu):
+    """No atomics: two fused steps from the same state give the same bits."""
+    p, c = params((orc.EULER, 2, 22, 4), precision=capi3.FP32)
+    capi3.init3(c)
+    rng = np.random.default_rng(8)
+    qs = [patch(rng, orc.EULER, *s) for s in FUSED_SIZES]
+    out = []
+    for _ in range(2):
+        capi3.set_patches3(FUSED_SIZES)
+        for k, q in enumerate(qs):
+            capi3.put_patch3(k, q)
+        capi3.step_patches3(1.0 / 40, 1.0 / 36, 1.0 / 44, 0.06 / 44)
+        out.append([capi3.get_patch3(k, q.shape) for k, q in enumerate(qs)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    capi3.finalize3()
