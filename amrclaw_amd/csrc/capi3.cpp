@@ -224,10 +224,9 @@ void plan_blocks_of(const std::vector<Patch3>& patches, int pbk, std::vector<Blo
 
 void plan_blocks() { plan_blocks_of(g3.patches, g3.plane_block, g3.blocks); }
 
-// n cells as near-equal parts of at most TILE3_MAX: (start, length)
-void split_tile3(int n, std::vector<std::pair<int, int>>& out) {
+// n cells as k near-equal parts: (start, length)
+void split_tile3(int n, int k, std::vector<std::pair<int, int>>& out) {
     out.clear();
-    const int k = (n + TILE3_MAX - 1) / TILE3_MAX;
     int s = 0;
     for (int i = 0; i < k; i++) {
         const int len = n / k + (i < n % k ? 1 : 0);
@@ -236,17 +235,82 @@ void split_tile3(int n, std::vector<std::pair<int, int>>& out) {
     }
 }
 
-// the fused step's tiles, largest first, and the LDS the largest needs
+// lane slots the fused kernel spends on a tile (its schedule: per sweep,
+// rows of lines in three classes, four waves of chunks of 64 / (tr + 3)
+// lines per round, 256 lanes per round)
+int64_t tile3_slots(int tx, int ty, int tz) {
+    const int t3[3][3] = {{tx, ty, tz}, {ty, tx, tz}, {tz, tx, ty}};
+    int64_t slots = 0;
+    for (const auto& t : t3) {
+        const int tr = t[0], tq = t[1], tp = t[2];
+        const int lpw = 64 / (tr + 3), K = (tq + 2 + lpw - 1) / lpw;
+        for (int c3 = 0; c3 < 3; c3++) {
+            const int nrows = (tp + 4 - c3) / 3, wpr = std::min(nrows, 4);
+            slots += (int64_t)((nrows * K + wpr - 1) / wpr) * 256;
+        }
+    }
+    return slots;
+}
+size_t tile3_bytes(int nv, int tx, int ty, int tz) {
+    return (size_t)nv * ((size_t)(tx + 4) * (ty + 4) * (tz + 4) + (size_t)tx * ty * tz) *
+           sizeof(float);
+}
+
+// the fused step's tiles, largest first, and the LDS the largest needs.  Each
+// grid is cut into kx x ky x kz near-equal tiles of at most TILE3_MAX cells
+// per direction (lines of at most 16 lanes) whose LDS lets two workgroups
+// share a CU; of those cuts, the one with the fewest lane slots.
 void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>& tiles,
                  size_t& lds) {
     tiles.clear();
     lds = 0;
+    const size_t cap = 160 * 1024 / 2;
     std::vector<std::pair<int, int>> sx, sy, sz;
     for (int p = 0; p < (int)patches.size(); p++) {
         const Patch3& pt = patches[p];
-        split_tile3(pt.mx, sx);
-        split_tile3(pt.my, sy);
-        split_tile3(pt.mz, sz);
+        const int n[3] = {pt.mx, pt.my, pt.mz};
+        int kbest[3] = {0, 0, 0};
+        int64_t best = -1;
+        int klo[3], khi[3];
+        for (int a = 0; a < 3; a++) {
+            klo[a] = (n[a] + TILE3_MAX - 1) / TILE3_MAX;
+            khi[a] = std::max(klo[a], (n[a] + 3) / 4);
+        }
+        for (int kx = klo[0]; kx <= khi[0]; kx++)
+            for (int ky = klo[1]; ky <= khi[1]; ky++)
+                for (int kz = klo[2]; kz <= khi[2]; kz++) {
+                    const int k[3] = {kx, ky, kz};
+                    // tile extents: n / k and n / k + 1 (the first n % k parts)
+                    int ext[3][2], cnt[3][2];
+                    for (int a = 0; a < 3; a++) {
+                        ext[a][0] = n[a] / k[a];
+                        ext[a][1] = ext[a][0] + 1;
+                        cnt[a][1] = n[a] % k[a];
+                        cnt[a][0] = k[a] - cnt[a][1];
+                    }
+                    const int mx_ = cnt[0][1] ? ext[0][1] : ext[0][0];
+                    const int my_ = cnt[1][1] ? ext[1][1] : ext[1][0];
+                    const int mz_ = cnt[2][1] ? ext[2][1] : ext[2][0];
+                    if (tile3_bytes(nv, mx_, my_, mz_) > cap) continue;
+                    int64_t slots = 0;
+                    for (int i = 0; i < 2; i++)
+                        for (int j = 0; j < 2; j++)
+                            for (int l = 0; l < 2; l++) {
+                                const int64_t c = (int64_t)cnt[0][i] * cnt[1][j] * cnt[2][l];
+                                if (c) slots += c * tile3_slots(ext[0][i], ext[1][j], ext[2][l]);
+                            }
+                    if (best < 0 || slots < best) {
+                        best = slots;
+                        kbest[0] = kx;
+                        kbest[1] = ky;
+                        kbest[2] = kz;
+                    }
+                }
+        if (best < 0)   // (a cut of tiles of at most 10^3 cells always fits)
+            for (int a = 0; a < 3; a++) kbest[a] = (n[a] + 9) / 10;
+        split_tile3(pt.mx, kbest[0], sx);
+        split_tile3(pt.my, kbest[1], sy);
+        split_tile3(pt.mz, kbest[2], sz);
         for (auto& z : sz)
             for (auto& y : sy)
                 for (auto& x : sx) {
@@ -259,9 +323,7 @@ void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>&
                     t.ty = (int16_t)y.second;
                     t.tz = (int16_t)z.second;
                     tiles.push_back(t);
-                    const size_t b = (size_t)nv * ((size_t)(t.tx + 4) * (t.ty + 4) * (t.tz + 4) +
-                                                   (size_t)t.tx * t.ty * t.tz) * sizeof(float);
-                    lds = std::max(lds, b);
+                    lds = std::max(lds, tile3_bytes(nv, t.tx, t.ty, t.tz));
                 }
     }
     std::stable_sort(tiles.begin(), tiles.end(), [](const Tile3& a, const Tile3& b) {

@@ -37,7 +37,7 @@
 
 namespace amrgpu {
 
-constexpr int TILE3_MAX = 10;        // cells per direction of a tile (lines of <= 13 lanes)
+constexpr int TILE3_MAX = 13;        // cells per direction of a tile (lines of <= 16 lanes)
 constexpr int TILE3_LANES = 256;     // four waves
 
 // a tile: grid `patch`, 0-based first interior cell (i0, j0, k0), extents
@@ -226,17 +226,22 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                 }
             }
             if (trans) {
+                // the splits both families use, once: of cqxx, A+dq and A-dq
+                // along icoor 2 and 3 (index 0: 2, 1: 3; [0] down-, [1] up-going)
+                T Cs[2][2][NV], As[2][2][NV], Ms[2][2][NV];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    if (cq) Sol::split(M.ph, d, 2 + i, ql, qrr, cache, cqxx, Cs[i][0], Cs[i][1]);
+                    Sol::split(M.ph, d, 2 + i, ql, qrr, cache, apdq, As[i][0], As[i][1]);
+                    Sol::split(M.ph, d, 2 + i, ql, qrr, cache, amdq, Ms[i][0], Ms[i][1]);
+                }
 #pragma unroll
                 for (int fam = 0; fam < 2; fam++) {
-                    const int icp = fam == 0 ? 2 : 3;
-                    const int ics = fam == 0 ? 3 : 2;
+                    // fam 0 = gadd: primary split along icoor 2; fam 1 = hadd
+                    const int ip = fam, is = 1 - fam;   // primary, secondary
+                    const int icp = 2 + ip;
                     const T w6 = fam == 0 ? g6 : h6;
-                    T xcm[NV], xcp[NV], scm[NV], scp[NV];
-                    T pm[NV], pp[NV], sm[NV], sp[NV], tmp[NV], rm[NV], rp[NV];
-                    if (cq) {
-                        Sol::split(M.ph, d, icp, ql, qrr, cache, cqxx, xcm, xcp);
-                        if (tt) Sol::split(M.ph, d, ics, ql, qrr, cache, cqxx, scm, scp);
-                    }
+                    T tmp[NV], rm[NV], rp[NV], sm[NV], sp[NV];
                     // right-cell partials of interface e (cell e): slot
                     // k = (s-1)*3 + (o+1), flux3.f:450-505
                     T v[6][NV];
@@ -244,19 +249,19 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                     for (int k = 0; k < 6; k++)
 #pragma unroll
                         for (int m = 0; m < NV; m++) v[k][m] = T(0);
-                    Sol::split(M.ph, d, icp, ql, qrr, cache, apdq, pm, pp);
 #pragma unroll
                     for (int m = 0; m < NV; m++) {
-                        v[1][m] = T(0) - half_t * pm[m];
-                        v[4][m] = T(0) - half_t * pp[m];
+                        v[1][m] = T(0) - half_t * As[ip][0][m];
+                        v[4][m] = T(0) - half_t * As[ip][1][m];
                     }
                     if (tt) {
-                        Sol::split(M.ph, d, ics, ql, qrr, cache, apdq, sm, sp);
 #pragma unroll
-                        for (int m = 0; m < NV; m++) tmp[m] = M.m4 == 2 ? sp[m] - T(3) * scp[m] : sp[m];
+                        for (int m = 0; m < NV; m++)
+                            tmp[m] = M.m4 == 2 ? As[is][1][m] - T(3) * Cs[is][1][m] : As[is][1][m];
                         Sol::split(M.ph, d, icp, ql, qrr, cache, tmp, rm, rp);
 #pragma unroll
-                        for (int m = 0; m < NV; m++) tmp[m] = M.m4 == 2 ? sm[m] - T(3) * scm[m] : sm[m];
+                        for (int m = 0; m < NV; m++)
+                            tmp[m] = M.m4 == 2 ? As[is][0][m] - T(3) * Cs[is][0][m] : As[is][0][m];
                         Sol::split(M.ph, d, icp, ql, qrr, cache, tmp, sm, sp);
 #pragma unroll
                         for (int m = 0; m < NV; m++) {
@@ -271,8 +276,8 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                     if (cq) {
 #pragma unroll
                         for (int m = 0; m < NV; m++) {
-                            v[4][m] = v[4][m] + dtdx * xcp[m];
-                            v[1][m] = v[1][m] + dtdx * xcm[m];
+                            v[4][m] = v[4][m] + dtdx * Cs[ip][1][m];
+                            v[1][m] = v[1][m] + dtdx * Cs[ip][0][m];
                         }
                     }
                     // cell e-1: the left neighbour's right-cell partials,
@@ -281,19 +286,19 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                     for (int k = 0; k < 6; k++)
 #pragma unroll
                         for (int m = 0; m < NV; m++) v[k][m] = lane_up(v[k][m], L);
-                    Sol::split(M.ph, d, icp, ql, qrr, cache, amdq, pm, pp);
 #pragma unroll
                     for (int m = 0; m < NV; m++) {
-                        v[1][m] = v[1][m] - half_t * pm[m];
-                        v[4][m] = v[4][m] - half_t * pp[m];
+                        v[1][m] = v[1][m] - half_t * Ms[ip][0][m];
+                        v[4][m] = v[4][m] - half_t * Ms[ip][1][m];
                     }
                     if (tt) {
-                        Sol::split(M.ph, d, ics, ql, qrr, cache, amdq, sm, sp);
 #pragma unroll
-                        for (int m = 0; m < NV; m++) tmp[m] = M.m4 == 2 ? sp[m] + T(3) * scp[m] : sp[m];
+                        for (int m = 0; m < NV; m++)
+                            tmp[m] = M.m4 == 2 ? Ms[is][1][m] + T(3) * Cs[is][1][m] : Ms[is][1][m];
                         Sol::split(M.ph, d, icp, ql, qrr, cache, tmp, rm, rp);
 #pragma unroll
-                        for (int m = 0; m < NV; m++) tmp[m] = M.m4 == 2 ? sm[m] + T(3) * scm[m] : sm[m];
+                        for (int m = 0; m < NV; m++)
+                            tmp[m] = M.m4 == 2 ? Ms[is][0][m] + T(3) * Cs[is][0][m] : Ms[is][0][m];
                         Sol::split(M.ph, d, icp, ql, qrr, cache, tmp, sm, sp);
 #pragma unroll
                         for (int m = 0; m < NV; m++) {
@@ -308,8 +313,8 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                     if (cq) {
 #pragma unroll
                         for (int m = 0; m < NV; m++) {
-                            v[4][m] = v[4][m] - dtdx * xcp[m];
-                            v[1][m] = v[1][m] - dtdx * xcm[m];
+                            v[4][m] = v[4][m] - dtdx * Cs[ip][1][m];
+                            v[1][m] = v[1][m] - dtdx * Cs[ip][0][m];
                         }
                     }
                     // slot (s, o) is the face Q+s-1 of cell row P+o (type

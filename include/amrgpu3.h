@@ -73,7 +73,7 @@ int amrgpu3_step_patches(double dx, double dy, double dz, double dt, double* cfl
 /* How amrgpu3_step_patches runs an FP32 batch.  0: the x-, y- and z-sweep
  * kernels accumulate the six flux arrays in HBM in step3.f's order, then the
  * update (the flux arrays stay readable through amrgpu3_get_fluxes).  1 (the
- * FP32 default): one fused kernel per tile of at most 10 cells per
+ * FP32 default): one fused kernel per tile of at most 13 cells per
  * direction, every face kept on chip and each face value added straight to
  * the two cells it separates; the same interface arithmetic, summed in
  * another order (agrees with mode 0 to FP32 rounding); no flux arrays.  FP64

@@ -180,7 +180,7 @@ FUSED_SIZES = [(32, 32, 32), (1, 2, 3), (10, 10, 10), (11, 21, 9), (20, 8, 26), 
 @pytest.mark.parametrize("case", CASES)
 def test_fused_batch_fp32_within_tolerance(gpu, case):
     """The fused FP32 step (amrgpu3_set_step_mode(1): one workgroup per tile
-    of at most 10^3 cells, every face on chip, each face value added to the
+    of at most 13 cells per direction, every face on chip, each face value added to the
     two cells it separates) on a batch of uneven grids, including grids
     thinner than a tile and grids cut into several tiles per direction:
     each grid within relative L1 1e-5 of the FP64 oracle for each of two
