@@ -50,6 +50,7 @@ struct Context3 {
     Tile3* d_tiles = nullptr;
     void* d_q2 = nullptr;
     size_t tile_lds = 0;
+    int xcd_start[TILE3_XCDS] = {}, xcd_count[TILE3_XCDS] = {};
     unsigned long long* d_cfl = nullptr;
     std::vector<unsigned long long> h_cfl;
     // flagging (flag2refine) and conck state
@@ -261,10 +262,12 @@ size_t tile3_bytes(int nv, int tx, int ty, int tz) {
 // per direction (lines of at most 16 lanes) whose LDS lets two workgroups
 // share a CU; of those cuts, the one with the fewest lane slots.
 void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>& tiles,
-                 size_t& lds) {
+                 size_t& lds, int xcd_start[TILE3_XCDS], int xcd_count[TILE3_XCDS]) {
     tiles.clear();
     lds = 0;
-    const size_t cap = 160 * 1024 / 2;
+    std::vector<std::vector<Tile3>> per(patches.size());
+    std::vector<int64_t> cost(patches.size(), 0);
+    const size_t cap = 160 * 1024 / TILE3_WG_PER_CU;
     std::vector<std::pair<int, int>> sx, sy, sz;
     for (int p = 0; p < (int)patches.size(); p++) {
         const Patch3& pt = patches[p];
@@ -306,8 +309,8 @@ void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>&
                         kbest[2] = kz;
                     }
                 }
-        if (best < 0)   // (a cut of tiles of at most 10^3 cells always fits)
-            for (int a = 0; a < 3; a++) kbest[a] = (n[a] + 9) / 10;
+        if (best < 0)   // (never: tiles of at most 4 cells per direction always fit)
+            for (int a = 0; a < 3; a++) kbest[a] = (n[a] + 3) / 4;
         split_tile3(pt.mx, kbest[0], sx);
         split_tile3(pt.my, kbest[1], sy);
         split_tile3(pt.mz, kbest[2], sz);
@@ -322,13 +325,32 @@ void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>&
                     t.tx = (int16_t)x.second;
                     t.ty = (int16_t)y.second;
                     t.tz = (int16_t)z.second;
-                    tiles.push_back(t);
+                    per[p].push_back(t);
+                    cost[p] += tile3_slots(t.tx, t.ty, t.tz);
                     lds = std::max(lds, tile3_bytes(nv, t.tx, t.ty, t.tz));
                 }
     }
-    std::stable_sort(tiles.begin(), tiles.end(), [](const Tile3& a, const Tile3& b) {
-        return a.tx * a.ty * a.tz > b.tx * b.ty * b.tz;
-    });
+    // whole grids to one XCD each (workgroups go to the XCDs in turn), so
+    // that neighbouring tiles share their halo rows in that XCD's L2: grids
+    // in decreasing cost to the least loaded XCD, then each XCD's tiles in
+    // one contiguous range, grid by grid
+    std::vector<int> order(patches.size());
+    for (int p = 0; p < (int)order.size(); p++) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&cost](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<int>> bin(TILE3_XCDS);
+    int64_t load[TILE3_XCDS] = {};
+    for (int p : order) {
+        int x = 0;
+        for (int y = 1; y < TILE3_XCDS; y++)
+            if (load[y] < load[x]) x = y;
+        bin[x].push_back(p);
+        load[x] += cost[p];
+    }
+    for (int x = 0; x < TILE3_XCDS; x++) {
+        xcd_start[x] = (int)tiles.size();
+        for (int p : bin[x]) tiles.insert(tiles.end(), per[p].begin(), per[p].end());
+        xcd_count[x] = (int)tiles.size() - xcd_start[x];
+    }
 }
 
 int define_batch(int npatch, const int* mx, const int* my, const int* mz) {
@@ -360,7 +382,7 @@ int define_batch(int npatch, const int* mx, const int* my, const int* mz) {
     HIPCHK3(hipMemsetAsync(g3.d_face, 0, std::max<int64_t>(1, fo) * g3.esize, g3.stream));
     if (g3.esize == 4) {
         // the fused step writes the second buffer
-        plan_tiles3(g3.patches, nv, g3.tiles, g3.tile_lds);
+        plan_tiles3(g3.patches, nv, g3.tiles, g3.tile_lds, g3.xcd_start, g3.xcd_count);
         HIPCHK3(hipMalloc(&g3.d_q2, std::max<int64_t>(1, qo) * g3.esize));
         HIPCHK3(hipMemsetAsync(g3.d_q2, 0, std::max<int64_t>(1, qo) * g3.esize, g3.stream));
         HIPCHK3(hipMalloc(&g3.d_tiles, std::max<size_t>(1, g3.tiles.size()) * sizeof(Tile3)));
@@ -481,6 +503,10 @@ int step_fused(double dx, double dy, double dz, double dt) {
     a.patches = g3.d_patches;
     a.tiles = g3.d_tiles;
     a.ntiles = (int)g3.tiles.size();
+    for (int x = 0; x < TILE3_XCDS; x++) {
+        a.xcd_start[x] = g3.xcd_start[x];
+        a.xcd_count[x] = g3.xcd_count[x];
+    }
     a.nv = g3.cfg.nvar;
     a.dtd[0] = dt / dx;
     a.dtd[1] = dt / dy;

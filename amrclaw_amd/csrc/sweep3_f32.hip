@@ -25,7 +25,10 @@ static hipError_t tile3_go(const Tile3Args& a, size_t lds, hipStream_t st) {
         attr_lds = lds;
     }
     if (a.ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((sweep3_tile_kernel<Sol>), dim3(a.ntiles), dim3(TILE3_LANES), lds, st, a);
+    int most = 0;
+    for (int x = 0; x < TILE3_XCDS; x++) most = a.xcd_count[x] > most ? a.xcd_count[x] : most;
+    hipLaunchKernelGGL((sweep3_tile_kernel<Sol>), dim3(most * TILE3_XCDS), dim3(TILE3_LANES), lds,
+                       st, a);
     return hipGetLastError();
 }
 

@@ -39,6 +39,8 @@ namespace amrgpu {
 
 constexpr int TILE3_MAX = 13;        // cells per direction of a tile (lines of <= 16 lanes)
 constexpr int TILE3_LANES = 256;     // four waves
+constexpr int TILE3_XCDS = 8;        // workgroups go to the XCDs in turn
+constexpr int TILE3_WG_PER_CU = 2;   // workgroups a CU holds (LDS and registers sized for it)
 
 // a tile: grid `patch`, 0-based first interior cell (i0, j0, k0), extents
 struct Tile3 {
@@ -53,6 +55,9 @@ struct Tile3Args {
     const Tile3* tiles;
     int32_t ntiles;
     int32_t nv;
+    // the tiles of XCD x are tiles[xcd_start[x] ..+ xcd_count[x]]: workgroup
+    // b runs on XCD b % 8 and takes its (b / 8)-th tile
+    int32_t xcd_start[TILE3_XCDS], xcd_count[TILE3_XCDS];
     double dtd[3];                  // dt/dx, dt/dy, dt/dz
     Method3 meth;
     unsigned long long* cfl_patch;  // per-patch max Courant number (double bits)
@@ -365,11 +370,13 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
 }
 
 template <class Sol>
-__global__ __launch_bounds__(TILE3_LANES, 2) void sweep3_tile_kernel(Tile3Args a) {
+__global__ __launch_bounds__(TILE3_LANES, TILE3_WG_PER_CU) void sweep3_tile_kernel(Tile3Args a) {
     using T = float;
     constexpr int NV = Sol::NV;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const Tile3 td = a.tiles[blockIdx.x];
+    const int xcd = blockIdx.x % TILE3_XCDS, k = blockIdx.x / TILE3_XCDS;
+    if (k >= a.xcd_count[xcd]) return;   // (the whole workgroup: no barrier reached)
+    const Tile3 td = a.tiles[a.xcd_start[xcd] + k];
     const Patch3 pt = a.patches[td.patch];
     const int tx = td.tx, ty = td.ty, tz = td.tz;
     const int bx = tx + 4, by = ty + 4, bz = tz + 4;

@@ -55,6 +55,8 @@ MEASURED_TRAFFIC = {
                                            "profiles/level_update_euler_fp32_4096.md"),
     ("euler2d", "mixed", 4096, "march"): (397129 * 2048 + 782660 * 1024,
                                           "profiles/level_update_euler_mixed_4096_march.md"),
+    ("euler3d", "fp32", 512, "fused"): (69543 * 2048 + 97207 * 1024,
+                                        "profiles/step3_euler_fp32_512_fused.md"),
     ("euler3d", "fp32", 512, "phases"): (
         (271400 + 875200 + 1415000 + 330700) * 2048 + (801800 + 1136000 + 1143000 + 98580) * 1024,
         "profiles/step3_euler_fp32_512.md"),
@@ -66,6 +68,7 @@ MEASURED_VALU = {
                                           "profiles/level_update_euler_mixed_4096_march.md"),
     ("euler2d", "mixed", 4096, "batched"): (757577540,
                                             "profiles/level_update_euler_mixed_4096.md"),
+    ("euler3d", "fp32", 512, "fused"): (1162758810, "profiles/step3_euler_fp32_512_fused.md"),
 }
 SIMD_COUNT = 1024                 # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9                  # peak engine clock
@@ -399,6 +402,7 @@ def bench3(a):
     achieved = cells * bpc / (step_kern / 1e3) / 1e9
     tkey = ("euler3d", p3, len(qs), "fused" if fused else "phases")
     tr = MEASURED_TRAFFIC.get(tkey) if not a.plane_block else None
+    vr = MEASURED_VALU.get(tkey) if not a.plane_block else None
     result["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": tr[0] if tr else None,
@@ -408,6 +412,13 @@ def bench3(a):
                           "kernel_ms": step_kern, "dominant": dom,
                           "bytes_per_cell_update": bpc,
                           "algorithmic_bytes_per_launch": cells * bpc}
+    if vr:
+        # the fused step is bound by its vector instructions, not by HBM
+        result["roofline"]["valu"] = {
+            "wave_instructions_per_launch": vr[0],
+            "lane_instructions_per_cell": vr[0] * 64 / cells,
+            "issue_frac": vr[0] * VALU_ISSUE_CYCLES / (SIMD_COUNT * step_kern * 1e-3 * CLOCK_HZ),
+            "source": vr[1]}
     if not a.no_compare and prec == capi3.FP32:
         print("bench3: FP64 batched", file=sys.stderr, flush=True)
         install(capi3.FP64)
