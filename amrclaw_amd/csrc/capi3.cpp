@@ -237,14 +237,14 @@ void split_tile3(int n, int k, std::vector<std::pair<int, int>>& out) {
 }
 
 // lane slots the fused kernel spends on a tile (its schedule: per sweep,
-// rows of lines in three classes, four waves of chunks of 64 / (tr + 3)
+// rows of lines in three classes, four waves of chunks of 64 / (tr + 1)
 // lines per round, 256 lanes per round)
 int64_t tile3_slots(int tx, int ty, int tz) {
     const int t3[3][3] = {{tx, ty, tz}, {ty, tx, tz}, {tz, tx, ty}};
     int64_t slots = 0;
     for (const auto& t : t3) {
         const int tr = t[0], tq = t[1], tp = t[2];
-        const int lpw = 64 / (tr + 3), K = (tq + 2 + lpw - 1) / lpw;
+        const int lpw = 64 / (tr + 1), K = (tq + 2 + lpw - 1) / lpw;
         for (int c3 = 0; c3 < 3; c3++) {
             const int nrows = (tp + 4 - c3) / 3, wpr = std::min(nrows, 4);
             slots += (int64_t)((nrows * K + wpr - 1) / wpr) * 256;
@@ -259,7 +259,7 @@ size_t tile3_bytes(int nv, int tx, int ty, int tz) {
 
 // the fused step's tiles, largest first, and the LDS the largest needs.  Each
 // grid is cut into kx x ky x kz near-equal tiles of at most TILE3_MAX cells
-// per direction (lines of at most 16 lanes) whose LDS lets two workgroups
+// per direction whose LDS lets two workgroups
 // share a CU; of those cuts, the one with the fewest lane slots.
 void plan_tiles3(const std::vector<Patch3>& patches, int nv, std::vector<Tile3>& tiles,
                  size_t& lds, int xcd_start[TILE3_XCDS], int xcd_count[TILE3_XCDS]) {

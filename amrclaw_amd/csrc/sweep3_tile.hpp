@@ -10,9 +10,10 @@
 //     LDS once;
 //   * each sweep solves the lines through the tile and one line on every
 //     side of it across the sweep (their transverse increments reach the
-//     tile's cells), tr + 3 lanes per line, one lane per interface:
-//     interfaces -1 .. tr+1 of a tile with tr cells along the line, the two
-//     outer ones only for the limiter's neighbour waves;
+//     tile's cells), tr + 1 lanes per line, one lane per interface:
+//     interfaces 0 .. tr of a tile with tr cells along the line (lanes 0
+//     and tr also solve interfaces -1 and tr+1, whose waves the limiter
+//     needs);
 //   * instead of accumulating faces and differencing them afterwards, every
 //     face contribution goes at once to the two cells it separates, in an
 //     LDS array dq of the tile's cells: a face value F between cells a < b
@@ -37,7 +38,7 @@
 
 namespace amrgpu {
 
-constexpr int TILE3_MAX = 13;        // cells per direction of a tile (lines of <= 16 lanes)
+constexpr int TILE3_MAX = 13;        // cells per direction of a tile
 constexpr int TILE3_LANES = 256;     // four waves
 constexpr int TILE3_XCDS = 8;        // workgroups go to the XCDs in turn
 constexpr int TILE3_WG_PER_CU = 2;   // workgroups a CU holds (LDS and registers sized for it)
@@ -82,7 +83,7 @@ __device__ __forceinline__ void wave_sync() {
 //
 // Schedule: the lines of the sweep form rows of constant P (P = -1 .. tp),
 // each of tq + 2 lines Q = -1 .. tq.  A wave takes a chunk of up to
-// lpw = 64 / (tr + 3) consecutive lines of one row, tr + 3 lanes per line.
+// lpw = 64 / (tr + 1) consecutive lines of one row, tr + 1 lanes per line.
 // The rows are taken in three classes ((P + 1) mod 3); in a round the four
 // waves take chunks of distinct rows of one class, which lie three or more
 // rows apart, so the cells their lines touch (P - 1 .. P + 1) are disjoint.
@@ -124,10 +125,13 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
     const T g6 = sixth * dtdx * dtdz_p;
     const T h6 = sixth * dtdx * dtdy_p;
 
-    const int L = tr + 3;                      // lanes of a line
+    // a line's lanes take its interfaces 0 .. tr; the limiter's outer
+    // neighbours (interfaces -1 and tr+1, waves only) are solved by lanes 0
+    // and tr themselves
+    const int L = tr + 1;                      // lanes of a line
     const int lpw = 64 / L;                    // lines of a chunk
     const int li = wl / L;                     // this lane's line in the chunk
-    const int e = wl - li * L - 1;             // interface e: between cells e-1, e
+    const int e = wl - li * L;                 // interface e: between cells e-1, e
     const int K = (tq + 2 + lpw - 1) / lpw;    // chunks per row
 #pragma unroll 1
     for (int c3 = 0; c3 < 3; c3++) {
@@ -179,6 +183,28 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                             wd[w][m] = lane_down(wave_[w][m], L);
                             wu[w][m] = lane_up(wave_[w][m], L);
                         }
+                    if (live && (e == 0 || e == tr)) {
+                        // the outer neighbour of the line's end: interface -1
+                        // (cells -2, -1) for lane 0, tr+1 (cells tr, tr+1) for
+                        // lane tr (both when tr = 0 cannot happen: tr >= 1)
+                        const int bx0 = e == 0 ? b0 - rs : b0 + rs;
+                        T xl[NV], xr[NV];
+#pragma unroll
+                        for (int m = 0; m < NV; m++) {
+                            xl[m] = qs[m * nbox + bx0];
+                            xr[m] = qs[m * nbox + bx0 + rs];
+                        }
+                        const typename Sol::Cache cx = Sol::prep(M.ph, xl, xr);
+                        T wx[NW][NV], sx_[NW], amx[NV], apx[NV];
+                        Sol::rpn3(M.ph, d, xl, xr, cx, wx, sx_, amx, apx);
+#pragma unroll
+                        for (int w = 0; w < NW; w++)
+#pragma unroll
+                            for (int m = 0; m < NV; m++) {
+                                if (e == 0) wu[w][m] = wx[w][m];
+                                else wd[w][m] = wx[w][m];
+                            }
+                    }
                     if (live && e >= 0 && e <= tr) {
 #pragma unroll
                         for (int w = 0; w < NW; w++) {
