@@ -104,6 +104,7 @@ struct Level {
     // aux arrays of the grids (FP64, see Context::auxk), set by
     // amrgpu_put_level_aux
     double* d_aux = nullptr;
+    bool aux_set = false;               // amrgpu_put_level_aux since the last layout
     double* d_partial = nullptr;        // conck total (one double; sized one per grid)
     // grid corners rnode(cornxlo/cornylo) (default ilo*hx + xlower)
     std::vector<double> cxlo, cylo;
@@ -617,6 +618,7 @@ int define_level(int level, int ngrids, const int* ilo, const int* ihi, const in
         if (g.auxk > 0) {
             HIPCHK(hipMalloc(&L.d_aux, (size_t)L.nelem * g.auxk * sizeof(double)));
             HIPCHK(dzero(L.d_aux, (size_t)L.nelem * g.auxk * sizeof(double)));
+            L.aux_set = false;
         }
         std::vector<int64_t> off(ngrids), cs(ngrids);
         std::vector<int32_t> pl(ngrids);
@@ -932,6 +934,10 @@ int ensure_regs() {
             if ((rc = upload(&L.d_sq_bufs, L.sq.bufs))) return rc;
             if (L.sq.scratch_elems > 0)
                 HIPCHK(hipMalloc(&L.d_sq_scratch, L.sq.scratch_elems * sizeof(double)));
+            // the aux mode's scratch too, here and not in the level step (which
+            // a hipGraph may capture)
+            if (g.auxk > 0 && L.sq.scratch_elems > 0)
+                HIPCHK(hipMalloc(&L.d_sq_aux, L.sq.scratch_elems * g.auxk * sizeof(double)));
         }
         if (!Rl.sv_grid_start.empty()) {
             // fluxsv entries that write their slot, grouped by the tile that
@@ -1093,7 +1099,8 @@ int run_saveqc_aux(int level) {
     const FillPlan& P = F.sq;
     if (P.ops.empty()) return AMRGPU_OK;
     const int64_t n = P.scratch_elems * g.auxk;
-    if (!F.d_sq_aux && n > 0) HIPCHK(hipMalloc(&F.d_sq_aux, (size_t)n * sizeof(double)));
+    if (!F.d_sq_aux && n > 0)
+        return fail(AMRGPU_ERR_STATE, "level %d: the saveqc aux scratch was not planned", level);
     if (n > 0) {
         const double needs = (double)10.e33f;                  // amr_module.f90:170
         unsigned long long bits;
@@ -1119,6 +1126,17 @@ int run_saveqc_aux(int level) {
         hipError_t e = launch_fill_stage((int)g.esize, a, g.stream);
         if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "saveqc aux: %s", hipGetErrorString(e));
     }
+    return AMRGPU_OK;
+}
+
+// With aux arrays: every level lo..hi with grids has had them handed over
+// since its layout last changed (a cleared capacity would divide by zero)
+int aux_ready(int lo, int hi) {
+    if (g.auxk == 0) return AMRGPU_OK;
+    for (int l = std::max(lo, 1); l <= hi && l < MAXLV && l <= g.mxnest; l++)
+        if (g.levels[l].ngrids > 0 && !g.levels[l].aux_set)
+            return fail(AMRGPU_ERR_STATE, "level %d has no aux arrays since its layout changed "
+                                          "(amrgpu_put_level_aux)", l);
     return AMRGPU_OK;
 }
 
@@ -1266,6 +1284,7 @@ int advance_level(int level, double* dtlevnew, double* cfl_level) {
         return AMRGPU_OK;
     }
     if (!(L.possk > 0)) return fail(AMRGPU_ERR_STATE, "level %d has no time step (amrgpu_set_time)", level);
+    if ((rc = aux_ready(level - 1, level + 1))) return rc;   // it, the one it fills from, its finer
     rc = g.graphs ? graph_level_step(level) : launch_level_step(level);
     if (rc) return rc;
     if (!g.gauges.empty()) {
@@ -1299,8 +1318,9 @@ int update_level(int level) {
     if (level + 1 >= MAXLV || level + 1 > g.mxnest) return AMRGPU_OK;
     Level& F = g.levels[level + 1];
     if (C.ngrids == 0 || F.ngrids == 0) return AMRGPU_OK;
-    int rc = ensure_regs();
+    int rc = aux_ready(level, level + 1);
     if (rc) return rc;
+    if ((rc = ensure_regs())) return rc;
     UpdateArgs a{};
     a.cells = C.d_ucells;
     a.ncells = C.nucells;
@@ -2033,6 +2053,7 @@ int amrgpu_put_level_aux(int level, const double* alloc_base, const int64_t* off
             for (int m = 0; m < na; m++) dst[(int64_t)m * pd.plane + c] = src[c * na + m];
     }
     HIPCHK(h2d(L.d_aux, h.data(), h.size() * sizeof(double)));
+    L.aux_set = true;
     return AMRGPU_OK;
 }
 
@@ -2121,6 +2142,7 @@ int amrgpu_conck(int level, double* totmass) {
     Level& L = g.levels[level];
     *totmass = 0.0;
     if (L.ngrids == 0) return AMRGPU_OK;
+    if ((rc = aux_ready(level, level))) return rc;
     hipError_t e = launch_conck((int)g.esize, L.d_patches, L.ngrids, L.buf[L.cur], L.d_partial, g.stream,
                                 L.d_aux, L.d_aux ? g.cfg.mcapa : 0, g.auxk);
     if (e != hipSuccess) return fail(AMRGPU_ERR_HIP, "conck: %s", hipGetErrorString(e));

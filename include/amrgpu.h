@@ -278,7 +278,8 @@ int amrgpu_plan_fill_get(amrgpu_fill_op* ops, amrgpu_fill_buf* bufs,
  * values per cell, component fastest, ghost cells included, as setaux
  * leaves them), at 0-based element offsets offsets[k] of alloc_base.  The
  * library keeps them until the level's layout changes (set them again after
- * every amrgpu_set_level of the level).  The capacity (mcapa) scales dtdx
+ * every amrgpu_set_level of the level: amrgpu_advance_level refuses a step
+ * that would read a level's aux arrays not set since).  The capacity (mcapa) scales dtdx
  * in the sweep and the update (step2.f90:91-95, stepgrid.f:166-172), the
  * reflux (upbnd.f:104-108), the averaging (update.f:147-161) and conck
  * (conck.f:56-60); solvers that read aux get the interface's cells' values. */

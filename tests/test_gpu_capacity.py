@@ -126,3 +126,32 @@ def test_capacity_composite_mass_conserved_on_device(gpu):
             assert abs(m - m0) <= 1e-13 * abs(m0), (m, m0)
     finally:
         capi.finalize()
+
+
+def test_level_step_refuses_aux_not_handed_over(gpu):
+    """A layout change clears a level's aux arrays (zero capacities would
+    divide the update): the level step that reads them is refused until
+    amrgpu_put_level_aux hands them over again, and then runs as before."""
+    print("capacity aux handover", flush=True)
+    H, O, D = _setup(CAPA_BCS[0], 0, capi.FP64)
+    try:
+        L2 = H.levels[1]
+        capi.set_level(2, L2["boxes"], L2["hx"], L2["hy"])
+        capi._check(capi.lib().amrgpu_put_level(2, 0, L2["qnew"], L2["off"]))
+        capi._check(capi.lib().amrgpu_put_level(2, 1, L2["qold"], L2["off"]))
+        capi.set_time(2, L2["time"], L2["possk"])
+        with pytest.raises(capi.AmrgpuError, match="no aux arrays"):
+            capi.advance_level(1)
+        with pytest.raises(capi.AmrgpuError, match="no aux arrays"):
+            capi.update_level(1)
+        with pytest.raises(capi.AmrgpuError, match="no aux arrays"):
+            capi.conck(2)
+        capi.put_level_aux(2, L2["aux"], L2["auxoff"])
+        lo, ld = [], []
+        berger_oliger(O, 1, H.nlev, 2, lo)
+        berger_oliger(D, 1, H.nlev, 2, ld)
+        assert lo == ld
+        for l in range(1, H.nlev + 1):
+            assert np.array_equal(capi.read_level(H, l, 0), O.H.levels[l - 1]["qnew"]), l
+    finally:
+        capi.finalize()
