@@ -38,7 +38,7 @@
 
 namespace amrgpu {
 
-constexpr int TILE3_MAX = 13;        // cells per direction of a tile
+constexpr int TILE3_MAX = 15;        // cells per direction of a tile (15: lines of 16 lanes fill a wave)
 constexpr int TILE3_LANES = 256;     // four waves
 constexpr int TILE3_XCDS = 8;        // workgroups go to the XCDs in turn
 constexpr int TILE3_WG_PER_CU = 2;   // workgroups a CU holds (LDS and registers sized for it)

@@ -174,7 +174,7 @@ def test_flag_patches3_matches_oracle(gpu, precision):
 
 
 FUSED_SIZES = [(32, 32, 32), (1, 2, 3), (10, 10, 10), (11, 21, 9), (20, 8, 26), (3, 30, 4),
-               (13, 6, 9), (16, 12, 22)]
+               (13, 6, 9), (16, 12, 22), (15, 15, 5), (14, 9, 15)]
 
 
 @pytest.mark.parametrize("case", CASES)
