@@ -78,6 +78,32 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// idx = tid + TILE3_LANES i written as digits (x, y, z, m) of the radices
+// (nx, ny, nz): stepped by TILE3_LANES with one carry per digit instead of
+// three integer divisions per element (a digit plus its step plus a carry
+// stays below twice its radix, so one subtraction brings it back)
+struct Digits3 {
+    int x, y, z, m, sx, sy, sz, sm, nx, ny, nz;
+    __device__ __forceinline__ Digits3(int idx, int nx_, int ny_, int nz_) : nx(nx_), ny(ny_), nz(nz_) {
+        int r = idx;
+        x = r % nx; r /= nx; y = r % ny; r /= ny; z = r % nz; m = r / nz;
+        r = TILE3_LANES;
+        sx = r % nx; r /= nx; sy = r % ny; r /= ny; sz = r % nz; sm = r / nz;
+    }
+    __device__ __forceinline__ void step() {
+        x += sx;
+        int c = x >= nx;
+        x -= c ? nx : 0;
+        y += sy + c;
+        c = y >= ny;
+        y -= c ? ny : 0;
+        z += sz + c;
+        c = z >= nz;
+        z -= c ? nz : 0;
+        m += sm + c;
+    }
+};
+
 // one sweep of a tile (D: 1 x, 2 y, 3 z, a template parameter so that every
 // component index of the solver is a compile-time constant).
 //
@@ -417,11 +443,10 @@ __global__ __launch_bounds__(TILE3_LANES, TILE3_WG_PER_CU) void sweep3_tile_kern
     // array index of local cell (-2, -2, -2): 1-based cell i0 + 1 - 2 sits
     // at index i0 + 1 - 2 + 1 = i0 (two ghost cells)
     const int64_t g0 = (int64_t)td.k0 * mij + (int64_t)td.j0 * mit + td.i0;
-    for (int idx = tid; idx < NV * nbox; idx += TILE3_LANES) {
-        const int m = idx / nbox, rem = idx - m * nbox;
-        const int z = rem / (bx * by), r2 = rem - z * bx * by;
-        const int y = r2 / bx, x = r2 - y * bx;
-        qs[idx] = qin[m * plane + g0 + (int64_t)z * mij + (int64_t)y * mit + x];
+    {
+        Digits3 u(tid, bx, by, bz);
+        for (int idx = tid; idx < NV * nbox; idx += TILE3_LANES, u.step())
+            qs[idx] = qin[u.m * plane + g0 + (int64_t)u.z * mij + (int64_t)u.y * mit + u.x];
     }
     for (int idx = tid; idx < NV * ncell; idx += TILE3_LANES) dq[idx] = T(0);
     __syncthreads();
@@ -435,12 +460,12 @@ __global__ __launch_bounds__(TILE3_LANES, TILE3_WG_PER_CU) void sweep3_tile_kern
 
     // q(t + dt) = q(t) + dq on the tile's cells
     T* qout = static_cast<T*>(a.qout) + pt.qoff;
-    for (int idx = tid; idx < NV * ncell; idx += TILE3_LANES) {
-        const int m = idx / ncell, rem = idx - m * ncell;
-        const int z = rem / (tx * ty), r2 = rem - z * tx * ty;
-        const int y = r2 / tx, x = r2 - y * tx;
-        const T v = qs[m * nbox + (z + 2) * bx * by + (y + 2) * bx + (x + 2)] + dq[idx];
-        qout[m * plane + g0 + (int64_t)(z + 2) * mij + (int64_t)(y + 2) * mit + (x + 2)] = v;
+    {
+        Digits3 u(tid, tx, ty, tz);
+        for (int idx = tid; idx < NV * ncell; idx += TILE3_LANES, u.step()) {
+            const T v = qs[u.m * nbox + (u.z + 2) * bx * by + (u.y + 2) * bx + (u.x + 2)] + dq[idx];
+            qout[u.m * plane + g0 + (int64_t)(u.z + 2) * mij + (int64_t)(u.y + 2) * mit + (u.x + 2)] = v;
+        }
     }
     double cd = double(cfl);
 #pragma unroll
