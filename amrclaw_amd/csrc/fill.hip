@@ -78,6 +78,29 @@ __device__ void fill_aux_op(const FillArgs& a, const FillOp& op, int tid) {
     }
 }
 
+// t = tid + 256 k written as digits (i, j, m) of the radices (ni, nj):
+// stepped by 256 with one carry per digit instead of two integer divisions
+// per element (a digit plus its step plus a carry stays below twice its
+// radix, so one subtraction brings it back)
+struct Digits2 {
+    int i, j, m, si, sj, sm, ni, nj;
+    __device__ __forceinline__ Digits2(int t, int ni_, int nj_)
+        : ni(ni_ > 0 ? ni_ : 1), nj(nj_ > 0 ? nj_ : 1) {   // (an empty op runs no element)
+        i = t % ni; t /= ni; j = t % nj; m = t / nj;
+        int r = 256;
+        si = r % ni; r /= ni; sj = r % nj; sm = r / nj;
+    }
+    __device__ __forceinline__ void step() {
+        i += si;
+        int c = i >= ni;
+        i -= c ? ni : 0;
+        j += sj + c;
+        c = j >= nj;
+        j -= c ? nj : 0;
+        m += sm + c;
+    }
+};
+
 template <typename S>
 __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
     const FillOp op = a.ops[blockIdx.x];
@@ -100,9 +123,9 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
         else if (op.flags & FOPF_FORCE_OLD) mode = TIME_OLD;
         else if (op.flags & FOPF_BIG) mode = a.big_old ? TIME_OLD : TIME_NEW;
         const double ai = a.alphai[op.src_level], ac = a.alphac[op.src_level];
-        for (int t = tid; t < ncell * nv; t += 256) {
-            const int m = t / ncell, c = t - m * ncell;
-            const int dj = c / op.ni, di = c - dj * op.ni;
+        Digits2 u(tid, op.ni, op.nj);
+        for (int t = tid; t < ncell * nv; t += 256, u.step()) {
+            const int m = u.m, dj = u.j, di = u.i;
             const int64_t s = (int64_t)m * sp.plane + (int64_t)(op.sj0 + dj) * sp.mitot + (op.si0 + di);
             double v;
             if (mode == TIME_NEW) v = (double)qn[s];
@@ -118,9 +141,10 @@ __global__ __launch_bounds__(256) void fill_stage_kernel(FillArgs a) {
         const double ylow_coarse = a.ylower + op.jplo * dyc;
         const double ratiox = (double)op.ratx, ratioy = (double)op.raty;
         const unsigned char* mask = a.masks + op.mask_off;
-        for (int c = tid; c < ncell; c += 256) {
+        Digits2 u(tid, op.ni, 1 << 30);   // (i, j) of cell c; j does not wrap
+        for (int c = tid; c < ncell; c += 256, u.step()) {
             if (mask[c]) continue;
-            const int jf = c / op.ni + 1, if_ = c - (jf - 1) * op.ni + 1;   // 1-based
+            const int jf = u.j + 1, if_ = u.i + 1;   // 1-based
             const int j_coarse = (int)floor((double)(jf + op.jlo - 1) / ratioy) - op.jplo + 1;
             const double ycent_coarse = ylow_coarse + ((double)j_coarse - .5) * dyc;
             const double ycent_fine = a.ylower + ((double)(jf - 1 + op.jlo) + .5) * dyf;
