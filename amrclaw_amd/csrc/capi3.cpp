@@ -46,6 +46,7 @@ struct Context3 {
     // wrote, which the step never changes) and its LDS size
     int step_mode = 0;                  // 0 phase kernels, 1 fused tiles
     bool fluxes_valid = false;          // the last step kept the flux arrays
+    bool last_fused = false;            // the last step was the fused one
     std::vector<Tile3> tiles;
     Tile3* d_tiles = nullptr;
     void* d_q2 = nullptr;
@@ -122,6 +123,7 @@ void free_batch() {
     g3.tiles.clear();
     g3.tile_lds = 0;
     g3.fluxes_valid = false;
+    g3.last_fused = false;
     hipFree(g3.d_cfl);
     hipFree(g3.d_flags);
     hipFree(g3.d_foff);
@@ -521,6 +523,7 @@ int step_fused(double dx, double dy, double dz, double dt) {
     if (rc) return rc;
     std::swap(g3.d_q, g3.d_q2);
     g3.fluxes_valid = false;
+    g3.last_fused = true;
     return AMRGPU_OK;
 }
 
@@ -551,6 +554,7 @@ int step_batch(double dx, double dy, double dz, double dt, double* cflgrid, bool
     int rc = step_io(io, dx, dy, dz, dt);
     if (rc) return rc;
     g3.fluxes_valid = true;
+    g3.last_fused = false;
     if (cflgrid) {
         HIPCHK3(d2h3(g3.h_cfl.data(), g3.d_cfl, g3.npatch * sizeof(unsigned long long)));
         for (int p = 0; p < g3.npatch; p++) {
@@ -564,8 +568,11 @@ int step_batch(double dx, double dy, double dz, double dt, double* cflgrid, bool
 
 int get_fluxes(int p, double* const out[6]) {
     if (!g3.fluxes_valid)
-        return fail3(AMRGPU_ERR_STATE, "no flux arrays: the last step was the fused one, which "
-                                       "keeps every face on chip (amrgpu3_set_step_mode(0))");
+        return g3.last_fused
+                   ? fail3(AMRGPU_ERR_STATE, "no flux arrays: the last step was the fused one, which "
+                                             "keeps every face on chip (amrgpu3_set_step_mode(0))")
+                   : fail3(AMRGPU_ERR_STATE, "no flux arrays: no step has run since the patches "
+                                             "were set (amrgpu3_step_patches)");
     const Patch3& pt = g3.patches[p];
     const int nv = g3.cfg.nvar;
     const Faces3 F = faces3(pt, nv);

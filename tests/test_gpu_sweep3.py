@@ -227,8 +227,10 @@ def test_fused_step_keeps_no_flux_arrays(gpu):
     q = patch(rng, orc.EULER, 6, 5, 4)
     capi3.set_patches3([(6, 5, 4)])
     capi3.put_patch3(0, q)
+    with pytest.raises(capi3.AmrgpuError, match="no step has run"):
+        capi3.get_fluxes3(0, q.shape)
     capi3.step_patches3(0.1, 0.1, 0.1, 0.002)
-    with pytest.raises(capi3.AmrgpuError):
+    with pytest.raises(capi3.AmrgpuError, match="fused"):
         capi3.get_fluxes3(0, q.shape)
     capi3.set_step_mode3(capi3.STEP_PHASES)
     capi3.step_patches3(0.1, 0.1, 0.1, 0.002)
