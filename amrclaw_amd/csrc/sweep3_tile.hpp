@@ -165,13 +165,15 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
         const int wpr = nrows < 4 ? nrows : 4;
         const int npairs = nrows * K;
         const int nrounds = (npairs + wpr - 1) / wpr;
+        // pair = rd * wpr + wave as (row, chunk) = (pair % nrows, pair / nrows),
+        // stepped by wpr <= nrows a round (one carry)
+        int prow = wave < wpr ? wave : 0, pchunk = 0;
 #pragma unroll 1
         for (int rd = 0; rd < nrounds; rd++) {
             const int pair = rd * wpr + wave;
             const bool wlive = wave < wpr && pair < npairs;
-            const int pc = wlive ? pair : 0;
-            const int P = -1 + c3 + 3 * (pc % nrows);
-            const int Qr = -1 + (pc / nrows) * lpw + li;
+            const int P = -1 + c3 + 3 * (wlive ? prow : 0);
+            const int Qr = -1 + (wlive ? pchunk : 0) * lpw + li;
             const bool live = wlive && li < lpw && Qr <= tq;
             const int Q = live ? Qr : -1;
             const int ec = live ? e : 0;
@@ -417,6 +419,10 @@ __device__ __forceinline__ void tile3_sweep(const Tile3Args& a, float* qs, float
                 wave_sync();
             }
             __syncthreads();   // the next round's waves may touch these cells
+            prow += wpr;
+            const int carry = prow >= nrows;
+            prow -= carry ? nrows : 0;
+            pchunk += carry;
         }
     }
 }
